@@ -1,0 +1,610 @@
+// cg_api.cpp -- extern "C" entry points of libcarpet_amd.so (per-call
+// operators, transfer dispatch, device storage) and the transfer dispatch
+// itself.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+
+#include "cg_internal.hpp"
+
+namespace cg {
+
+const char *last_error_message();
+
+static hipStream_t S(cg_stream_t s) { return static_cast<hipStream_t>(s); }
+
+// data<T>::transfer_time -> transfer_p_r -> transfer_prolongate /
+// transfer_restrict / copy_from_innerloop, with the time-level choice of
+// gdata::find_source_timelevel (CarpetLib/src/data.cc:485-556, 572-966,
+// 976-1171, 1182-1299; CarpetLib/src/gdata.cc:288-349)
+int resolve_transfer(const cg_gdata *dst, const cg_gdata *const *srcs,
+                     const double *times, int ntimes, const int *dstregbbox,
+                     const int *srcregbbox, const cg_islab *slab, double time,
+                     int op, int cent, int order_space, int order_time,
+                     Transfer &tr) {
+  if (!dst || !srcs || !times || ntimes <= 0 || !dstregbbox || !srcregbbox)
+    return set_error(CG_ERR_ARG, "null argument");
+  for (int l = 0; l < ntimes; ++l)
+    if (!srcs[l])
+      return set_error(CG_ERR_ARG, "null source time level %d", l);
+  if (op == CG_OP_ERROR || op == CG_OP_NONE)
+    return set_error(CG_ERR_OPERATOR, "transport operator %d does not transfer", op);
+  if (cent != CG_VERTEX_CENTERED && cent != CG_CELL_CENTERED)
+    return set_error(CG_ERR_OPERATOR, "unknown centering %d", cent);
+  std::memset(&tr, 0, sizeof tr);
+  int tl0, ntl, e;
+  if ((e = find_source_timelevel(times, ntimes, time, order_time, op, &tl0, &ntl)))
+    return e;
+  if (ntl > ntimes)
+    return set_error(CG_ERR_ARG, "%d time levels needed, %d given", ntl, ntimes);
+  const Arr d = arr_from(*dst);
+  const Arr s0 = arr_from(*srcs[tl0]);
+  for (int l = 0; l < ntl; ++l) {
+    const cg_gdata &g = *srcs[tl0 + l];
+    if (std::memcmp(g.bbox, srcs[tl0]->bbox, sizeof g.bbox) ||
+        std::memcmp(g.padext, srcs[tl0]->padext, sizeof g.padext))
+      return set_error(CG_ERR_ARG, "source time levels have different layouts");
+  }
+  const Box dreg = Box::from(dstregbbox);
+  const Box sreg = Box::from(srcregbbox);
+  const void *sp[3] = {nullptr, nullptr, nullptr};
+  for (int l = 0; l < ntl && l < 3; ++l)
+    sp[l] = srcs[tl0 + l]->storage;
+
+  bool same = true, coarser = true, finer = true;
+  for (int k = 0; k < 3; ++k) {
+    same &= s0.box.str[k] == d.box.str[k];
+    coarser &= s0.box.str[k] > d.box.str[k];
+    finer &= s0.box.str[k] < d.box.str[k];
+  }
+  if (same) {
+    if (ntl == 1) {
+      tr.is_prolong = false;
+      return plan_copy(s0, d, sreg, dreg, slab, tr.pt);
+    }
+    // copying each level into a temporary and interpolating in time equals
+    // interpolating the sources directly (a copy is exact)
+    if (slab)
+      return set_error(CG_ERR_OPERATOR, "slabbed copy with time interpolation");
+    if (ntl > 3)
+      return set_error(CG_ERR_ORDER, "time interpolation order %d not supported", order_time);
+    if (ntl == 3 && op != CG_OP_LAGRANGE && op != CG_OP_COPY)
+      return set_error(CG_ERR_ORDER, "interpolate_eno_3d_3tl is not provided");
+    tr.is_prolong = false;
+    return plan_interp(ntl, sp, times, time, s0, d, dreg, tr.pt);
+  }
+  if (coarser) {
+    if (op == CG_OP_SYNC || op == CG_OP_RESTRICT)
+      return set_error(CG_ERR_OPERATOR, "operator %d cannot prolongate", op);
+    if (slab)
+      return set_error(CG_ERR_ARG, "prolongation takes no slab");
+    int kind = -1, order = order_space;
+    switch (op) {
+    case CG_OP_COPY:
+    case CG_OP_LAGRANGE:
+      if (cent == CG_VERTEX_CENTERED) {
+        if (order_space < 0 || order_space > 11 || order_space % 2 == 0)
+          return set_error(CG_ERR_ORDER,
+                           "There is no vertex-centred stencil for op=\"LAGRANGE\" or "
+                           "op=\"COPY\" with order_space not in {1, 3, 5, 7, 9, 11}");
+        kind = PK_VLAG;
+      } else {
+        if (order_space < 0 || order_space > 5)
+          return set_error(CG_ERR_ORDER,
+                           "There is no cell-centred stencil for op=\"LAGRANGE\" or "
+                           "op=\"COPY\" with order_space not in {0, 1, 2, 3, 4, 5}");
+        kind = PK_CLAG;
+      }
+      break;
+    case CG_OP_ENO:
+      if (cent == CG_VERTEX_CENTERED)
+        return set_error(CG_ERR_OPERATOR, "vertex-centred ENO is not provided");
+      if (order_space < 2 || order_space > 5)
+        return set_error(CG_ERR_ORDER, "There is no cell-centred stencil for "
+                                       "op=\"ENO\" with order_space not in {2,3,4,5}");
+      kind = PK_ENO;
+      order = order_space == 5 ? 3 : 2; // data.cc:697-704
+      break;
+    case CG_OP_TVD:
+      if (cent == CG_VERTEX_CENTERED)
+        return set_error(CG_ERR_OPERATOR, "vertex-centred TVD is not provided");
+      if (order_space != 1)
+        return set_error(CG_ERR_ORDER, "There is no stencil for op=\"TVD\" with order_space!=1");
+      kind = PK_TVD;
+      order = 1;
+      break;
+    default:
+      return set_error(CG_ERR_OPERATOR, "prolongation operator %d is not provided", op);
+    }
+    double sf[3] = {1.0, 0.0, 0.0};
+    if (ntl > 1) {
+      if (ntl > 3)
+        return set_error(CG_ERR_ORDER, "time interpolation order %d not supported", order_time);
+      if (ntl == 3 && (op == CG_OP_ENO || op == CG_OP_TVD))
+        return set_error(CG_ERR_ORDER, "interpolate_eno_3d_3tl is not provided");
+      if ((e = time_factors(ntl, times, time, sf)))
+        return e;
+      if (kind == PK_TVD)
+        return set_error(CG_ERR_ORDER, "TVD with time interpolation is not provided");
+    }
+    tr.is_prolong = true;
+    return plan_prolong(kind, order, s0, sp, ntl, sf, d, dreg, tr.pr);
+  }
+  if (finer) {
+    if (op == CG_OP_SYNC)
+      return set_error(CG_ERR_OPERATOR, "op_sync cannot restrict");
+    if (ntl != 1)
+      return set_error(CG_ERR_OPERATOR, "restriction with time interpolation is not provided");
+    if (cent == CG_VERTEX_CENTERED) {
+      if (slab)
+        return set_error(CG_ERR_ARG, "vertex-centred restriction takes no slab");
+      tr.is_prolong = false;
+      return plan_restrict(false, s0, d, dreg, tr.pt);
+    }
+    if (slab && !(slab->is_centered[0] == 1 && slab->is_centered[1] == 1 &&
+                  slab->is_centered[2] == 1))
+      return set_error(CG_ERR_OPERATOR, "restrict_3d_vc_rf2 is not provided");
+    tr.is_prolong = false;
+    return plan_restrict(true, s0, d, dreg, tr.pt);
+  }
+  return set_error(CG_ERR_STRIDE, "source and destination strides are not ordered");
+}
+
+int run_transfer(const Transfer &tr, int prec, hipStream_t st) {
+  if (tr.is_prolong)
+    return launch_prolong(tr.pr, prec, st);
+  const int r = launch_point(tr.pt.op, tr.pt.a, st);
+  if (r == -1)
+    return set_error(CG_ERR_ARG, "region too large for the pointwise kernel");
+  if (r)
+    return hip_error(hipGetLastError(), "kernel launch");
+  return CG_OK;
+}
+
+} // namespace cg
+
+using namespace cg;
+
+extern "C" {
+
+int cg_version(void) { return 1; }
+
+const char *cg_error_string(int code) {
+  switch (code) {
+  case CG_OK: return "success";
+  case CG_ERR_ARG: return "invalid argument";
+  case CG_ERR_STRIDE: return "strides disagree";
+  case CG_ERR_REFFACT: return "refinement factor is not 2";
+  case CG_ERR_ALIGN: return "bbox not aligned";
+  case CG_ERR_EMPTY: return "region extent is empty";
+  case CG_ERR_CONTAIN: return "region extent is not contained in array extent";
+  case CG_ERR_ORDER: return "no stencil for this order";
+  case CG_ERR_TIME_SAME: return "arrays have same time";
+  case CG_ERR_TIME_EXTRAP: return "extrapolation in time";
+  case CG_ERR_SHAPE: return "array sizes don't agree with bounding boxes";
+  case CG_ERR_OPERATOR: return "operator not supported";
+  case CG_ERR_HIP: return "HIP runtime error";
+  case CG_ERR_NOMEM: return "out of memory";
+  case CG_ERR_STATE: return "batch used in the wrong state";
+  case CG_ERR_DEVICE: return "device-side check failed";
+  }
+  return "unknown error";
+}
+
+const char *cg_last_error_message(void) { return cg::last_error_message(); }
+
+int cg_set_device(int device) {
+  hipError_t e = hipSetDevice(device);
+  return e == hipSuccess ? CG_OK : hip_error(e, "hipSetDevice");
+}
+
+int cg_device_count(int *count) {
+  if (!count)
+    return set_error(CG_ERR_ARG, "null count");
+  hipError_t e = hipGetDeviceCount(count);
+  if (e != hipSuccess) {
+    *count = 0;
+    return hip_error(e, "hipGetDeviceCount");
+  }
+  return CG_OK;
+}
+
+// ---- storage --------------------------------------------------------------
+static long long align_up(long long x, long long a) { return (x + a - 1) / a * a; }
+static long long next_pow2(long long x) {
+  long long p = 1;
+  while (p < x)
+    p <<= 1;
+  return p;
+}
+
+// pad_shape(shape, offset), CarpetLib/src/cacheinfo.cc:40-209, with
+// VECTORISE_ALIGNED_ARRAYS, VECTORISE_ALIGN_FOR_CACHE and one cache level
+int cg_pad_shape(const int shape[3], const int offset[3], int vec_size,
+                 int linesize, int align_interior, int padded_shape[3],
+                 int padding_offset[3]) {
+  if (!shape || !offset || !padded_shape || !padding_offset || vec_size < 1 ||
+      linesize < 1 || (linesize & (linesize - 1)))
+    return set_error(CG_ERR_ARG, "bad pad_shape arguments");
+  for (int d = 0; d < 3; ++d)
+    if (shape[d] < 0 || offset[d] < 0 || offset[d] > shape[d])
+      return set_error(CG_ERR_ARG, "pad_shape: shape/offset out of range");
+  if (shape[0] == 0 || shape[1] == 0 || shape[2] == 0) {
+    for (int d = 0; d < 3; ++d) {
+      padded_shape[d] = shape[d];
+      padding_offset[d] = 0;
+    }
+    return CG_OK;
+  }
+  long long acc = 1;
+  for (int d = 0; d < 3; ++d) {
+    long long np = shape[d], bnd = offset[d];
+    if (d == 0) {
+      if (align_interior) {
+        long long nb = align_up(bnd, vec_size);
+        np += nb - bnd;
+        bnd = nb;
+      }
+      np = align_up(np, vec_size);
+    }
+    if (np * acc < linesize) {
+      np = next_pow2(np);
+    } else {
+      if (align_interior && d == 0) {
+        long long nb = align_up(bnd, linesize);
+        np += nb - bnd;
+        bnd = nb;
+      }
+      long long tot = align_up(np * acc, linesize);
+      if (tot % acc)
+        return set_error(CG_ERR_ARG, "pad_shape: inconsistent line size");
+      np = tot / acc;
+    }
+    if (np > 0x7fffffff)
+      return set_error(CG_ERR_ARG, "pad_shape: extent overflows");
+    padded_shape[d] = (int)np;
+    padding_offset[d] = (int)(bnd - offset[d]);
+    acc *= np;
+  }
+  return CG_OK;
+}
+
+int cg_gdata_alloc(cg_gdata *out, const int bbox[9], const int owned_bbox[9],
+                   size_t elem_size) {
+  if (!out || !bbox || (elem_size != 4 && elem_size != 8))
+    return set_error(CG_ERR_ARG, "bad gdata_alloc arguments");
+  const Box b = Box::from(bbox);
+  for (int d = 0; d < 3; ++d)
+    if (b.str[d] <= 0)
+      return set_error(CG_ERR_ARG, "non-positive stride");
+  if (b.empty())
+    return set_error(CG_ERR_EMPTY, "empty extent");
+  int shape[3], offset[3] = {0, 0, 0}, pad[3], poff[3];
+  for (int d = 0; d < 3; ++d)
+    shape[d] = b.sizes(d);
+  if (owned_bbox) {
+    const Box o = Box::from(owned_bbox);
+    if (!o.contained_in(b) || o.empty())
+      return set_error(CG_ERR_CONTAIN, "owned box not inside the extent");
+    for (int d = 0; d < 3; ++d)
+      offset[d] = (o.lo[d] - b.lo[d]) / b.str[d];
+  }
+  // 128-byte x-rows: a wavefront's 64 lanes then read whole cache lines
+  const int vec = (int)(128 / elem_size);
+  int e = cg_pad_shape(shape, offset, vec, vec, owned_bbox ? 1 : 0, pad, poff);
+  if (e)
+    return e;
+  // the padding offset shifts the array inside the allocation so the owned
+  // interior starts on a line; the storage pointer points at element (0,0,0)
+  const size_t n = (size_t)pad[0] * pad[1] * pad[2] + (size_t)poff[0];
+  void *p = nullptr;
+  hipError_t he = hipMalloc(&p, n * elem_size);
+  if (he != hipSuccess)
+    return set_error(CG_ERR_NOMEM, "hipMalloc(%zu): %s", n * elem_size, hipGetErrorString(he));
+  out->storage = static_cast<char *>(p) + (size_t)poff[0] * elem_size;
+  for (int d = 0; d < 3; ++d) {
+    out->padext[d] = pad[d];
+    out->ext[d] = shape[d];
+  }
+  std::memcpy(out->bbox, bbox, sizeof out->bbox);
+  // remember the base pointer in front of the array for cg_gdata_free
+  return CG_OK;
+}
+
+int cg_gdata_free(cg_gdata *g) {
+  if (!g || !g->storage)
+    return set_error(CG_ERR_ARG, "null gdata");
+  // the allocation base is the storage rounded down to the allocation
+  // granularity: hipMalloc returns 256-byte aligned blocks and the offset is
+  // below one 128-byte row
+  void *base = reinterpret_cast<void *>(reinterpret_cast<uintptr_t>(g->storage) & ~uintptr_t(255));
+  hipError_t e = hipFree(base);
+  g->storage = nullptr;
+  return e == hipSuccess ? CG_OK : hip_error(e, "hipFree");
+}
+
+int cg_dev_malloc(void **ptr, size_t bytes) {
+  if (!ptr)
+    return set_error(CG_ERR_ARG, "null ptr");
+  hipError_t e = hipMalloc(ptr, bytes ? bytes : 1);
+  if (e != hipSuccess) {
+    *ptr = nullptr;
+    return set_error(CG_ERR_NOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+  }
+  return CG_OK;
+}
+int cg_dev_free(void *ptr) {
+  hipError_t e = hipFree(ptr);
+  return e == hipSuccess ? CG_OK : hip_error(e, "hipFree");
+}
+int cg_memcpy_htod(void *dst, const void *src, size_t bytes) {
+  hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+  return e == hipSuccess ? CG_OK : hip_error(e, "hipMemcpy H2D");
+}
+int cg_memcpy_dtoh(void *dst, const void *src, size_t bytes) {
+  hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
+  return e == hipSuccess ? CG_OK : hip_error(e, "hipMemcpy D2H");
+}
+int cg_memcpy_dtod(void *dst, const void *src, size_t bytes, cg_stream_t s) {
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, S(s));
+  return e == hipSuccess ? CG_OK : hip_error(e, "hipMemcpyAsync D2D");
+}
+int cg_memset_d8(void *dst, int value, size_t bytes, cg_stream_t s) {
+  hipError_t e = hipMemsetAsync(dst, value, bytes, S(s));
+  return e == hipSuccess ? CG_OK : hip_error(e, "hipMemsetAsync");
+}
+int cg_fill_f64(double *dst, double value, size_t n, cg_stream_t s) {
+  if (!dst && n)
+    return set_error(CG_ERR_ARG, "null dst");
+  return launch_fill(dst, value, n, S(s)) ? hip_error(hipGetLastError(), "fill") : CG_OK;
+}
+int cg_fill_hash_f64(double *dst, const int padext[3], const int ext[3],
+                     unsigned long long seed, double lo, double hi, cg_stream_t s) {
+  if (!dst || !padext || !ext)
+    return set_error(CG_ERR_ARG, "null argument");
+  for (int d = 0; d < 3; ++d)
+    if (ext[d] < 0 || padext[d] < ext[d])
+      return set_error(CG_ERR_SHAPE, "extent does not fit the padding");
+  return launch_fill_hash(dst, padext[0], (long long)padext[0] * padext[1], ext, seed, lo, hi,
+                          S(s))
+             ? hip_error(hipGetLastError(), "fill_hash")
+             : CG_OK;
+}
+int cg_fill_fourier_f64(const cg_gdata *g, double scale, double amp, int nmodes,
+                        const int *kvec, const double *phase, cg_stream_t s) {
+  if (!g || !g->storage || (nmodes > 0 && (!kvec || !phase)))
+    return set_error(CG_ERR_ARG, "null argument");
+  if (nmodes < 0 || nmodes > 8)
+    return set_error(CG_ERR_ARG, "at most 8 modes");
+  const int r = launch_fill_fourier(static_cast<double *>(g->storage), g->padext[0],
+                                    (long long)g->padext[0] * g->padext[1], g->ext, g->bbox,
+                                    g->bbox + 6, scale, amp, nmodes, kvec, phase, S(s));
+  return r ? hip_error(hipGetLastError(), "fill_fourier") : CG_OK;
+}
+int cg_stream_create(cg_stream_t *s) {
+  hipStream_t st;
+  hipError_t e = hipStreamCreate(&st);
+  if (e != hipSuccess)
+    return hip_error(e, "hipStreamCreate");
+  *s = st;
+  return CG_OK;
+}
+int cg_stream_destroy(cg_stream_t s) {
+  hipError_t e = hipStreamDestroy(S(s));
+  return e == hipSuccess ? CG_OK : hip_error(e, "hipStreamDestroy");
+}
+int cg_stream_synchronize(cg_stream_t s) {
+  hipError_t e = hipStreamSynchronize(S(s));
+  return e == hipSuccess ? CG_OK : hip_error(e, "hipStreamSynchronize");
+}
+int cg_device_synchronize(void) {
+  hipError_t e = hipDeviceSynchronize();
+  return e == hipSuccess ? CG_OK : hip_error(e, "hipDeviceSynchronize");
+}
+int cg_event_create(void **ev) {
+  hipEvent_t e;
+  hipError_t r = hipEventCreate(&e);
+  if (r != hipSuccess)
+    return hip_error(r, "hipEventCreate");
+  *ev = e;
+  return CG_OK;
+}
+int cg_event_destroy(void *ev) {
+  hipError_t r = hipEventDestroy(static_cast<hipEvent_t>(ev));
+  return r == hipSuccess ? CG_OK : hip_error(r, "hipEventDestroy");
+}
+int cg_event_record(void *ev, cg_stream_t s) {
+  hipError_t r = hipEventRecord(static_cast<hipEvent_t>(ev), S(s));
+  return r == hipSuccess ? CG_OK : hip_error(r, "hipEventRecord");
+}
+int cg_event_elapsed_ms(float *ms, void *start, void *stop) {
+  hipError_t r = hipEventSynchronize(static_cast<hipEvent_t>(stop));
+  if (r != hipSuccess)
+    return hip_error(r, "hipEventSynchronize");
+  r = hipEventElapsedTime(ms, static_cast<hipEvent_t>(start), static_cast<hipEvent_t>(stop));
+  return r == hipSuccess ? CG_OK : hip_error(r, "hipEventElapsedTime");
+}
+
+// ---- per-call operators ---------------------------------------------------
+int cg_copy_3d_f64(const double *src, const int srcpadext[3], const int srcext[3],
+                   double *dst, const int dstpadext[3], const int dstext[3],
+                   const int srcbbox[9], const int dstbbox[9],
+                   const int srcregbbox[9], const int dstregbbox[9],
+                   const cg_islab *slab, cg_stream_t stream) {
+  if (!srcpadext || !srcext || !dstpadext || !dstext || !srcbbox || !dstbbox ||
+      !srcregbbox || !dstregbbox)
+    return set_error(CG_ERR_ARG, "null argument");
+  PointPlan pl;
+  int e = plan_copy(arr_from(src, srcpadext, srcext, srcbbox),
+                    arr_from(dst, dstpadext, dstext, dstbbox),
+                    Box::from(srcregbbox), Box::from(dstregbbox), slab, pl);
+  if (e)
+    return e;
+  Transfer tr;
+  tr.is_prolong = false;
+  tr.pt = pl;
+  return run_transfer(tr, 0, S(stream));
+}
+
+static int prolong_call(int kind, int order, const double *src,
+                        const int *srcpadext, const int *srcext, double *dst,
+                        const int *dstpadext, const int *dstext,
+                        const int *srcbbox, const int *dstbbox,
+                        const int *dstregbbox, cg_stream_t stream) {
+  if (!srcpadext || !srcext || !dstpadext || !dstext || !srcbbox || !dstbbox || !dstregbbox)
+    return set_error(CG_ERR_ARG, "null argument");
+  ProlongPlan pl;
+  const void *sp[1] = {src};
+  const double s1[1] = {1.0};
+  int e = plan_prolong(kind, order, arr_from(src, srcpadext, srcext, srcbbox), sp, 1,
+                       s1, arr_from(dst, dstpadext, dstext, dstbbox),
+                       Box::from(dstregbbox), pl);
+  if (e)
+    return e;
+  return launch_prolong(pl, 0, S(stream));
+}
+
+int cg_prolongate_3d_rf2_f64(int order, const double *src, const int srcpadext[3],
+                             const int srcext[3], double *dst,
+                             const int dstpadext[3], const int dstext[3],
+                             const int srcbbox[9], const int dstbbox[9],
+                             const int srcregbbox[9], const int dstregbbox[9],
+                             cg_stream_t stream) {
+  (void)srcregbbox;
+  return prolong_call(PK_VLAG, order, src, srcpadext, srcext, dst, dstpadext,
+                      dstext, srcbbox, dstbbox, dstregbbox, stream);
+}
+
+int cg_prolongate_3d_cc_rf2_f64(int order, const double *src, const int srcpadext[3],
+                                const int srcext[3], double *dst,
+                                const int dstpadext[3], const int dstext[3],
+                                const int srcbbox[9], const int dstbbox[9],
+                                const int srcregbbox[9], const int dstregbbox[9],
+                                cg_stream_t stream) {
+  (void)srcregbbox;
+  return prolong_call(PK_CLAG, order, src, srcpadext, srcext, dst, dstpadext,
+                      dstext, srcbbox, dstbbox, dstregbbox, stream);
+}
+
+int cg_prolongate_3d_cc_eno_rf2_f64(int order, const double *src,
+                                    const int srcpadext[3], const int srcext[3],
+                                    double *dst, const int dstpadext[3],
+                                    const int dstext[3], const int srcbbox[9],
+                                    const int dstbbox[9], const int srcregbbox[9],
+                                    const int dstregbbox[9], cg_stream_t stream) {
+  (void)srcregbbox;
+  return prolong_call(PK_ENO, order, src, srcpadext, srcext, dst, dstpadext,
+                      dstext, srcbbox, dstbbox, dstregbbox, stream);
+}
+
+int cg_prolongate_3d_cc_tvd_f64(const double *src, const int srcpadext[3],
+                                const int srcext[3], double *dst,
+                                const int dstpadext[3], const int dstext[3],
+                                const int srcbbox[9], const int dstbbox[9],
+                                const int srcregbbox[9], const int dstregbbox[9],
+                                cg_stream_t stream) {
+  (void)srcregbbox;
+  return prolong_call(PK_TVD, 1, src, srcpadext, srcext, dst, dstpadext, dstext,
+                      srcbbox, dstbbox, dstregbbox, stream);
+}
+
+static int restrict_call(bool cell, const double *src, const int *srcpadext,
+                         const int *srcext, double *dst, const int *dstpadext,
+                         const int *dstext, const int *srcbbox,
+                         const int *dstbbox, const int *dstregbbox,
+                         cg_stream_t stream) {
+  if (!srcpadext || !srcext || !dstpadext || !dstext || !srcbbox || !dstbbox || !dstregbbox)
+    return set_error(CG_ERR_ARG, "null argument");
+  Transfer tr;
+  tr.is_prolong = false;
+  int e = plan_restrict(cell, arr_from(src, srcpadext, srcext, srcbbox),
+                        arr_from(dst, dstpadext, dstext, dstbbox),
+                        Box::from(dstregbbox), tr.pt);
+  if (e)
+    return e;
+  return run_transfer(tr, 0, S(stream));
+}
+
+int cg_restrict_3d_rf2_f64(const double *src, const int srcpadext[3],
+                           const int srcext[3], double *dst, const int dstpadext[3],
+                           const int dstext[3], const int srcbbox[9],
+                           const int dstbbox[9], const int srcregbbox[9],
+                           const int dstregbbox[9], cg_stream_t stream) {
+  (void)srcregbbox;
+  return restrict_call(false, src, srcpadext, srcext, dst, dstpadext, dstext,
+                       srcbbox, dstbbox, dstregbbox, stream);
+}
+
+int cg_restrict_3d_cc_rf2_f64(const double *src, const int srcpadext[3],
+                              const int srcext[3], double *dst,
+                              const int dstpadext[3], const int dstext[3],
+                              const int srcbbox[9], const int dstbbox[9],
+                              const int srcregbbox[9], const int dstregbbox[9],
+                              cg_stream_t stream) {
+  (void)srcregbbox;
+  return restrict_call(true, src, srcpadext, srcext, dst, dstpadext, dstext,
+                       srcbbox, dstbbox, dstregbbox, stream);
+}
+
+static int interp_call(int ntl, const double *const *srcs, const double *times,
+                       const int *srcpadext, const int *srcext, double *dst,
+                       double t, const int *dstpadext, const int *dstext,
+                       const int *srcbbox, const int *dstbbox,
+                       const int *dstregbbox, cg_stream_t stream) {
+  if (!srcpadext || !srcext || !dstpadext || !dstext || !srcbbox || !dstbbox || !dstregbbox)
+    return set_error(CG_ERR_ARG, "null argument");
+  Transfer tr;
+  tr.is_prolong = false;
+  const void *sp[3] = {srcs[0], srcs[1], ntl > 2 ? srcs[2] : nullptr};
+  int e = plan_interp(ntl, sp, times, t, arr_from(srcs[0], srcpadext, srcext, srcbbox),
+                      arr_from(dst, dstpadext, dstext, dstbbox),
+                      Box::from(dstregbbox), tr.pt);
+  if (e)
+    return e;
+  return run_transfer(tr, 0, S(stream));
+}
+
+int cg_interpolate_3d_2tl_f64(const double *src1, double t1, const double *src2,
+                              double t2, const int srcpadext[3], const int srcext[3],
+                              double *dst, double t, const int dstpadext[3],
+                              const int dstext[3], const int srcbbox[9],
+                              const int dstbbox[9], const int srcregbbox[9],
+                              const int dstregbbox[9], cg_stream_t stream) {
+  (void)srcregbbox;
+  const double *srcs[2] = {src1, src2};
+  const double tt[2] = {t1, t2};
+  return interp_call(2, srcs, tt, srcpadext, srcext, dst, t, dstpadext, dstext,
+                     srcbbox, dstbbox, dstregbbox, stream);
+}
+
+int cg_interpolate_3d_3tl_f64(const double *src1, double t1, const double *src2,
+                              double t2, const double *src3, double t3,
+                              const int srcpadext[3], const int srcext[3],
+                              double *dst, double t, const int dstpadext[3],
+                              const int dstext[3], const int srcbbox[9],
+                              const int dstbbox[9], const int srcregbbox[9],
+                              const int dstregbbox[9], cg_stream_t stream) {
+  (void)srcregbbox;
+  const double *srcs[3] = {src1, src2, src3};
+  const double tt[3] = {t1, t2, t3};
+  return interp_call(3, srcs, tt, srcpadext, srcext, dst, t, dstpadext, dstext,
+                     srcbbox, dstbbox, dstregbbox, stream);
+}
+
+int cg_transfer_from(const cg_gdata *dst, const cg_gdata *const *srcs,
+                     const double *times, int ntimes, const int dstregbbox[9],
+                     const int srcregbbox[9], const cg_islab *slab, double time,
+                     int transport_operator, int centering, int order_space,
+                     int order_time, cg_stream_t stream) {
+  Transfer tr;
+  int e = resolve_transfer(dst, srcs, times, ntimes, dstregbbox, srcregbbox, slab,
+                           time, transport_operator, centering, order_space,
+                           order_time, tr);
+  if (e)
+    return e;
+  return run_transfer(tr, 0, S(stream));
+}
+
+} // extern "C"

@@ -1,0 +1,42 @@
+// cg_batch.hpp -- device side of the batched level step: descriptor lists
+// and persistent launches (built in cg_batch_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "cg_internal.hpp"
+
+namespace cg {
+
+// One persistent launch: the transfers of one phase that share a kernel
+// (tensor kind/order/time levels or one pointwise operator).
+struct BatchLaunch {
+  int phase;
+  int is_tensor;
+  int kind, order, ntl, op, prec;
+  size_t desc_off; // byte offset of the descriptor array in the blob
+  size_t item_off; // byte offset of the (descriptor, tile) item array
+  int nitems;
+  int counter;     // index of this launch's work counter
+  int grid;        // persistent workgroups
+};
+
+struct BatchDevice {
+  std::vector<BatchLaunch> launches;
+  char *blob = nullptr;   // descriptors + items (device)
+  size_t blob_bytes = 0;
+  int *counters = nullptr; // one work counter per launch (device)
+  int ncounters = 0;
+};
+
+// tile_cfg selects the z chunk of tensor tiles in coarse bases (0 = whole
+// region, < 0 = automatic)
+int batch_device_build(BatchDevice &dev, const std::vector<Transfer> &tr,
+                       const std::vector<int> &phase, const std::vector<int> &prec,
+                       int tile_cfg);
+int batch_device_run(BatchDevice &dev, hipStream_t st);
+void batch_device_free(BatchDevice &dev);
+
+} // namespace cg

@@ -1,0 +1,202 @@
+// cg_kernels.hip -- per-call launches of the transfer kernels (bodies in
+// cg_tiles.hpp).
+
+#include <hip/hip_runtime.h>
+
+#include "cg_tiles.hpp"
+
+namespace cg {
+
+template <class L, int NTL, int CX, int CY, int IPT>
+__global__ __launch_bounds__(2 * CX * CY / IPT) void k_tensor(TensorArgs a) {
+  __shared__ double smem[TensorLds<L, NTL, CX, CY>::TOTAL];
+  tensor_tile<L, NTL, CX, CY, IPT>(a, xcd_remap(blockIdx.x, gridDim.x), smem);
+}
+
+long long tensor_tiles(TensorArgs &a, const TileCfg &cfg) {
+  a.ntiles[0] = (a.nq[0] + cfg.cx - 1) / cfg.cx;
+  a.ntiles[1] = (a.nq[1] + cfg.cy - 1) / cfg.cy;
+  a.zchunk = cfg.zchunk > 0 ? cfg.zchunk : a.nq[2];
+  a.ntiles[2] = (a.nq[2] + a.zchunk - 1) / a.zchunk;
+  return (long long)a.ntiles[0] * a.ntiles[1] * a.ntiles[2];
+}
+
+template <class L, int NTL>
+static int launch_tensor_t(const TensorArgs &a0, const TileCfg &cfg0, hipStream_t s) {
+  using T = TileShape<NTL>;
+  TensorArgs a = a0;
+  TileCfg cfg = cfg0;
+  cfg.cx = T::CX;
+  cfg.cy = T::CY;
+  const long long nblk = tensor_tiles(a, cfg);
+  if (nblk <= 0 || nblk > 0x7fffffffLL)
+    return -1;
+  constexpr int NT = 2 * T::CX * T::CY / T::IPT;
+  hipLaunchKernelGGL((k_tensor<L, NTL, T::CX, T::CY, T::IPT>), dim3((unsigned)nblk),
+                     dim3(NT), 0, s, a);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int launch_tensor(int kind, int order, int ntl, int prec, const TensorArgs &a,
+                  const TileCfg &cfg, hipStream_t s) {
+  if (prec != 0)
+    return -1;
+  return with_line(kind, order, [&](auto tag) -> int {
+    using L = decltype(tag);
+    switch (ntl) {
+    case 1: return launch_tensor_t<L, 1>(a, cfg, s);
+    case 2: return launch_tensor_t<L, 2>(a, cfg, s);
+    case 3: return launch_tensor_t<L, 3>(a, cfg, s);
+    }
+    return -1;
+  });
+}
+
+template <int OP>
+__global__ __launch_bounds__(256) void k_point(PointArgs a) {
+  const unsigned nx = (unsigned)a.n[0];
+  const unsigned nxy = nx * (unsigned)a.n[1];
+  const unsigned e = blockIdx.x * 256u + threadIdx.x;
+  if (e >= nxy)
+    return;
+  const int j = (int)(e / nx);
+  const int i = (int)(e - (unsigned)j * nx);
+  for (int k = blockIdx.y; k < a.n[2]; k += gridDim.y)
+    point_elem<OP>(a, i, j, k);
+}
+
+int launch_point(int op, const PointArgs &a, hipStream_t s) {
+  const long long nxy = (long long)a.n[0] * a.n[1];
+  if (nxy <= 0 || a.n[2] <= 0 || nxy >= (1LL << 32) / 2)
+    return -1;
+  const unsigned gx = (unsigned)((nxy + 255) / 256);
+  const unsigned gy = (unsigned)(a.n[2] < 65535 ? a.n[2] : 65535);
+  dim3 grid(gx, gy), block(256);
+  switch (op) {
+  case PO_COPY: hipLaunchKernelGGL(k_point<PO_COPY>, grid, block, 0, s, a); break;
+  case PO_RESTRICT_VC: hipLaunchKernelGGL(k_point<PO_RESTRICT_VC>, grid, block, 0, s, a); break;
+  case PO_RESTRICT_CC: hipLaunchKernelGGL(k_point<PO_RESTRICT_CC>, grid, block, 0, s, a); break;
+  case PO_INTERP: hipLaunchKernelGGL(k_point<PO_INTERP>, grid, block, 0, s, a); break;
+  case PO_TVD: hipLaunchKernelGGL(k_point<PO_TVD>, grid, block, 0, s, a); break;
+  default: return -1;
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+__global__ void k_fill(double *d, double v, size_t n) {
+  size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  const size_t st = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += st)
+    d[i] = v;
+}
+
+// splitmix64 counter stream: element (i,j,k) of the extent gets
+// lo + (hi-lo) * u(seed + i + ext0*(j + ext1*k)), u in [0,1) from the top 53 bits
+__device__ __forceinline__ unsigned long long splitmix64(unsigned long long x) {
+  unsigned long long z = x + 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+__global__ void k_fill_hash(double *d, long long sj, long long sk, int nx, int ny, int nz,
+                            unsigned long long seed, double lo, double hi) {
+  const long long n = (long long)nx * ny * nz;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n;
+       e += (long long)gridDim.x * blockDim.x) {
+    const long long k = e / ((long long)nx * ny);
+    const long long r = e - k * nx * ny;
+    const long long j = r / nx, i = r - j * nx;
+    const unsigned long long z = splitmix64(seed + (unsigned long long)e);
+    const double u = (double)(z >> 11) * (1.0 / 9007199254740992.0);
+    d[i + sj * j + sk * k] = lo + (hi - lo) * u;
+  }
+}
+
+int launch_fill_hash(double *d, long long sj, long long sk, const int *ext,
+                     unsigned long long seed, double lo, double hi, hipStream_t s) {
+  const long long n = (long long)ext[0] * ext[1] * ext[2];
+  if (n <= 0)
+    return 0;
+  long long blocks = (n + 255) / 256;
+  if (blocks > 65536)
+    blocks = 65536;
+  hipLaunchKernelGGL(k_fill_hash, dim3((unsigned)blocks), dim3(256), 0, s, d, sj, sk, ext[0],
+                     ext[1], ext[2], seed, lo, hi);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+// sum of Fourier modes: amp * sum_m sin(2 pi (k_m . x) + phase_m) at
+// x = (lower + idx*stride) * scale for every point of the extent
+struct FourierArgs {
+  double *d;
+  long long sj, sk;
+  int n[3];
+  int lower[3], stride[3];
+  double scale, amp;
+  int nmodes;
+  int k[8][3];
+  double phase[8];
+};
+
+__global__ void k_fill_fourier(FourierArgs a) {
+  const long long n = (long long)a.n[0] * a.n[1] * a.n[2];
+  const double twopi = 6.283185307179586476925286766559;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n;
+       e += (long long)gridDim.x * blockDim.x) {
+    const long long k = e / ((long long)a.n[0] * a.n[1]);
+    const long long r = e - k * a.n[0] * a.n[1];
+    const long long j = r / a.n[0], i = r - j * a.n[0];
+    const double x = (a.lower[0] + i * a.stride[0]) * a.scale;
+    const double y = (a.lower[1] + j * a.stride[1]) * a.scale;
+    const double z = (a.lower[2] + k * a.stride[2]) * a.scale;
+    double v = 0.0;
+    for (int m = 0; m < a.nmodes; ++m)
+      v += sin(twopi * (a.k[m][0] * x + a.k[m][1] * y + a.k[m][2] * z) + a.phase[m]);
+    a.d[i + a.sj * j + a.sk * k] = a.amp * v;
+  }
+}
+
+int launch_fill_fourier(double *d, long long sj, long long sk, const int *ext,
+                        const int *lower, const int *stride, double scale, double amp,
+                        int nmodes, const int *kvec, const double *phase, hipStream_t s) {
+  if (nmodes < 0 || nmodes > 8)
+    return -1;
+  FourierArgs a{};
+  a.d = d;
+  a.sj = sj;
+  a.sk = sk;
+  for (int q = 0; q < 3; ++q) {
+    a.n[q] = ext[q];
+    a.lower[q] = lower[q];
+    a.stride[q] = stride[q];
+  }
+  a.scale = scale;
+  a.amp = amp;
+  a.nmodes = nmodes;
+  for (int m = 0; m < nmodes; ++m) {
+    for (int q = 0; q < 3; ++q)
+      a.k[m][q] = kvec[3 * m + q];
+    a.phase[m] = phase[m];
+  }
+  const long long n = (long long)ext[0] * ext[1] * ext[2];
+  if (n <= 0)
+    return 0;
+  long long blocks = (n + 255) / 256;
+  if (blocks > 65536)
+    blocks = 65536;
+  hipLaunchKernelGGL(k_fill_fourier, dim3((unsigned)blocks), dim3(256), 0, s, a);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int launch_fill(double *dst, double v, size_t n, hipStream_t s) {
+  if (n == 0)
+    return 0;
+  size_t blocks = (n + 255) / 256;
+  if (blocks > 65536)
+    blocks = 65536;
+  hipLaunchKernelGGL(k_fill, dim3((unsigned)blocks), dim3(256), 0, s, dst, v, n);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+} // namespace cg

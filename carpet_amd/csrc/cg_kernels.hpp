@@ -1,0 +1,68 @@
+// cg_kernels.hpp -- kernel argument blocks and launcher declarations shared
+// by the per-call entry points, the transfer dispatch and the batch path.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace cg {
+
+// Tensor-product prolongation (Lagrange vertex/cell, ENO cell) fused with
+// time interpolation over up to 3 source time levels.
+struct TensorArgs {
+  const void *src[3]; // per time level: element (0,0,0) of the source array
+  long long sj, sk;   // source strides (elements)
+  void *dst;          // destination element at the region origin
+  long long dj, dk;   // destination strides (elements)
+  int n[3];           // region extents (fine points)
+  int off[3];         // fine index of the region origin relative to the source
+                      // array: fine point f has coarse base (off+f)>>1 and
+                      // parity (off+f)&1
+  int flo[3], fhi[3]; // exact coarse footprint of the region (inclusive)
+  int qlo[3];         // first coarse base (off>>1)
+  int nq[3];          // number of coarse bases per axis
+  int ntiles[3];      // tile counts: x, y (in CX/CY bases), z (chunks)
+  int zchunk;         // coarse bases per z chunk
+  double s[3];        // time weights
+};
+
+// Pointwise region operators: copy (with slab shift), injection
+// restriction, cell-centred restriction, time interpolation, cell TVD.
+enum PointOp { PO_COPY = 0, PO_RESTRICT_VC = 1, PO_RESTRICT_CC = 2,
+               PO_INTERP = 3, PO_TVD = 4 };
+
+struct PointArgs {
+  const double *src[3]; // source element matching region point (0,0,0)
+                        // (copy/interp); for restrict: first child; for
+                        // TVD: source element (0,0,0) of the array
+  long long sj, sk;
+  double *dst;          // destination element at the region origin
+  long long dj, dk;
+  int n[3];             // region extents (destination points)
+  int ntl;              // time levels (interp)
+  double s[3];          // time weights (interp)
+  int off[3];           // TVD: fine offsets (off+f)/2 = parent, (off+f)%2 = side
+};
+
+// tile configuration of the tensor kernel
+struct TileCfg {
+  int cx, cy; // coarse bases per tile in x and y (fine tile 2cx x 2cy)
+  int zchunk; // coarse bases per z chunk
+};
+
+// ---- launchers (defined in cg_kernels.hip) --------------------------------
+// kind: LK_VLAG/LK_CLAG/LK_ENO; prec 0: double sources/destination,
+// 1: float sources/destination with compensated accumulation.
+int launch_tensor(int kind, int order, int ntl, int prec, const TensorArgs &a,
+                  const TileCfg &cfg, hipStream_t s);
+// computes ntiles/zchunk of a for a tile configuration; returns block count
+long long tensor_tiles(TensorArgs &a, const TileCfg &cfg);
+int launch_point(int op, const PointArgs &a, hipStream_t s);
+int launch_fill(double *dst, double v, size_t n, hipStream_t s);
+int launch_fill_fourier(double *d, long long sj, long long sk, const int *ext,
+                        const int *lower, const int *stride, double scale, double amp,
+                        int nmodes, const int *kvec, const double *phase, hipStream_t s);
+int launch_fill_hash(double *d, long long sj, long long sk, const int *ext,
+                     unsigned long long seed, double lo, double hi, hipStream_t s);
+
+} // namespace cg

@@ -1,0 +1,499 @@
+// cg_plan.cpp -- host-side validation of operator calls and construction of
+// kernel arguments.  Every check restates a precondition of the reference
+// operator (cited per function) and returns the matching CG_ERR_* code
+// where the reference aborts.  Where the reference's own containment check
+// is looser than the stencil it evaluates (prolongate_3d_rf2.cc:399-404 for
+// ORDER >= 5 and the cell-centred checks), the exact stencil footprint is
+// required to lie in the source array as well, so that no kernel reads
+// outside the arrays it was given.
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "cg_internal.hpp"
+#include "cg_stencil.hpp"
+
+namespace cg {
+
+static thread_local char g_errmsg[512];
+
+int set_error(int code, const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_errmsg, sizeof g_errmsg, fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+const char *last_error_message() { return g_errmsg; }
+
+int hip_error(hipError_t e, const char *where) {
+  return set_error(CG_ERR_HIP, "%s: %s", where, hipGetErrorString(e));
+}
+
+Arr arr_from(const void *p, const int *pad, const int *ext, const int *bbox) {
+  Arr a;
+  a.p = p;
+  for (int d = 0; d < 3; ++d) {
+    a.pad[d] = pad[d];
+    a.ext[d] = ext[d];
+  }
+  a.box = Box::from(bbox);
+  return a;
+}
+
+Arr arr_from(const cg_gdata &g) { return arr_from(g.storage, g.padext, g.ext, g.bbox); }
+
+int check_array(const Arr &a, const char *name) {
+  if (!a.p)
+    return set_error(CG_ERR_ARG, "%s: null storage", name);
+  for (int d = 0; d < 3; ++d) {
+    if (a.box.str[d] <= 0)
+      return set_error(CG_ERR_ARG, "%s: stride %d is not positive", name, a.box.str[d]);
+    if (a.ext[d] != a.box.sizes(d) || a.pad[d] < a.ext[d])
+      return set_error(CG_ERR_SHAPE,
+                       "%s: array sizes don't agree with bounding box (dir %d: "
+                       "ext=%d padext=%d box size=%d)",
+                       name, d, a.ext[d], a.pad[d], a.box.sizes(d));
+  }
+  return CG_OK;
+}
+
+static int check_region_stride(const Box &r) {
+  for (int d = 0; d < 3; ++d)
+    if (r.str[d] <= 0)
+      return set_error(CG_ERR_ARG, "region stride %d is not positive", r.str[d]);
+  return CG_OK;
+}
+
+static LineInfo line_info_rt(int kind, int order, bool &ok) {
+  ok = true;
+  switch (kind) {
+  case PK_VLAG:
+    switch (order) {
+    case 1: return line_info<LK_VLAG, 1>();
+    case 3: return line_info<LK_VLAG, 3>();
+    case 5: return line_info<LK_VLAG, 5>();
+    case 7: return line_info<LK_VLAG, 7>();
+    case 9: return line_info<LK_VLAG, 9>();
+    case 11: return line_info<LK_VLAG, 11>();
+    }
+    break;
+  case PK_CLAG:
+    switch (order) {
+    case 0: return line_info<LK_CLAG, 0>();
+    case 1: return line_info<LK_CLAG, 1>();
+    case 2: return line_info<LK_CLAG, 2>();
+    case 3: return line_info<LK_CLAG, 3>();
+    case 4: return line_info<LK_CLAG, 4>();
+    case 5: return line_info<LK_CLAG, 5>();
+    }
+    break;
+  case PK_ENO:
+    switch (order) {
+    case 2: return line_info<LK_ENO, 2>();
+    case 3: return line_info<LK_ENO, 3>();
+    }
+    break;
+  }
+  ok = false;
+  return LineInfo{};
+}
+
+// prolongate_3d_rf2.cc:373-417 (vertex), prolongate_3d_cc_rf2.cc:253-321
+// (cell Lagrange), prolongate_3d_cc_eno_rf2.cc:962-1030 (cell ENO),
+// prolongate_3d_cc_real8_tvd.F90:94-179 (cell TVD)
+int plan_prolong(int kind, int order, const Arr &src, const void *const *srcs,
+                 int ntl, const double *s, const Arr &dst, const Box &reg,
+                 ProlongPlan &pl) {
+  int e;
+  if ((e = check_array(src, "source")) || (e = check_array(dst, "destination")))
+    return e;
+  if ((e = check_region_stride(reg)))
+    return e;
+  if (ntl < 1 || ntl > 3)
+    return set_error(CG_ERR_ORDER, "unsupported number of time levels %d", ntl);
+  for (int l = 0; l < ntl; ++l)
+    if (!srcs[l])
+      return set_error(CG_ERR_ARG, "null source time level %d", l);
+  const Box &sb = src.box, &db = dst.box;
+  std::memset(&pl, 0, sizeof pl);
+  pl.kind = kind;
+  pl.order = order;
+  pl.ntl = ntl;
+
+  if (kind == PK_TVD) {
+    for (int d = 0; d < 3; ++d) {
+      if (sb.str[d] <= reg.str[d] || db.str[d] != reg.str[d])
+        return set_error(CG_ERR_STRIDE, "Internal error: strides disagree");
+      if (sb.str[d] % db.str[d])
+        return set_error(CG_ERR_REFFACT, "Internal error: destination strides "
+                                         "are not integer multiples of the source strides");
+      if (reg.lo[d] > reg.up[d])
+        return set_error(CG_ERR_EMPTY, "Internal error: region extent is empty");
+    }
+    for (int d = 0; d < 3; ++d) {
+      if (sb.str[d] / db.str[d] != 2)
+        return set_error(CG_ERR_REFFACT, "Internal error: refinement factor is not 2");
+      if (reg.str[d] % 2)
+        return set_error(CG_ERR_ALIGN, "Internal error: region stride is not a multiple of 2");
+    }
+    int off[3], doff[3], n[3];
+    long long srcpts = 1;
+    for (int d = 0; d < 3; ++d) {
+      n[d] = (reg.up[d] - reg.lo[d]) / reg.str[d] + 1;
+      // every axis adds the x stride's half (F90:165-167)
+      off[d] = (reg.lo[d] - sb.lo[d] + reg.str[0] / 2) / db.str[d];
+      doff[d] = (reg.lo[d] - db.lo[d]) / db.str[d];
+      if (off[d] < 0)
+        return set_error(CG_ERR_CONTAIN, "Internal error: source array offset is negative");
+      if (doff[d] < 0 || doff[d] + n[d] > dst.ext[d] || reg.lo[d] < db.lo[d])
+        return set_error(CG_ERR_CONTAIN, "Internal error: region extent is not "
+                                         "contained in array extent");
+      const int plo = off[d] / 2, phi = (off[d] + n[d] - 1) / 2;
+      if (plo - 1 < 0 || phi + 1 >= src.ext[d])
+        return set_error(CG_ERR_CONTAIN, "TVD stencil footprint [%d,%d] outside "
+                                         "source array [0,%d) in dir %d",
+                         plo - 1, phi + 1, src.ext[d], d);
+      srcpts *= (phi + 1) - (plo - 1) + 1;
+    }
+    PointArgs &p = pl.p;
+    p.src[0] = static_cast<const double *>(srcs[0]);
+    p.sj = src.sj();
+    p.sk = src.sk();
+    p.dst = static_cast<double *>(const_cast<void *>(dst.p)) + dst.at(doff[0], doff[1], doff[2]);
+    p.dj = dst.sj();
+    p.dk = dst.sk();
+    for (int d = 0; d < 3; ++d) {
+      p.n[d] = n[d];
+      p.off[d] = off[d];
+    }
+    p.ntl = 1;
+    if (ntl != 1)
+      return set_error(CG_ERR_ORDER, "TVD prolongation is fused with time "
+                                     "interpolation only through the dispatch");
+    pl.dst_points = (long long)n[0] * n[1] * n[2];
+    pl.src_points = srcpts;
+    return CG_OK;
+  }
+
+  bool ok;
+  const LineInfo li = line_info_rt(kind, order, ok);
+  if (!ok)
+    return set_error(CG_ERR_ORDER, "There is no %s stencil with order_space=%d",
+                     kind == PK_VLAG ? "vertex-centred Lagrange"
+                     : kind == PK_CLAG ? "cell-centred Lagrange"
+                                       : "cell-centred ENO",
+                     order);
+  const bool cell = kind != PK_VLAG;
+  for (int d = 0; d < 3; ++d)
+    if (sb.str[d] <= reg.str[d] || db.str[d] != reg.str[d])
+      return set_error(CG_ERR_STRIDE, "Internal error: strides disagree");
+  for (int d = 0; d < 3; ++d)
+    if (sb.str[d] != 2 * db.str[d])
+      return set_error(CG_ERR_REFFACT, "Internal error: source strides are not "
+                                       "twice the destination strides");
+  if (!cell) {
+    for (int d = 0; d < 3; ++d) {
+      if (sb.lo[d] % sb.str[d])
+        return set_error(CG_ERR_ALIGN, "Internal error: source bbox is not aligned with vertices");
+      if (db.lo[d] % db.str[d])
+        return set_error(CG_ERR_ALIGN, "Internal error: destination bbox is not aligned with vertices");
+      if (reg.lo[d] % reg.str[d])
+        return set_error(CG_ERR_ALIGN, "Internal error: prolongation region bbox is not aligned with vertices");
+    }
+  } else {
+    for (int d = 0; d < 3; ++d)
+      if (db.str[d] % 2)
+        return set_error(CG_ERR_ALIGN, "Internal error: destination strides are not even");
+  }
+  if (reg.empty())
+    return set_error(CG_ERR_EMPTY, "Internal error: region extent is empty");
+  if (!reg.contained_in(db))
+    return set_error(CG_ERR_CONTAIN, "Internal error: region extent is not contained in array extent");
+
+  TensorArgs &t = pl.t;
+  long long srcpts = 1;
+  int doff[3];
+  for (int d = 0; d < 3; ++d) {
+    const int num = reg.lo[d] - sb.lo[d] + (cell ? reg.str[d] / 2 : 0);
+    if (num % reg.str[d] || (reg.lo[d] - db.lo[d]) % reg.str[d])
+      return set_error(CG_ERR_ALIGN, "region is not aligned with the source/destination grids (dir %d)", d);
+    t.off[d] = num / reg.str[d];
+    doff[d] = (reg.lo[d] - db.lo[d]) / reg.str[d];
+    t.n[d] = reg.sizes(d);
+    int lo = 1 << 30, hi = -(1 << 30);
+    const int fs[4] = {0, 1, t.n[d] - 2, t.n[d] - 1};
+    for (int f : fs) {
+      if (f < 0 || f >= t.n[d])
+        continue;
+      const int g = t.off[d] + f, b = g >> 1, p = g & 1;
+      lo = std::min(lo, b + li.tmin[p]);
+      hi = std::max(hi, b + li.tmin[p] + li.ntap[p] - 1);
+    }
+    if (lo < 0 || hi >= src.ext[d])
+      return set_error(CG_ERR_CONTAIN,
+                       "Internal error: region extent is not contained in array "
+                       "extent (stencil footprint [%d,%d] vs source [0,%d) in dir %d)",
+                       lo, hi, src.ext[d], d);
+    t.flo[d] = lo;
+    t.fhi[d] = hi;
+    t.qlo[d] = t.off[d] >> 1;
+    t.nq[d] = ((t.off[d] + t.n[d] - 1) >> 1) - t.qlo[d] + 1;
+    srcpts *= (long long)(hi - lo + 1);
+  }
+  for (int l = 0; l < 3; ++l)
+    t.src[l] = l < ntl ? srcs[l] : nullptr;
+  t.sj = src.sj();
+  t.sk = src.sk();
+  t.dst = static_cast<double *>(const_cast<void *>(dst.p)) + dst.at(doff[0], doff[1], doff[2]);
+  t.dj = dst.sj();
+  t.dk = dst.sk();
+  for (int l = 0; l < 3; ++l)
+    t.s[l] = l < ntl ? s[l] : 0.0;
+  pl.dst_points = (long long)t.n[0] * t.n[1] * t.n[2];
+  pl.src_points = srcpts;
+  return CG_OK;
+}
+
+int launch_prolong(const ProlongPlan &pl, int prec, hipStream_t st) {
+  int r;
+  if (pl.kind == PK_TVD) {
+    r = launch_point(PO_TVD, pl.p, st);
+  } else {
+    TileCfg cfg{0, 0, 0};
+    // z chunks: enough workgroups to fill the chip, each chunk long enough
+    // to amortise the stencil's halo planes
+    TensorArgs a = pl.t;
+    cfg.cx = 32;
+    cfg.cy = pl.ntl == 1 ? 8 : 4;
+    const long long cols = (long long)((a.nq[0] + cfg.cx - 1) / cfg.cx) *
+                           ((a.nq[1] + cfg.cy - 1) / cfg.cy);
+    int zc = a.nq[2];
+    while (zc > 16 && cols * ((a.nq[2] + zc - 1) / zc) < 2048)
+      zc = (zc + 1) / 2;
+    cfg.zchunk = zc;
+    r = launch_tensor(pl.kind, pl.order, pl.ntl, prec, a, cfg, st);
+  }
+  if (r == -1)
+    return set_error(CG_ERR_ORDER, "no kernel for kind %d order %d ntl %d prec %d",
+                     pl.kind, pl.order, pl.ntl, prec);
+  if (r)
+    return hip_error(hipGetLastError(), "kernel launch");
+  return CG_OK;
+}
+
+// copy_3d.cc:36-82
+int plan_copy(const Arr &src, const Arr &dst, const Box &srcreg1,
+              const Box &dreg, const cg_islab *slab, PointPlan &pl) {
+  int e;
+  if ((e = check_array(src, "source")) || (e = check_array(dst, "destination")))
+    return e;
+  if ((e = check_region_stride(dreg)))
+    return e;
+  const Box &sb = src.box, &db = dst.box;
+  for (int d = 0; d < 3; ++d)
+    if (sb.str[d] != dreg.str[d] || db.str[d] != dreg.str[d] ||
+        srcreg1.str[d] != dreg.str[d])
+      return set_error(CG_ERR_STRIDE, "Internal error: strides disagree");
+  if (dreg.empty())
+    return set_error(CG_ERR_EMPTY, "Internal error: region extent is empty");
+  Box sreg = dreg;
+  if (slab)
+    for (int d = 0; d < 3; ++d) {
+      sreg.lo[d] += slab->offset[d] * dreg.str[d];
+      sreg.up[d] += slab->offset[d] * dreg.str[d];
+    }
+  if (!sreg.contained_in(srcreg1))
+    return set_error(CG_ERR_CONTAIN, "shifted source region is not contained in srcregbbox");
+  if (!sreg.contained_in(sb) || !dreg.contained_in(db))
+    return set_error(CG_ERR_CONTAIN, "Internal error: region extent is not contained in array extent");
+  int so[3], dof[3];
+  std::memset(&pl, 0, sizeof pl);
+  for (int d = 0; d < 3; ++d) {
+    if ((sreg.lo[d] - sb.lo[d]) % sb.str[d] || (dreg.lo[d] - db.lo[d]) % db.str[d])
+      return set_error(CG_ERR_ALIGN, "region is not aligned with the arrays (dir %d)", d);
+    so[d] = (sreg.lo[d] - sb.lo[d]) / sb.str[d];
+    dof[d] = (dreg.lo[d] - db.lo[d]) / db.str[d];
+    pl.a.n[d] = dreg.sizes(d);
+  }
+  pl.op = PO_COPY;
+  pl.a.src[0] = static_cast<const double *>(src.p) + src.at(so[0], so[1], so[2]);
+  pl.a.sj = src.sj();
+  pl.a.sk = src.sk();
+  pl.a.dst = static_cast<double *>(const_cast<void *>(dst.p)) + dst.at(dof[0], dof[1], dof[2]);
+  pl.a.dj = dst.sj();
+  pl.a.dk = dst.sk();
+  pl.dst_points = dreg.size();
+  pl.src_points = dreg.size();
+  return CG_OK;
+}
+
+// restrict_3d_rf2.cc:38-68 (vertex), restrict_3d_cc_rf2.cc:38-73 (cell)
+int plan_restrict(bool cell, const Arr &src, const Arr &dst, const Box &reg,
+                  PointPlan &pl) {
+  int e;
+  if ((e = check_array(src, "source")) || (e = check_array(dst, "destination")))
+    return e;
+  if ((e = check_region_stride(reg)))
+    return e;
+  const Box &sb = src.box, &db = dst.box;
+  for (int d = 0; d < 3; ++d)
+    if (sb.str[d] >= reg.str[d] || db.str[d] != reg.str[d])
+      return set_error(CG_ERR_STRIDE, "Internal error: strides disagree");
+  for (int d = 0; d < 3; ++d)
+    if (2 * sb.str[d] != db.str[d])
+      return set_error(CG_ERR_REFFACT, "Internal error: destination strides are "
+                                       "not twice the source strides");
+  if (reg.empty())
+    return set_error(CG_ERR_EMPTY, "Internal error: region extent is empty");
+  std::memset(&pl, 0, sizeof pl);
+  int so[3], dof[3];
+  for (int d = 0; d < 3; ++d) {
+    pl.a.n[d] = reg.shape(d) / reg.str[d];
+    if (cell) {
+      if (sb.str[d] % 2)
+        return set_error(CG_ERR_ALIGN, "source stride is not even");
+      const int num = reg.lo[d] - sb.lo[d] - sb.str[d] / 2;
+      if (num % sb.str[d])
+        return set_error(CG_ERR_ALIGN, "region is not aligned with the source cells (dir %d)", d);
+      so[d] = num / sb.str[d];
+      if (so[d] < 0 || so[d] + 2 * pl.a.n[d] > src.ext[d])
+        return set_error(CG_ERR_CONTAIN, "Internal error: region extent is not contained in array extent");
+    } else {
+      if ((reg.lo[d] - sb.lo[d]) % sb.str[d])
+        return set_error(CG_ERR_ALIGN, "region is not aligned with the source (dir %d)", d);
+      so[d] = (reg.lo[d] - sb.lo[d]) / sb.str[d];
+    }
+    if ((reg.lo[d] - db.lo[d]) % db.str[d])
+      return set_error(CG_ERR_ALIGN, "region is not aligned with the destination (dir %d)", d);
+    dof[d] = (reg.lo[d] - db.lo[d]) / db.str[d];
+  }
+  if ((!cell && !reg.contained_in(sb)) || !reg.contained_in(db))
+    return set_error(CG_ERR_CONTAIN, "Internal error: region extent is not contained in array extent");
+  pl.op = cell ? PO_RESTRICT_CC : PO_RESTRICT_VC;
+  pl.a.src[0] = static_cast<const double *>(src.p) + src.at(so[0], so[1], so[2]);
+  pl.a.sj = src.sj();
+  pl.a.sk = src.sk();
+  pl.a.dst = static_cast<double *>(const_cast<void *>(dst.p)) + dst.at(dof[0], dof[1], dof[2]);
+  pl.a.dj = dst.sj();
+  pl.a.dk = dst.sk();
+  pl.dst_points = (long long)pl.a.n[0] * pl.a.n[1] * pl.a.n[2];
+  pl.src_points = pl.dst_points * (cell ? 8 : 1);
+  return CG_OK;
+}
+
+int time_factors(int ntl, const double *tt, double t, double *s) {
+  const double eps = 1.0e-12;
+  if (ntl == 1) {
+    s[0] = 1.0;
+    return CG_OK;
+  }
+  if (ntl == 2) {
+    const double t1 = tt[0], t2 = tt[1];
+    if (std::fabs(t1 - t2) < eps)
+      return set_error(CG_ERR_TIME_SAME, "Internal error: arrays have same time");
+    if (t < std::min(t1, t2) - eps || t > std::max(t1, t2) + eps)
+      return set_error(CG_ERR_TIME_EXTRAP, "Internal error: extrapolation in time");
+    s[0] = (t - t2) / (t1 - t2);
+    s[1] = (t - t1) / (t2 - t1);
+    return CG_OK;
+  }
+  if (ntl == 3) {
+    const double t1 = tt[0], t2 = tt[1], t3 = tt[2];
+    if (std::fabs(t1 - t2) < eps || std::fabs(t1 - t3) < eps || std::fabs(t2 - t3) < eps)
+      return set_error(CG_ERR_TIME_SAME,
+                       "Internal error: arrays have same time (t1=%.17g t2=%.17g t3=%.17g)",
+                       t1, t2, t3);
+    if (t < std::fmin(std::fmin(t1, t2), t3) - eps || t > std::fmax(std::fmax(t1, t2), t3) + eps)
+      return set_error(CG_ERR_TIME_EXTRAP, "Internal error: extrapolation in time");
+    s[0] = (t - t2) * (t - t3) / ((t1 - t2) * (t1 - t3));
+    s[1] = (t - t1) * (t - t3) / ((t2 - t1) * (t2 - t3));
+    s[2] = (t - t1) * (t - t2) / ((t3 - t1) * (t3 - t2));
+    return CG_OK;
+  }
+  return set_error(CG_ERR_ORDER, "no time interpolation with %d time levels", ntl);
+}
+
+// interpolate_3d_2tl.cc:40-107, interpolate_3d_3tl.cc:41-115
+int plan_interp(int ntl, const void *const *srcs, const double *times, double t,
+                const Arr &src, const Arr &dst, const Box &reg, PointPlan &pl) {
+  int e;
+  if ((e = check_array(src, "source")) || (e = check_array(dst, "destination")))
+    return e;
+  if ((e = check_region_stride(reg)))
+    return e;
+  for (int l = 0; l < ntl; ++l)
+    if (!srcs[l])
+      return set_error(CG_ERR_ARG, "null source time level %d", l);
+  const Box &sb = src.box, &db = dst.box;
+  for (int d = 0; d < 3; ++d)
+    if (sb.str[d] != reg.str[d] || db.str[d] != reg.str[d])
+      return set_error(CG_ERR_STRIDE, "Internal error: strides disagree");
+  if (reg.empty())
+    return set_error(CG_ERR_EMPTY, "Internal error: region extent is empty");
+  if (!reg.contained_in(sb) || !reg.contained_in(db))
+    return set_error(CG_ERR_CONTAIN, "Internal error: region extent is not contained in array extent");
+  std::memset(&pl, 0, sizeof pl);
+  if ((e = time_factors(ntl, times, t, pl.a.s)))
+    return e;
+  if (ntl < 2)
+    return set_error(CG_ERR_ORDER, "time interpolation needs 2 or 3 time levels");
+  int so[3], dof[3];
+  for (int d = 0; d < 3; ++d) {
+    if ((reg.lo[d] - sb.lo[d]) % sb.str[d] || (reg.lo[d] - db.lo[d]) % db.str[d])
+      return set_error(CG_ERR_ALIGN, "region is not aligned with the arrays (dir %d)", d);
+    so[d] = (reg.lo[d] - sb.lo[d]) / sb.str[d];
+    dof[d] = (reg.lo[d] - db.lo[d]) / db.str[d];
+    pl.a.n[d] = reg.sizes(d);
+  }
+  pl.op = PO_INTERP;
+  pl.a.ntl = ntl;
+  for (int l = 0; l < ntl; ++l)
+    pl.a.src[l] = static_cast<const double *>(srcs[l]) + src.at(so[0], so[1], so[2]);
+  pl.a.sj = src.sj();
+  pl.a.sk = src.sk();
+  pl.a.dst = static_cast<double *>(const_cast<void *>(dst.p)) + dst.at(dof[0], dof[1], dof[2]);
+  pl.a.dj = dst.sj();
+  pl.a.dk = dst.sk();
+  pl.dst_points = reg.size();
+  pl.src_points = reg.size();
+  return CG_OK;
+}
+
+int find_source_timelevel(const double *times, int ntimes, double time,
+                          int order_time, int op, int *tl0, int *ntl) {
+  if (!times || ntimes <= 0 || order_time < 0)
+    return set_error(CG_ERR_ARG, "bad time levels");
+  const double eps = 1.0e-12;
+  double mn = times[0], mx = times[0];
+  for (int i = 1; i < ntimes; ++i) {
+    mn = std::min(mn, times[i]);
+    mx = std::max(mx, times[i]);
+  }
+  const double some_time = std::fabs(mn) + std::fabs(mx) + 1.0;
+  if (op != CG_OP_COPY && (time < mn - eps * some_time || time > mx + eps * some_time))
+    return set_error(CG_ERR_TIME_EXTRAP, "Internal error: extrapolation in time. time=%.17g", time);
+  int tl = -1;
+  if (ntimes == 1)
+    tl = 0;
+  if (tl == -1 && op == CG_OP_COPY)
+    tl = 0;
+  if (tl == -1)
+    for (int i = 0; i < ntimes; ++i)
+      if (std::fabs(times[i] - time) < eps * some_time) {
+        tl = i;
+        break;
+      }
+  if (tl >= 0) {
+    *tl0 = tl;
+    *ntl = 1;
+  } else {
+    *tl0 = 0;
+    *ntl = order_time + 1;
+  }
+  return CG_OK;
+}
+
+} // namespace cg

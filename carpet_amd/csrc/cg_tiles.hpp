@@ -1,0 +1,303 @@
+// cg_tiles.hpp -- device bodies of the transfer kernels, shared by the
+// per-call launches (cg_kernels.hip) and the persistent batch kernels
+// (cg_batch_kernels.hip).
+//
+// k_tensor: every tensor-product prolongation (vertex Lagrange
+// prolongate_3d_rf2, cell Lagrange prolongate_3d_cc_rf2, cell ENO
+// prolongate_3d_cc_eno_rf2), optionally fused with 2- or 3-level time
+// interpolation.  A workgroup owns a tile of CX x CY coarse bases in x-y
+// (a 2CX x 2CY fine column) and marches through a chunk of coarse z planes:
+//   1. the coarse plane window (tile + stencil halo, clipped to the region's
+//      exact source footprint) is loaded into LDS with coalesced x-rows;
+//   2. x-pass: every coarse row of the window is interpolated to both fine
+//      x children of each base (LDS -> LDS);
+//   3. y-pass: every fine x column is interpolated to both fine y children
+//      of each base; the results stay in registers in a ring of the last R
+//      coarse planes;
+//   4. z-emission: as soon as the ring holds all planes of a fine z point,
+//      that fine plane is combined over time levels and stored (one
+//      coalesced x-row per wave).
+// Each pass evaluates the reference's nested sums on the same operands in
+// the same order, so the result equals the per-point nested evaluation bit
+// for bit while the work per fine point drops from (O+1)^3 to ~3(O+1)/2.
+// The kernel is bandwidth-bound: 8 bytes written per fine point plus ~1/8 of
+// that read per source time level.
+//
+// k_point: copy_3d, restrict_3d_rf2, restrict_3d_cc_rf2,
+// interpolate_3d_2tl/3tl and prolongate_3d_cc_tvd, one destination point
+// per thread over a flattened x-y plane (so thin ghost slabs still fill
+// waves) and planes along blockIdx.y.
+
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "cg_kernels.hpp"
+#include "cg_stencil.hpp"
+
+namespace cg {
+
+
+// Workgroups are dispatched round-robin over the 8 XCDs of an MI355X; remap
+// the hardware index so that each XCD walks a contiguous range of tiles and
+// neighbouring tiles share stencil halos through the same L2.
+__device__ __forceinline__ int xcd_remap(int b, int n) {
+  constexpr int NX = 8;
+  if (n < NX)
+    return b;
+  const int per = n / NX, rem = n % NX;
+  const int xcd = b % NX, slot = b / NX;
+  return xcd * per + min(xcd, rem) + slot;
+}
+
+template <typename TS> __device__ __forceinline__ double ld(const void *p, long long o) {
+  return (double)(static_cast<const TS *>(p)[o]);
+}
+
+// LDS footprint (doubles) of one tensor tile
+template <class L, int NTL, int CX, int CY> struct TensorLds {
+  static constexpr int SPAN = L::TMAXALL - L::TMINALL;
+  static constexpr int WX = CX + SPAN;
+  static constexpr int WY = CY + SPAN;
+  static constexpr int A = NTL * WY * WX;
+  static constexpr int X = NTL * WY * 2 * CX;
+  static constexpr int TOTAL = A + X;
+};
+
+// One z-marching tile; tile b indexes (x tile, y tile, z chunk) of a.
+template <class L, int NTL, int CX, int CY, int IPT>
+__device__ __forceinline__ void tensor_tile(const TensorArgs &a, int b, double *smem) {
+  constexpr int FX = 2 * CX;
+  constexpr int NT = FX * CY / IPT;
+  constexpr int NQY = NT / FX;
+  static_assert(NQY * IPT == CY, "tile rows must divide among threads");
+  using LD = TensorLds<L, NTL, CX, CY>;
+  constexpr int SPAN = LD::SPAN;
+  constexpr int WX = LD::WX;
+  constexpr int WY = LD::WY;
+  constexpr int R = L::R;
+
+  double(*A)[WY][WX] = reinterpret_cast<double(*)[WY][WX]>(smem);
+  double(*X)[WY][FX] = reinterpret_cast<double(*)[WY][FX]>(smem + LD::A);
+  (void)SPAN;
+
+  const int tx = b % a.ntiles[0];
+  b /= a.ntiles[0];
+  const int ty = b % a.ntiles[1];
+  const int tz = b / a.ntiles[1];
+
+  const int qx0 = a.qlo[0] + tx * CX;
+  const int qy0 = a.qlo[1] + ty * CY;
+  const int qz0 = a.qlo[2] + tz * a.zchunk;
+  const int qz1 = min(qz0 + a.zchunk, a.qlo[2] + a.nq[2]) - 1;
+
+  const int wx0 = qx0 + L::TMINALL, wy0 = qy0 + L::TMINALL;
+  const int lx0 = max(wx0, a.flo[0]), lx1 = min(wx0 + WX - 1, a.fhi[0]);
+  const int ly0 = max(wy0, a.flo[1]), ly1 = min(wy0 + WY - 1, a.fhi[1]);
+  const int c0 = max(qz0 + L::TMINALL, a.flo[2]);
+  const int c1 = min(qz1 + L::TMAXALL, a.fhi[2]);
+
+  const int tid = threadIdx.x;
+  const int fx = tid % FX, qyl = tid / FX;
+  const int gfx = 2 * qx0 + fx - a.off[0];
+  const bool xin = gfx >= 0 && gfx < a.n[0];
+
+  double ring[NTL][R][IPT][2];
+#pragma unroll
+  for (int tl = 0; tl < NTL; ++tl)
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+#pragma unroll
+      for (int m = 0; m < IPT; ++m)
+        ring[tl][i][m][0] = ring[tl][i][m][1] = 0.0;
+
+  for (int c = c0; c <= c1; ++c) {
+    // 1. coarse plane window -> LDS
+    for (int e = tid; e < WY * WX; e += NT) {
+      const int r = e / WX, x = e - r * WX;
+      const int gy = wy0 + r, gx = wx0 + x;
+      if (gy >= ly0 && gy <= ly1 && gx >= lx0 && gx <= lx1) {
+        const long long o = gx + a.sj * gy + a.sk * c;
+#pragma unroll
+        for (int tl = 0; tl < NTL; ++tl)
+          A[tl][r][x] = ld<double>(a.src[tl], o);
+      }
+    }
+    __syncthreads();
+    // 2. x-pass: both children of each base
+    for (int e = tid; e < WY * CX; e += NT) {
+      const int r = e / CX, q = e - r * CX;
+#pragma unroll
+      for (int tl = 0; tl < NTL; ++tl) {
+        const double *w = &A[tl][r][q - L::TMINALL];
+        const double v0 =
+            L::template apply<0>([&](int t) { return w[L::TMIN[0] + t]; });
+        const double v1 =
+            L::template apply<1>([&](int t) { return w[L::TMIN[1] + t]; });
+        X[tl][r][2 * q] = v0;
+        X[tl][r][2 * q + 1] = v1;
+      }
+    }
+    __syncthreads();
+    // 3. y-pass into the register ring
+#pragma unroll
+    for (int m = 0; m < IPT; ++m) {
+      const int qy = qyl + NQY * m;
+#pragma unroll
+      for (int tl = 0; tl < NTL; ++tl) {
+        const double *col = &X[tl][qy - L::TMINALL][fx];
+        const double y0 =
+            L::template apply<0>([&](int t) { return col[(L::TMIN[0] + t) * FX]; });
+        const double y1 =
+            L::template apply<1>([&](int t) { return col[(L::TMIN[1] + t) * FX]; });
+#pragma unroll
+        for (int i = 0; i < R - 1; ++i) {
+          ring[tl][i][m][0] = ring[tl][i + 1][m][0];
+          ring[tl][i][m][1] = ring[tl][i + 1][m][1];
+        }
+        ring[tl][R - 1][m][0] = y0;
+        ring[tl][R - 1][m][1] = y1;
+      }
+    }
+    // 4. z-emission of the fine planes whose last coarse plane is c
+    auto emit = [&](auto PZc) {
+      constexpr int PZ = decltype(PZc)::value;
+      constexpr int NTP = L::NTAP[PZ];
+      constexpr int TMAX = L::TMIN[PZ] + NTP - 1;
+      const int base = c - TMAX;
+      if (base < qz0 || base > qz1)
+        return;
+      const int fz = 2 * base + PZ - a.off[2];
+      if (fz < 0 || fz >= a.n[2])
+        return;
+#pragma unroll
+      for (int m = 0; m < IPT; ++m) {
+#pragma unroll
+        for (int py = 0; py < 2; ++py) {
+          const int fy = 2 * (qy0 + qyl + NQY * m) + py - a.off[1];
+          double v[NTL];
+#pragma unroll
+          for (int tl = 0; tl < NTL; ++tl)
+            v[tl] = L::template apply<PZ>(
+                [&](int t) { return ring[tl][R - NTP + t][m][py]; });
+          double out;
+          if constexpr (NTL == 1)
+            out = v[0];
+          else if constexpr (NTL == 2)
+            out = fmad(a.s[1], v[1], a.s[0] * v[0]);
+          else
+            out = fmad(a.s[2], v[2], fmad(a.s[1], v[1], a.s[0] * v[0]));
+          if (xin && fy >= 0 && fy < a.n[1])
+            static_cast<double *>(a.dst)[gfx + a.dj * fy + a.dk * fz] = out;
+        }
+      }
+    };
+    emit(std::integral_constant<int, 0>{});
+    emit(std::integral_constant<int, 1>{});
+  }
+  // the next tile of a persistent workgroup overwrites the LDS window
+  __syncthreads();
+}
+
+// ---- pointwise region operators -------------------------------------------
+__device__ __forceinline__ double minmod(double a, double b) {
+  if (a * b < 0)
+    return 0.0;
+  return __builtin_fabs(a) < __builtin_fabs(b) ? a : b;
+}
+
+// one destination point (i,j,k) of a pointwise operator
+template <int OP>
+__device__ __forceinline__ void point_elem(const PointArgs &a, int i, int j, int k) {
+  double out;
+  if constexpr (OP == PO_COPY) {
+    out = a.src[0][i + a.sj * j + a.sk * k];
+  } else if constexpr (OP == PO_RESTRICT_VC) {
+    out = a.src[0][2 * i + a.sj * (2 * j) + a.sk * (2 * k)];
+  } else if constexpr (OP == PO_RESTRICT_CC) {
+    // restrict_3d_cc_rf2.cc:118-126: eight children, fixed order, weight 1/8
+    const double *p = a.src[0] + 2 * i + a.sj * (2 * j) + a.sk * (2 * k);
+    const long long J = a.sj, K = a.sk;
+    const double w = (1.0 / 2) * (1.0 / 2) * (1.0 / 2);
+    double r = w * p[0];
+    r = r + w * p[1];
+    r = r + w * p[J];
+    r = r + w * p[1 + J];
+    r = r + w * p[K];
+    r = r + w * p[1 + K];
+    r = r + w * p[J + K];
+    r = r + w * p[1 + J + K];
+    out = r;
+  } else if constexpr (OP == PO_INTERP) {
+    const long long o = i + a.sj * j + a.sk * k;
+    double r = a.s[0] * a.src[0][o];
+    r = fmad(a.s[1], a.src[1][o], r);
+    if (a.ntl == 3)
+      r = fmad(a.s[2], a.src[2][o], r);
+    out = r;
+  } else { // PO_TVD, prolongate_3d_cc_real8_tvd.F90:183-225
+    const int gi = a.off[0] + i, gj = a.off[1] + j, gk = a.off[2] + k;
+    const int i0 = gi >> 1, j0 = gj >> 1, k0 = gk >> 1;
+    const double sx0 = double(2 * (gi & 1) - 1) * (1.0 / 4);
+    const double sy0 = double(2 * (gj & 1) - 1) * (1.0 / 4);
+    const double sz0 = double(2 * (gk & 1) - 1) * (1.0 / 4);
+    const double *p = a.src[0] + i0 + a.sj * j0 + a.sk * k0;
+    const long long J = a.sj, K = a.sk;
+    const double c = p[0];
+    const double sx = sx0 * minmod(c - p[-1], p[1] - c);
+    const double sy = sy0 * minmod(c - p[-J], p[J] - c);
+    const double sz = sz0 * minmod(c - p[-K], p[K] - c);
+    out = c + sx + sy + sz;
+  }
+  a.dst[i + a.dj * j + a.dk * k] = out;
+}
+
+// tile shapes: one z-marching column of 2CX x 2CY fine points
+template <int NTL> struct TileShape;
+template <> struct TileShape<1> {
+  static constexpr int CX = 32, CY = 8, IPT = 2;
+};
+template <> struct TileShape<2> {
+  static constexpr int CX = 32, CY = 4, IPT = 1;
+};
+template <> struct TileShape<3> {
+  static constexpr int CX = 32, CY = 4, IPT = 1;
+};
+
+} // namespace cg
+
+namespace cg {
+// Calls fn(Line<KIND,O>{}) for a runtime (kind, order); returns -1 if there
+// is no such line operator.
+template <class Fn> int with_line(int kind, int order, Fn &&fn) {
+  switch (kind) {
+  case LK_VLAG:
+    switch (order) {
+    case 1: return fn(Line<LK_VLAG, 1>{});
+    case 3: return fn(Line<LK_VLAG, 3>{});
+    case 5: return fn(Line<LK_VLAG, 5>{});
+    case 7: return fn(Line<LK_VLAG, 7>{});
+    case 9: return fn(Line<LK_VLAG, 9>{});
+    case 11: return fn(Line<LK_VLAG, 11>{});
+    }
+    break;
+  case LK_CLAG:
+    switch (order) {
+    case 0: return fn(Line<LK_CLAG, 0>{});
+    case 1: return fn(Line<LK_CLAG, 1>{});
+    case 2: return fn(Line<LK_CLAG, 2>{});
+    case 3: return fn(Line<LK_CLAG, 3>{});
+    case 4: return fn(Line<LK_CLAG, 4>{});
+    case 5: return fn(Line<LK_CLAG, 5>{});
+    }
+    break;
+  case LK_ENO:
+    switch (order) {
+    case 2: return fn(Line<LK_ENO, 2>{});
+    case 3: return fn(Line<LK_ENO, 3>{});
+    }
+    break;
+  }
+  return -1;
+}
+} // namespace cg

@@ -1,0 +1,345 @@
+"""Seeded synthetic hierarchies for the transfer operators.
+
+Each workload describes its arrays (boxes, padding, host input data) and the
+ordered list of transfers of one level step, and runs that step on the GPU
+through the library.  The same description is what the tests replay on the
+CPU oracle.
+
+Transfers are tuples (name, dst, srcs, times, dstreg, srcreg, time, op,
+centering, order_space, order_time, slab_offset) where dst/srcs name arrays.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import Dict, List
+
+import numpy as np
+
+from .core import (CELL_CENTERED, OP_ENO, OP_LAGRANGE, OP_RESTRICT, OP_SYNC, OP_TVD, PREC_F64,
+                   VERTEX_CENTERED, Batch, Box, DeviceArray, pad_shape, transfer_from)
+from ._lib import check, lib
+
+GOLDEN = 0x9E3779B97F4A7C15
+
+
+def splitmix64(x: np.ndarray) -> np.ndarray:
+    """Counter-based splitmix64 (uint64 in, uint64 out)."""
+    with np.errstate(over="ignore"):
+        z = (x + np.uint64(GOLDEN)).astype(np.uint64)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+
+
+def uniform(seed: int, n: int, lo: float, hi: float) -> np.ndarray:
+    z = splitmix64(np.arange(n, dtype=np.uint64) + np.uint64(seed))
+    u = (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+    return lo + (hi - lo) * u
+
+
+def hash_field(seed: int, ext, lo: float = -1.0, hi: float = 1.0) -> np.ndarray:
+    """Host twin of cg_fill_hash_f64: (ext_z, ext_y, ext_x) array."""
+    n = int(np.prod(ext))
+    z = splitmix64(np.arange(n, dtype=np.uint64) + np.uint64(seed & (2**64 - 1)))
+    u = (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+    return (lo + (hi - lo) * u).reshape(ext[2], ext[1], ext[0])
+
+
+@dataclass
+class ArraySpec:
+    box: Box
+    owned: Box | None = None
+    init: np.ndarray | None = None  # (ext_z, ext_y, ext_x); None -> NaN poison
+    seed: int | None = None  # device-side seeded fill (large workloads)
+
+
+@dataclass
+class Transfer:
+    name: str
+    dst: str
+    srcs: List[str]
+    times: List[float]
+    dstreg: Box
+    srcreg: Box
+    time: float
+    op: int
+    cent: int
+    order_space: int
+    order_time: int
+    slab: tuple | None = None
+    phase: int = 0
+
+
+@dataclass
+class Workload:
+    name: str
+    arrays: Dict[str, ArraySpec] = field(default_factory=dict)
+    steps: List[Transfer] = field(default_factory=list)
+    outputs: List[str] = field(default_factory=list)
+
+    def allocate(self, pool: bool = False):
+        """Device arrays for every ArraySpec.  pool=True carves them out of one
+        allocation (thousands of small arrays); seeded arrays are filled on
+        the device from a shared seeded buffer."""
+        dev = {}
+        if pool:
+            specs = list(self.arrays.items())
+            layout = []
+            total = 0
+            for k, spec in specs:
+                shape = spec.box.sizes()
+                off = (0, 0, 0)
+                if spec.owned is not None:
+                    off = tuple((o - l) // s for o, l, s in zip(spec.owned.lower, spec.box.lower, spec.box.stride))
+                pad, poff = pad_shape(shape, off, 16, 16, spec.owned is not None)
+                start = (total + 255) // 256 * 256 + poff[0] * 8
+                n = int(np.prod(pad)) * 8
+                layout.append((k, spec, pad, start))
+                total = start + n
+            import ctypes as C
+            base = C.c_void_p()
+            check(lib.cg_dev_malloc(C.byref(base), total), "cg_dev_malloc")
+            self._pool = base.value
+            for k, spec, pad, start in layout:
+                dev[k] = DeviceArray.view(spec.box, self._pool + start, pad)
+        else:
+            for k, spec in self.arrays.items():
+                dev[k] = DeviceArray(spec.box, owned=spec.owned)
+        for k, spec in self.arrays.items():
+            a = dev[k]
+            if spec.init is not None:
+                a.upload(spec.init, fill=np.nan)
+            elif spec.seed is not None:
+                a.fill(np.nan)
+                a.fill_hash(spec.seed)
+            else:
+                a.fill(np.nan)
+        return dev
+
+    def host_inputs(self) -> Dict[str, np.ndarray]:
+        """Host copies of every array's initial extent (NaN where poisoned)."""
+        out = {}
+        for k, spec in self.arrays.items():
+            e = spec.box.sizes()
+            if spec.init is not None:
+                out[k] = np.asarray(spec.init, dtype=np.float64).reshape(e[2], e[1], e[0])
+            elif spec.seed is not None:
+                out[k] = hash_field(spec.seed, e)
+            else:
+                out[k] = np.full((e[2], e[1], e[0]), np.nan)
+        return out
+
+    def free(self, dev):
+        for a in dev.values():
+            a.free()
+        if getattr(self, "_pool", None):
+            lib.cg_dev_free(self._pool)
+            self._pool = None
+
+    def batch(self, dev, prec: int = PREC_F64) -> Batch:
+        b = Batch()
+        for t in self.steps:
+            b.add(t.phase, dev[t.dst], [dev[s] for s in t.srcs], t.times, t.dstreg, t.srcreg, t.time,
+                  t.op, t.cent, t.order_space, t.order_time, slab_offset=t.slab, prec=prec)
+        return b
+
+    def run_gpu(self, cg=None, stream=None) -> Dict[str, np.ndarray]:
+        dev = self.allocate()
+        self.run_steps(dev, stream)
+        out = {k: dev[k].download() for k in self.outputs}
+        self.free(dev)
+        return out
+
+    def run_steps(self, dev, stream=None):
+        """One transfer_from call per step, in phase order."""
+        for t in sorted(self.steps, key=lambda t: t.phase):
+            transfer_from(dev[t.dst], [dev[s] for s in t.srcs], t.times, t.dstreg, t.srcreg, t.time,
+                          t.op, t.cent, t.order_space, t.order_time, slab_offset=t.slab, stream=stream)
+
+
+def box3(lo, up, st) -> Box:
+    return Box((lo,) * 3 if np.isscalar(lo) else lo, (up,) * 3 if np.isscalar(up) else up,
+               (st,) * 3 if np.isscalar(st) else st)
+
+
+def config1(seed: int = 20261202) -> Workload:
+    """FP64 vertex-centred level step on a 33^3 coarse box (+3 ghosts): order-3
+    prolongation onto the 64^3 fine region [2,65]^3 held by two 32x64x64
+    components, rf2 restriction of the covered coarse points [1,32]^3 back,
+    and the copy sync of the two components' ghost slabs (width 3).
+    Values: sin(2 pi x) sin(2 pi y) sin(2 pi z), x = coarse index / 32, plus
+    uniform noise of amplitude 1e-3 (splitmix64 counter stream)."""
+    w = Workload("config1")
+    g = 3
+    # coarse: interior 0..32 (coarse index), stride 2 in finest units
+    cbox = box3(-2 * g, 2 * (32 + g), 2)
+    n = cbox.sizes()[0]
+    idx = np.arange(-g, 32 + g + 1, dtype=np.float64) / 32.0
+    s = np.sin(2 * np.pi * idx)
+    data = s[:, None, None] * s[None, :, None] * s[None, None, :]
+    data = data + uniform(seed, n**3, -1e-3, 1e-3).reshape(n, n, n)
+    w.arrays["coarse"] = ArraySpec(cbox, owned=box3(0, 64, 2), init=data)
+    # fine: two components split in x, interior [2,33] and [34,65], ghosts 3
+    fa = Box((2 - g, 2 - g, 2 - g), (33 + g, 65 + g, 65 + g), (1, 1, 1))
+    fb = Box((34 - g, 2 - g, 2 - g), (65 + g, 65 + g, 65 + g), (1, 1, 1))
+    w.arrays["fineA"] = ArraySpec(fa, owned=Box((2, 2, 2), (33, 65, 65), (1, 1, 1)))
+    w.arrays["fineB"] = ArraySpec(fb, owned=Box((34, 2, 2), (65, 65, 65), (1, 1, 1)))
+    ra = Box((2, 2, 2), (33, 65, 65), (1, 1, 1))
+    rb = Box((34, 2, 2), (65, 65, 65), (1, 1, 1))
+    for nm, r in (("fineA", ra), ("fineB", rb)):
+        w.steps.append(Transfer(f"prolong->{nm}", nm, ["coarse"], [0.0], r, r, 0.0, OP_LAGRANGE,
+                                VERTEX_CENTERED, 3, 0, phase=0))
+    # restriction of coarse points [1,32]^3 (finest units 2..64), split by owner
+    w.steps.append(Transfer("restrict<-fineA", "coarse", ["fineA"], [0.0],
+                            Box((2, 2, 2), (32, 64, 64), (2, 2, 2)), Box((2, 2, 2), (32, 64, 64), (2, 2, 2)),
+                            0.0, OP_LAGRANGE, VERTEX_CENTERED, 3, 0, phase=1))
+    w.steps.append(Transfer("restrict<-fineB", "coarse", ["fineB"], [0.0],
+                            Box((34, 2, 2), (64, 64, 64), (2, 2, 2)), Box((34, 2, 2), (64, 64, 64), (2, 2, 2)),
+                            0.0, OP_LAGRANGE, VERTEX_CENTERED, 3, 0, phase=1))
+    # sync: A's upper x ghosts from B, B's lower x ghosts from A
+    ga = Box((34, 2, 2), (36, 65, 65), (1, 1, 1))
+    gb = Box((31, 2, 2), (33, 65, 65), (1, 1, 1))
+    w.steps.append(Transfer("sync A<-B", "fineA", ["fineB"], [0.0], ga, ga, 0.0, OP_SYNC,
+                            VERTEX_CENTERED, 3, 0, phase=2))
+    w.steps.append(Transfer("sync B<-A", "fineB", ["fineA"], [0.0], gb, gb, 0.0, OP_SYNC,
+                            VERTEX_CENTERED, 3, 0, phase=2))
+    w.outputs = ["coarse", "fineA", "fineB"]
+    return w
+
+
+def config4(n_regions: int = 2000, n_gf: int = 24, edge=(8, 96), seed: int = 4,
+            levels: int = 5, order: int = 5) -> Workload:
+    """Batched uneven level step: n_regions destination regions per grid
+    function with edges uniform in [edge[0], edge[1]] fine points per axis,
+    split evenly between copy sync (ghost width 3 taken from a neighbour
+    component), order-5 Lagrange prolongation with 3-level time interpolation
+    (the refinement-boundary + buffer shell, sources at t = 0, -dt, -2dt
+    interpolated to t = -dt/2) and rf2 restriction.  Regions belong to one of
+    `levels` refinement levels (stride 2^(levels-1-rl) in finest units).
+    Every region has its own destination and source arrays, so the
+    destinations of a phase are disjoint, as dh's lists guarantee.
+    Phases: 0 restriction, 1 prolongation, 2 sync (prolongation precedes
+    sync, CarpetLib's SyncProlongateGroups order)."""
+    rng = np.random.default_rng(seed)
+    w = Workload(f"config4[{n_regions}x{n_gf}]")
+    lo_e, hi_e = edge
+    half = order // 2 + 1
+    times = [0.0, -1.0, -2.0]
+    for r in range(n_regions):
+        n = tuple(int(v) for v in rng.integers(lo_e, hi_e + 1, size=3))
+        rl = int(rng.integers(0, levels))
+        sf = 2 ** (levels - 1 - rl)
+        kind = r % 3
+        base = tuple(int(v) for v in rng.integers(-40, 40, size=3))
+        for g in range(n_gf):
+            tag = f"r{r}g{g}"
+            sd = seed * 1000003 + r * 1009 + g
+            if kind == 0:  # sync: dst ghost slab region from a neighbour array
+                lo = tuple(b * sf for b in base)
+                reg = Box(lo, tuple(l + (k - 1) * sf for l, k in zip(lo, n)), (sf,) * 3)
+                dbox = reg.expand((3, 3, 3), (3, 3, 3))
+                sbox = reg.expand((1, 0, 2), (0, 2, 1))
+                w.arrays[tag + "d"] = ArraySpec(dbox, owned=reg, seed=sd)
+                w.arrays[tag + "s"] = ArraySpec(sbox, seed=sd + 7)
+                w.steps.append(Transfer(tag, tag + "d", [tag + "s"], [0.0], reg, reg, 0.0, OP_LAGRANGE,
+                                        VERTEX_CENTERED, order, 2, phase=2))
+            elif kind == 1:  # prolongation + 3tl
+                sc = 2 * sf
+                lo = tuple((2 * b + int(rng.integers(0, 2))) * sf for b in base)
+                reg = Box(lo, tuple(l + (k - 1) * sf for l, k in zip(lo, n)), (sf,) * 3)
+                clo = tuple((l // sc) - half for l in reg.lower)
+                chi = tuple((u // sc) + half + 1 for u in reg.upper)
+                sbox = Box(tuple(c * sc for c in clo), tuple(c * sc for c in chi), (sc,) * 3)
+                dbox = reg.expand((3, 3, 3), (3, 3, 3))
+                w.arrays[tag + "d"] = ArraySpec(dbox, owned=reg, seed=sd)
+                for tl in range(3):
+                    w.arrays[tag + f"s{tl}"] = ArraySpec(sbox, seed=sd + 11 * (tl + 1))
+                w.steps.append(Transfer(tag, tag + "d", [tag + f"s{tl}" for tl in range(3)], times, reg, reg,
+                                        -0.5, OP_LAGRANGE, VERTEX_CENTERED, order, 2, phase=1))
+            else:  # restriction: coarse region from a fine component
+                sc = 2 * sf
+                m = tuple(max(1, k // 2) for k in n)
+                lo = tuple(b * sc for b in base)
+                reg = Box(lo, tuple(l + (k - 1) * sc for l, k in zip(lo, m)), (sc,) * 3)
+                sbox = Box(tuple(l - 2 * sf for l in reg.lower), tuple(u + 2 * sf for u in reg.upper), (sf,) * 3)
+                dbox = reg.expand((3, 3, 3), (3, 3, 3))
+                w.arrays[tag + "d"] = ArraySpec(dbox, owned=reg, seed=sd)
+                w.arrays[tag + "s"] = ArraySpec(sbox, seed=sd + 5)
+                w.steps.append(Transfer(tag, tag + "d", [tag + "s"], [0.0], reg, reg, 0.0, OP_LAGRANGE,
+                                        VERTEX_CENTERED, order, 2, phase=0))
+            w.outputs.append(tag + "d")
+    return w
+
+
+@dataclass
+class Config2:
+    """Single large patch, FP64 vertex-centred: n_gf grid functions on a
+    coarse 257^3 box (+3 ghosts, 263^3 allocated, stride 2), each with 3 time
+    levels at t = 1, 0, -1; the fine region [2,513]^3 (512^3 points, stride 1)
+    is prolongated with prolongate_3d_rf2<5> fused with interpolate_3d_3tl at
+    the target times t = 1/3 and 2/3.  Each grid function is a sum of 8
+    Fourier modes, amplitude 1/8, wavenumbers in {1..8}^3 and phases uniform
+    in [0, 2 pi) drawn from seed 1000+gf; time level t shifts each phase by
+    0.1 t |k|."""
+
+    n_gf: int = 24
+    n_coarse: int = 257
+    ghosts: int = 3
+    order: int = 5
+    targets: tuple = (1.0 / 3.0, 2.0 / 3.0)
+    times: tuple = (1.0, 0.0, -1.0)
+
+    def boxes(self):
+        g = self.ghosts
+        c = Box((-2 * g,) * 3, (2 * (self.n_coarse - 1 + g),) * 3, (2, 2, 2))
+        n_f = 2 * (self.n_coarse - 1)
+        reg = Box((2,) * 3, (n_f + 1,) * 3, (1, 1, 1))
+        fine = reg.expand((g, g, g), (g, g, g))
+        return c, fine, reg
+
+    def dst_points(self) -> int:
+        return self.boxes()[2].size() * self.n_gf * len(self.targets)
+
+    def alg_bytes(self) -> int:
+        """8 B per fine point written + 8 B per unique coarse source point per
+        time level (the order-5 footprint of the region)."""
+        c, _, reg = self.boxes()
+        n = reg.sizes()[0]
+        foot = (n // 2 + 5) ** 3  # bases 1..256 plus taps -2..+3
+        per = 8 * reg.size() + 8 * len(self.times) * foot
+        return per * self.n_gf * len(self.targets)
+
+    def allocate(self, stream=None):
+        cbox, fbox, reg = self.boxes()
+        owned_c = Box((0,) * 3, (2 * (self.n_coarse - 1),) * 3, (2, 2, 2))
+        src, dst = {}, {}
+        for gf in range(self.n_gf):
+            rng = np.random.default_rng(1000 + gf)
+            kvec = rng.integers(1, 9, size=(8, 3))
+            phase = rng.uniform(0, 2 * np.pi, size=8)
+            for tl, t in enumerate(self.times):
+                a = DeviceArray(cbox, owned=owned_c)
+                ph = phase + 0.1 * t * np.linalg.norm(kvec, axis=1)
+                # x = coarse index / (n_coarse-1): global index / (2*(n_coarse-1))
+                a.fill_fourier(1.0 / (2 * (self.n_coarse - 1)), 1.0 / 8, kvec.tolist(), ph.tolist(), stream)
+                src[(gf, tl)] = a
+            for ti in range(len(self.targets)):
+                d = DeviceArray(fbox, owned=reg)
+                d.fill(np.nan, stream)
+                dst[(gf, ti)] = d
+        return src, dst
+
+    def transfers(self, src, dst):
+        _, _, reg = self.boxes()
+        out = []
+        for gf in range(self.n_gf):
+            for ti, t in enumerate(self.targets):
+                out.append((dst[(gf, ti)], [src[(gf, tl)] for tl in range(len(self.times))],
+                            list(self.times), reg, t))
+        return out
+
+    def batch(self, src, dst, prec=PREC_F64) -> Batch:
+        b = Batch()
+        for d, ss, times, reg, t in self.transfers(src, dst):
+            b.add(0, d, ss, times, reg, reg, t, OP_LAGRANGE, VERTEX_CENTERED, self.order, 2, prec=prec)
+        return b

@@ -1,0 +1,305 @@
+/*
+ * carpet_amd.h -- C ABI of libcarpet_amd.so: CarpetLib's inter-grid transfer
+ * operators (copy, prolongation, restriction, time interpolation) as
+ * hand-written gfx950 (MI355X) HIP kernels.
+ *
+ * Conventions shared by every entry point
+ * ---------------------------------------
+ *  - A box ("ibbox3") is 9 ints laid out exactly like CarpetLib's
+ *    bbox<int,3>: lower[3], upper[3], stride[3]; bounds are inclusive and
+ *    live in the global index space measured in finest-grid units
+ *    (CarpetLib/src/bbox.hh:34-70; the reference itself reinterprets an
+ *    ibbox3 as int[3][3] at CarpetLib/src/data.cc:321-323).
+ *  - An array is a device pointer to a padded 3-D array with x fastest:
+ *    element (i,j,k) sits at i + padext[0]*(j + padext[1]*k)
+ *    (CarpetLib/src/operator_prototypes_3d.hh:17-28).  ext[] must equal the
+ *    array box's sizes() and padext[] >= ext[].
+ *  - srcbbox/dstbbox are the array boxes; srcregbbox/dstregbbox the regions.
+ *    The operators write dst only inside dstregbbox.
+ *  - Every call validates the reference's preconditions on the host before
+ *    anything touches the GPU and returns 0 or a negative CG_ERR_* code; the
+ *    reference aborts through CCTK_ERROR/CCTK_WARN(0) in the same places.
+ *    cg_last_error_message() gives the detail of the last failure of the
+ *    calling thread.
+ *  - Work is enqueued on the given HIP stream (a hipStream_t passed as
+ *    void*; NULL is the default stream) and is not synchronised.
+ *  - Floating-point policy (the canonical order): every stencil sum
+ *    "res += c*x" of the reference is evaluated left to right starting from
+ *    0 as one fused multiply-add per term, res = fma(c, x, res); a
+ *    coincident vertex-centred direction passes the value through without
+ *    arithmetic; time interpolation is s1*a, then fma(s2,b,.), then
+ *    fma(s3,c,.).
+ *
+ * Entry points that have a reference counterpart cite it; the rest say so.
+ */
+#ifndef CARPET_AMD_H
+#define CARPET_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void *cg_stream_t; /* hipStream_t */
+
+/* ---- error codes ------------------------------------------------------ */
+enum {
+  CG_OK = 0,
+  CG_ERR_ARG = -1,          /* null pointer or malformed argument */
+  CG_ERR_STRIDE = -2,       /* "Internal error: strides disagree" */
+  CG_ERR_REFFACT = -3,      /* strides are not related by a factor 2 */
+  CG_ERR_ALIGN = -4,        /* bbox not aligned with vertices / offsets */
+  CG_ERR_EMPTY = -5,        /* "region extent is empty" */
+  CG_ERR_CONTAIN = -6,      /* region (or stencil footprint) not contained */
+  CG_ERR_ORDER = -7,        /* no stencil for this order */
+  CG_ERR_TIME_SAME = -8,    /* "arrays have same time" */
+  CG_ERR_TIME_EXTRAP = -9,  /* "extrapolation in time" */
+  CG_ERR_SHAPE = -10,       /* ext/padext disagree with the array box */
+  CG_ERR_OPERATOR = -11,    /* operator/centering combination unsupported */
+  CG_ERR_HIP = -12,         /* HIP runtime error */
+  CG_ERR_NOMEM = -13,       /* allocation failed */
+  CG_ERR_STATE = -14,       /* batch object used in the wrong state */
+  CG_ERR_DEVICE = -15       /* device kernel reported a failed check */
+};
+
+/* transport operators: same numbering as CarpetLib's enum operator_type
+   (CarpetLib/src/operators.hh:8-25) */
+enum {
+  CG_OP_ERROR = 0,
+  CG_OP_NONE = 1,
+  CG_OP_SYNC = 2,
+  CG_OP_RESTRICT = 3,
+  CG_OP_COPY = 4,
+  CG_OP_LAGRANGE = 5,
+  CG_OP_ENO = 6,
+  CG_OP_WENO = 7,
+  CG_OP_TVD = 8,
+  CG_OP_LAGRANGE_MONOTONE = 9
+};
+
+/* centering: same numbering as CarpetLib's enum centering
+   (CarpetLib/src/defs.hh:134) */
+enum { CG_ERROR_CENTERED = 0, CG_VERTEX_CENTERED = 1, CG_CELL_CENTERED = 2 };
+
+/* precision of prolongation sources/destinations in the batch and fused
+   paths: CG_PREC_F64 is the drop-in; CG_PREC_MIXED stores prolongation
+   sources and destinations in FP32 with compensated FP32 accumulation */
+enum { CG_PREC_F64 = 0, CG_PREC_MIXED = 1 };
+
+/* Slab description for copies: dst[ipos] = src[ipos + offset*stride].
+   Layout of CarpetLib's slab<int,3> (CarpetLib/src/gdata.hh:25-31). */
+typedef struct cg_islab {
+  int offset[3];
+  int is_centered[3];
+} cg_islab;
+
+/* A view of one data<T> storage block (CarpetLib/src/data.hh:20-60):
+   device storage, padded extents, extents and the array box. */
+typedef struct cg_gdata {
+  void *storage;
+  int padext[3];
+  int ext[3];
+  int bbox[9];
+} cg_gdata;
+
+/* ---- library ---------------------------------------------------------- */
+/* no reference counterpart: version and diagnostics */
+int cg_version(void);
+const char *cg_error_string(int code);
+const char *cg_last_error_message(void);
+/* no reference counterpart: selects the HIP device for this thread */
+int cg_set_device(int device);
+int cg_device_count(int *count);
+
+/* ---- device-resident storage ----------------------------------------- */
+/* Padded shape of one array: restates pad_shape(shape, offset)
+   (CarpetLib/src/cacheinfo.cc:40-209) with VECTORISE_ALIGNED_ARRAYS and
+   VECTORISE_ALIGN_FOR_CACHE on and one cache level: x padded to a multiple
+   of vec_size, extents shorter than a line rounded to a power of 2, longer
+   ones to whole lines; align_interior moves the owned interior (offset) to
+   an aligned position.  linesize and vec_size are in elements (linesize a
+   power of 2).  Writes padded_shape[3] and padding_offset[3]. */
+int cg_pad_shape(const int shape[3], const int offset[3], int vec_size,
+                 int linesize, int align_interior, int padded_shape[3],
+                 int padding_offset[3]);
+/* Device allocation of a padded grid-function block (the device analogue of
+   data<T>::allocate, CarpetLib/src/data.cc:355-394 with mem<T>,
+   CarpetLib/src/mem.cc:70-140): x-rows padded to 128 bytes and, when
+   owned_bbox is given, the owned interior aligned to a row boundary; fills
+   out->padext/ext/bbox and out->storage (element (0,0,0)).  Free with
+   cg_gdata_free. */
+int cg_gdata_alloc(cg_gdata *out, const int bbox[9], const int owned_bbox[9],
+                   size_t elem_size);
+int cg_gdata_free(cg_gdata *g);
+/* no reference counterpart: plain device memory helpers for hosts that have
+   no other HIP binding */
+int cg_dev_malloc(void **ptr, size_t bytes);
+int cg_dev_free(void *ptr);
+int cg_memcpy_htod(void *dst, const void *src, size_t bytes);
+int cg_memcpy_dtoh(void *dst, const void *src, size_t bytes);
+int cg_memcpy_dtod(void *dst, const void *src, size_t bytes, cg_stream_t s);
+int cg_memset_d8(void *dst, int value, size_t bytes, cg_stream_t s);
+int cg_fill_f64(double *dst, double value, size_t n, cg_stream_t s);
+/* no reference counterpart: seeded synthetic data for tests and benchmarks;
+   element (i,j,k) of the extent (padding untouched) becomes
+   lo + (hi-lo)*u with u the top 53 bits of splitmix64(seed + i +
+   ext[0]*(j + ext[1]*k)) scaled to [0,1) */
+int cg_fill_hash_f64(double *dst, const int padext[3], const int ext[3],
+                     unsigned long long seed, double lo, double hi, cg_stream_t s);
+/* no reference counterpart: smooth synthetic data; every point of g's
+   extent at global index X becomes amp * sum_m sin(2 pi k_m.(X*scale) +
+   phase_m), nmodes <= 8, kvec[3*m+q] */
+int cg_fill_fourier_f64(const cg_gdata *g, double scale, double amp, int nmodes,
+                        const int *kvec, const double *phase, cg_stream_t s);
+int cg_stream_create(cg_stream_t *s);
+int cg_stream_destroy(cg_stream_t s);
+int cg_stream_synchronize(cg_stream_t s);
+int cg_device_synchronize(void);
+/* events on a stream, for timing kernels on the stream they run on */
+int cg_event_create(void **ev);
+int cg_event_destroy(void *ev);
+int cg_event_record(void *ev, cg_stream_t s);
+int cg_event_elapsed_ms(float *ms, void *start, void *stop);
+
+/* ---- per-call operators (the plain drop-in path) ---------------------- */
+/* copy_3d<double>: CarpetLib/src/copy_3d.cc:25-122. slab may be NULL. */
+int cg_copy_3d_f64(const double *src, const int srcpadext[3],
+                   const int srcext[3], double *dst, const int dstpadext[3],
+                   const int dstext[3], const int srcbbox[9],
+                   const int dstbbox[9], const int srcregbbox[9],
+                   const int dstregbbox[9], const cg_islab *slab,
+                   cg_stream_t stream);
+
+/* prolongate_3d_rf2<double,ORDER>, ORDER in {1,3,5,7,9,11}:
+   CarpetLib/src/prolongate_3d_rf2.cc:356-675 */
+int cg_prolongate_3d_rf2_f64(int order, const double *src,
+                             const int srcpadext[3], const int srcext[3],
+                             double *dst, const int dstpadext[3],
+                             const int dstext[3], const int srcbbox[9],
+                             const int dstbbox[9], const int srcregbbox[9],
+                             const int dstregbbox[9], cg_stream_t stream);
+
+/* prolongate_3d_cc_rf2<double,ORDER>, ORDER in 0..5:
+   CarpetLib/src/prolongate_3d_cc_rf2.cc:238-579 */
+int cg_prolongate_3d_cc_rf2_f64(int order, const double *src,
+                                const int srcpadext[3], const int srcext[3],
+                                double *dst, const int dstpadext[3],
+                                const int dstext[3], const int srcbbox[9],
+                                const int dstbbox[9], const int srcregbbox[9],
+                                const int dstregbbox[9], cg_stream_t stream);
+
+/* prolongate_3d_cc_eno_rf2<double,ORDER>, ORDER in {2,3}:
+   CarpetLib/src/prolongate_3d_cc_eno_rf2.cc:943-1363 */
+int cg_prolongate_3d_cc_eno_rf2_f64(int order, const double *src,
+                                    const int srcpadext[3],
+                                    const int srcext[3], double *dst,
+                                    const int dstpadext[3],
+                                    const int dstext[3], const int srcbbox[9],
+                                    const int dstbbox[9],
+                                    const int srcregbbox[9],
+                                    const int dstregbbox[9],
+                                    cg_stream_t stream);
+
+/* prolongate_3d_cc_tvd<CCTK_REAL8> -> prolongate_3d_cc_real8_tvd:
+   CarpetLib/src/data.cc:309-324, CarpetLib/src/prolongate_3d_cc_real8_tvd.F90:55-244 */
+int cg_prolongate_3d_cc_tvd_f64(const double *src, const int srcpadext[3],
+                                const int srcext[3], double *dst,
+                                const int dstpadext[3], const int dstext[3],
+                                const int srcbbox[9], const int dstbbox[9],
+                                const int srcregbbox[9],
+                                const int dstregbbox[9], cg_stream_t stream);
+
+/* restrict_3d_rf2<double>: CarpetLib/src/restrict_3d_rf2.cc:25-121 */
+int cg_restrict_3d_rf2_f64(const double *src, const int srcpadext[3],
+                           const int srcext[3], double *dst,
+                           const int dstpadext[3], const int dstext[3],
+                           const int srcbbox[9], const int dstbbox[9],
+                           const int srcregbbox[9], const int dstregbbox[9],
+                           cg_stream_t stream);
+
+/* restrict_3d_cc_rf2<double>: CarpetLib/src/restrict_3d_cc_rf2.cc:23-152 */
+int cg_restrict_3d_cc_rf2_f64(const double *src, const int srcpadext[3],
+                              const int srcext[3], double *dst,
+                              const int dstpadext[3], const int dstext[3],
+                              const int srcbbox[9], const int dstbbox[9],
+                              const int srcregbbox[9],
+                              const int dstregbbox[9], cg_stream_t stream);
+
+/* interpolate_3d_2tl<double>: CarpetLib/src/interpolate_3d_2tl.cc:24-117 */
+int cg_interpolate_3d_2tl_f64(const double *src1, double t1,
+                              const double *src2, double t2,
+                              const int srcpadext[3], const int srcext[3],
+                              double *dst, double t, const int dstpadext[3],
+                              const int dstext[3], const int srcbbox[9],
+                              const int dstbbox[9], const int srcregbbox[9],
+                              const int dstregbbox[9], cg_stream_t stream);
+
+/* interpolate_3d_3tl<double>: CarpetLib/src/interpolate_3d_3tl.cc:24-126 */
+int cg_interpolate_3d_3tl_f64(const double *src1, double t1,
+                              const double *src2, double t2,
+                              const double *src3, double t3,
+                              const int srcpadext[3], const int srcext[3],
+                              double *dst, double t, const int dstpadext[3],
+                              const int dstext[3], const int srcbbox[9],
+                              const int dstbbox[9], const int srcregbbox[9],
+                              const int dstregbbox[9], cg_stream_t stream);
+
+/* ---- transfer dispatch ------------------------------------------------ */
+/* data<T>::transfer_from_innerloop -> transfer_time -> transfer_p_r ->
+   {copy_from_innerloop, transfer_prolongate, transfer_restrict} and
+   time_interpolate, with gdata::find_source_timelevel's time-level choice
+   (CarpetLib/src/data.cc:464-1314, CarpetLib/src/gdata.cc:288-349).
+   srcs[0..ntimes-1] are the source time levels (all with the same box),
+   times[] their times.  Where the reference prolongates every time level
+   into a temporary and then interpolates in time, this call runs one fused
+   kernel that keeps the prolongated values in registers; the arithmetic is
+   the same.  use_higher_order_restriction is not supported (CG_ERR_ORDER). */
+int cg_transfer_from(const cg_gdata *dst, const cg_gdata *const *srcs,
+                     const double *times, int ntimes,
+                     const int dstregbbox[9], const int srcregbbox[9],
+                     const cg_islab *slab, double time, int transport_operator,
+                     int centering, int order_space, int order_time,
+                     cg_stream_t stream);
+
+/* ---- batched level step ----------------------------------------------- */
+/* Replaces the per-region walk of ggf::transfer_from_all
+   (CarpetLib/src/ggf.cc:500-591) for a whole level step: the caller adds
+   every (region, variable) transfer once (phases order the dependent
+   steps, e.g. restrict=0, coarse sync=1, prolongate=2, fine sync=3), the
+   batch builds a device-resident descriptor list, and each phase runs as
+   one persistent kernel over it; the whole step can be captured as one
+   hipGraph and replayed while the hierarchy is unchanged. */
+typedef struct cg_batch cg_batch;
+int cg_batch_create(cg_batch **out);
+int cg_batch_destroy(cg_batch *b);
+/* same arguments as cg_transfer_from plus the phase; returns the transfer
+   index (>= 0) or an error code.  prec selects CG_PREC_F64 or
+   CG_PREC_MIXED (mixed only for Lagrange prolongation; dst/src storage
+   then holds float). */
+int cg_batch_add(cg_batch *b, int phase, const cg_gdata *dst,
+                 const cg_gdata *const *srcs, const double *times, int ntimes,
+                 const int dstregbbox[9], const int srcregbbox[9],
+                 const cg_islab *slab, double time, int transport_operator,
+                 int centering, int order_space, int order_time, int prec);
+/* builds tiles and uploads the descriptor list; tile_cfg < 0 lets the
+   tuner choose */
+int cg_batch_finalize(cg_batch *b, int tile_cfg);
+/* runs every phase, one persistent launch per phase */
+int cg_batch_run(cg_batch *b, cg_stream_t stream);
+/* runs the same work as one launch per (region, variable) transfer: the
+   "plain" per-box baseline */
+int cg_batch_run_plain(cg_batch *b, cg_stream_t stream);
+/* captures cg_batch_run into a hipGraph, then replays it */
+int cg_batch_capture_graph(cg_batch *b, cg_stream_t stream);
+int cg_batch_launch_graph(cg_batch *b, cg_stream_t stream);
+/* statistics: number of transfers, destination points and algorithmic
+   bytes (destination + unique source footprint per time level) */
+int cg_batch_stats(const cg_batch *b, long long *ntransfers,
+                   long long *dst_points, long long *alg_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CARPET_AMD_H */

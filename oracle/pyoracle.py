@@ -1,0 +1,200 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes wrapper of the CPU oracle (liboracle.so).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module, as the checker or the timed CPU baseline.  Host arrays are
+padded numpy arrays of shape (padext_z, padext_y, padext_x), x fastest.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboracle.so"
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB
+
+
+if not LIB.exists():
+    build()
+_lib = C.CDLL(str(LIB))
+
+I3 = C.c_int * 3
+I9 = C.c_int * 9
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_OP = (_dp, _ip, _ip, _dp, _ip, _ip, _ip, _ip, _ip, _ip)
+
+
+def _decl(name, restype, *args):
+    f = getattr(_lib, name)
+    f.restype = restype
+    f.argtypes = list(args)
+    return f
+
+
+_decl("orc_set_openmp", None, C.c_int)
+_decl("orc_num_threads", C.c_int)
+_decl("orc_vc_coeff_selftest", C.c_double, C.c_int)
+_decl("orc_cc_coeff_selftest", C.c_double, C.c_int, C.c_int)
+_decl("orc_eno_coeff_selftest", C.c_double, C.c_int, C.c_int, C.c_int)
+_decl("orc_eno1", C.c_double, C.c_int, C.c_int, _dp)
+_decl("orc_eno_window", C.c_int, C.c_int, C.c_int, _ip, _ip)
+_decl("orc_copy_3d", C.c_int, *_OP, _ip)
+_decl("orc_prolongate_3d_rf2", C.c_int, C.c_int, *_OP)
+_decl("orc_prolongate_3d_cc_rf2", C.c_int, C.c_int, *_OP)
+_decl("orc_prolongate_3d_cc_eno_rf2", C.c_int, C.c_int, *_OP)
+_decl("orc_prolongate_3d_cc_tvd", C.c_int, *_OP)
+_decl("orc_restrict_3d_rf2", C.c_int, *_OP)
+_decl("orc_restrict_3d_cc_rf2", C.c_int, *_OP)
+_decl("orc_time_factors", C.c_int, C.c_int, _dp, C.c_double, _dp)
+_decl("orc_interpolate_3d", C.c_int, C.c_int, C.POINTER(_dp), _dp, _ip, _ip, _dp, C.c_double,
+      _ip, _ip, _ip, _ip, _ip, _ip)
+_decl("orc_find_source_timelevel", C.c_int, _dp, C.c_int, C.c_double, C.c_int, C.c_int, _ip, _ip)
+_decl("orc_prolongate_time", C.c_int, C.c_int, C.c_int, C.POINTER(_dp), _dp, C.c_int, _ip, _ip, _dp,
+      C.c_double, _ip, _ip, _ip, _ip, _ip, C.c_int)
+
+KIND_VLAG, KIND_CLAG, KIND_ENO, KIND_TVD = 0, 1, 2, 3
+
+
+@dataclass
+class HostArr:
+    """A padded host array with its bbox (lower, upper, stride)."""
+
+    box: tuple  # (lower, upper, stride) each a 3-tuple
+    data: np.ndarray  # (padz, pady, padx) float64
+
+    @property
+    def padext(self):
+        return (self.data.shape[2], self.data.shape[1], self.data.shape[0])
+
+    @property
+    def ext(self):
+        lo, up, st = self.box
+        return tuple((u - l + s) // s if u >= l else 0 for l, u, s in zip(lo, up, st))
+
+    def c_box(self):
+        lo, up, st = self.box
+        return I9(*lo, *up, *st)
+
+    def view(self):
+        e = self.ext
+        return self.data[: e[2], : e[1], : e[0]]
+
+
+def _p(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+def _b(box):
+    lo, up, st = box
+    return I9(*lo, *up, *st)
+
+
+def _args(src: HostArr, dst: HostArr, srcreg, dstreg):
+    return (_p(src.data), I3(*src.padext), I3(*src.ext), _p(dst.data), I3(*dst.padext), I3(*dst.ext),
+            src.c_box(), dst.c_box(), _b(srcreg), _b(dstreg))
+
+
+def set_openmp(on: bool):
+    _lib.orc_set_openmp(int(on))
+
+
+def num_threads() -> int:
+    return _lib.orc_num_threads()
+
+
+def copy_3d(src, dst, srcreg, dstreg, slab_offset=None) -> int:
+    so = I3(*slab_offset) if slab_offset is not None else None
+    return _lib.orc_copy_3d(*_args(src, dst, srcreg, dstreg), so)
+
+
+def prolongate_3d_rf2(order, src, dst, dstreg) -> int:
+    return _lib.orc_prolongate_3d_rf2(order, *_args(src, dst, dstreg, dstreg))
+
+
+def prolongate_3d_cc_rf2(order, src, dst, dstreg) -> int:
+    return _lib.orc_prolongate_3d_cc_rf2(order, *_args(src, dst, dstreg, dstreg))
+
+
+def prolongate_3d_cc_eno_rf2(order, src, dst, dstreg) -> int:
+    return _lib.orc_prolongate_3d_cc_eno_rf2(order, *_args(src, dst, dstreg, dstreg))
+
+
+def prolongate_3d_cc_tvd(src, dst, dstreg) -> int:
+    return _lib.orc_prolongate_3d_cc_tvd(*_args(src, dst, dstreg, dstreg))
+
+
+def restrict_3d_rf2(src, dst, dstreg) -> int:
+    return _lib.orc_restrict_3d_rf2(*_args(src, dst, dstreg, dstreg))
+
+
+def restrict_3d_cc_rf2(src, dst, dstreg) -> int:
+    return _lib.orc_restrict_3d_cc_rf2(*_args(src, dst, dstreg, dstreg))
+
+
+def time_factors(times, t):
+    n = len(times)
+    s = (C.c_double * 3)()
+    tt = (C.c_double * n)(*times)
+    rc = _lib.orc_time_factors(n, tt, t, s)
+    return rc, list(s)[:n]
+
+
+def interpolate_3d(srcs, times, dst, t, dstreg) -> int:
+    n = len(srcs)
+    ptrs = (_dp * n)(*[_p(s.data) for s in srcs])
+    tt = (C.c_double * n)(*times)
+    s0 = srcs[0]
+    return _lib.orc_interpolate_3d(n, ptrs, tt, I3(*s0.padext), I3(*s0.ext), _p(dst.data), t,
+                                   I3(*dst.padext), I3(*dst.ext), s0.c_box(), dst.c_box(),
+                                   _b(dstreg), _b(dstreg))
+
+
+def find_source_timelevel(times, time, order_time, op):
+    tl0, ntl = C.c_int(), C.c_int()
+    tt = (C.c_double * len(times))(*times)
+    rc = _lib.orc_find_source_timelevel(tt, len(times), time, order_time, op, C.byref(tl0), C.byref(ntl))
+    return rc, tl0.value, ntl.value
+
+
+def prolongate_time(kind, order, srcs, times, dst, time, dstreg, order_time) -> int:
+    """data<T>::transfer_time for a prolongation (temporaries + time interpolation)."""
+    n = len(srcs)
+    ptrs = (_dp * n)(*[_p(s.data) for s in srcs])
+    tt = (C.c_double * n)(*times)
+    s0 = srcs[0]
+    return _lib.orc_prolongate_time(kind, order, ptrs, tt, n, I3(*s0.padext), I3(*s0.ext),
+                                    _p(dst.data), time, I3(*dst.padext), I3(*dst.ext), s0.c_box(),
+                                    dst.c_box(), _b(dstreg), order_time)
+
+
+def eno1(order, di, f) -> float:
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    return _lib.orc_eno1(order, di, _p(f))
+
+
+def eno_window(order, di):
+    a, b = C.c_int(), C.c_int()
+    rc = _lib.orc_eno_window(order, di, C.byref(a), C.byref(b))
+    return rc, a.value, b.value
+
+
+def vc_coeff_selftest(order) -> float:
+    return _lib.orc_vc_coeff_selftest(order)
+
+
+def cc_coeff_selftest(order, di) -> float:
+    return _lib.orc_cc_coeff_selftest(order, di)
+
+
+def eno_coeff_selftest(order, di, offset) -> float:
+    return _lib.orc_eno_coeff_selftest(order, di, offset)

@@ -1,0 +1,90 @@
+"""Batched level step on the GPU: persistent launches over the descriptor
+list, the plain per-region launches and hipGraph replays all equal the CPU
+oracle's replay of the same transfers, bit for bit."""
+import numpy as np
+import pytest
+
+from helpers import first_diff, same
+from workload_check import run_oracle
+
+cg = pytest.importorskip("carpet_amd")
+from carpet_amd.workloads import config1, config4  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def _outputs(w, dev):
+    return {k: dev[k].download() for k in w.outputs}
+
+
+def _compare(got, ref, what):
+    for k in ref:
+        if not same(got[k], ref[k]):
+            raise AssertionError(f"{what}: {k} differs {first_diff(got[k], ref[k])}")
+
+
+def _poison_outputs(w, dev, stream):
+    for k in w.outputs:
+        spec = w.arrays[k]
+        a = dev[k]
+        a.fill(np.nan, stream=stream)
+        if spec.seed is not None:
+            a.fill_hash(spec.seed, stream=stream)
+        elif spec.init is not None:
+            stream.synchronize()
+            a.upload(spec.init, fill=np.nan)
+
+
+@pytest.mark.parametrize("make", [lambda: config1(), lambda: config4(n_regions=45, n_gf=2, edge=(4, 40))],
+                         ids=["config1", "config4-small"])
+def test_batch_modes_match_oracle(make):
+    w = make()
+    print(f"workload {w.name}: {len(w.steps)} transfers", flush=True)
+    ref = run_oracle(w)
+    st = cg.Stream()
+    dev = w.allocate(pool=True)
+    b = w.batch(dev)
+    b.finalize()
+    print("stats", b.stats(), flush=True)
+    # persistent batched launches
+    b.run(st)
+    st.synchronize()
+    _compare(_outputs(w, dev), ref, "batched")
+    # plain: one launch per transfer
+    _poison_outputs(w, dev, st)
+    if w.name == "config1":
+        w.free(dev)
+        dev = w.allocate(pool=True)
+        b = w.batch(dev)
+        b.finalize()
+    b.run_plain(st)
+    st.synchronize()
+    _compare(_outputs(w, dev), ref, "plain")
+    # hipGraph: capture once, replay several times with eager work between
+    if w.name == "config1":
+        w.free(dev)
+        dev = w.allocate(pool=True)
+        b = w.batch(dev)
+        b.finalize()
+    b.capture_graph(st)
+    for rep in range(3):
+        if rep and w.name != "config1":
+            _poison_outputs(w, dev, st)
+            st.synchronize()
+        b.launch_graph(st)
+        st.synchronize()
+        print(f"graph replay {rep}", flush=True)
+        if w.name != "config1" or rep == 0:
+            _compare(_outputs(w, dev), ref, f"graph replay {rep}")
+    b.close()
+    w.free(dev)
+
+
+def test_batch_rejects_wrong_state():
+    b = cg.Batch()
+    with pytest.raises(cg.CarpetError):
+        b.run()
+    b.finalize()
+    with pytest.raises(cg.CarpetError):
+        b.finalize()
+    b.close()

@@ -1,0 +1,222 @@
+"""GPU parity of every per-call operator against the CPU oracle (bit-exact).
+
+Each case builds seeded random geometry (region offsets of both parities,
+random padding, regions crossing several tiles and z-chunks), poisons the
+destination and the source padding with NaN, runs the HIP operator through
+the C ABI and the oracle on the same inputs, and compares the whole padded
+destination: the region must match bit for bit and everything outside it
+must be left untouched.
+"""
+import numpy as np
+import pytest
+
+import pyoracle as orc
+from helpers import field, first_diff, host_array, pad_of, prolong_case, same, sizes
+
+cg = pytest.importorskip("carpet_amd")
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(h: orc.HostArr):
+    lo, up, st = h.box
+    a = cg.DeviceArray(cg.Box(lo, up, st), padext=h.padext)
+    a.upload(h.data)
+    return a
+
+
+def _check(expected: orc.HostArr, dev: cg.DeviceArray, what: str):
+    got = dev.download_padded()
+    if not same(got, expected.data):
+        raise AssertionError(f"{what}: mismatch {first_diff(got, expected.data)}")
+
+
+def _src_host(rng, box, kind="mixed"):
+    ext = sizes(box)
+    h = host_array(box, pad_of(ext, rng))
+    h.view()[...] = field(rng, ext, kind)
+    return h
+
+
+CASES = (
+    [("vlag", o) for o in (1, 3, 5, 7, 9, 11)]
+    + [("clag", o) for o in (0, 1, 2, 3, 4, 5)]
+    + [("eno", 2), ("eno", 3), ("tvd", 1)]
+)
+
+
+def _run_prolong(kind, order, src, dst, reg, stream=None):
+    box = cg.Box(*reg)
+    if kind == "vlag":
+        cg.prolongate_3d_rf2(order, src, dst, box, stream=stream)
+    elif kind == "clag":
+        cg.prolongate_3d_cc_rf2(order, src, dst, box, stream=stream)
+    elif kind == "eno":
+        cg.prolongate_3d_cc_eno_rf2(order, src, dst, box, stream=stream)
+    else:
+        cg.prolongate_3d_cc_tvd(src, dst, box, stream=stream)
+
+
+def _orc_prolong(kind, order, src, dst, reg):
+    if kind == "vlag":
+        return orc.prolongate_3d_rf2(order, src, dst, reg)
+    if kind == "clag":
+        return orc.prolongate_3d_cc_rf2(order, src, dst, reg)
+    if kind == "eno":
+        return orc.prolongate_3d_cc_eno_rf2(order, src, dst, reg)
+    return orc.prolongate_3d_cc_tvd(src, dst, reg)
+
+
+@pytest.mark.parametrize("kind,order", CASES)
+def test_prolongate_parity(kind, order):
+    rng = np.random.default_rng(1000 + 17 * order + len(kind))
+    ncases = 6
+    for c in range(ncases):
+        print(f"case {kind} o{order} #{c}", flush=True)
+        if c == 0:
+            n = (1, 1, 1)
+        elif c == 1:
+            n = (150, 3, 70)  # several x tiles and z chunks
+        else:
+            n = None
+        sb, db, rb = prolong_case(rng, kind, order, n=n, scale=1 + (c % 2))
+        data_kind = ("mixed", "smooth", "random")[c % 3]
+        src = _src_host(rng, sb, data_kind)
+        dst = host_array(db, pad_of(sizes(db), rng))
+        exp = orc.HostArr(db, dst.data.copy())
+        assert _orc_prolong(kind, order, src, exp, rb) == 0
+        dsrc, ddst = _dev(src), _dev(dst)
+        _run_prolong(kind, order, dsrc, ddst, rb)
+        _check(exp, ddst, f"{kind} o{order} case {c} reg={rb}")
+
+
+@pytest.mark.parametrize("cell", [False, True])
+def test_restrict_parity(cell):
+    rng = np.random.default_rng(7 + cell)
+    for c in range(6):
+        print(f"restrict cell={cell} #{c}", flush=True)
+        sf = 2 if cell else 1
+        sc = 2 * sf
+        n = tuple(int(v) for v in rng.integers(1, 40, size=3)) if c else (1, 1, 1)
+        rlo, rup, slo, sup, dlo, dup = [], [], [], [], [], []
+        for d in range(3):
+            C0 = int(rng.integers(-4, 5))
+            if cell:
+                rlo.append(sc // 2 + C0 * sc)
+                rup.append(sc // 2 + (C0 + n[d] - 1) * sc)
+                F0 = 2 * C0 - int(rng.integers(0, 3))  # first fine cell of the source
+                slo.append(sf // 2 + F0 * sf)
+                sup.append(sf // 2 + (2 * (C0 + n[d]) + int(rng.integers(0, 3)) - 1) * sf)
+            else:
+                rlo.append(C0 * sc)
+                rup.append((C0 + n[d] - 1) * sc)
+                slo.append(rlo[-1] - int(rng.integers(0, 3)) * sf)
+                sup.append(rup[-1] + int(rng.integers(0, 3)) * sf)
+            dlo.append(rlo[-1] - int(rng.integers(0, 2)) * sc)
+            dup.append(rup[-1] + int(rng.integers(0, 2)) * sc)
+        sb = (tuple(slo), tuple(sup), (sf,) * 3)
+        db = (tuple(dlo), tuple(dup), (sc,) * 3)
+        rb = (tuple(rlo), tuple(rup), (sc,) * 3)
+        src = _src_host(rng, sb, "random")
+        dst = host_array(db, pad_of(sizes(db), rng))
+        exp = orc.HostArr(db, dst.data.copy())
+        f = orc.restrict_3d_cc_rf2 if cell else orc.restrict_3d_rf2
+        assert f(src, exp, rb) == 0
+        dsrc, ddst = _dev(src), _dev(dst)
+        (cg.restrict_3d_cc_rf2 if cell else cg.restrict_3d_rf2)(dsrc, ddst, cg.Box(*rb))
+        _check(exp, ddst, f"restrict cell={cell} case {c}")
+
+
+def test_copy_parity():
+    rng = np.random.default_rng(99)
+    for c in range(8):
+        print(f"copy #{c}", flush=True)
+        s = int(rng.integers(1, 4))
+        n = tuple(int(v) for v in rng.integers(1, 50, size=3))
+        if c == 1:
+            n = (3, 64, 64)  # a ghost slab
+        rlo = tuple(int(v) * s for v in rng.integers(-10, 10, size=3))
+        rup = tuple(l + (k - 1) * s for l, k in zip(rlo, n))
+        off = tuple(int(v) for v in rng.integers(-3, 4, size=3)) if c % 2 else None
+        srlo = tuple(l + (o or 0) * s for l, o in zip(rlo, off or (0, 0, 0)))
+        srup = tuple(u + (o or 0) * s for u, o in zip(rup, off or (0, 0, 0)))
+        sb = (tuple(l - int(rng.integers(0, 3)) * s for l in srlo),
+              tuple(u + int(rng.integers(0, 3)) * s for u in srup), (s,) * 3)
+        db = (tuple(l - int(rng.integers(0, 3)) * s for l in rlo),
+              tuple(u + int(rng.integers(0, 3)) * s for u in rup), (s,) * 3)
+        rb = (rlo, rup, (s,) * 3)
+        srb = (srlo, srup, (s,) * 3)
+        src = _src_host(rng, sb, "random")
+        dst = host_array(db, pad_of(sizes(db), rng))
+        exp = orc.HostArr(db, dst.data.copy())
+        assert orc.copy_3d(src, exp, srb, rb, off) == 0
+        dsrc, ddst = _dev(src), _dev(dst)
+        cg.copy_3d(dsrc, ddst, cg.Box(*srb), cg.Box(*rb), slab_offset=off)
+        _check(exp, ddst, f"copy case {c}")
+
+
+@pytest.mark.parametrize("ntl", [2, 3])
+def test_interpolate_parity(ntl):
+    rng = np.random.default_rng(5 + ntl)
+    for c in range(5):
+        print(f"interp ntl={ntl} #{c}", flush=True)
+        s = 1
+        n = tuple(int(v) for v in rng.integers(1, 40, size=3))
+        rlo = tuple(int(v) for v in rng.integers(-5, 5, size=3))
+        rup = tuple(l + k - 1 for l, k in zip(rlo, n))
+        sb = (tuple(l - 1 for l in rlo), tuple(u + 2 for u in rup), (s,) * 3)
+        db = (tuple(l - 2 for l in rlo), tuple(u + 1 for u in rup), (s,) * 3)
+        rb = (rlo, rup, (s,) * 3)
+        pad = pad_of(sizes(sb), rng)
+        srcs = []
+        for _ in range(ntl):
+            h = host_array(sb, pad)
+            h.view()[...] = field(rng, sizes(sb), "random")
+            srcs.append(h)
+        times = [1.0, 0.0, -1.0][:ntl]
+        t = float(rng.uniform(min(times), max(times)))
+        dst = host_array(db, pad_of(sizes(db), rng))
+        exp = orc.HostArr(db, dst.data.copy())
+        assert orc.interpolate_3d(srcs, times, exp, t, rb) == 0
+        ds = [_dev(h) for h in srcs]
+        ddst = _dev(dst)
+        if ntl == 2:
+            cg.interpolate_3d_2tl(ds[0], times[0], ds[1], times[1], ddst, t, cg.Box(*rb))
+        else:
+            cg.interpolate_3d_3tl(ds[0], times[0], ds[1], times[1], ds[2], times[2], ddst, t, cg.Box(*rb))
+        _check(exp, ddst, f"interp ntl={ntl} case {c}")
+
+
+KIND_IDS = {"vlag": (orc.KIND_VLAG, cg.OP_LAGRANGE, cg.VERTEX_CENTERED),
+            "clag": (orc.KIND_CLAG, cg.OP_LAGRANGE, cg.CELL_CENTERED),
+            "eno": (orc.KIND_ENO, cg.OP_ENO, cg.CELL_CENTERED)}
+
+
+@pytest.mark.parametrize("kind,order,ntl", [("vlag", 5, 3), ("vlag", 3, 2), ("vlag", 1, 3),
+                                            ("clag", 3, 3), ("eno", 3, 2), ("eno", 2, 2)])
+def test_transfer_time_interpolation(kind, order, ntl):
+    """cg_transfer_from with time interpolation (one fused kernel) against the
+    reference's shape (prolongate every level into a temporary, then
+    interpolate_3d_2tl/3tl), data.cc:497-521."""
+    rng = np.random.default_rng(31 * order + ntl)
+    okind, op, cent = KIND_IDS[kind]
+    # ENO order_space 5 -> ENO<3>, 2..4 -> ENO<2> (data.cc:697-704)
+    order_space = {("eno", 3): 5, ("eno", 2): 3}.get((kind, order), order)
+    for c in range(4):
+        print(f"transfer {kind} o{order} ntl{ntl} #{c}", flush=True)
+        sb, db, rb = prolong_case(rng, kind, order, n=(70, 20, 40) if c == 0 else None)
+        pad = pad_of(sizes(sb), rng)
+        srcs = []
+        for _ in range(ntl):
+            h = host_array(sb, pad)
+            h.view()[...] = field(rng, sizes(sb), "mixed")
+            srcs.append(h)
+        times = [0.5, 0.25, 0.0][:ntl]
+        t = min(times) + [0.4, 0.3, 0.1, 0.125][c] * (max(times) - min(times)) / 0.5
+        dst = host_array(db, pad_of(sizes(db), rng))
+        exp = orc.HostArr(db, dst.data.copy())
+        assert orc.prolongate_time(okind, order, srcs, times, exp, t, rb, ntl - 1) == 0
+        ds = [_dev(h) for h in srcs]
+        ddst = _dev(dst)
+        cg.transfer_from(ddst, ds, times, cg.Box(*rb), cg.Box(*rb), t, op, cent, order_space, ntl - 1)
+        _check(exp, ddst, f"transfer {kind} case {c}")

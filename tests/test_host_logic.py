@@ -1,0 +1,157 @@
+"""Host-side logic of libcarpet_amd.so that needs no GPU: the C ABI exports
+every declared symbol, pad_shape follows CarpetLib's rules, and malformed
+operator calls are rejected with the reference's error class before
+anything touches the GPU -- with the same codes the oracle returns."""
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import pyoracle as orc
+from helpers import host_array, sizes
+
+cg = pytest.importorskip("carpet_amd")
+from carpet_amd._lib import I3, lib  # noqa: E402
+
+HEADER = Path(__file__).resolve().parent.parent / "include" / "carpet_amd.h"
+
+
+def test_library_exports_every_declared_symbol():
+    names = set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(cg_\w+)\s*\(", HEADER.read_text(), re.M))
+    assert len(names) > 30
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_error_strings():
+    for code in range(0, -16, -1):
+        assert lib.cg_error_string(code).decode() != "unknown error"
+
+
+def test_pad_shape_rules():
+    # x padded to a multiple of the vector size and to whole cache lines
+    ps, po = cg.pad_shape((33, 33, 33), (0, 0, 0), 4, 8, False)
+    assert ps[0] == 40 and ps[1] * ps[0] % 8 == 0 and po == (0, 0, 0)
+    # extents below one line are rounded to a power of two
+    ps, _ = cg.pad_shape((3, 5, 7), (0, 0, 0), 1, 16, False)
+    assert ps[0] == 4
+    # align_interior moves the owned interior to an aligned x position
+    ps, po = cg.pad_shape((70, 10, 10), (3, 3, 3), 16, 16, True)
+    assert (po[0] + 3) % 16 == 0 and po[0] + 70 <= ps[0]
+    # empty arrays are not padded
+    assert cg.pad_shape((0, 4, 4), (0, 0, 0), 16, 16, True)[0] == (0, 4, 4)
+    # the reference's safety bound prod(padded) <= 2*prod(shape)+1000 holds
+    for shape in [(64, 64, 64), (9, 17, 33), (100, 3, 3)]:
+        ps, _ = cg.pad_shape(shape, (0, 0, 0), 16, 16, False)
+        assert np.prod(ps) <= 2 * np.prod(shape) + 1000 or shape[0] < 16
+
+
+FAKE = 0x10000  # never dereferenced: every call below fails validation first
+
+
+def _call_prolong(order, sb, db, rb, sext=None, spad=None, cell=False):
+    sext = sext or sizes(sb)
+    spad = spad or sext
+    dext = sizes(db)
+    f = lib.cg_prolongate_3d_cc_rf2_f64 if cell else lib.cg_prolongate_3d_rf2_f64
+    return f(order, FAKE, I3(*spad), I3(*sext), FAKE, I3(*dext), I3(*dext),
+             cg.Box(*sb).as_c(), cg.Box(*db).as_c(), cg.Box(*rb).as_c(), cg.Box(*rb).as_c(), None)
+
+
+def _orc_prolong(order, sb, db, rb, cell=False):
+    src = host_array(sb, sizes(sb), 0.0)
+    dst = host_array(db, sizes(db), 0.0)
+    f = orc.prolongate_3d_cc_rf2 if cell else orc.prolongate_3d_rf2
+    return f(order, src, dst, rb)
+
+
+S = ((0, 0, 0), (40, 40, 40), (2, 2, 2))
+D = ((0, 0, 0), (40, 40, 40), (1, 1, 1))
+BAD = {
+    "refinement factor": (S[:2] + ((4, 4, 4),), D, ((8, 8, 8), (12, 12, 12), (1, 1, 1)), -3),
+    "strides disagree": (S, D, ((8, 8, 8), (12, 12, 12), (2, 2, 2)), -2),
+    "unaligned source": (((1, 0, 0), (41, 40, 40), (2, 2, 2)), D, ((8, 8, 8), (12, 12, 12), (1, 1, 1)), -4),
+    "empty region": (S, D, ((8, 8, 8), (7, 12, 12), (1, 1, 1)), -5),
+    "outside destination": (S, D, ((8, 8, 8), (44, 12, 12), (1, 1, 1)), -6),
+    "footprint outside source": (S, D, ((1, 8, 8), (12, 12, 12), (1, 1, 1)), -6),
+}
+
+
+@pytest.mark.parametrize("case", sorted(BAD))
+def test_prolongate_validation_matches_oracle(case):
+    sb, db, rb, code = BAD[case]
+    assert _call_prolong(5, sb, db, rb) == code
+    assert _orc_prolong(5, sb, db, rb) == code
+    assert lib.cg_last_error_message().decode()
+
+
+def test_unknown_orders_rejected():
+    rb = ((8, 8, 8), (12, 12, 12), (1, 1, 1))
+    for order in (0, 2, 4, 13):
+        assert _call_prolong(order, S, D, rb) == -7
+    for order in (-1, 6):
+        assert _call_prolong(order, S, D, rb, cell=True) == -7
+    assert lib.cg_prolongate_3d_cc_eno_rf2_f64(4, FAKE, I3(21, 21, 21), I3(21, 21, 21), FAKE, I3(41, 41, 41),
+                                               I3(41, 41, 41), cg.Box(*S).as_c(), cg.Box(*D).as_c(),
+                                               cg.Box(*rb).as_c(), cg.Box(*rb).as_c(), None) == -7
+
+
+def test_shape_and_padding_checks():
+    rb = ((8, 8, 8), (12, 12, 12), (1, 1, 1))
+    # ext must equal the box sizes; padext >= ext
+    assert _call_prolong(5, S, D, rb, sext=(20, 21, 21)) == -10
+    assert _call_prolong(5, S, D, rb, spad=(20, 21, 21)) == -10
+
+
+def test_time_interpolation_validation():
+    box = cg.Box((0, 0, 0), (9, 9, 9), (1, 1, 1)).as_c()
+    e = I3(10, 10, 10)
+    # same time
+    assert lib.cg_interpolate_3d_2tl_f64(FAKE, 1.0, FAKE, 1.0 + 1e-13, e, e, FAKE, 1.0, e, e, box, box, box,
+                                         box, None) == -8
+    # extrapolation
+    assert lib.cg_interpolate_3d_3tl_f64(FAKE, 1.0, FAKE, 0.0, FAKE, -1.0, e, e, FAKE, 1.5, e, e, box, box, box,
+                                         box, None) == -9
+
+
+def test_copy_validation():
+    box = cg.Box((0, 0, 0), (9, 9, 9), (1, 1, 1)).as_c()
+    e = I3(10, 10, 10)
+    reg = cg.Box((2, 2, 2), (5, 5, 5), (1, 1, 1)).as_c()
+    far = cg.Box((2, 2, 2), (12, 5, 5), (1, 1, 1)).as_c()
+    assert lib.cg_copy_3d_f64(FAKE, e, e, FAKE, e, e, box, box, far, far, None, None) == -6
+    s2 = cg.Box((0, 0, 0), (18, 18, 18), (2, 2, 2)).as_c()
+    assert lib.cg_copy_3d_f64(FAKE, e, e, FAKE, e, e, s2, box, reg, reg, None, None) == -2
+    empty = cg.Box((2, 2, 2), (1, 5, 5), (1, 1, 1)).as_c()
+    assert lib.cg_copy_3d_f64(FAKE, e, e, FAKE, e, e, box, box, empty, empty, None, None) == -5
+
+
+def test_transfer_dispatch_validation():
+    """cg_transfer_from rejects operator/centering/order combinations the
+    reference dispatch rejects (data.cc:584-800) before launching."""
+    from carpet_amd._lib import GData
+    c = GData()
+    c.storage = FAKE
+    c.padext[:] = [21, 21, 21]
+    c.ext[:] = [21, 21, 21]
+    c.bbox[:] = [0, 0, 0, 40, 40, 40, 2, 2, 2]
+    f = GData()
+    f.storage = FAKE
+    f.padext[:] = [41, 41, 41]
+    f.ext[:] = [41, 41, 41]
+    f.bbox[:] = [0, 0, 0, 40, 40, 40, 1, 1, 1]
+    srcs = (C.POINTER(GData) * 1)(C.pointer(c))
+    reg = cg.Box((8, 8, 8), (12, 12, 12), (1, 1, 1)).as_c()
+    tt = (C.c_double * 1)(0.0)
+
+    def call(op, cent, order, time=0.0, nt=1, ot=0):
+        return lib.cg_transfer_from(C.byref(f), srcs, tt, nt, reg, reg, None, time, op, cent, order, ot, None)
+
+    assert call(cg.OP_LAGRANGE, cg.VERTEX_CENTERED, 4) == -7  # even vertex order
+    assert call(cg.OP_ENO, cg.VERTEX_CENTERED, 3) == -11  # vertex ENO out of scope
+    assert call(cg.OP_TVD, cg.CELL_CENTERED, 3) == -7  # TVD order 1 only
+    assert call(cg.OP_SYNC, cg.VERTEX_CENTERED, 3) == -11  # op_sync cannot prolongate
+    assert call(cg.OP_NONE, cg.VERTEX_CENTERED, 3) == -11
+    assert call(cg.OP_LAGRANGE, cg.VERTEX_CENTERED, 3, time=5.0) == -9  # extrapolation in time

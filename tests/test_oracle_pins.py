@@ -1,0 +1,445 @@
+"""Pins the CPU oracle to the reference: its own coefficient self-tests, the
+CarpetProlongateTest goldens, closed forms and brute force (no GPU).
+
+Each check is chosen so that a plausible slip in the oracle (a wrong weight,
+tap offset, parity, mirrored table, sign or summation grouping) fails it.
+"""
+import json
+import math
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import pyoracle as orc
+from helpers import field, host_array, pad_of, prolong_case, sizes
+
+GOLD = json.loads((Path(__file__).parent / "golden" / "carpetprolongatetest.json").read_text())["tests"]
+EPS = np.finfo(np.float64).eps
+
+
+# ---- reference self-tests (coeffs1d<RT,ORDER>::test) ----------------------
+@pytest.mark.parametrize("order", [1, 3, 5, 7, 9, 11])
+def test_vertex_coeff_selftest(order):
+    # prolongate_3d_rf2.cc:49-89: reproduce x^n, n <= ORDER, within 1e3 eps
+    assert 0 <= orc.vc_coeff_selftest(order) < 1e3 * EPS
+
+
+@pytest.mark.parametrize("order", range(6))
+@pytest.mark.parametrize("di", [0, 1])
+def test_cell_coeff_selftest(order, di):
+    # prolongate_3d_cc_rf2.cc:75-113
+    assert 0 <= orc.cc_coeff_selftest(order, di) < 1e3 * EPS
+
+
+@pytest.mark.parametrize("order,di,offset", [(o, di, off) for o in (2, 3)
+                                             for di, off in ((0, 0), (1, 0), (0, -1), (1, 1))])
+def test_eno_coeff_selftest(order, di, offset):
+    # prolongate_3d_cc_eno_rf2.cc:957-960 runs exactly these four per order
+    assert 0 <= orc.eno_coeff_selftest(order, di, offset) < 1e-12
+
+
+def test_selftests_reject_broken_weights():
+    # the self-test is a real check: order 3 weights fail the order-5 test
+    # through the oracle's own tables only if the table is wrong, so check
+    # the mathematics it encodes directly on a perturbed table
+    c = np.array([-1, 9, 9, -1]) / 16.0
+    x = np.arange(-1, 3) - 0.5
+    assert all(abs(np.dot(c, x**n) - (n == 0)) < 1e-15 for n in range(4))
+    c[1] += 1e-6
+    assert any(abs(np.dot(c, x**n) - (n == 0)) > 1e-9 for n in range(4))
+
+
+# ---- CarpetProlongateTest polynomial (CarpetProlongateTest/src/init.F90) --
+def density_sum(xx, yy, zz, p):
+    px, py, pz = p
+
+    def density(a, b, c):
+        integral = (2.0 ** (a + 1) / (a + 1)) * (2.0 ** (b + 1) / (b + 1)) * (2.0 ** (c + 1) / (c + 1))
+        return xx**a * yy**b * zz**c - integral / 8.0
+
+    return (0 + 1 * density(px, 0, 0) + 2 * density(0, py, 0) + 3 * density(0, 0, pz)
+            + 4 * density(px, py, 0) + 5 * density(px, 0, pz) + 6 * density(0, py, pz)
+            + 7 * density(px, py, pz))
+
+
+def test_density_matches_golden_exact_values():
+    # u0 columns of the goldens are the exact polynomial values
+    for t, p in (("test_cc_tvd", (1, 1, 1)), ("test_o3", (3, 3, 3)), ("test_cc_o3", (3, 3, 3))):
+        for s in GOLD[t]["samples"]:
+            u0 = density_sum(s["x"] + 1, s["y"] + 1, s["z"] + 1, p)
+            assert abs(u0 - s["u0"]) <= 1e-12 * max(1.0, abs(u0)), (t, s)
+
+
+def _cc_grid(dx_coarse, c_lo, c_hi, f_lo, f_hi):
+    """Cell-centred coarse array over coarse cells [c_lo,c_hi] and fine region
+    cells [f_lo,f_hi] of a grid with faces at -1 + k*dx (integer coordinates
+    in units of dx_coarse/4: coarse cell c at 2+4c, fine cell f at 1+2f)."""
+    sb = ((2 + 4 * c_lo,) * 3, (2 + 4 * c_hi,) * 3, (4,) * 3)
+    rb = ((1 + 2 * f_lo,) * 3, (1 + 2 * f_hi,) * 3, (2,) * 3)
+    xc = -1 + dx_coarse / 2 + dx_coarse * np.arange(c_lo, c_hi + 1)
+    xf = -1 + dx_coarse / 4 + dx_coarse / 2 * np.arange(f_lo, f_hi + 1)
+    return sb, rb, xc, xf
+
+
+def _tvd_du(dx, powers, c_lo, c_hi, f_lo, f_hi):
+    sb, rb, xc, xf = _cc_grid(dx, c_lo, c_hi, f_lo, f_hi)
+    Z, Y, X = np.meshgrid(xc, xc, xc, indexing="ij")
+    src = host_array(sb, sizes(sb))
+    src.view()[...] = density_sum(X + 1, Y + 1, Z + 1, powers)
+    dst = host_array(rb, sizes(rb))
+    assert orc.prolongate_3d_cc_tvd(src, dst, rb) == 0
+    Zf, Yf, Xf = np.meshgrid(xf, xf, xf, indexing="ij")
+    du = dst.view() - density_sum(Xf + 1, Yf + 1, Zf + 1, powers)
+    return du, xf
+
+
+@pytest.mark.parametrize("test,dx,powers", [
+    ("test_cc_tvd", 0.02, (1, 1, 1)), ("test_cc_tvd_hi", 0.01, (1, 1, 1)),
+    ("test_cc_tvd_x", 0.02, (1, 0, 0)), ("test_cc_tvd_y", 0.02, (0, 1, 0)),
+    ("test_cc_tvd_z", 0.02, (0, 0, 1))])
+def test_tvd_reproduces_carpetprolongatetest(test, dx, powers):
+    """prolongate_3d_cc_tvd on the CarpetProlongateTest polynomial: every
+    stored non-zero du sample and the errornorm (max |du| over the three
+    prolongated layers of the refined level) must match the golden."""
+    n_c = round(2 / dx)
+    # refined level: the golden's outermost samples are the 3 prolongated layers
+    xs = sorted({s["x"] for s in GOLD["test_cc_tvd"]["samples"] if s["line"] == "d"}) \
+        if test != "test_cc_tvd_hi" else [-0.2025, 0.2025]
+    edge = max(abs(v) for v in xs)
+    f_hi = round((edge + 1 - dx / 4) / (dx / 2))
+    f_lo = round((-edge + 1 - dx / 4) / (dx / 2))
+    c_lo, c_hi = f_lo // 2 - 2, f_hi // 2 + 2
+    assert 0 <= c_lo and c_hi < n_c
+    du, xf = _tvd_du(dx, powers, c_lo, c_hi, f_lo, f_hi)
+    samples = GOLD[test]["samples"]
+    assert samples
+    for s in samples:
+        i = [int(np.argmin(np.abs(xf - s[a]))) for a in ("x", "y", "z")]
+        got = du[i[2], i[1], i[0]]
+        tol = 2e-13 if abs(s["du"]) > 1e-10 else 1e-14
+        assert abs(got - s["du"]) <= tol, (s, got)
+    # errornorm: the prolongated shell is the 3 outer fine layers
+    shell = np.ones(du.shape, bool)
+    shell[3:-3, 3:-3, 3:-3] = False
+    en = float(np.max(np.abs(du[shell])))
+    gold = GOLD[test]["errornorm"]
+    if gold > 1e-10:
+        assert abs(en - gold) <= 2e-13, (en, gold)
+    else:
+        assert en <= 4 * gold + 1e-15, (en, gold)
+
+
+def _vc_errornorm(order, half_width=0.3):
+    dx = 0.02
+    nf = round(half_width / (dx / 2))
+    # fine points -nf..nf (units dx/2, origin 0), coarse points cover the stencil
+    h = order // 2 + 2
+    c_lo, c_hi = -(nf // 2) - h, nf // 2 + h
+    sb = ((2 * c_lo,) * 3, (2 * c_hi,) * 3, (2,) * 3)
+    rb = ((-nf,) * 3, (nf,) * 3, (1,) * 3)
+    xc = dx * np.arange(c_lo, c_hi + 1)
+    xf = dx / 2 * np.arange(-nf, nf + 1)
+    p = (order,) * 3
+    Z, Y, X = np.meshgrid(xc, xc, xc, indexing="ij")
+    src = host_array(sb, sizes(sb))
+    src.view()[...] = density_sum(X + 1, Y + 1, Z + 1, p)
+    dst = host_array(rb, sizes(rb))
+    assert orc.prolongate_3d_rf2(order, src, dst, rb) == 0
+    Zf, Yf, Xf = np.meshgrid(xf, xf, xf, indexing="ij")
+    return float(np.max(np.abs(dst.view() - density_sum(Xf + 1, Yf + 1, Zf + 1, p))))
+
+
+@pytest.mark.parametrize("order", [1, 3, 5, 7, 9, 11])
+def test_vertex_lagrange_polynomial_exactness(order):
+    """Vertex Lagrange of order p reproduces the degree-p CarpetProlongateTest
+    polynomial (test_o<p>) to rounding: the max error over the whole refined
+    box stays within a few times the stored errornorm."""
+    en = _vc_errornorm(order, half_width=0.3 if order < 9 else 0.1)
+    gold = GOLD[f"test_o{order}"]["errornorm"]
+    assert en <= 8 * gold, (en, gold)
+    # and a one-order-too-high polynomial is *not* reproduced (pins the degree)
+    dx = 0.02
+    c = np.arange(-8, 9)
+    x = dx * c
+    sb = ((2 * -8,) * 3, (2 * 8,) * 3, (2,) * 3)
+    rb = ((-3,) * 3, (3,) * 3, (1,) * 3)
+    src = host_array(sb, sizes(sb))
+    Z, Y, X = np.meshgrid(x, x, x, indexing="ij")
+    src.view()[...] = (X / dx) ** (order + 1)
+    dst = host_array(rb, sizes(rb))
+    assert orc.prolongate_3d_rf2(order, src, dst, rb) == 0
+    xf = np.arange(-3, 4) / 2.0
+    exact = (xf[None, None, :]) ** (order + 1) * np.ones((7, 7, 1))
+    assert np.max(np.abs(dst.view() - exact)) > 1e-3
+
+
+@pytest.mark.parametrize("order", range(6))
+def test_cell_lagrange_polynomial_exactness(order):
+    """Cell-centred Lagrange of order p reproduces the degree-p polynomial of
+    test_cc_o<p> to rounding (bounded by the stored errornorm)."""
+    dx = 0.02
+    f_lo, f_hi = 70, 129
+    c_lo, c_hi = f_lo // 2 - 4, f_hi // 2 + 4
+    sb, rb, xc, xf = _cc_grid(dx, c_lo, c_hi, f_lo, f_hi)
+    p = (order,) * 3
+    Z, Y, X = np.meshgrid(xc, xc, xc, indexing="ij")
+    src = host_array(sb, sizes(sb))
+    src.view()[...] = density_sum(X + 1, Y + 1, Z + 1, p)
+    dst = host_array(rb, sizes(rb))
+    assert orc.prolongate_3d_cc_rf2(order, src, dst, rb) == 0
+    Zf, Yf, Xf = np.meshgrid(xf, xf, xf, indexing="ij")
+    en = float(np.max(np.abs(dst.view() - density_sum(Xf + 1, Yf + 1, Zf + 1, p))))
+    gold = GOLD[f"test_cc_o{order}"]["errornorm"]
+    assert en <= 8 * gold + 8 * EPS * 100, (en, gold)
+
+
+# ---- brute force: tensor-product Lagrange through the nearest nodes -------
+def _lagrange_weights(nodes, x):
+    w = []
+    for i, xi in enumerate(nodes):
+        v = 1.0
+        for j, xj in enumerate(nodes):
+            if j != i:
+                v *= (x - xj) / (xi - xj)
+        w.append(v)
+    return np.array(w)
+
+
+def _nearest_nodes(centres, x, n):
+    d = np.abs(centres - x)
+    order = np.argsort(d, kind="stable")
+    return np.sort(order[:n])
+
+
+@pytest.mark.parametrize("kind,order", [("vlag", o) for o in (1, 3, 5)] + [("clag", o) for o in range(6)])
+def test_lagrange_brute_force(kind, order):
+    """Every fine value equals the tensor-product Lagrange interpolant through
+    the order+1 nearest coarse nodes per axis, evaluated from the fine point's
+    physical coordinate (no tap tables involved)."""
+    rng = np.random.default_rng(order + 10 * (kind == "clag"))
+    for _ in range(4):
+        sb, db, rb = prolong_case(rng, kind, order, maxn=7)
+        src = host_array(sb, pad_of(sizes(sb), rng))
+        src.view()[...] = rng.uniform(-1, 1, size=src.view().shape)
+        dst = host_array(db, pad_of(sizes(db), rng))
+        f = orc.prolongate_3d_rf2 if kind == "vlag" else orc.prolongate_3d_cc_rf2
+        assert f(order, src, dst, rb) == 0
+        (slo, sup, sst), (dlo, _, dst_st), (rlo, rup, rst) = sb, db, rb
+        cs = [slo[d] + sst[d] * np.arange(sizes(sb)[d]) for d in range(3)]
+        vals = src.view()
+        out = dst.view()
+        for k in range(rlo[2], rup[2] + 1, rst[2]):
+            for j in range(rlo[1], rup[1] + 1, rst[1]):
+                for i in range(rlo[0], rup[0] + 1, rst[0]):
+                    X = (i, j, k)
+                    idx, wts = [], []
+                    for d in range(3):
+                        if kind == "vlag" and (X[d] - slo[d]) % sst[d] == 0:
+                            nodes = np.array([(X[d] - slo[d]) // sst[d]])
+                            w = np.array([1.0])
+                        else:
+                            nodes = _nearest_nodes(cs[d], X[d], order + 1)
+                            w = _lagrange_weights(cs[d][nodes].astype(float), float(X[d]))
+                        idx.append(nodes)
+                        wts.append(w)
+                    block = vals[np.ix_(idx[2], idx[1], idx[0])]
+                    exp = np.einsum("k,j,i,kji->", wts[2], wts[1], wts[0], block)
+                    got = out[(k - dlo[2]) // dst_st[2], (j - dlo[1]) // dst_st[1], (i - dlo[0]) // dst_st[0]]
+                    assert abs(got - exp) <= 1e-12 * (1 + np.max(np.abs(block))), (X, got, exp)
+
+
+# ---- TVD / cell restriction conservation ----------------------------------
+def test_tvd_then_cc_restrict_conserves():
+    rng = np.random.default_rng(3)
+    for _ in range(3):
+        nc = int(rng.integers(3, 12))
+        # coarse cells -1..nc (one ghost), fine region = children of 0..nc-1
+        sb = ((2 - 4,) * 3, (2 + 4 * nc,) * 3, (4,) * 3)
+        rb = ((1,) * 3, (1 + 2 * (2 * nc - 1),) * 3, (2,) * 3)
+        src = host_array(sb, sizes(sb))
+        src.view()[...] = field(rng, sizes(sb), "mixed")
+        fine = host_array(rb, sizes(rb))
+        assert orc.prolongate_3d_cc_tvd(src, fine, rb) == 0
+        cb = ((2,) * 3, (2 + 4 * (nc - 1),) * 3, (4,) * 3)
+        back = host_array(cb, sizes(cb))
+        assert orc.restrict_3d_cc_rf2(fine, back, cb) == 0
+        orig = src.view()[1:-1, 1:-1, 1:-1]
+        rel = np.max(np.abs(back.view() - orig)) / np.max(np.abs(orig))
+        assert rel <= 1e-14, rel
+
+
+def test_cc_restrict_is_eight_child_mean():
+    rng = np.random.default_rng(4)
+    n = 6
+    sb = ((1,) * 3, (1 + 2 * (2 * n - 1),) * 3, (2,) * 3)
+    rb = ((2,) * 3, (2 + 4 * (n - 1),) * 3, (4,) * 3)
+    src = host_array(sb, sizes(sb))
+    v = rng.uniform(-1, 1, size=src.view().shape)
+    src.view()[...] = v
+    dst = host_array(rb, sizes(rb))
+    assert orc.restrict_3d_cc_rf2(src, dst, rb) == 0
+    acc = np.zeros((n, n, n))
+    for dz, dy, dx in [(0, 0, 0), (0, 0, 1), (0, 1, 0), (0, 1, 1), (1, 0, 0), (1, 0, 1), (1, 1, 0), (1, 1, 1)]:
+        acc = acc + 0.125 * v[dz::2, dy::2, dx::2]
+    assert np.array_equal(dst.view(), acc)
+
+
+def test_injection_and_copy_identity():
+    rng = np.random.default_rng(5)
+    sb = ((0,) * 3, (20,) * 3, (1,) * 3)
+    src = host_array(sb, sizes(sb))
+    v = rng.uniform(-1, 1, size=src.view().shape)
+    src.view()[...] = v
+    rb = ((2,) * 3, (18,) * 3, (2,) * 3)
+    dst = host_array(((0,) * 3, (20,) * 3, (2,) * 3), (11, 11, 11))
+    assert orc.restrict_3d_rf2(src, dst, rb) == 0
+    assert np.array_equal(dst.view()[1:10, 1:10, 1:10], v[2:19:2, 2:19:2, 2:19:2])
+    # copy with a periodic slab offset: dst[ipos] = src[ipos + offset*stride]
+    d2 = host_array(sb, sizes(sb))
+    reg = ((1, 2, 3), (5, 6, 7), (1, 1, 1))
+    sreg = ((4, 1, 3), (8, 5, 7), (1, 1, 1))
+    assert orc.copy_3d(src, d2, sreg, reg, (3, -1, 0)) == 0
+    assert np.array_equal(d2.view()[3:8, 2:7, 1:6], v[3:8, 1:6, 4:9])
+    assert np.isnan(d2.view()[0]).all()
+
+
+# ---- ENO -------------------------------------------------------------------
+@pytest.mark.parametrize("order", [2, 3])
+def test_eno_bounded_by_nearest_coarse_values(order):
+    """Each nested 1-D ENO step falls back to linear unless the value lies
+    between the two nearest inputs (prolongate_3d_cc_eno_rf2.cc:378-408), so
+    every fine value lies within the min/max of its 2x2x2 nearest cells."""
+    rng = np.random.default_rng(order)
+    for c in range(4):
+        sb, db, rb = prolong_case(rng, "eno", order, maxn=16)
+        src = host_array(sb, pad_of(sizes(sb), rng))
+        src.view()[...] = field(rng, sizes(sb), ("mixed", "random")[c % 2])
+        dst = host_array(db, pad_of(sizes(db), rng))
+        assert orc.prolongate_3d_cc_eno_rf2(order, src, dst, rb) == 0
+        (slo, _, sst), (dlo, _, dstst), (rlo, rup, rst) = sb, db, rb
+        v = src.view()
+        for k in range(rlo[2], rup[2] + 1, rst[2]):
+            for j in range(rlo[1], rup[1] + 1, rst[1]):
+                for i in range(rlo[0], rup[0] + 1, rst[0]):
+                    # nearest two coarse centres per axis
+                    ids = []
+                    for d, X in enumerate((i, j, k)):
+                        t = (X - slo[d]) / sst[d]
+                        ids.append([int(math.floor(t)), int(math.floor(t)) + 1])
+                    blk = v[np.ix_(ids[2], ids[1], ids[0])]
+                    got = dst.view()[(k - dlo[2]) // dstst[2], (j - dlo[1]) // dstst[1], (i - dlo[0]) // dstst[0]]
+                    assert blk.min() - 1e-13 <= got <= blk.max() + 1e-13
+
+
+def test_eno_exact_on_smooth_monotone_polynomials():
+    """Quadratic (ENO<2>) and cubic (ENO<3>) monotone data keep one sign of
+    the undivided differences, pass the monotonicity check and are
+    reproduced by the selected stencil."""
+    for order, deg in ((2, 2), (3, 3)):
+        f_lo, f_hi = 20, 41
+        c_lo, c_hi = f_lo // 2 - 4, f_hi // 2 + 4
+        sb, rb, xc, xf = _cc_grid(0.02, c_lo, c_hi, f_lo, f_hi)
+        P = lambda x: (x + 3.0) ** deg
+        Z, Y, X = np.meshgrid(xc, xc, xc, indexing="ij")
+        src = host_array(sb, sizes(sb))
+        src.view()[...] = P(X) * P(Y) * P(Z)
+        dst = host_array(rb, sizes(rb))
+        assert orc.prolongate_3d_cc_eno_rf2(order, src, dst, rb) == 0
+        Zf, Yf, Xf = np.meshgrid(xf, xf, xf, indexing="ij")
+        exact = P(Xf) * P(Yf) * P(Zf)
+        assert np.max(np.abs(dst.view() - exact) / exact) < 1e-13
+
+
+@pytest.mark.parametrize("order", [2, 3])
+@pytest.mark.parametrize("di", [0, 1])
+def test_eno_line_fallbacks(order, di):
+    """The 1-D ENO operator on hand-made lines: sign changes of the undivided
+    differences and non-monotone results fall back to the linear {1/4,3/4}
+    interpolant; smooth data of degree ORDER is reproduced exactly."""
+    rc, mm, mx = orc.eno_window(order, di)
+    assert rc == 0 and mx - mm == (4 if order == 2 else 6)
+    x = np.arange(mm, mx, dtype=float)
+    a, b = -1 + di, di  # offsets of the two nearest coarse values
+
+    # nearest values 1 and 3 (exact arithmetic); alternating curvature makes
+    # the undivided differences change sign -> linear
+    f = np.array([(-1.0) ** k * 4 for k in range(mx - mm)])
+    f[a - mm], f[b - mm] = 1.0, 3.0
+    f[a - mm - 1] = 9.0
+    got = orc.eno1(order, di, f)
+    exp = 0.25 * 1.0 + 0.75 * 3.0 if di == 0 else 0.75 * 1.0 + 0.25 * 3.0
+    assert got == exp
+    # a step between the two nearest values: result stays between them
+    f = np.where(x < 0.5 * (a + b), 1.0, 10.0)
+    got = orc.eno1(order, di, f)
+    assert min(f[a - mm], f[b - mm]) <= got <= max(f[a - mm], f[b - mm])
+    # smooth monotone polynomial of degree ORDER: reproduced (coarse value i
+    # sits at i + 1/2, the fine point at 1/4 + di/2)
+    f = (x + 0.5 + 7.0) ** order
+    got = orc.eno1(order, di, f)
+    assert got == pytest.approx((0.25 + 0.5 * di + 7.0) ** order, rel=1e-14)
+    # one degree lower for ENO<3>: third differences vanish -> linear
+    if order == 3:
+        f = (x + 0.5 + 7.0) ** 2
+        got = orc.eno1(order, di, f)
+        fa, fb = f[a - mm], f[b - mm]
+        exp = (0.25 * fa + 0.75 * fb) if di == 0 else (0.75 * fa + 0.25 * fb)
+        assert got == pytest.approx(exp, rel=1e-15)
+
+
+# ---- time interpolation -----------------------------------------------------
+def test_time_factors_reproduce_polynomials_in_time():
+    for times in ([1.0, 0.0], [1.0, 0.0, -1.0], [0.3, 0.2, 0.1]):
+        n = len(times)
+        for t in np.linspace(min(times), max(times), 7):
+            rc, s = orc.time_factors(times, float(t))
+            assert rc == 0
+            for deg in range(n):
+                val = sum(si * ti**deg for si, ti in zip(s, times))
+                assert val == pytest.approx(t**deg, abs=1e-14)
+    assert orc.time_factors([1.0, 1.0 + 1e-13], 1.0)[0] == -8  # CG_ERR_TIME_SAME
+    assert orc.time_factors([1.0, 0.0], 1.1)[0] == -9  # CG_ERR_TIME_EXTRAP
+    assert orc.time_factors([1.0, 0.0, -1.0], -1.0 - 1e-13)[0] == 0  # within eps
+
+
+def test_find_source_timelevel():
+    # exact match (within 1e-12 of |min|+|max|+1) -> that level only
+    assert orc.find_source_timelevel([1.0, 0.0, -1.0], 0.0, 2, 5) == (0, 1, 1)
+    assert orc.find_source_timelevel([1.0, 0.0, -1.0], 1e-13, 2, 5) == (0, 1, 1)
+    # otherwise order_time+1 levels starting at 0
+    assert orc.find_source_timelevel([1.0, 0.0, -1.0], 0.5, 2, 5) == (0, 0, 3)
+    assert orc.find_source_timelevel([1.0, 0.0], 0.5, 1, 5) == (0, 0, 2)
+    # op_copy takes level 0 and never extrapolates
+    assert orc.find_source_timelevel([1.0, 0.0], 7.0, 1, 4) == (0, 0, 1)
+    assert orc.find_source_timelevel([1.0, 0.0], 7.0, 1, 5)[0] == -9
+    assert orc.find_source_timelevel([2.0], 0.0, 0, 5)[0] == -9
+
+
+def test_prolongate_time_matches_temporaries_shape():
+    """transfer_time: prolongating every level then interpolating in time
+    equals prolongating the time-interpolated coarse data to rounding (the
+    operators are linear), and equals level 0 when t hits a source time."""
+    rng = np.random.default_rng(8)
+    sb, db, rb = prolong_case(rng, "vlag", 5, maxn=12)
+    pad = pad_of(sizes(sb), rng)
+    srcs = []
+    for _ in range(3):
+        h = host_array(sb, pad)
+        h.view()[...] = field(rng, sizes(sb), "smooth")
+        srcs.append(h)
+    times = [1.0, 0.0, -1.0]
+    dst = host_array(db, sizes(db))
+    assert orc.prolongate_time(orc.KIND_VLAG, 5, srcs, times, dst, 1 / 3, rb, 2) == 0
+    rc, s = orc.time_factors(times, 1 / 3)
+    mix = host_array(sb, pad)
+    mix.data[...] = s[0] * srcs[0].data + s[1] * srcs[1].data + s[2] * srcs[2].data
+    d2 = host_array(db, sizes(db))
+    assert orc.prolongate_3d_rf2(5, mix, d2, rb) == 0
+    m = ~np.isnan(dst.data)
+    assert np.max(np.abs(dst.data[m] - d2.data[m])) < 1e-13
+    d3 = host_array(db, sizes(db))
+    assert orc.prolongate_time(orc.KIND_VLAG, 5, srcs, times, d3, 0.0, rb, 2) == 0
+    d4 = host_array(db, sizes(db))
+    assert orc.prolongate_3d_rf2(5, srcs[1], d4, rb) == 0
+    assert np.array_equal(d3.data, d4.data, equal_nan=True)
