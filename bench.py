@@ -77,6 +77,24 @@ def cpu_baseline(target_seconds: float = 12.0):
                       f"{n[0]}x{n[1]}x{n[2]} fine points, {reps} repetitions in {el:.1f} s"}
 
 
+def calib_copy(n: int = 512):
+    """One copy_3d of an n^3 FP64 array (known traffic: 8 n^3 bytes read and
+    written) so that a profiled run can scale its HBM counters."""
+    import numpy as np
+
+    import carpet_amd as cg
+
+    box = cg.Box((0, 0, 0), (n - 1,) * 3, (1, 1, 1))
+    a, b = cg.DeviceArray(box), cg.DeviceArray(box)
+    a.fill(1.0)
+    b.fill(0.0)
+    cg.copy_3d(a, b, box, box)
+    cg.synchronize()
+    a.free()
+    b.free()
+    return 8 * n**3
+
+
 def run_config2(steps: int, warmup: int, n_gf: int, plain_steps: int):
     import carpet_amd as cg
     from carpet_amd.workloads import Config2
@@ -88,6 +106,7 @@ def run_config2(steps: int, warmup: int, n_gf: int, plain_steps: int):
     b = cfg.batch(src, dst)
     b.finalize()
     stats = b.stats()
+    print(b.describe(), file=sys.stderr, flush=True)
     b.capture_graph(st)
     for _ in range(warmup):
         b.launch_graph(st)
@@ -120,6 +139,46 @@ def run_config2(steps: int, warmup: int, n_gf: int, plain_steps: int):
     return cfg, stats, ms_step, wall, ms_plain
 
 
+def run_config4(n_regions: int, n_gf: int, steps: int, warmup: int, plain_steps: int):
+    """Batched uneven hierarchy: n_regions regions (copy sync / O=5 prolongation
+    with 3 time levels / rf2 restriction, edges uniform in [8,96]) per grid
+    function; batched graph replay vs one launch per transfer."""
+    import carpet_amd as cg
+    from carpet_amd.workloads import config4
+
+    w = config4(n_regions=n_regions, n_gf=n_gf)
+    t0 = time.perf_counter()
+    dev = w.allocate(pool=True)
+    st = cg.Stream()
+    b = w.batch(dev)
+    b.finalize()
+    print(f"# config4: {len(w.steps)} transfers, {len(w.arrays)} arrays, set up in "
+          f"{time.perf_counter() - t0:.1f} s\n" + b.describe(), file=sys.stderr, flush=True)
+    stats = b.stats()
+    b.capture_graph(st)
+    for _ in range(warmup):
+        b.launch_graph(st)
+    st.synchronize()
+    e0, e1 = cg.Event(), cg.Event()
+    e0.record(st)
+    for _ in range(steps):
+        b.launch_graph(st)
+    e1.record(st)
+    st.synchronize()
+    ms_b = e0.elapsed_ms(e1) / steps
+    b.run_plain(st)
+    st.synchronize()
+    e0.record(st)
+    for _ in range(plain_steps):
+        b.run_plain(st)
+    e1.record(st)
+    st.synchronize()
+    ms_p = e0.elapsed_ms(e1) / plain_steps
+    b.close()
+    w.free(dev)
+    return stats, ms_b, ms_p
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -129,13 +188,45 @@ def main(argv=None):
     ap.add_argument("--plain-steps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--config", type=int, default=2, choices=(2, 4),
+                    help="2: single large patch (default, the headline line); 4: batched uneven hierarchy")
+    ap.add_argument("--n-regions", type=int, default=2000)
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="run one 1 GiB copy_3d first (HBM counter calibration under rocprofv3)")
     args = ap.parse_args(argv)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the transfer operators of a level step run on a single "
               "MI355X here (sharding across GPUs is a design note, not a measured path); "
               "run with --gpus 1", file=sys.stderr)
         return 2
+    if args.config == 4:
+        n_gf = args.n_gf
+        stats, ms_b, ms_p = run_config4(args.n_regions, n_gf, args.steps, args.warmup, args.plain_steps)
+        pts, alg = stats["dst_points"], stats["alg_bytes"]
+        line = {"metric": METRIC, "value": round(pts / (ms_b * 1e-3) / 1e9, 3), "unit": "Gpoints/s",
+                "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_b, 4),
+                "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+                "data": "synthetic",
+                "config": {"workload": f"batched uneven level step: {args.n_regions} regions x {n_gf} grid "
+                                       "functions (edges U[8,96]; copy sync, O=5 prolongation + 3 time levels, "
+                                       "rf2 restriction), batched persistent launches + hipGraph",
+                           "transfers": stats["transfers"], "dst_points_per_step": pts,
+                           "algorithmic_bytes_per_step": alg},
+                "gbytes_per_s": round(alg / (ms_b * 1e-3) / 1e9, 1),
+                "plain": {"ms_per_step": round(ms_p, 4), "gpoints_per_s": round(pts / (ms_p * 1e-3) / 1e9, 3),
+                          "launches_per_step": stats["transfers"]},
+                "batched_vs_plain": round(ms_p / ms_b, 3),
+                "roofline": {"bound": "hbm", "achieved": round(alg / (ms_b * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                             "unit": "GB/s", "frac": round(alg / (ms_b * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                             "traffic": None}}
+        print(json.dumps(line))
+        return 0
+    if args.calib_copy:
+        nbytes = calib_copy()
+        print(f"# calibration copy: {nbytes} bytes read and written", file=sys.stderr, flush=True)
+    print(f"# config2: {args.n_gf} grid functions, {args.steps} steps", file=sys.stderr, flush=True)
     cfg, stats, ms_step, wall, ms_plain = run_config2(args.steps, args.warmup, args.n_gf, args.plain_steps)
+    print(f"# timed: {ms_step:.3f} ms/step batched, {ms_plain:.3f} ms/step plain", file=sys.stderr, flush=True)
     pts = cfg.dst_points()
     alg = cfg.alg_bytes()
     gpts = pts / (ms_step * 1e-3) / 1e9

@@ -30,6 +30,10 @@ from .core import (  # noqa: F401
     prolongate_3d_cc_rf2,
     prolongate_3d_cc_tvd,
     prolongate_3d_rf2,
+    prolongate_3d_rf2_mixed,
+    prolongate_3d_cc_rf2_mixed,
+    relative_errors,
+    transfer_from_mixed,
     restrict_3d_cc_rf2,
     restrict_3d_rf2,
     synchronize,

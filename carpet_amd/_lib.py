@@ -107,3 +107,9 @@ def check(code: int, where: str) -> int:
     if code < 0:
         raise CarpetError(code, where)
     return code
+
+_decl("cg_prolongate_3d_rf2_mixed", i, i, *_OPARGS, vp)
+_decl("cg_prolongate_3d_cc_rf2_mixed", i, i, *_OPARGS, vp)
+_decl("cg_transfer_from_mixed", i, _GP, C.POINTER(_GP), C.POINTER(C.c_double), i, ip, ip, dp, i, i, i, i, vp)
+_decl("cg_batch_autotune", i, vp, vp, i, C.c_char_p)
+_decl("cg_batch_describe", i, vp, C.c_char_p, C.c_size_t)

@@ -377,6 +377,16 @@ class Batch:
     def launch_graph(self, stream: Stream):
         check(lib.cg_batch_launch_graph(self.handle, _sh(stream)), "cg_batch_launch_graph")
 
+    def autotune(self, stream=None, reps: int = 3, cache_path: Optional[str] = None):
+        """Searches the work-item sizes per launch class (cg_batch_autotune)."""
+        check(lib.cg_batch_autotune(self.handle, _sh(stream), reps,
+                                    cache_path.encode() if cache_path else None), "cg_batch_autotune")
+
+    def describe(self) -> str:
+        buf = C.create_string_buffer(1 << 16)
+        check(lib.cg_batch_describe(self.handle, buf, len(buf)), "cg_batch_describe")
+        return buf.value.decode()
+
     def stats(self):
         n, p, b = C.c_longlong(), C.c_longlong(), C.c_longlong()
         check(lib.cg_batch_stats(self.handle, C.byref(n), C.byref(p), C.byref(b)), "cg_batch_stats")
@@ -392,3 +402,40 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+# ---- mixed precision (FP32 storage, compensated FP32 sums, FP64 time) -----
+def prolongate_3d_rf2_mixed(order: int, src: DeviceArray, dst: DeviceArray, dstregbbox: Box,
+                            stream=None):
+    assert src.dtype == np.float32 and dst.dtype == np.float32
+    check(lib.cg_prolongate_3d_rf2_mixed(order, *_arr(src), *_arr(dst), src.box.as_c(), dst.box.as_c(),
+                                         dstregbbox.as_c(), dstregbbox.as_c(), _sh(stream)),
+          "prolongate_3d_rf2_mixed")
+
+
+def prolongate_3d_cc_rf2_mixed(order: int, src: DeviceArray, dst: DeviceArray, dstregbbox: Box,
+                               stream=None):
+    assert src.dtype == np.float32 and dst.dtype == np.float32
+    check(lib.cg_prolongate_3d_cc_rf2_mixed(order, *_arr(src), *_arr(dst), src.box.as_c(),
+                                            dst.box.as_c(), dstregbbox.as_c(), dstregbbox.as_c(),
+                                            _sh(stream)),
+          "prolongate_3d_cc_rf2_mixed")
+
+
+def transfer_from_mixed(dst: DeviceArray, srcs: Sequence[DeviceArray], times: Sequence[float],
+                        dstregbbox: Box, srcregbbox: Box, time: float, transport_operator: int,
+                        centering: int, order_space: int, order_time: int, stream=None):
+    tt = (C.c_double * len(times))(*times)
+    check(lib.cg_transfer_from_mixed(C.byref(dst.g), _gptrs(srcs), tt, len(times), dstregbbox.as_c(),
+                                     srcregbbox.as_c(), time, transport_operator, centering,
+                                     order_space, order_time, _sh(stream)),
+          "transfer_from_mixed")
+
+
+def relative_errors(approx: np.ndarray, exact: np.ndarray):
+    """Normwise max and RMS relative error of approx against exact."""
+    a = np.asarray(approx, dtype=np.float64)
+    e = np.asarray(exact, dtype=np.float64)
+    scale = float(np.max(np.abs(e))) or 1.0
+    d = a - e
+    return float(np.max(np.abs(d)) / scale), float(np.sqrt(np.mean(d * d)) / (np.sqrt(np.mean(e * e)) or 1.0))

@@ -23,7 +23,7 @@ int resolve_transfer(const cg_gdata *dst, const cg_gdata *const *srcs,
                      const double *times, int ntimes, const int *dstregbbox,
                      const int *srcregbbox, const cg_islab *slab, double time,
                      int op, int cent, int order_space, int order_time,
-                     Transfer &tr) {
+                     Transfer &tr, int es) {
   if (!dst || !srcs || !times || ntimes <= 0 || !dstregbbox || !srcregbbox)
     return set_error(CG_ERR_ARG, "null argument");
   for (int l = 0; l < ntimes; ++l)
@@ -53,12 +53,17 @@ int resolve_transfer(const cg_gdata *dst, const cg_gdata *const *srcs,
   for (int l = 0; l < ntl && l < 3; ++l)
     sp[l] = srcs[tl0 + l]->storage;
 
+  if (es != 8 && es != 4)
+    return set_error(CG_ERR_ARG, "element size must be 8 or 4");
   bool same = true, coarser = true, finer = true;
   for (int k = 0; k < 3; ++k) {
     same &= s0.box.str[k] == d.box.str[k];
     coarser &= s0.box.str[k] > d.box.str[k];
     finer &= s0.box.str[k] < d.box.str[k];
   }
+  if (es != 8 && !coarser)
+    return set_error(CG_ERR_OPERATOR, "mixed precision applies to prolongation only; copies, "
+                                      "restriction and time interpolation stay FP64");
   if (same) {
     if (ntl == 1) {
       tr.is_prolong = false;
@@ -130,7 +135,7 @@ int resolve_transfer(const cg_gdata *dst, const cg_gdata *const *srcs,
         return set_error(CG_ERR_ORDER, "TVD with time interpolation is not provided");
     }
     tr.is_prolong = true;
-    return plan_prolong(kind, order, s0, sp, ntl, sf, d, dreg, tr.pr);
+    return plan_prolong(kind, order, s0, sp, ntl, sf, d, dreg, tr.pr, es);
   }
   if (finer) {
     if (op == CG_OP_SYNC)
@@ -154,7 +159,7 @@ int resolve_transfer(const cg_gdata *dst, const cg_gdata *const *srcs,
 
 int run_transfer(const Transfer &tr, int prec, hipStream_t st) {
   if (tr.is_prolong)
-    return launch_prolong(tr.pr, prec, st);
+    return launch_prolong(tr.pr, tr.pr.es == 4 ? 1 : prec, st);
   const int r = launch_point(tr.pt.op, tr.pt.a, st);
   if (r == -1)
     return set_error(CG_ERR_ARG, "region too large for the pointwise kernel");
@@ -591,6 +596,54 @@ int cg_interpolate_3d_3tl_f64(const double *src1, double t1, const double *src2,
   const double tt[3] = {t1, t2, t3};
   return interp_call(3, srcs, tt, srcpadext, srcext, dst, t, dstpadext, dstext,
                      srcbbox, dstbbox, dstregbbox, stream);
+}
+
+static int prolong_mixed_call(int kind, int order, const float *src, const int *srcpadext,
+                              const int *srcext, float *dst, const int *dstpadext,
+                              const int *dstext, const int *srcbbox, const int *dstbbox,
+                              const int *dstregbbox, cg_stream_t stream) {
+  if (!srcpadext || !srcext || !dstpadext || !dstext || !srcbbox || !dstbbox || !dstregbbox)
+    return set_error(CG_ERR_ARG, "null argument");
+  ProlongPlan pl;
+  const void *sp[1] = {src};
+  const double s1[1] = {1.0};
+  int e = plan_prolong(kind, order, arr_from(src, srcpadext, srcext, srcbbox), sp, 1, s1,
+                       arr_from(dst, dstpadext, dstext, dstbbox), Box::from(dstregbbox), pl, 4);
+  if (e)
+    return e;
+  return launch_prolong(pl, 1, S(stream));
+}
+
+int cg_prolongate_3d_rf2_mixed(int order, const float *src, const int srcpadext[3],
+                               const int srcext[3], float *dst, const int dstpadext[3],
+                               const int dstext[3], const int srcbbox[9], const int dstbbox[9],
+                               const int srcregbbox[9], const int dstregbbox[9],
+                               cg_stream_t stream) {
+  (void)srcregbbox;
+  return prolong_mixed_call(PK_VLAG, order, src, srcpadext, srcext, dst, dstpadext, dstext,
+                            srcbbox, dstbbox, dstregbbox, stream);
+}
+
+int cg_prolongate_3d_cc_rf2_mixed(int order, const float *src, const int srcpadext[3],
+                                  const int srcext[3], float *dst, const int dstpadext[3],
+                                  const int dstext[3], const int srcbbox[9], const int dstbbox[9],
+                                  const int srcregbbox[9], const int dstregbbox[9],
+                                  cg_stream_t stream) {
+  (void)srcregbbox;
+  return prolong_mixed_call(PK_CLAG, order, src, srcpadext, srcext, dst, dstpadext, dstext,
+                            srcbbox, dstbbox, dstregbbox, stream);
+}
+
+int cg_transfer_from_mixed(const cg_gdata *dst, const cg_gdata *const *srcs,
+                           const double *times, int ntimes, const int dstregbbox[9],
+                           const int srcregbbox[9], double time, int transport_operator,
+                           int centering, int order_space, int order_time, cg_stream_t stream) {
+  Transfer tr;
+  int e = resolve_transfer(dst, srcs, times, ntimes, dstregbbox, srcregbbox, nullptr, time,
+                           transport_operator, centering, order_space, order_time, tr, 4);
+  if (e)
+    return e;
+  return run_transfer(tr, 1, S(stream));
 }
 
 int cg_transfer_from(const cg_gdata *dst, const cg_gdata *const *srcs,

@@ -11,8 +11,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
 #include <cstring>
+#include <fstream>
 #include <new>
+#include <sstream>
 #include <vector>
 
 #include "cg_batch.hpp"
@@ -25,6 +28,8 @@ struct cg_batch {
   std::vector<int> phase;
   std::vector<int> prec;
   cg::BatchDevice dev;
+  cg::TuneMap tuned;
+  int tile_cfg = -1;
   bool finalized = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -67,11 +72,9 @@ int cg_batch_add(cg_batch *b, int phase, const cg_gdata *dst,
   Transfer tr;
   int e = resolve_transfer(dst, srcs, times, ntimes, dstregbbox, srcregbbox, slab,
                            time, transport_operator, centering, order_space,
-                           order_time, tr);
+                           order_time, tr, prec == CG_PREC_MIXED ? 4 : 8);
   if (e)
     return e;
-  if (prec == CG_PREC_MIXED && !(tr.is_prolong && (tr.pr.kind == PK_VLAG)))
-    return set_error(CG_ERR_OPERATOR, "mixed precision applies to Lagrange prolongation only");
   b->transfers.push_back(tr);
   b->phase.push_back(phase);
   b->prec.push_back(prec);
@@ -83,11 +86,131 @@ int cg_batch_finalize(cg_batch *b, int tile_cfg) {
     return set_error(CG_ERR_ARG, "null batch");
   if (b->finalized)
     return set_error(CG_ERR_STATE, "batch already finalized");
-  int e = batch_device_build(b->dev, b->transfers, b->phase, b->prec, tile_cfg);
+  b->tile_cfg = tile_cfg;
+  int e = batch_device_build(b->dev, b->transfers, b->phase, b->prec, tile_cfg, &b->tuned);
   if (e)
     return e;
   b->finalized = true;
   return CG_OK;
+}
+
+// ---- autotuner --------------------------------------------------------------
+static void drop_graph(cg_batch *b) {
+  if (b->exec)
+    (void)hipGraphExecDestroy(b->exec);
+  if (b->graph)
+    (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+}
+
+static void load_cache(const char *path, TuneMap &m) {
+  if (!path)
+    return;
+  std::ifstream f(path);
+  std::string line;
+  while (std::getline(f, line)) {
+    const size_t bar = line.rfind('|');
+    if (line.empty() || line[0] == '#' || bar == std::string::npos)
+      continue;
+    int z = 0, p = 0;
+    if (sscanf(line.c_str() + bar + 1, "%d %d", &z, &p) == 2)
+      m[line.substr(0, bar)] = {z, p};
+  }
+}
+
+static int save_cache(const char *path, const TuneMap &m) {
+  if (!path)
+    return CG_OK;
+  std::ofstream f(path);
+  if (!f)
+    return set_error(CG_ERR_ARG, "cannot write tuner cache %s", path);
+  f << "# carpet_amd batch tuner cache: <class key>|<zchunk> <points per item>\n";
+  for (const auto &kv : m)
+    f << kv.first << "|" << kv.second.first << " " << kv.second.second << "\n";
+  return CG_OK;
+}
+
+static int time_launch(cg_batch *b, int idx, hipStream_t st, int reps, float *ms) {
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+    return hip_error(hipGetLastError(), "hipEventCreate");
+  int e = batch_device_run_one(b->dev, idx, st); // warm-up
+  float best = 1e30f;
+  for (int r = 0; r < reps && !e; ++r) {
+    (void)hipEventRecord(e0, st);
+    e = batch_device_run_one(b->dev, idx, st);
+    (void)hipEventRecord(e1, st);
+    if (!e && hipEventSynchronize(e1) == hipSuccess) {
+      float t = 0;
+      (void)hipEventElapsedTime(&t, e0, e1);
+      best = std::min(best, t);
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *ms = best;
+  return e;
+}
+
+extern "C" int cg_batch_autotune(cg_batch *b, cg_stream_t stream, int reps,
+                                 const char *cache_path) {
+  if (!b)
+    return set_error(CG_ERR_ARG, "null batch");
+  if (!b->finalized)
+    return set_error(CG_ERR_STATE, "batch not finalized");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  reps = std::max(1, reps);
+  load_cache(cache_path, b->tuned);
+  drop_graph(b);
+  // current choices for every class, then search the classes the cache lacks
+  for (size_t i = 0; i < b->dev.launches.size(); ++i) {
+    const BatchLaunch l = b->dev.launches[i];
+    if (b->tuned.count(l.key))
+      continue;
+    std::vector<std::pair<int, int>> cands;
+    if (l.is_tensor)
+      for (int z : {4, 8, 16, 32, 64, 0})
+        cands.push_back({z, l.pt});
+    else
+      for (int p : {1024, 4096, 16384, 65536})
+        cands.push_back({l.zchunk, p});
+    float best = 1e30f;
+    std::pair<int, int> bestc = cands[0];
+    for (const auto &c : cands) {
+      TuneMap trial = b->tuned;
+      trial[l.key] = c;
+      int e = batch_device_build(b->dev, b->transfers, b->phase, b->prec, b->tile_cfg, &trial);
+      if (e)
+        return e;
+      float ms = 0;
+      if ((e = time_launch(b, (int)i, st, reps, &ms)))
+        return e;
+      if (ms < best) {
+        best = ms;
+        bestc = c;
+      }
+    }
+    b->tuned[l.key] = bestc;
+  }
+  int e = batch_device_build(b->dev, b->transfers, b->phase, b->prec, b->tile_cfg, &b->tuned);
+  if (e)
+    return e;
+  return save_cache(cache_path, b->tuned);
+}
+
+extern "C" int cg_batch_describe(const cg_batch *b, char *buf, size_t len) {
+  if (!b || !buf || !len)
+    return set_error(CG_ERR_ARG, "null argument");
+  std::ostringstream o;
+  for (const BatchLaunch &l : b->dev.launches)
+    o << "phase " << l.phase << " [" << l.key << "] items " << l.nitems << " grid " << l.grid
+      << " zchunk " << l.zchunk << " pt " << l.pt << "\n";
+  const std::string s = o.str();
+  const size_t n = std::min(len - 1, s.size());
+  std::memcpy(buf, s.data(), n);
+  buf[n] = 0;
+  return (int)s.size();
 }
 
 int cg_batch_run(cg_batch *b, cg_stream_t stream) {

@@ -4,6 +4,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <map>
+#include <string>
+#include <utility>
 #include <vector>
 
 #include "cg_internal.hpp"
@@ -18,10 +21,17 @@ struct BatchLaunch {
   int kind, order, ntl, op, prec;
   size_t desc_off; // byte offset of the descriptor array in the blob
   size_t item_off; // byte offset of the (descriptor, tile) item array
+  size_t seg_off;  // byte offset of the 9 segment bounds (one segment per XCD)
   int nitems;
-  int counter;     // index of this launch's work counter
+  int counter;     // index of this launch's block of 8 work counters
   int grid;        // persistent workgroups
+  int zchunk;      // tensor work item: coarse z bases per chunk (0 = all)
+  int pt;          // pointwise work item: destination points
+  std::string key; // tuner cache key (class + box-size histogram bucket)
 };
+
+// tuner cache: key -> (zchunk, pt)
+using TuneMap = std::map<std::string, std::pair<int, int>>;
 
 struct BatchDevice {
   std::vector<BatchLaunch> launches;
@@ -35,8 +45,10 @@ struct BatchDevice {
 // region, < 0 = automatic)
 int batch_device_build(BatchDevice &dev, const std::vector<Transfer> &tr,
                        const std::vector<int> &phase, const std::vector<int> &prec,
-                       int tile_cfg);
+                       int tile_cfg, const TuneMap *tuned = nullptr);
 int batch_device_run(BatchDevice &dev, hipStream_t st);
+// runs launch idx alone (its counter cleared first)
+int batch_device_run_one(BatchDevice &dev, int idx, hipStream_t st);
 void batch_device_free(BatchDevice &dev);
 
 } // namespace cg

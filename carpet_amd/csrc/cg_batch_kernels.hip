@@ -23,56 +23,77 @@
 
 namespace cg {
 
-// pointwise work item: PT consecutive destination points (x fastest)
-constexpr int PT = 4096;
+// default work items: pointwise = PT_DEFAULT consecutive destination points
+// (x fastest); tensor = one x/y tile over ZCHUNK_DEFAULT coarse z bases
+constexpr int PT_DEFAULT = 4096;
+constexpr int ZCHUNK_DEFAULT = 16;
 
 struct Item {
   unsigned desc;
   unsigned tile;
 };
 
-__device__ __forceinline__ int dequeue(int *counter, int *s_item) {
-  if (threadIdx.x == 0)
-    *s_item = atomicAdd(counter, 1);
+// Work items are split into 8 contiguous, cost-balanced segments, one per
+// XCD (workgroups are dispatched round-robin over the XCDs, so workgroup b
+// runs on XCD b % 8): neighbouring tiles -- which share stencil halos --
+// are then processed at the same time on the same L2.  A workgroup takes
+// items from its own segment's counter and, once that is exhausted, from
+// the other segments', so the load still balances.  Returns -1 when every
+// segment is exhausted; every workgroup gets there, so the grid drains.
+__device__ __forceinline__ int dequeue(int *ctr8, const int *seg, int *s_item) {
+  if (threadIdx.x == 0) {
+    int it = -1;
+    const int home = blockIdx.x & 7;
+    for (int k = 0; k < 8 && it < 0; ++k) {
+      const int sg = (home + k) & 7;
+      const int n = seg[sg + 1] - seg[sg];
+      if (n <= 0 || __hip_atomic_load(&ctr8[sg], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= n)
+        continue;
+      const int i = atomicAdd(&ctr8[sg], 1);
+      if (i < n)
+        it = seg[sg] + i;
+    }
+    *s_item = it;
+  }
   __syncthreads();
   const int it = *s_item;
   __syncthreads();
   return it;
 }
 
-template <class L, int NTL, int CX, int CY, int IPT>
+template <class L, int NTL, int CX, int CY, int IPT, typename TS>
 __global__ __launch_bounds__(2 * CX * CY / IPT) void k_persist_tensor(
     const TensorArgs *__restrict__ descs, const Item *__restrict__ items,
-    int nitems, int *counter) {
+    const int *__restrict__ seg, int *counter) {
   __shared__ double smem[TensorLds<L, NTL, CX, CY>::TOTAL];
   __shared__ int s_item;
   for (;;) {
-    const int it = dequeue(counter, &s_item);
-    if (it >= nitems)
+    const int it = dequeue(counter, seg, &s_item);
+    if (it < 0)
       return;
     const Item w = items[it];
     const TensorArgs a = descs[w.desc];
-    tensor_tile<L, NTL, CX, CY, IPT>(a, (int)w.tile, smem);
+    tensor_tile<L, NTL, CX, CY, IPT, TS>(a, (int)w.tile, smem);
   }
 }
 
 template <int OP>
 __global__ __launch_bounds__(256) void k_persist_point(
     const PointArgs *__restrict__ descs, const Item *__restrict__ items,
-    int nitems, int *counter) {
+    const int *__restrict__ seg, int *counter, int pt) {
   __shared__ int s_item;
   for (;;) {
-    const int it = dequeue(counter, &s_item);
-    if (it >= nitems)
+    const int it = dequeue(counter, seg, &s_item);
+    if (it < 0)
       return;
     const Item w = items[it];
     const PointArgs a = descs[w.desc];
     const unsigned nx = (unsigned)a.n[0];
     const unsigned nxy = nx * (unsigned)a.n[1];
     const unsigned tot = nxy * (unsigned)a.n[2];
-    const unsigned e0 = w.tile * (unsigned)PT;
+    const unsigned e0 = w.tile * (unsigned)pt;
 #pragma unroll 4
-    for (int m = 0; m < PT / 256; ++m) {
+    for (int m = 0; m < pt / 256; ++m) {
       const unsigned e = e0 + m * 256u + threadIdx.x;
       if (e >= tot)
         break;
@@ -103,9 +124,11 @@ struct Group {
   }
 };
 
-template <class L, int NTL> const void *persist_tensor_fn() {
+template <class L, int NTL> const void *persist_tensor_fn(int prec) {
   using T = TileShape<NTL>;
-  return reinterpret_cast<const void *>(&k_persist_tensor<L, NTL, T::CX, T::CY, T::IPT>);
+  if (prec)
+    return reinterpret_cast<const void *>(&k_persist_tensor<L, NTL, T::CX, T::CY, T::IPT, float>);
+  return reinterpret_cast<const void *>(&k_persist_tensor<L, NTL, T::CX, T::CY, T::IPT, double>);
 }
 
 const void *persist_point_fn(int op) {
@@ -128,9 +151,9 @@ const void *launch_fn(const BatchLaunch &l, int *threads) {
   with_line(l.kind, l.order, [&](auto tag) -> int {
     using L = decltype(tag);
     switch (l.ntl) {
-    case 1: fn = persist_tensor_fn<L, 1>(); *threads = 2 * TileShape<1>::CX * TileShape<1>::CY / TileShape<1>::IPT; break;
-    case 2: fn = persist_tensor_fn<L, 2>(); *threads = 2 * TileShape<2>::CX * TileShape<2>::CY / TileShape<2>::IPT; break;
-    case 3: fn = persist_tensor_fn<L, 3>(); *threads = 2 * TileShape<3>::CX * TileShape<3>::CY / TileShape<3>::IPT; break;
+    case 1: fn = persist_tensor_fn<L, 1>(l.prec); *threads = 2 * TileShape<1>::CX * TileShape<1>::CY / TileShape<1>::IPT; break;
+    case 2: fn = persist_tensor_fn<L, 2>(l.prec); *threads = 2 * TileShape<2>::CX * TileShape<2>::CY / TileShape<2>::IPT; break;
+    case 3: fn = persist_tensor_fn<L, 3>(l.prec); *threads = 2 * TileShape<3>::CX * TileShape<3>::CY / TileShape<3>::IPT; break;
     }
     return 0;
   });
@@ -151,9 +174,18 @@ void batch_device_free(BatchDevice &dev) {
   dev.ncounters = 0;
 }
 
+static int ilog2(long long v) {
+  int r = 0;
+  while (v > 1) {
+    v >>= 1;
+    ++r;
+  }
+  return r;
+}
+
 int batch_device_build(BatchDevice &dev, const std::vector<Transfer> &tr,
                        const std::vector<int> &phase, const std::vector<int> &prec,
-                       int tile_cfg) {
+                       int tile_cfg, const TuneMap *tuned) {
   batch_device_free(dev);
   // group transfers by (phase, kernel)
   std::vector<Group> groups;
@@ -201,8 +233,8 @@ int batch_device_build(BatchDevice &dev, const std::vector<Transfer> &tr,
     return off;
   };
   for (const Group &g : groups) {
-    if (g.prec != CG_PREC_F64)
-      return set_error(CG_ERR_OPERATOR, "mixed precision is not available in the batch yet");
+    if (g.prec != CG_PREC_F64 && (!g.is_tensor || g.kind == LK_ENO))
+      return set_error(CG_ERR_OPERATOR, "mixed precision applies to Lagrange prolongation only");
     BatchLaunch l{};
     l.phase = g.phase;
     l.is_tensor = g.is_tensor;
@@ -211,56 +243,29 @@ int batch_device_build(BatchDevice &dev, const std::vector<Transfer> &tr,
     l.ntl = g.ntl;
     l.op = g.op;
     l.prec = g.prec;
-    std::vector<Item> items;
-    std::vector<double> cost;
-    if (g.is_tensor) {
-      std::vector<TensorArgs> descs;
-      for (int m : g.members) {
-        TensorArgs a = tr[m].pr.t;
-        TileCfg cfg;
-        cfg.cx = g.ntl == 1 ? TileShape<1>::CX : TileShape<3>::CX;
-        cfg.cy = g.ntl == 1 ? TileShape<1>::CY : TileShape<3>::CY;
-        cfg.zchunk = tile_cfg < 0 ? 16 : tile_cfg;
-        const long long nt = tensor_tiles(a, cfg);
-        const unsigned d = (unsigned)descs.size();
-        descs.push_back(a);
-        for (long long t = 0; t < nt; ++t) {
-          items.push_back(Item{d, (unsigned)t});
-          // cost ~ fine points of the tile
-          const int tz = (int)(t / ((long long)a.ntiles[0] * a.ntiles[1]));
-          const int nz = std::min(a.zchunk, a.nq[2] - tz * a.zchunk);
-          cost.push_back((double)nz * std::min(cfg.cx, a.nq[0]) * std::min(cfg.cy, a.nq[1]));
-        }
-      }
-      l.desc_off = append(descs.data(), descs.size() * sizeof(TensorArgs));
-    } else {
-      std::vector<PointArgs> descs;
-      for (int m : g.members) {
-        const PointArgs &a = tr[m].is_prolong ? tr[m].pr.p : tr[m].pt.a;
-        const long long tot = (long long)a.n[0] * a.n[1] * a.n[2];
-        if (tot >= (1LL << 32))
-          return set_error(CG_ERR_ARG, "region too large for the batch");
-        const unsigned d = (unsigned)descs.size();
-        descs.push_back(a);
-        const long long nt = (tot + PT - 1) / PT;
-        for (long long t = 0; t < nt; ++t) {
-          items.push_back(Item{d, (unsigned)t});
-          cost.push_back((double)std::min<long long>(PT, tot - t * PT));
-        }
-      }
-      l.desc_off = append(descs.data(), descs.size() * sizeof(PointArgs));
+    // key of the tuner cache: kernel class + box-size histogram bucket
+    // (log2 of the median destination points per transfer) + count bucket
+    {
+      std::vector<long long> pts;
+      for (int m : g.members)
+        pts.push_back(tr[m].dst_points());
+      std::nth_element(pts.begin(), pts.begin() + pts.size() / 2, pts.end());
+      char key[128];
+      snprintf(key, sizeof key, "%s k%d o%d t%d p%d op%d h%d n%d", g.is_tensor ? "tensor" : "point",
+               g.kind, g.order, g.ntl, g.prec, g.op, ilog2(pts[pts.size() / 2]),
+               ilog2((long long)g.members.size()));
+      l.key = key;
     }
-    // largest tiles first
-    std::vector<int> ord(items.size());
-    for (size_t i = 0; i < ord.size(); ++i)
-      ord[i] = (int)i;
-    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return cost[a] > cost[b]; });
-    std::vector<Item> sorted(items.size());
-    for (size_t i = 0; i < ord.size(); ++i)
-      sorted[i] = items[ord[i]];
-    l.item_off = append(sorted.data(), sorted.size() * sizeof(Item));
-    l.nitems = (int)sorted.size();
-    l.counter = (int)dev.launches.size();
+    l.zchunk = tile_cfg < 0 ? ZCHUNK_DEFAULT : tile_cfg;
+    l.pt = PT_DEFAULT;
+    if (tuned) {
+      auto it = tuned->find(l.key);
+      if (it != tuned->end()) {
+        l.zchunk = it->second.first;
+        l.pt = it->second.second;
+      }
+    }
+    // persistent grid: occupancy x CUs, a multiple of the 8 XCDs
     int threads = 0;
     const void *fn = launch_fn(l, &threads);
     if (!fn)
@@ -269,7 +274,91 @@ int batch_device_build(BatchDevice &dev, const std::vector<Transfer> &tr,
     he = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, 0);
     if (he != hipSuccess)
       return hip_error(he, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
-    l.grid = std::max(1, std::min(l.nitems, std::max(1, per_cu) * ncu));
+    const int full_grid = std::max(8, (std::max(1, per_cu) * ncu) / 8 * 8);
+    // transfers largest first; a transfer's items in spatial order (x tile
+    // fastest, then y, then z chunk), so consecutive items are neighbours
+    std::vector<int> members = g.members;
+    std::stable_sort(members.begin(), members.end(), [&](int x, int y) {
+      return tr[x].dst_points() > tr[y].dst_points();
+    });
+    std::vector<Item> items;
+    std::vector<double> cost;
+    if (g.is_tensor) {
+      TileCfg cfg;
+      cfg.cx = g.ntl == 1 ? TileShape<1>::CX : TileShape<3>::CX;
+      cfg.cy = g.ntl == 1 ? TileShape<1>::CY : TileShape<3>::CY;
+      // z chunk: whole columns (no halo planes) when that already gives the
+      // grid a few items per workgroup, else the longest chunk that does
+      if (tile_cfg < 0 && !(tuned && tuned->count(l.key))) {
+        l.zchunk = 0;
+        for (int zc : {0, 64, 32, 16, 8}) {
+          l.zchunk = zc;
+          long long tiles = 0;
+          for (int m : members) {
+            TensorArgs a = tr[m].pr.t;
+            cfg.zchunk = zc;
+            tiles += tensor_tiles(a, cfg);
+          }
+          if (tiles >= 4LL * full_grid)
+            break;
+        }
+      }
+      std::vector<TensorArgs> descs;
+      for (int m : members) {
+        TensorArgs a = tr[m].pr.t;
+        cfg.zchunk = l.zchunk;
+        const long long nt = tensor_tiles(a, cfg);
+        const unsigned d = (unsigned)descs.size();
+        descs.push_back(a);
+        for (long long t = 0; t < nt; ++t) {
+          items.push_back(Item{d, (unsigned)t});
+          // cost ~ coarse planes x bases of the tile
+          const int tz = (int)(t / ((long long)a.ntiles[0] * a.ntiles[1]));
+          const int nz = std::min(a.zchunk, a.nq[2] - tz * a.zchunk) + 4;
+          cost.push_back((double)nz * std::min(cfg.cx, a.nq[0]) * std::min(cfg.cy, a.nq[1]));
+        }
+      }
+      l.desc_off = append(descs.data(), descs.size() * sizeof(TensorArgs));
+    } else {
+      std::vector<PointArgs> descs;
+      for (int m : members) {
+        const PointArgs &a = tr[m].is_prolong ? tr[m].pr.p : tr[m].pt.a;
+        const long long tot = (long long)a.n[0] * a.n[1] * a.n[2];
+        if (tot >= (1LL << 32))
+          return set_error(CG_ERR_ARG, "region too large for the batch");
+        const unsigned d = (unsigned)descs.size();
+        descs.push_back(a);
+        const long long nt = (tot + l.pt - 1) / l.pt;
+        for (long long t = 0; t < nt; ++t) {
+          items.push_back(Item{d, (unsigned)t});
+          cost.push_back((double)std::min<long long>(l.pt, tot - t * l.pt));
+        }
+      }
+      l.desc_off = append(descs.data(), descs.size() * sizeof(PointArgs));
+    }
+    l.item_off = append(items.data(), items.size() * sizeof(Item));
+    l.nitems = (int)items.size();
+    // 8 contiguous segments of equal cost, one per XCD
+    {
+      double total = 0;
+      for (double c : cost)
+        total += c;
+      int segb[9];
+      segb[0] = 0;
+      double acc = 0;
+      int k = 1;
+      for (int i = 0; i < l.nitems && k < 8; ++i) {
+        acc += cost[i];
+        while (k < 8 && acc >= total * k / 8.0)
+          segb[k++] = i + 1;
+      }
+      while (k < 9)
+        segb[k++] = l.nitems;
+      segb[8] = l.nitems;
+      l.seg_off = append(segb, sizeof segb);
+    }
+    l.counter = (int)dev.launches.size();
+    l.grid = std::min(full_grid, std::max(8, (l.nitems + 7) / 8 * 8));
     dev.launches.push_back(l);
   }
   dev.ncounters = (int)dev.launches.size();
@@ -282,21 +371,41 @@ int batch_device_build(BatchDevice &dev, const std::vector<Transfer> &tr,
   he = hipMemcpy(dev.blob, host.data(), host.size(), hipMemcpyHostToDevice);
   if (he != hipSuccess)
     return hip_error(he, "batch descriptor upload");
-  he = hipMalloc(&dev.counters, sizeof(int) * std::max(1, dev.ncounters));
+  he = hipMalloc(&dev.counters, sizeof(int) * 8 * std::max(1, dev.ncounters));
   if (he != hipSuccess)
     return set_error(CG_ERR_NOMEM, "batch counters: %s", hipGetErrorString(he));
   return CG_OK;
 }
 
+static int run_launch(BatchDevice &dev, const BatchLaunch &l, hipStream_t st);
+
 int batch_device_run(BatchDevice &dev, hipStream_t st) {
   if (dev.launches.empty())
     return CG_OK;
-  hipLaunchKernelGGL(k_zero_counters, dim3((dev.ncounters + 255) / 256), dim3(256), 0, st,
-                     dev.counters, dev.ncounters);
+  hipLaunchKernelGGL(k_zero_counters, dim3((8 * dev.ncounters + 255) / 256), dim3(256), 0, st,
+                     dev.counters, 8 * dev.ncounters);
   for (const BatchLaunch &l : dev.launches) {
+    int e = run_launch(dev, l, st);
+    if (e)
+      return e;
+  }
+  return CG_OK;
+}
+
+int batch_device_run_one(BatchDevice &dev, int idx, hipStream_t st) {
+  if (idx < 0 || idx >= (int)dev.launches.size())
+    return set_error(CG_ERR_ARG, "no launch %d", idx);
+  hipLaunchKernelGGL(k_zero_counters, dim3((8 * dev.ncounters + 255) / 256), dim3(256), 0, st,
+                     dev.counters, 8 * dev.ncounters);
+  return run_launch(dev, dev.launches[idx], st);
+}
+
+static int run_launch(BatchDevice &dev, const BatchLaunch &l, hipStream_t st) {
+  {
     const void *descs = dev.blob + l.desc_off;
     const Item *items = reinterpret_cast<const Item *>(dev.blob + l.item_off);
-    int *ctr = dev.counters + l.counter;
+    const int *seg = reinterpret_cast<const int *>(dev.blob + l.seg_off);
+    int *ctr = dev.counters + 8 * l.counter;
     int rc = 0;
     if (l.is_tensor) {
       rc = with_line(l.kind, l.order, [&](auto tag) -> int {
@@ -304,9 +413,14 @@ int batch_device_run(BatchDevice &dev, hipStream_t st) {
         auto go = [&](auto ntlc) {
           constexpr int NTL = decltype(ntlc)::value;
           using T = TileShape<NTL>;
-          hipLaunchKernelGGL((k_persist_tensor<L, NTL, T::CX, T::CY, T::IPT>), dim3(l.grid),
-                             dim3(2 * T::CX * T::CY / T::IPT), 0, st,
-                             static_cast<const TensorArgs *>(descs), items, l.nitems, ctr);
+          if (l.prec)
+            hipLaunchKernelGGL((k_persist_tensor<L, NTL, T::CX, T::CY, T::IPT, float>), dim3(l.grid),
+                               dim3(2 * T::CX * T::CY / T::IPT), 0, st,
+                               static_cast<const TensorArgs *>(descs), items, seg, ctr);
+          else
+            hipLaunchKernelGGL((k_persist_tensor<L, NTL, T::CX, T::CY, T::IPT, double>), dim3(l.grid),
+                               dim3(2 * T::CX * T::CY / T::IPT), 0, st,
+                               static_cast<const TensorArgs *>(descs), items, seg, ctr);
           return 0;
         };
         switch (l.ntl) {
@@ -319,11 +433,11 @@ int batch_device_run(BatchDevice &dev, hipStream_t st) {
     } else {
       const PointArgs *pd = static_cast<const PointArgs *>(descs);
       switch (l.op) {
-      case PO_COPY: hipLaunchKernelGGL(k_persist_point<PO_COPY>, dim3(l.grid), dim3(256), 0, st, pd, items, l.nitems, ctr); break;
-      case PO_RESTRICT_VC: hipLaunchKernelGGL(k_persist_point<PO_RESTRICT_VC>, dim3(l.grid), dim3(256), 0, st, pd, items, l.nitems, ctr); break;
-      case PO_RESTRICT_CC: hipLaunchKernelGGL(k_persist_point<PO_RESTRICT_CC>, dim3(l.grid), dim3(256), 0, st, pd, items, l.nitems, ctr); break;
-      case PO_INTERP: hipLaunchKernelGGL(k_persist_point<PO_INTERP>, dim3(l.grid), dim3(256), 0, st, pd, items, l.nitems, ctr); break;
-      case PO_TVD: hipLaunchKernelGGL(k_persist_point<PO_TVD>, dim3(l.grid), dim3(256), 0, st, pd, items, l.nitems, ctr); break;
+      case PO_COPY: hipLaunchKernelGGL(k_persist_point<PO_COPY>, dim3(l.grid), dim3(256), 0, st, pd, items, seg, ctr, l.pt); break;
+      case PO_RESTRICT_VC: hipLaunchKernelGGL(k_persist_point<PO_RESTRICT_VC>, dim3(l.grid), dim3(256), 0, st, pd, items, seg, ctr, l.pt); break;
+      case PO_RESTRICT_CC: hipLaunchKernelGGL(k_persist_point<PO_RESTRICT_CC>, dim3(l.grid), dim3(256), 0, st, pd, items, seg, ctr, l.pt); break;
+      case PO_INTERP: hipLaunchKernelGGL(k_persist_point<PO_INTERP>, dim3(l.grid), dim3(256), 0, st, pd, items, seg, ctr, l.pt); break;
+      case PO_TVD: hipLaunchKernelGGL(k_persist_point<PO_TVD>, dim3(l.grid), dim3(256), 0, st, pd, items, seg, ctr, l.pt); break;
       default: rc = -1;
       }
     }

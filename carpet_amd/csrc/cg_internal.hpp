@@ -74,13 +74,15 @@ struct ProlongPlan {
   PointArgs p;    // TVD
   long long dst_points;
   long long src_points; // unique source footprint per time level
+  int es;               // element size (8 FP64, 4 mixed)
 };
 
 // validates and builds a prolongation plan (srcs[0..ntl-1] share src's
 // layout); s[] are the time weights
+// es: element size of the arrays (8 = FP64 drop-in, 4 = mixed precision)
 int plan_prolong(int kind, int order, const Arr &src, const void *const *srcs,
                  int ntl, const double *s, const Arr &dst, const Box &reg,
-                 ProlongPlan &plan);
+                 ProlongPlan &plan, int es = 8);
 int launch_prolong(const ProlongPlan &plan, int prec, hipStream_t st);
 
 struct PointPlan {
@@ -116,7 +118,7 @@ int resolve_transfer(const cg_gdata *dst, const cg_gdata *const *srcs,
                      const double *times, int ntimes, const int *dstregbbox,
                      const int *srcregbbox, const cg_islab *slab, double time,
                      int op, int cent, int order_space, int order_time,
-                     Transfer &tr);
+                     Transfer &tr, int es = 8);
 int run_transfer(const Transfer &tr, int prec, hipStream_t st);
 
 Arr arr_from(const void *p, const int *pad, const int *ext, const int *bbox);

@@ -106,7 +106,9 @@ static LineInfo line_info_rt(int kind, int order, bool &ok) {
 // prolongate_3d_cc_real8_tvd.F90:94-179 (cell TVD)
 int plan_prolong(int kind, int order, const Arr &src, const void *const *srcs,
                  int ntl, const double *s, const Arr &dst, const Box &reg,
-                 ProlongPlan &pl) {
+                 ProlongPlan &pl, int es) {
+  if (es != 8 && !(es == 4 && (kind == PK_VLAG || kind == PK_CLAG)))
+    return set_error(CG_ERR_OPERATOR, "mixed precision applies to Lagrange prolongation only");
   int e;
   if ((e = check_array(src, "source")) || (e = check_array(dst, "destination")))
     return e;
@@ -122,6 +124,7 @@ int plan_prolong(int kind, int order, const Arr &src, const void *const *srcs,
   pl.kind = kind;
   pl.order = order;
   pl.ntl = ntl;
+  pl.es = es;
 
   if (kind == PK_TVD) {
     for (int d = 0; d < 3; ++d) {
@@ -247,13 +250,14 @@ int plan_prolong(int kind, int order, const Arr &src, const void *const *srcs,
     t.src[l] = l < ntl ? srcs[l] : nullptr;
   t.sj = src.sj();
   t.sk = src.sk();
-  t.dst = static_cast<double *>(const_cast<void *>(dst.p)) + dst.at(doff[0], doff[1], doff[2]);
+  t.dst = static_cast<char *>(const_cast<void *>(dst.p)) + (long long)es * dst.at(doff[0], doff[1], doff[2]);
   t.dj = dst.sj();
   t.dk = dst.sk();
   for (int l = 0; l < 3; ++l)
     t.s[l] = l < ntl ? s[l] : 0.0;
   pl.dst_points = (long long)t.n[0] * t.n[1] * t.n[2];
   pl.src_points = srcpts;
+  pl.es = es;
   return CG_OK;
 }
 

@@ -75,6 +75,28 @@ template <> struct CW<5> {
 // fused multiply-add in the canonical order
 CG_HD double fmad(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
+// Compensated FP32 dot product sum_t c(t)*v(t) (Ogita, Rump and Oishi,
+// "Accurate sum and dot product", SIAM J. Sci. Comput. 26 (2005), Dot2):
+// every product is split exactly with an FMA (TwoProduct) and every partial
+// sum with Knuth's TwoSum; the errors are summed separately and added at the
+// end, so the result is as accurate as if computed in twice the working
+// precision and then rounded to FP32.
+template <int N, class C, class V> CG_HD float dot2f(C c, V v) {
+  float p = c(0) * v(0);
+  float s = __builtin_fmaf(c(0), v(0), -p);
+#pragma unroll
+  for (int t = 1; t < N; ++t) {
+    const float h = c(t) * v(t);
+    const float r = __builtin_fmaf(c(t), v(t), -h);
+    const float q = p + h;
+    const float z = q - p;
+    const float e = (p - (q - z)) + (h - z);
+    p = q;
+    s = s + (e + r);
+  }
+  return p + s;
+}
+
 // ---- line operator traits ------------------------------------------------
 // Members: TMIN[2], NTAP[2], TMINALL, TMAXALL, R (widest window),
 // template <int P, class V> static double apply(V v): v(t) gives the t-th
@@ -101,6 +123,14 @@ template <int O> struct Line<LK_VLAG, O> {
       return r;
     }
   }
+  // mixed precision: FP32 values, compensated FP32 accumulation (the weights
+  // are dyadic fractions, exact in FP32)
+  template <int P, class V> static CG_HD float apply_f(V v) {
+    if constexpr (P == 0)
+      return v(0);
+    else
+      return dot2f<N>([](int t) { return (float)VW<O>::c[t]; }, v);
+  }
 };
 
 // cell-centred Lagrange (coeffs1d<RT,ORDER,di>: imin = -(O+1)/2 + (O odd ?
@@ -118,6 +148,9 @@ template <int O> struct Line<LK_CLAG, O> {
     for (int t = 0; t < N; ++t)
       r = fmad(CW<O>::c[P == 0 ? t : N - 1 - t], v(t), r);
     return r;
+  }
+  template <int P, class V> static CG_HD float apply_f(V v) {
+    return dot2f<N>([](int t) { return (float)CW<O>::c[P == 0 ? t : N - 1 - t]; }, v);
   }
 };
 
@@ -155,6 +188,9 @@ template <int O> struct Line<LK_ENO, O> {
     }
     return r;
   }
+
+  // ENO stays FP64 in the mixed configuration
+  template <int P, class V> static CG_HD float apply_f(V v) { return (float)apply<P>(v); }
 
   template <int P, class V> static CG_HD double apply(V v) {
     constexpr int mm = EnoW<O>::MM[P];

@@ -65,21 +65,40 @@ template <class L, int NTL, int CX, int CY> struct TensorLds {
 };
 
 // One z-marching tile; tile b indexes (x tile, y tile, z chunk) of a.
-template <class L, int NTL, int CX, int CY, int IPT>
+// line operator in the storage precision: FP64 (the drop-in) or FP32 with
+// compensated accumulation (mixed)
+template <class L, int P, typename TS, class V> __device__ __forceinline__ TS line_apply(V v) {
+  if constexpr (sizeof(TS) == 8)
+    return L::template apply<P>(v);
+  else
+    return L::template apply_f<P>(v);
+}
+
+// compile-time loop: f(integral_constant<int,I>) for I = 0..N-1 while f
+// returns true
+template <int I, int N, class F> __device__ __forceinline__ bool unroll_while(F &f) {
+  if constexpr (I < N) {
+    if (!f(std::integral_constant<int, I>{}))
+      return false;
+    return unroll_while<I + 1, N>(f);
+  }
+  return true;
+}
+
+template <class L, int NTL, int CX, int CY, int IPT, typename TS = double>
 __device__ __forceinline__ void tensor_tile(const TensorArgs &a, int b, double *smem) {
   constexpr int FX = 2 * CX;
   constexpr int NT = FX * CY / IPT;
   constexpr int NQY = NT / FX;
   static_assert(NQY * IPT == CY, "tile rows must divide among threads");
   using LD = TensorLds<L, NTL, CX, CY>;
-  constexpr int SPAN = LD::SPAN;
   constexpr int WX = LD::WX;
   constexpr int WY = LD::WY;
   constexpr int R = L::R;
+  constexpr int NLD = (WY * WX + NT - 1) / NT; // window elements per thread
 
-  double(*A)[WY][WX] = reinterpret_cast<double(*)[WY][WX]>(smem);
-  double(*X)[WY][FX] = reinterpret_cast<double(*)[WY][FX]>(smem + LD::A);
-  (void)SPAN;
+  TS(*A)[WY][WX] = reinterpret_cast<TS(*)[WY][WX]>(smem);
+  TS(*X)[WY][FX] = reinterpret_cast<TS(*)[WY][FX]>(reinterpret_cast<TS *>(smem) + LD::A);
 
   const int tx = b % a.ntiles[0];
   b /= a.ntiles[0];
@@ -102,64 +121,84 @@ __device__ __forceinline__ void tensor_tile(const TensorArgs &a, int b, double *
   const int gfx = 2 * qx0 + fx - a.off[0];
   const bool xin = gfx >= 0 && gfx < a.n[0];
 
-  double ring[NTL][R][IPT][2];
+  // this thread's share of the plane window: offsets into the source plane
+  // (or -1 outside the clipped footprint)
+  long long woff[NLD];
+#pragma unroll
+  for (int k = 0; k < NLD; ++k) {
+    const int e = tid + k * NT;
+    const int r = e / WX, x = e - r * WX;
+    const int gy = wy0 + r, gx = wx0 + x;
+    woff[k] = (e < WY * WX && gy >= ly0 && gy <= ly1 && gx >= lx0 && gx <= lx1)
+                  ? gx + a.sj * gy
+                  : -1;
+  }
+  // next plane's window, loaded one plane ahead of its use
+  TS pre[NTL][NLD];
+  auto fetch = [&](int c) {
+#pragma unroll
+    for (int k = 0; k < NLD; ++k)
+      if (woff[k] >= 0)
+#pragma unroll
+        for (int tl = 0; tl < NTL; ++tl)
+          pre[tl][k] = static_cast<const TS *>(a.src[tl])[woff[k] + a.sk * c];
+  };
+
+  // ring of y-pass results: slot j holds the planes c with (c - c0) % R == j
+  TS ring[NTL][R][IPT][2];
 #pragma unroll
   for (int tl = 0; tl < NTL; ++tl)
 #pragma unroll
     for (int i = 0; i < R; ++i)
 #pragma unroll
       for (int m = 0; m < IPT; ++m)
-        ring[tl][i][m][0] = ring[tl][i][m][1] = 0.0;
+        ring[tl][i][m][0] = ring[tl][i][m][1] = TS(0);
 
-  for (int c = c0; c <= c1; ++c) {
-    // 1. coarse plane window -> LDS
-    for (int e = tid; e < WY * WX; e += NT) {
-      const int r = e / WX, x = e - r * WX;
-      const int gy = wy0 + r, gx = wx0 + x;
-      if (gy >= ly0 && gy <= ly1 && gx >= lx0 && gx <= lx1) {
-        const long long o = gx + a.sj * gy + a.sk * c;
+  if (c0 <= c1)
+    fetch(c0);
+  int c = c0;
+  auto plane = [&](auto Jc) -> bool {
+    constexpr int J = decltype(Jc)::value;
+    if (c > c1)
+      return false;
+    // 1. window of plane c -> LDS; start loading plane c+1
+#pragma unroll
+    for (int k = 0; k < NLD; ++k)
+      if (woff[k] >= 0) {
+        const int e = tid + k * NT;
 #pragma unroll
         for (int tl = 0; tl < NTL; ++tl)
-          A[tl][r][x] = ld<double>(a.src[tl], o);
+          (&A[tl][0][0])[e] = pre[tl][k];
       }
-    }
+    if (c < c1)
+      fetch(c + 1);
     __syncthreads();
     // 2. x-pass: both children of each base
     for (int e = tid; e < WY * CX; e += NT) {
       const int r = e / CX, q = e - r * CX;
 #pragma unroll
       for (int tl = 0; tl < NTL; ++tl) {
-        const double *w = &A[tl][r][q - L::TMINALL];
-        const double v0 =
-            L::template apply<0>([&](int t) { return w[L::TMIN[0] + t]; });
-        const double v1 =
-            L::template apply<1>([&](int t) { return w[L::TMIN[1] + t]; });
+        const TS *w = &A[tl][r][q - L::TMINALL];
+        const TS v0 = line_apply<L, 0, TS>([&](int t) { return w[L::TMIN[0] + t]; });
+        const TS v1 = line_apply<L, 1, TS>([&](int t) { return w[L::TMIN[1] + t]; });
         X[tl][r][2 * q] = v0;
         X[tl][r][2 * q + 1] = v1;
       }
     }
     __syncthreads();
-    // 3. y-pass into the register ring
+    // 3. y-pass into ring slot J
 #pragma unroll
     for (int m = 0; m < IPT; ++m) {
       const int qy = qyl + NQY * m;
 #pragma unroll
       for (int tl = 0; tl < NTL; ++tl) {
-        const double *col = &X[tl][qy - L::TMINALL][fx];
-        const double y0 =
-            L::template apply<0>([&](int t) { return col[(L::TMIN[0] + t) * FX]; });
-        const double y1 =
-            L::template apply<1>([&](int t) { return col[(L::TMIN[1] + t) * FX]; });
-#pragma unroll
-        for (int i = 0; i < R - 1; ++i) {
-          ring[tl][i][m][0] = ring[tl][i + 1][m][0];
-          ring[tl][i][m][1] = ring[tl][i + 1][m][1];
-        }
-        ring[tl][R - 1][m][0] = y0;
-        ring[tl][R - 1][m][1] = y1;
+        const TS *col = &X[tl][qy - L::TMINALL][fx];
+        ring[tl][J][m][0] = line_apply<L, 0, TS>([&](int t) { return col[(L::TMIN[0] + t) * FX]; });
+        ring[tl][J][m][1] = line_apply<L, 1, TS>([&](int t) { return col[(L::TMIN[1] + t) * FX]; });
       }
     }
-    // 4. z-emission of the fine planes whose last coarse plane is c
+    // 4. z-emission of the fine planes whose last coarse plane is c; their
+    //    planes c-NTP+1..c sit in slots (J-NTP+1+t) mod R
     auto emit = [&](auto PZc) {
       constexpr int PZ = decltype(PZc)::value;
       constexpr int NTP = L::NTAP[PZ];
@@ -178,8 +217,9 @@ __device__ __forceinline__ void tensor_tile(const TensorArgs &a, int b, double *
           double v[NTL];
 #pragma unroll
           for (int tl = 0; tl < NTL; ++tl)
-            v[tl] = L::template apply<PZ>(
-                [&](int t) { return ring[tl][R - NTP + t][m][py]; });
+            v[tl] = (double)line_apply<L, PZ, TS>(
+                [&](int t) { return ring[tl][(J - NTP + 1 + t + 2 * R) % R][m][py]; });
+          // time interpolation in FP64 in both precisions
           double out;
           if constexpr (NTL == 1)
             out = v[0];
@@ -188,12 +228,16 @@ __device__ __forceinline__ void tensor_tile(const TensorArgs &a, int b, double *
           else
             out = fmad(a.s[2], v[2], fmad(a.s[1], v[1], a.s[0] * v[0]));
           if (xin && fy >= 0 && fy < a.n[1])
-            static_cast<double *>(a.dst)[gfx + a.dj * fy + a.dk * fz] = out;
+            __builtin_nontemporal_store((TS)out, &static_cast<TS *>(a.dst)[gfx + a.dj * fy + a.dk * fz]);
         }
       }
     };
     emit(std::integral_constant<int, 0>{});
     emit(std::integral_constant<int, 1>{});
+    ++c;
+    return true;
+  };
+  while (unroll_while<0, R>(plane)) {
   }
   // the next tile of a persistent workgroup overwrites the LDS window
   __syncthreads();

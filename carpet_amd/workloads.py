@@ -343,3 +343,46 @@ class Config2:
         for d, ss, times, reg, t in self.transfers(src, dst):
             b.add(0, d, ss, times, reg, reg, t, OP_LAGRANGE, VERTEX_CENTERED, self.order, 2, prec=prec)
         return b
+
+
+@dataclass
+class Config3:
+    """Cell-centred conservative set: n_var FP64 variables on a coarse 192^3
+    box (+3 ghosts, 198^3 allocated; cell c centred at 2+4c in finest
+    units) refined x2 onto the fine 384^3 region (all children of the
+    interior cells) by prolongate_3d_cc_tvd and prolongate_3d_cc_eno_rf2
+    (order_space 5 -> ENO<3>, 3 -> ENO<2>), then restricted back with
+    restrict_3d_cc_rf2.  Variable 0 is rho = 1 + 0.1 prod_d sin(2 pi x_d)
+    times 10 on one side of the plane n.x = d (n uniform on the sphere,
+    d ~ U(0.3, 0.7), seed 3); the others are smooth products of sines."""
+
+    n_var: int = 8
+    n: int = 192
+    ghosts: int = 3
+
+    def boxes(self):
+        g, n = self.ghosts, self.n
+        c = Box((2 - 4 * g,) * 3, (2 + 4 * (n - 1 + g),) * 3, (4, 4, 4))
+        owned = Box((2,) * 3, (2 + 4 * (n - 1),) * 3, (4, 4, 4))
+        reg = Box((1,) * 3, (1 + 2 * (2 * n - 1),) * 3, (2, 2, 2))
+        fine = reg.expand((g, g, g), (g, g, g))
+        return c, owned, fine, reg
+
+    def coarse_data(self, var: int) -> np.ndarray:
+        c, _, _, _ = self.boxes()
+        m = c.sizes()[0]
+        x = (np.arange(m) - self.ghosts + 0.5) / self.n
+        if var == 0:
+            s = np.sin(2 * np.pi * x)
+            rho = 1 + 0.1 * s[:, None, None] * s[None, :, None] * s[None, None, :]
+            rng = np.random.default_rng(3)
+            nv = rng.normal(size=3)
+            nv /= np.linalg.norm(nv)
+            d = rng.uniform(0.3, 0.7)
+            # n.x with x = (x, y, z) on the (z, y, x) array
+            plane = nv[0] * x[None, None, :] + nv[1] * x[None, :, None] + nv[2] * x[:, None, None]
+            return np.where(plane > d, 10.0 * rho, rho)
+        k = 1 + var % 4
+        a = np.sin(2 * np.pi * k * x + 0.3 * var)
+        b = np.cos(2 * np.pi * (k + 1) * x - 0.1 * var)
+        return 1.0 + 0.2 * a[:, None, None] * b[None, :, None] * a[None, None, :]

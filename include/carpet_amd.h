@@ -245,6 +245,30 @@ int cg_interpolate_3d_3tl_f64(const double *src1, double t1,
                               const int dstbbox[9], const int srcregbbox[9],
                               const int dstregbbox[9], cg_stream_t stream);
 
+/* ---- mixed precision (no reference counterpart: the FP32 form beside the
+   FP64 drop-in).  Sources and destination hold float; every 1-D sum runs in
+   FP32 with compensated (TwoProduct/TwoSum, Dot2) accumulation; the fused
+   time interpolation runs in FP64 and the result is rounded to FP32 once.
+   Same arguments and validation as the FP64 entry points. */
+int cg_prolongate_3d_rf2_mixed(int order, const float *src, const int srcpadext[3],
+                               const int srcext[3], float *dst, const int dstpadext[3],
+                               const int dstext[3], const int srcbbox[9],
+                               const int dstbbox[9], const int srcregbbox[9],
+                               const int dstregbbox[9], cg_stream_t stream);
+int cg_prolongate_3d_cc_rf2_mixed(int order, const float *src, const int srcpadext[3],
+                                  const int srcext[3], float *dst, const int dstpadext[3],
+                                  const int dstext[3], const int srcbbox[9],
+                                  const int dstbbox[9], const int srcregbbox[9],
+                                  const int dstregbbox[9], cg_stream_t stream);
+/* cg_transfer_from for float storage: Lagrange prolongation only (with
+   time interpolation in FP64); copies, restrictions, TVD and ENO stay FP64
+   and are rejected here (CG_ERR_OPERATOR) */
+int cg_transfer_from_mixed(const cg_gdata *dst, const cg_gdata *const *srcs,
+                           const double *times, int ntimes, const int dstregbbox[9],
+                           const int srcregbbox[9], double time, int transport_operator,
+                           int centering, int order_space, int order_time,
+                           cg_stream_t stream);
+
 /* ---- transfer dispatch ------------------------------------------------ */
 /* data<T>::transfer_from_innerloop -> transfer_time -> transfer_p_r ->
    {copy_from_innerloop, transfer_prolongate, transfer_restrict} and
@@ -293,6 +317,21 @@ int cg_batch_run_plain(cg_batch *b, cg_stream_t stream);
 /* captures cg_batch_run into a hipGraph, then replays it */
 int cg_batch_capture_graph(cg_batch *b, cg_stream_t stream);
 int cg_batch_launch_graph(cg_batch *b, cg_stream_t stream);
+/* Autotuner: times each persistent launch of the finalized batch under
+   candidate work-item sizes (tensor tiles: z chunk of 4/8/16/32/64/all
+   coarse bases; pointwise items: 1024..65536 points) with HIP events and
+   keeps the fastest per launch class.  Choices are keyed on (kernel class,
+   order, time levels, precision, log2 of the median region size, log2 of
+   the region count) and kept in the text file cache_path (may be NULL),
+   so a later batch with the same key reuses them without a search.  The
+   search executes each launch (one phase's class of transfers) reps times
+   on its own, outside phase order, so call it before the step whose
+   results are kept, or on scratch data.  Drops a captured graph. */
+int cg_batch_autotune(cg_batch *b, cg_stream_t stream, int reps,
+                      const char *cache_path);
+/* human-readable summary of the persistent launches (phase, class key,
+   work items, grid, tile choice); returns the full length */
+int cg_batch_describe(const cg_batch *b, char *buf, size_t len);
 /* statistics: number of transfers, destination points and algorithmic
    bytes (destination + unique source footprint per time level) */
 int cg_batch_stats(const cg_batch *b, long long *ntransfers,
