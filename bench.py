@@ -95,7 +95,7 @@ def calib_copy(n: int = 512):
     return 8 * n**3
 
 
-def run_config2(steps: int, warmup: int, n_gf: int, plain_steps: int):
+def run_config2(steps: int, warmup: int, n_gf: int, plain_steps: int, tune_cache=None):
     import carpet_amd as cg
     from carpet_amd.workloads import Config2
 
@@ -105,6 +105,8 @@ def run_config2(steps: int, warmup: int, n_gf: int, plain_steps: int):
     st.synchronize()
     b = cfg.batch(src, dst)
     b.finalize()
+    if tune_cache:
+        b.autotune(st, reps=3, cache_path=tune_cache)
     stats = b.stats()
     print(b.describe(), file=sys.stderr, flush=True)
     b.capture_graph(st)
@@ -139,7 +141,8 @@ def run_config2(steps: int, warmup: int, n_gf: int, plain_steps: int):
     return cfg, stats, ms_step, wall, ms_plain
 
 
-def run_config4(n_regions: int, n_gf: int, steps: int, warmup: int, plain_steps: int):
+def run_config4(n_regions: int, n_gf: int, steps: int, warmup: int, plain_steps: int,
+                tune_cache=None):
     """Batched uneven hierarchy: n_regions regions (copy sync / O=5 prolongation
     with 3 time levels / rf2 restriction, edges uniform in [8,96]) per grid
     function; batched graph replay vs one launch per transfer."""
@@ -152,6 +155,8 @@ def run_config4(n_regions: int, n_gf: int, steps: int, warmup: int, plain_steps:
     st = cg.Stream()
     b = w.batch(dev)
     b.finalize()
+    if tune_cache:
+        b.autotune(st, reps=3, cache_path=tune_cache)
     print(f"# config4: {len(w.steps)} transfers, {len(w.arrays)} arrays, set up in "
           f"{time.perf_counter() - t0:.1f} s\n" + b.describe(), file=sys.stderr, flush=True)
     stats = b.stats()
@@ -191,6 +196,9 @@ def main(argv=None):
     ap.add_argument("--config", type=int, default=2, choices=(2, 4),
                     help="2: single large patch (default, the headline line); 4: batched uneven hierarchy")
     ap.add_argument("--n-regions", type=int, default=2000)
+    ap.add_argument("--tune-cache", default=None,
+                    help="batch tuner cache file: classes it names take its parameters, the others are "
+                         "searched (cg_batch_autotune) before the timed steps")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run one 1 GiB copy_3d first (HBM counter calibration under rocprofv3)")
     args = ap.parse_args(argv)
@@ -201,7 +209,8 @@ def main(argv=None):
         return 2
     if args.config == 4:
         n_gf = args.n_gf
-        stats, ms_b, ms_p = run_config4(args.n_regions, n_gf, args.steps, args.warmup, args.plain_steps)
+        stats, ms_b, ms_p = run_config4(args.n_regions, n_gf, args.steps, args.warmup, args.plain_steps,
+                                        args.tune_cache)
         pts, alg = stats["dst_points"], stats["alg_bytes"]
         line = {"metric": METRIC, "value": round(pts / (ms_b * 1e-3) / 1e9, 3), "unit": "Gpoints/s",
                 "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_b, 4),
@@ -225,7 +234,8 @@ def main(argv=None):
         nbytes = calib_copy()
         print(f"# calibration copy: {nbytes} bytes read and written", file=sys.stderr, flush=True)
     print(f"# config2: {args.n_gf} grid functions, {args.steps} steps", file=sys.stderr, flush=True)
-    cfg, stats, ms_step, wall, ms_plain = run_config2(args.steps, args.warmup, args.n_gf, args.plain_steps)
+    cfg, stats, ms_step, wall, ms_plain = run_config2(args.steps, args.warmup, args.n_gf, args.plain_steps,
+                                                      args.tune_cache)
     print(f"# timed: {ms_step:.3f} ms/step batched, {ms_plain:.3f} ms/step plain", file=sys.stderr, flush=True)
     pts = cfg.dst_points()
     alg = cfg.alg_bytes()

@@ -5,7 +5,7 @@
 // Transfers are resolved and validated on the host exactly as
 // cg_transfer_from would (same plans, same error codes).  "plain" runs them
 // as one launch each, the shape of the reference's per-region operator
-// calls.  "batched" runs each phase as one persistent launch over a
+// calls.  "batched" runs each (phase, kernel class) as one launch over a
 // device-resident descriptor list (cg_batch_kernels.hip).
 
 #include <hip/hip_runtime.h>
@@ -113,9 +113,9 @@ static void load_cache(const char *path, TuneMap &m) {
     const size_t bar = line.rfind('|');
     if (line.empty() || line[0] == '#' || bar == std::string::npos)
       continue;
-    int z = 0, p = 0;
-    if (sscanf(line.c_str() + bar + 1, "%d %d", &z, &p) == 2)
-      m[line.substr(0, bar)] = {z, p};
+    int z = 0, p = 0, w = 0;
+    if (sscanf(line.c_str() + bar + 1, "%d %d %d", &z, &p, &w) >= 2)
+      m[line.substr(0, bar)] = Tune{z, p, w};
   }
 }
 
@@ -125,9 +125,11 @@ static int save_cache(const char *path, const TuneMap &m) {
   std::ofstream f(path);
   if (!f)
     return set_error(CG_ERR_ARG, "cannot write tuner cache %s", path);
-  f << "# carpet_amd batch tuner cache: <class key>|<zchunk> <points per item>\n";
+  f << "# carpet_amd batch tuner cache: <class key>|<zchunk> <points per item> "
+       "<workgroups per CU: 0 resident capacity, -1 one per item>\n";
   for (const auto &kv : m)
-    f << kv.first << "|" << kv.second.first << " " << kv.second.second << "\n";
+    f << kv.first << "|" << kv.second.zchunk << " " << kv.second.pt << " " << kv.second.wpc
+      << "\n";
   return CG_OK;
 }
 
@@ -161,6 +163,9 @@ extern "C" int cg_batch_autotune(cg_batch *b, cg_stream_t stream, int reps,
     return set_error(CG_ERR_STATE, "batch not finalized");
   hipStream_t st = static_cast<hipStream_t>(stream);
   reps = std::max(1, reps);
+  // classes named in the cache take its parameters, every other class is
+  // searched (the launches run on the batch's own arrays)
+  b->tuned.clear();
   load_cache(cache_path, b->tuned);
   drop_graph(b);
   // current choices for every class, then search the classes the cache lacks
@@ -168,16 +173,17 @@ extern "C" int cg_batch_autotune(cg_batch *b, cg_stream_t stream, int reps,
     const BatchLaunch l = b->dev.launches[i];
     if (b->tuned.count(l.key))
       continue;
-    std::vector<std::pair<int, int>> cands;
+    // item size first (one workgroup per item), then persistent grids
+    std::vector<Tune> cands;
     if (l.is_tensor)
       for (int z : {4, 8, 16, 32, 64, 0})
-        cands.push_back({z, l.pt});
+        cands.push_back({z, l.pt, -1});
     else
-      for (int p : {1024, 4096, 16384, 65536})
-        cands.push_back({l.zchunk, p});
+      for (int p : {2048, 4096, 16384, 65536})
+        cands.push_back({l.zchunk, p, -1});
     float best = 1e30f;
-    std::pair<int, int> bestc = cands[0];
-    for (const auto &c : cands) {
+    Tune bestc = cands[0];
+    auto trial_run = [&](const Tune &c) -> int {
       TuneMap trial = b->tuned;
       trial[l.key] = c;
       int e = batch_device_build(b->dev, b->transfers, b->phase, b->prec, b->tile_cfg, &trial);
@@ -190,7 +196,15 @@ extern "C" int cg_batch_autotune(cg_batch *b, cg_stream_t stream, int reps,
         best = ms;
         bestc = c;
       }
-    }
+      return CG_OK;
+    };
+    for (const Tune &c : cands)
+      if (int e = trial_run(c))
+        return e;
+    const Tune base = bestc;
+    for (int w : {0, 2})
+      if (int e = trial_run(Tune{base.zchunk, base.pt, w}))
+        return e;
     b->tuned[l.key] = bestc;
   }
   int e = batch_device_build(b->dev, b->transfers, b->phase, b->prec, b->tile_cfg, &b->tuned);
@@ -205,7 +219,7 @@ extern "C" int cg_batch_describe(const cg_batch *b, char *buf, size_t len) {
   std::ostringstream o;
   for (const BatchLaunch &l : b->dev.launches)
     o << "phase " << l.phase << " [" << l.key << "] items " << l.nitems << " grid " << l.grid
-      << " zchunk " << l.zchunk << " pt " << l.pt << "\n";
+      << " zchunk " << l.zchunk << " pt " << l.pt << " wpc " << l.wpc << "\n";
   const std::string s = o.str();
   const size_t n = std::min(len - 1, s.size());
   std::memcpy(buf, s.data(), n);

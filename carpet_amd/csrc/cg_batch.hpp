@@ -27,11 +27,15 @@ struct BatchLaunch {
   int grid;        // persistent workgroups
   int zchunk;      // tensor work item: coarse z bases per chunk (0 = all)
   int pt;          // pointwise work item: destination points
+  int wpc;         // workgroups per CU of the grid (-1 = one per item)
   std::string key; // tuner cache key (class + box-size histogram bucket)
 };
 
-// tuner cache: key -> (zchunk, pt)
-using TuneMap = std::map<std::string, std::pair<int, int>>;
+// tuned parameters of one launch class; wpc 0 = resident capacity
+struct Tune {
+  int zchunk, pt, wpc;
+};
+using TuneMap = std::map<std::string, Tune>;
 
 struct BatchDevice {
   std::vector<BatchLaunch> launches;

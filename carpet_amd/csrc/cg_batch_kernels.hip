@@ -1,16 +1,22 @@
-// cg_batch_kernels.hip -- persistent kernels of the batched level step.
+// cg_batch_kernels.hip -- kernels of the batched level step.
 //
-// The transfers of one phase that share a kernel become one launch of a
-// persistent grid (occupancy x CUs workgroups).  Each workgroup takes work
-// items -- (descriptor, tile) pairs, sorted by decreasing cost -- from a
-// device counter until the list is exhausted, so thousands of uneven
-// regions and variables run as one launch without per-region launch
-// overhead and the biggest tiles start first.  The loop ends for every
-// workgroup once the counter passes the item count, so the grid always
-// drains.  Phases are separate launches in stream order (a kernel boundary
-// makes one phase's results visible to the next across all XCD L2s), and
-// the counters are cleared by a small kernel at the start of the step, so
-// the whole step can be captured as one hipGraph.
+// The transfers of one phase that share a kernel become one launch over a
+// device-resident list of work items -- (descriptor, tile) pairs, transfers
+// by decreasing size, each transfer's tiles in spatial order -- arranged in
+// 8 segments, one per XCD, every transfer striped over all 8 in slabs of
+// equal cost.  Two launch modes:
+//  * grid mode (default): one workgroup per item; workgroup b runs on XCD
+//    b % 8 (the hardware dispatches workgroups round-robin over the XCDs)
+//    and takes slot b / 8 of that XCD's segment, so neighbouring tiles,
+//    which share stencil halos, meet in the same L2;
+//  * persistent mode: resident-capacity workgroups take items from per-XCD
+//    counters, their own segment first, then the others' (work stealing);
+//    the loop ends for every workgroup once all counters pass their
+//    segment ends, so the grid always drains.
+// Phases are separate launches in stream order (a kernel boundary makes one
+// phase's results visible to the next across all XCD L2s), and the counters
+// are cleared by a small kernel at the start of the step, so the whole step
+// can be captured as one hipGraph.
 
 #include <hip/hip_runtime.h>
 
@@ -56,7 +62,9 @@ __device__ __forceinline__ int dequeue(int *ctr8, const int *seg, int *s_item) {
     *s_item = it;
   }
   __syncthreads();
-  const int it = *s_item;
+  // wave-uniform: the item's descriptor is then read with scalar loads and
+  // held in SGPRs, as a kernel argument would be
+  const int it = __builtin_amdgcn_readfirstlane(*s_item);
   __syncthreads();
   return it;
 }
@@ -77,6 +85,41 @@ __global__ __launch_bounds__(2 * CX * CY / IPT) void k_persist_tensor(
   }
 }
 
+constexpr int PU = 8; // points per thread and round of the pointwise kernel
+
+// points [tile*pt, tile*pt + pt) of a region, x fastest
+template <int OP>
+__device__ __forceinline__ void point_item(const PointArgs &a, unsigned tile, int pt) {
+  const unsigned nx = (unsigned)a.n[0];
+  const unsigned nxy = nx * (unsigned)a.n[1];
+  const unsigned tot = nxy * (unsigned)a.n[2]; // < 2^31 (checked on the host)
+  const unsigned e0 = tile * (unsigned)pt;
+  const unsigned e1 = min(tot, e0 + (unsigned)pt);
+  // PU points per thread per round: all loads first, then all stores, so
+  // that PU loads per thread are in flight at once
+  for (unsigned eb = e0; eb < e1; eb += 256u * PU) {
+    double v[PU];
+    long long o[PU];
+#pragma unroll
+    for (int u = 0; u < PU; ++u) {
+      const unsigned e = eb + u * 256u + threadIdx.x;
+      o[u] = -1;
+      if (e < e1) {
+        const unsigned k = fastdiv(e, a.fxy);
+        const unsigned r = e - k * nxy;
+        const unsigned j = fastdiv(r, a.fx);
+        const unsigned i = r - j * nx;
+        v[u] = point_value<OP>(a, (int)i, (int)j, (int)k);
+        o[u] = (long long)i + a.dj * j + a.dk * k;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < PU; ++u)
+      if (o[u] >= 0)
+        __builtin_nontemporal_store(v[u], a.dst + o[u]);
+  }
+}
+
 template <int OP>
 __global__ __launch_bounds__(256) void k_persist_point(
     const PointArgs *__restrict__ descs, const Item *__restrict__ items,
@@ -87,23 +130,40 @@ __global__ __launch_bounds__(256) void k_persist_point(
     if (it < 0)
       return;
     const Item w = items[it];
-    const PointArgs a = descs[w.desc];
-    const unsigned nx = (unsigned)a.n[0];
-    const unsigned nxy = nx * (unsigned)a.n[1];
-    const unsigned tot = nxy * (unsigned)a.n[2];
-    const unsigned e0 = w.tile * (unsigned)pt;
-#pragma unroll 4
-    for (int m = 0; m < pt / 256; ++m) {
-      const unsigned e = e0 + m * 256u + threadIdx.x;
-      if (e >= tot)
-        break;
-      const unsigned k = e / nxy;
-      const unsigned r = e - k * nxy;
-      const unsigned j = r / nx;
-      const unsigned i = r - j * nx;
-      point_elem<OP>(a, (int)i, (int)j, (int)k);
-    }
+    point_item<OP>(descs[w.desc], w.tile, pt);
   }
+}
+
+// Grid mode: one workgroup per item, no loop and no counters.  Workgroup b
+// runs on XCD b % 8 and takes slot b / 8 of that XCD's segment; segments
+// shorter than the longest leave their last workgroups without an item.
+__device__ __forceinline__ int grid_item(const int *__restrict__ seg) {
+  const int x = blockIdx.x & 7, slot = blockIdx.x >> 3;
+  const int i = seg[x] + slot;
+  return i < seg[x + 1] ? i : -1;
+}
+
+template <class L, int NTL, int CX, int CY, int IPT, typename TS>
+__global__ __launch_bounds__(2 * CX * CY / IPT) void k_grid_tensor(
+    const TensorArgs *__restrict__ descs, const Item *__restrict__ items,
+    const int *__restrict__ seg) {
+  __shared__ double smem[TensorLds<L, NTL, CX, CY>::TOTAL];
+  const int it = grid_item(seg);
+  if (it < 0)
+    return;
+  const Item w = items[it];
+  tensor_tile<L, NTL, CX, CY, IPT, TS>(descs[w.desc], (int)w.tile, smem);
+}
+
+template <int OP>
+__global__ __launch_bounds__(256) void k_grid_point(const PointArgs *__restrict__ descs,
+                                                    const Item *__restrict__ items,
+                                                    const int *__restrict__ seg, int pt) {
+  const int it = grid_item(seg);
+  if (it < 0)
+    return;
+  const Item w = items[it];
+  point_item<OP>(descs[w.desc], w.tile, pt);
 }
 
 __global__ void k_zero_counters(int *c, int n) {
@@ -151,9 +211,9 @@ const void *launch_fn(const BatchLaunch &l, int *threads) {
   with_line(l.kind, l.order, [&](auto tag) -> int {
     using L = decltype(tag);
     switch (l.ntl) {
-    case 1: fn = persist_tensor_fn<L, 1>(l.prec); *threads = 2 * TileShape<1>::CX * TileShape<1>::CY / TileShape<1>::IPT; break;
-    case 2: fn = persist_tensor_fn<L, 2>(l.prec); *threads = 2 * TileShape<2>::CX * TileShape<2>::CY / TileShape<2>::IPT; break;
-    case 3: fn = persist_tensor_fn<L, 3>(l.prec); *threads = 2 * TileShape<3>::CX * TileShape<3>::CY / TileShape<3>::IPT; break;
+    case 1: fn = persist_tensor_fn<L, 1>(l.prec); *threads = tile_threads<1>(); break;
+    case 2: fn = persist_tensor_fn<L, 2>(l.prec); *threads = tile_threads<2>(); break;
+    case 3: fn = persist_tensor_fn<L, 3>(l.prec); *threads = tile_threads<3>(); break;
     }
     return 0;
   });
@@ -172,6 +232,31 @@ void batch_device_free(BatchDevice &dev) {
   dev.launches.clear();
   dev.blob_bytes = 0;
   dev.ncounters = 0;
+}
+
+// Workgroups of fn that one CU holds at once, from the kernel's registers
+// and LDS and the device's per-CU LDS: each SIMD has 512 VGPRs per lane
+// (allocated in blocks of 8) and at most 8 waves; a workgroup's waves are
+// spread over the 4 SIMDs.  The runtime's occupancy query is the fallback
+// and the bound when it reports more.
+static int resident_per_cu(const void *fn, int threads, int dev) {
+  int api = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&api, fn, threads, 0) != hipSuccess)
+    api = 1;
+  hipFuncAttributes fa;
+  int lds_cu = 0;
+  if (hipFuncGetAttributes(&fa, fn) != hipSuccess ||
+      hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) !=
+          hipSuccess ||
+      fa.numRegs <= 0)
+    return std::max(1, api);
+  const int waves = (threads + 63) / 64;
+  const int per_simd = (waves + 3) / 4;
+  const int vgprs = (fa.numRegs + 7) / 8 * 8;
+  const int by_vgpr = std::min(8, 512 / vgprs) / per_simd;
+  const int by_lds = fa.sharedSizeBytes > 0 ? lds_cu / (int)fa.sharedSizeBytes : 32;
+  const int by_waves = 32 / waves;
+  return std::max({1, api, std::min({by_vgpr, by_lds, by_waves})});
 }
 
 static int ilog2(long long v) {
@@ -258,23 +343,23 @@ int batch_device_build(BatchDevice &dev, const std::vector<Transfer> &tr,
     }
     l.zchunk = tile_cfg < 0 ? ZCHUNK_DEFAULT : tile_cfg;
     l.pt = PT_DEFAULT;
+    l.wpc = -1;
     if (tuned) {
       auto it = tuned->find(l.key);
       if (it != tuned->end()) {
-        l.zchunk = it->second.first;
-        l.pt = it->second.second;
+        l.zchunk = it->second.zchunk;
+        l.pt = it->second.pt;
+        l.wpc = it->second.wpc;
       }
     }
-    // persistent grid: occupancy x CUs, a multiple of the 8 XCDs
+    // persistent grid: resident workgroups per CU x CUs, a multiple of the
+    // 8 XCDs
     int threads = 0;
     const void *fn = launch_fn(l, &threads);
     if (!fn)
       return set_error(CG_ERR_ORDER, "no persistent kernel for this transfer class");
-    int per_cu = 0;
-    he = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, 0);
-    if (he != hipSuccess)
-      return hip_error(he, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
-    const int full_grid = std::max(8, (std::max(1, per_cu) * ncu) / 8 * 8);
+    const int per_cu = resident_per_cu(fn, threads, dev_id);
+    const int full_grid = std::max(8, (per_cu * ncu) / 8 * 8);
     // transfers largest first; a transfer's items in spatial order (x tile
     // fastest, then y, then z chunk), so consecutive items are neighbours
     std::vector<int> members = g.members;
@@ -283,10 +368,11 @@ int batch_device_build(BatchDevice &dev, const std::vector<Transfer> &tr,
     });
     std::vector<Item> items;
     std::vector<double> cost;
+    std::vector<int> first; // first item of each transfer, plus the end
     if (g.is_tensor) {
       TileCfg cfg;
-      cfg.cx = g.ntl == 1 ? TileShape<1>::CX : TileShape<3>::CX;
-      cfg.cy = g.ntl == 1 ? TileShape<1>::CY : TileShape<3>::CY;
+      int tthreads = 0;
+      tile_shape(g.ntl, &cfg.cx, &cfg.cy, &tthreads);
       // z chunk: whole columns (no halo planes) when that already gives the
       // grid a few items per workgroup, else the longest chunk that does
       if (tile_cfg < 0 && !(tuned && tuned->count(l.key))) {
@@ -310,24 +396,29 @@ int batch_device_build(BatchDevice &dev, const std::vector<Transfer> &tr,
         const long long nt = tensor_tiles(a, cfg);
         const unsigned d = (unsigned)descs.size();
         descs.push_back(a);
+        first.push_back((int)items.size());
         for (long long t = 0; t < nt; ++t) {
           items.push_back(Item{d, (unsigned)t});
-          // cost ~ coarse planes x bases of the tile
+          // cost ~ coarse planes (with the halo) x tile area: a tile's
+          // lanes are busy whether or not the region fills it
           const int tz = (int)(t / ((long long)a.ntiles[0] * a.ntiles[1]));
-          const int nz = std::min(a.zchunk, a.nq[2] - tz * a.zchunk) + 4;
-          cost.push_back((double)nz * std::min(cfg.cx, a.nq[0]) * std::min(cfg.cy, a.nq[1]));
+          const int nz = std::min(a.zchunk, a.nq[2] - tz * a.zchunk) + 5;
+          cost.push_back((double)nz * cfg.cx * cfg.cy);
         }
       }
       l.desc_off = append(descs.data(), descs.size() * sizeof(TensorArgs));
     } else {
       std::vector<PointArgs> descs;
       for (int m : members) {
-        const PointArgs &a = tr[m].is_prolong ? tr[m].pr.p : tr[m].pt.a;
+        PointArgs a = tr[m].is_prolong ? tr[m].pr.p : tr[m].pt.a;
         const long long tot = (long long)a.n[0] * a.n[1] * a.n[2];
-        if (tot >= (1LL << 32))
+        if (tot >= (1LL << 31) - l.pt)
           return set_error(CG_ERR_ARG, "region too large for the batch");
+        a.fx = make_fastdiv((unsigned)a.n[0]);
+        a.fxy = make_fastdiv((unsigned)a.n[0] * (unsigned)a.n[1]);
         const unsigned d = (unsigned)descs.size();
         descs.push_back(a);
+        first.push_back((int)items.size());
         const long long nt = (tot + l.pt - 1) / l.pt;
         for (long long t = 0; t < nt; ++t) {
           items.push_back(Item{d, (unsigned)t});
@@ -336,29 +427,67 @@ int batch_device_build(BatchDevice &dev, const std::vector<Transfer> &tr,
       }
       l.desc_off = append(descs.data(), descs.size() * sizeof(PointArgs));
     }
-    l.item_off = append(items.data(), items.size() * sizeof(Item));
+    first.push_back((int)items.size());
     l.nitems = (int)items.size();
-    // 8 contiguous segments of equal cost, one per XCD
+    // One segment per XCD.  Every transfer is striped over the XCDs: its
+    // items are cut into 8 contiguous parts of equal cost (spatial slabs,
+    // e.g. ranges of y tiles) and part x goes to XCD x, so all XCDs work on
+    // the same transfer at the same time, as one launch per transfer would,
+    // with each slab's halos shared in one L2.  A transfer with fewer than
+    // 8 items gives each item to the XCD with the least work so far.
+    int segb[9];
     {
-      double total = 0;
-      for (double c : cost)
-        total += c;
-      int segb[9];
-      segb[0] = 0;
-      double acc = 0;
-      int k = 1;
-      for (int i = 0; i < l.nitems && k < 8; ++i) {
-        acc += cost[i];
-        while (k < 8 && acc >= total * k / 8.0)
-          segb[k++] = i + 1;
+      std::vector<Item> part[8];
+      double load[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      for (size_t m = 0; m + 1 < first.size(); ++m) {
+        const int i0 = first[m], i1 = first[m + 1];
+        if (i1 - i0 < 8) {
+          for (int i = i0; i < i1; ++i) {
+            const int x = (int)(std::min_element(load, load + 8) - load);
+            part[x].push_back(items[i]);
+            load[x] += cost[i];
+          }
+          continue;
+        }
+        double total = 0;
+        for (int i = i0; i < i1; ++i)
+          total += cost[i];
+        double acc = 0;
+        int x = 0;
+        for (int i = i0; i < i1; ++i) {
+          // item i belongs to the part its cost midpoint falls in
+          const double mid = acc + 0.5 * cost[i];
+          while (x < 7 && mid >= total * (x + 1) / 8.0)
+            ++x;
+          part[x].push_back(items[i]);
+          load[x] += cost[i];
+          acc += cost[i];
+        }
       }
-      while (k < 9)
-        segb[k++] = l.nitems;
-      segb[8] = l.nitems;
+      items.clear();
+      segb[0] = 0;
+      for (int x = 0; x < 8; ++x) {
+        items.insert(items.end(), part[x].begin(), part[x].end());
+        segb[x + 1] = (int)items.size();
+      }
       l.seg_off = append(segb, sizeof segb);
     }
+    l.item_off = append(items.data(), items.size() * sizeof(Item));
     l.counter = (int)dev.launches.size();
-    l.grid = std::min(full_grid, std::max(8, (l.nitems + 7) / 8 * 8));
+    if (l.wpc < 0) {
+      // grid mode: 8 x the longest segment
+      long long longest = 1;
+      for (int x = 0; x < 8; ++x)
+        longest = std::max<long long>(longest, segb[x + 1] - segb[x]);
+      if (8 * longest > 0x7fffffffLL)
+        return set_error(CG_ERR_ARG, "too many work items for one launch");
+      l.grid = (int)(8 * longest);
+    } else {
+      const long long all = std::max(8, (l.nitems + 7) / 8 * 8);
+      const long long grid =
+          l.wpc > 0 ? std::max(8LL, (long long)l.wpc * ncu / 8 * 8) : full_grid;
+      l.grid = (int)std::min(grid, all);
+    }
     dev.launches.push_back(l);
   }
   dev.ncounters = (int)dev.launches.size();
@@ -382,8 +511,13 @@ static int run_launch(BatchDevice &dev, const BatchLaunch &l, hipStream_t st);
 int batch_device_run(BatchDevice &dev, hipStream_t st) {
   if (dev.launches.empty())
     return CG_OK;
-  hipLaunchKernelGGL(k_zero_counters, dim3((8 * dev.ncounters + 255) / 256), dim3(256), 0, st,
-                     dev.counters, 8 * dev.ncounters);
+  // only persistent launches use the work counters
+  bool persistent = false;
+  for (const BatchLaunch &l : dev.launches)
+    persistent = persistent || l.wpc >= 0;
+  if (persistent)
+    hipLaunchKernelGGL(k_zero_counters, dim3((8 * dev.ncounters + 255) / 256), dim3(256), 0, st,
+                       dev.counters, 8 * dev.ncounters);
   for (const BatchLaunch &l : dev.launches) {
     int e = run_launch(dev, l, st);
     if (e)
@@ -401,52 +535,64 @@ int batch_device_run_one(BatchDevice &dev, int idx, hipStream_t st) {
 }
 
 static int run_launch(BatchDevice &dev, const BatchLaunch &l, hipStream_t st) {
-  {
-    const void *descs = dev.blob + l.desc_off;
-    const Item *items = reinterpret_cast<const Item *>(dev.blob + l.item_off);
-    const int *seg = reinterpret_cast<const int *>(dev.blob + l.seg_off);
-    int *ctr = dev.counters + 8 * l.counter;
-    int rc = 0;
-    if (l.is_tensor) {
-      rc = with_line(l.kind, l.order, [&](auto tag) -> int {
-        using L = decltype(tag);
-        auto go = [&](auto ntlc) {
-          constexpr int NTL = decltype(ntlc)::value;
-          using T = TileShape<NTL>;
-          if (l.prec)
-            hipLaunchKernelGGL((k_persist_tensor<L, NTL, T::CX, T::CY, T::IPT, float>), dim3(l.grid),
-                               dim3(2 * T::CX * T::CY / T::IPT), 0, st,
-                               static_cast<const TensorArgs *>(descs), items, seg, ctr);
-          else
-            hipLaunchKernelGGL((k_persist_tensor<L, NTL, T::CX, T::CY, T::IPT, double>), dim3(l.grid),
-                               dim3(2 * T::CX * T::CY / T::IPT), 0, st,
-                               static_cast<const TensorArgs *>(descs), items, seg, ctr);
-          return 0;
-        };
-        switch (l.ntl) {
-        case 1: return go(std::integral_constant<int, 1>{});
-        case 2: return go(std::integral_constant<int, 2>{});
-        case 3: return go(std::integral_constant<int, 3>{});
-        }
-        return -1;
-      });
-    } else {
-      const PointArgs *pd = static_cast<const PointArgs *>(descs);
-      switch (l.op) {
-      case PO_COPY: hipLaunchKernelGGL(k_persist_point<PO_COPY>, dim3(l.grid), dim3(256), 0, st, pd, items, seg, ctr, l.pt); break;
-      case PO_RESTRICT_VC: hipLaunchKernelGGL(k_persist_point<PO_RESTRICT_VC>, dim3(l.grid), dim3(256), 0, st, pd, items, seg, ctr, l.pt); break;
-      case PO_RESTRICT_CC: hipLaunchKernelGGL(k_persist_point<PO_RESTRICT_CC>, dim3(l.grid), dim3(256), 0, st, pd, items, seg, ctr, l.pt); break;
-      case PO_INTERP: hipLaunchKernelGGL(k_persist_point<PO_INTERP>, dim3(l.grid), dim3(256), 0, st, pd, items, seg, ctr, l.pt); break;
-      case PO_TVD: hipLaunchKernelGGL(k_persist_point<PO_TVD>, dim3(l.grid), dim3(256), 0, st, pd, items, seg, ctr, l.pt); break;
-      default: rc = -1;
+  const void *descs = dev.blob + l.desc_off;
+  const Item *items = reinterpret_cast<const Item *>(dev.blob + l.item_off);
+  const int *seg = reinterpret_cast<const int *>(dev.blob + l.seg_off);
+  int *ctr = dev.counters + 8 * l.counter;
+  const bool grid_mode = l.wpc < 0;
+  int rc = 0;
+  if (l.is_tensor) {
+    rc = with_line(l.kind, l.order, [&](auto tag) -> int {
+      using L = decltype(tag);
+      auto go = [&](auto ntlc, auto tsc) {
+        constexpr int NTL = decltype(ntlc)::value;
+        using TS = decltype(tsc);
+        using T = TileShape<NTL>;
+        const TensorArgs *td = static_cast<const TensorArgs *>(descs);
+        const dim3 block(2 * T::CX * T::CY / T::IPT);
+        if (grid_mode)
+          hipLaunchKernelGGL((k_grid_tensor<L, NTL, T::CX, T::CY, T::IPT, TS>), dim3(l.grid), block,
+                             0, st, td, items, seg);
+        else
+          hipLaunchKernelGGL((k_persist_tensor<L, NTL, T::CX, T::CY, T::IPT, TS>), dim3(l.grid),
+                             block, 0, st, td, items, seg, ctr);
+        return 0;
+      };
+      auto by_prec = [&](auto ntlc) {
+        return l.prec ? go(ntlc, float{}) : go(ntlc, double{});
+      };
+      switch (l.ntl) {
+      case 1: return by_prec(std::integral_constant<int, 1>{});
+      case 2: return by_prec(std::integral_constant<int, 2>{});
+      case 3: return by_prec(std::integral_constant<int, 3>{});
       }
+      return -1;
+    });
+  } else {
+    const PointArgs *pd = static_cast<const PointArgs *>(descs);
+    auto go = [&](auto opc) {
+      constexpr int OP = decltype(opc)::value;
+      if (grid_mode)
+        hipLaunchKernelGGL(k_grid_point<OP>, dim3(l.grid), dim3(256), 0, st, pd, items, seg, l.pt);
+      else
+        hipLaunchKernelGGL(k_persist_point<OP>, dim3(l.grid), dim3(256), 0, st, pd, items, seg, ctr,
+                           l.pt);
+      return 0;
+    };
+    switch (l.op) {
+    case PO_COPY: rc = go(std::integral_constant<int, PO_COPY>{}); break;
+    case PO_RESTRICT_VC: rc = go(std::integral_constant<int, PO_RESTRICT_VC>{}); break;
+    case PO_RESTRICT_CC: rc = go(std::integral_constant<int, PO_RESTRICT_CC>{}); break;
+    case PO_INTERP: rc = go(std::integral_constant<int, PO_INTERP>{}); break;
+    case PO_TVD: rc = go(std::integral_constant<int, PO_TVD>{}); break;
+    default: rc = -1;
     }
-    if (rc)
-      return set_error(CG_ERR_ORDER, "no persistent kernel for this transfer class");
-    hipError_t he = hipGetLastError();
-    if (he != hipSuccess)
-      return hip_error(he, "persistent launch");
   }
+  if (rc)
+    return set_error(CG_ERR_ORDER, "no batch kernel for this transfer class");
+  hipError_t he = hipGetLastError();
+  if (he != hipSuccess)
+    return hip_error(he, "batch launch");
   return CG_OK;
 }
 

@@ -26,6 +26,25 @@ struct TensorArgs {
   double s[3];        // time weights
 };
 
+// Division by a fixed d >= 1 for dividends n < 2^31 as a multiply and a
+// shift: m = ceil(2^(31+l) / d), l = ceil(log2 d), q = (n * m) >> (31 + l)
+// (Granlund & Montgomery, "Division by invariant integers using
+// multiplication", PLDI 1994, Thm 4.2 with N = 31: m*d - 2^(31+l) < 2^l).
+// m < 2^32 for every d, so the product is one 32x32->64 multiply.
+struct FastDiv {
+  unsigned m;
+  int s;
+};
+inline FastDiv make_fastdiv(unsigned d) {
+  int l = 0;
+  while ((1ull << l) < d)
+    ++l;
+  return FastDiv{(unsigned)(((1ull << (31 + l)) + d - 1) / d), 31 + l};
+}
+__device__ __forceinline__ unsigned fastdiv(unsigned n, FastDiv f) {
+  return (unsigned)(((unsigned long long)n * f.m) >> f.s);
+}
+
 // Pointwise region operators: copy (with slab shift), injection
 // restriction, cell-centred restriction, time interpolation, cell TVD.
 enum PointOp { PO_COPY = 0, PO_RESTRICT_VC = 1, PO_RESTRICT_CC = 2,
@@ -42,6 +61,7 @@ struct PointArgs {
   int ntl;              // time levels (interp)
   double s[3];          // time weights (interp)
   int off[3];           // TVD: fine offsets (off+f)/2 = parent, (off+f)%2 = side
+  FastDiv fx, fxy;      // division by n[0] and n[0]*n[1] (batch work items)
 };
 
 // tile configuration of the tensor kernel

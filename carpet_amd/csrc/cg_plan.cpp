@@ -12,6 +12,7 @@
 #include <cstring>
 
 #include "cg_internal.hpp"
+#include "cg_tiles.hpp"
 #include "cg_stencil.hpp"
 
 namespace cg {
@@ -267,15 +268,15 @@ int launch_prolong(const ProlongPlan &pl, int prec, hipStream_t st) {
     r = launch_point(PO_TVD, pl.p, st);
   } else {
     TileCfg cfg{0, 0, 0};
-    // z chunks: enough workgroups to fill the chip, each chunk long enough
-    // to amortise the stencil's halo planes
+    // z chunks: enough threads to fill the chip (2^19: 8 resident waves per
+    // SIMD), each chunk long enough to amortise the stencil's halo planes
     TensorArgs a = pl.t;
-    cfg.cx = 32;
-    cfg.cy = pl.ntl == 1 ? 8 : 4;
+    int threads = 0;
+    tile_shape(pl.ntl, &cfg.cx, &cfg.cy, &threads);
     const long long cols = (long long)((a.nq[0] + cfg.cx - 1) / cfg.cx) *
                            ((a.nq[1] + cfg.cy - 1) / cfg.cy);
     int zc = a.nq[2];
-    while (zc > 16 && cols * ((a.nq[2] + zc - 1) / zc) < 2048)
+    while (zc > 16 && cols * ((a.nq[2] + zc - 1) / zc) * threads < (1LL << 19))
       zc = (zc + 1) / 2;
     cfg.zchunk = zc;
     r = launch_tensor(pl.kind, pl.order, pl.ntl, prec, a, cfg, st);

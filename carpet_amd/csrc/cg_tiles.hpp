@@ -85,7 +85,8 @@ template <int I, int N, class F> __device__ __forceinline__ bool unroll_while(F 
   return true;
 }
 
-template <class L, int NTL, int CX, int CY, int IPT, typename TS = double>
+// PF: planes loaded ahead of their use (register buffers per plane)
+template <class L, int NTL, int CX, int CY, int IPT, typename TS = double, int PF = 1>
 __device__ __forceinline__ void tensor_tile(const TensorArgs &a, int b, double *smem) {
   constexpr int FX = 2 * CX;
   constexpr int NT = FX * CY / IPT;
@@ -96,6 +97,10 @@ __device__ __forceinline__ void tensor_tile(const TensorArgs &a, int b, double *
   constexpr int WY = LD::WY;
   constexpr int R = L::R;
   constexpr int NLD = (WY * WX + NT - 1) / NT; // window elements per thread
+  // planes per unrolled round: ring slot and prefetch slot are both
+  // compile-time constants within a round
+  constexpr int U = (R % PF == 0) ? R : R * PF;
+  static_assert(U % PF == 0 && U % R == 0, "unroll must cover ring and prefetch slots");
 
   TS(*A)[WY][WX] = reinterpret_cast<TS(*)[WY][WX]>(smem);
   TS(*X)[WY][FX] = reinterpret_cast<TS(*)[WY][FX]>(reinterpret_cast<TS *>(smem) + LD::A);
@@ -133,15 +138,17 @@ __device__ __forceinline__ void tensor_tile(const TensorArgs &a, int b, double *
                   ? gx + a.sj * gy
                   : -1;
   }
-  // next plane's window, loaded one plane ahead of its use
-  TS pre[NTL][NLD];
-  auto fetch = [&](int c) {
+  // windows of the next PF planes, loaded PF planes ahead of their use;
+  // plane c sits in buffer (c - c0) % PF
+  TS pre[PF][NTL][NLD];
+  auto fetch = [&](int c, auto slotc) {
+    constexpr int SL = decltype(slotc)::value;
 #pragma unroll
     for (int k = 0; k < NLD; ++k)
       if (woff[k] >= 0)
 #pragma unroll
         for (int tl = 0; tl < NTL; ++tl)
-          pre[tl][k] = static_cast<const TS *>(a.src[tl])[woff[k] + a.sk * c];
+          pre[SL][tl][k] = static_cast<const TS *>(a.src[tl])[woff[k] + a.sk * c];
   };
 
   // ring of y-pass results: slot j holds the planes c with (c - c0) % R == j
@@ -154,24 +161,30 @@ __device__ __forceinline__ void tensor_tile(const TensorArgs &a, int b, double *
       for (int m = 0; m < IPT; ++m)
         ring[tl][i][m][0] = ring[tl][i][m][1] = TS(0);
 
-  if (c0 <= c1)
-    fetch(c0);
+  auto prologue = [&](auto Sc) -> bool {
+    constexpr int S = decltype(Sc)::value;
+    if (c0 + S <= c1)
+      fetch(c0 + S, std::integral_constant<int, S>{});
+    return true;
+  };
+  unroll_while<0, PF>(prologue);
   int c = c0;
-  auto plane = [&](auto Jc) -> bool {
-    constexpr int J = decltype(Jc)::value;
+  auto plane = [&](auto Uc) -> bool {
+    constexpr int J = decltype(Uc)::value % R;  // ring slot of plane c
+    constexpr int SL = decltype(Uc)::value % PF; // prefetch buffer of plane c
     if (c > c1)
       return false;
-    // 1. window of plane c -> LDS; start loading plane c+1
+    // 1. window of plane c -> LDS; start loading plane c+PF into its buffer
 #pragma unroll
     for (int k = 0; k < NLD; ++k)
       if (woff[k] >= 0) {
         const int e = tid + k * NT;
 #pragma unroll
         for (int tl = 0; tl < NTL; ++tl)
-          (&A[tl][0][0])[e] = pre[tl][k];
+          (&A[tl][0][0])[e] = pre[SL][tl][k];
       }
-    if (c < c1)
-      fetch(c + 1);
+    if (c + PF <= c1)
+      fetch(c + PF, std::integral_constant<int, SL>{});
     __syncthreads();
     // 2. x-pass: both children of each base
     for (int e = tid; e < WY * CX; e += NT) {
@@ -237,7 +250,7 @@ __device__ __forceinline__ void tensor_tile(const TensorArgs &a, int b, double *
     ++c;
     return true;
   };
-  while (unroll_while<0, R>(plane)) {
+  while (unroll_while<0, U>(plane)) {
   }
   // the next tile of a persistent workgroup overwrites the LDS window
   __syncthreads();
@@ -250,9 +263,10 @@ __device__ __forceinline__ double minmod(double a, double b) {
   return __builtin_fabs(a) < __builtin_fabs(b) ? a : b;
 }
 
-// one destination point (i,j,k) of a pointwise operator
+// value of destination point (i,j,k) of a pointwise operator (loads only,
+// so that a caller can issue the loads of several points before storing)
 template <int OP>
-__device__ __forceinline__ void point_elem(const PointArgs &a, int i, int j, int k) {
+__device__ __forceinline__ double point_value(const PointArgs &a, int i, int j, int k) {
   double out;
   if constexpr (OP == PO_COPY) {
     out = a.src[0][i + a.sj * j + a.sk * k];
@@ -293,20 +307,39 @@ __device__ __forceinline__ void point_elem(const PointArgs &a, int i, int j, int
     const double sz = sz0 * minmod(c - p[-K], p[K] - c);
     out = c + sx + sy + sz;
   }
-  a.dst[i + a.dj * j + a.dk * k] = out;
+  return out;
 }
 
-// tile shapes: one z-marching column of 2CX x 2CY fine points
+template <int OP>
+__device__ __forceinline__ void point_elem(const PointArgs &a, int i, int j, int k) {
+  a.dst[i + a.dj * j + a.dk * k] = point_value<OP>(a, i, j, k);
+}
+
+// tile shapes: one z-marching column of 2CX x 2CY fine points, 2CX*CY/IPT
+// threads.  With 2-3 time levels the register ring is 2-3x larger, so each
+// thread keeps one base row (IPT 1, 512 threads); 8 base rows per tile cut
+// the y halo of the x-pass and of the window loads to (8+span)/8.
 template <int NTL> struct TileShape;
 template <> struct TileShape<1> {
   static constexpr int CX = 32, CY = 8, IPT = 2;
 };
 template <> struct TileShape<2> {
-  static constexpr int CX = 32, CY = 4, IPT = 1;
+  static constexpr int CX = 32, CY = 8, IPT = 1;
 };
 template <> struct TileShape<3> {
-  static constexpr int CX = 32, CY = 4, IPT = 1;
+  static constexpr int CX = 32, CY = 8, IPT = 1;
 };
+template <int NTL> constexpr int tile_threads() {
+  return 2 * TileShape<NTL>::CX * TileShape<NTL>::CY / TileShape<NTL>::IPT;
+}
+// host view of the tile shape for ntl time levels
+inline void tile_shape(int ntl, int *cx, int *cy, int *threads) {
+  switch (ntl) {
+  case 1: *cx = TileShape<1>::CX; *cy = TileShape<1>::CY; *threads = tile_threads<1>(); break;
+  case 2: *cx = TileShape<2>::CX; *cy = TileShape<2>::CY; *threads = tile_threads<2>(); break;
+  default: *cx = TileShape<3>::CX; *cy = TileShape<3>::CY; *threads = tile_threads<3>(); break;
+  }
+}
 
 } // namespace cg
 

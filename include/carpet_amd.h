@@ -291,9 +291,11 @@ int cg_transfer_from(const cg_gdata *dst, const cg_gdata *const *srcs,
    (CarpetLib/src/ggf.cc:500-591) for a whole level step: the caller adds
    every (region, variable) transfer once (phases order the dependent
    steps, e.g. restrict=0, coarse sync=1, prolongate=2, fine sync=3), the
-   batch builds a device-resident descriptor list, and each phase runs as
-   one persistent kernel over it; the whole step can be captured as one
-   hipGraph and replayed while the hierarchy is unchanged. */
+   batch builds a device-resident descriptor list, and each (phase, kernel
+   class) runs as one launch over it -- one workgroup per work item, or a
+   persistent grid taking items from per-XCD counters; the whole step can
+   be captured as one hipGraph and replayed while the hierarchy is
+   unchanged. */
 typedef struct cg_batch cg_batch;
 int cg_batch_create(cg_batch **out);
 int cg_batch_destroy(cg_batch *b);
@@ -309,7 +311,7 @@ int cg_batch_add(cg_batch *b, int phase, const cg_gdata *dst,
 /* builds tiles and uploads the descriptor list; tile_cfg < 0 lets the
    tuner choose */
 int cg_batch_finalize(cg_batch *b, int tile_cfg);
-/* runs every phase, one persistent launch per phase */
+/* runs every phase, one launch per (phase, kernel class) */
 int cg_batch_run(cg_batch *b, cg_stream_t stream);
 /* runs the same work as one launch per (region, variable) transfer: the
    "plain" per-box baseline */
@@ -317,20 +319,23 @@ int cg_batch_run_plain(cg_batch *b, cg_stream_t stream);
 /* captures cg_batch_run into a hipGraph, then replays it */
 int cg_batch_capture_graph(cg_batch *b, cg_stream_t stream);
 int cg_batch_launch_graph(cg_batch *b, cg_stream_t stream);
-/* Autotuner: times each persistent launch of the finalized batch under
-   candidate work-item sizes (tensor tiles: z chunk of 4/8/16/32/64/all
-   coarse bases; pointwise items: 1024..65536 points) with HIP events and
-   keeps the fastest per launch class.  Choices are keyed on (kernel class,
-   order, time levels, precision, log2 of the median region size, log2 of
-   the region count) and kept in the text file cache_path (may be NULL),
-   so a later batch with the same key reuses them without a search.  The
+/* Autotuner: times each launch of the finalized batch under candidate
+   work-item sizes (tensor tiles: z chunk of 4/8/16/32/64/all coarse bases;
+   pointwise items: 2048..65536 points), one workgroup per item, then the
+   best size on persistent grids of resident capacity and of 2 workgroups
+   per CU, with HIP events, and keeps the fastest per launch class.
+   Choices are keyed on (kernel class, order, time levels, precision, log2
+   of the median region size, log2 of the region count) and kept in the
+   text file cache_path (may be NULL; lines "<key>|<zchunk> <points>
+   <workgroups per CU: -1 one per item, 0 resident capacity>"): classes the
+   file names take its parameters, the others are searched.  The
    search executes each launch (one phase's class of transfers) reps times
    on its own, outside phase order, so call it before the step whose
    results are kept, or on scratch data.  Drops a captured graph. */
 int cg_batch_autotune(cg_batch *b, cg_stream_t stream, int reps,
                       const char *cache_path);
-/* human-readable summary of the persistent launches (phase, class key,
-   work items, grid, tile choice); returns the full length */
+/* human-readable summary of the launches (phase, class key, work items,
+   grid, tile choice, grid mode); returns the full length */
 int cg_batch_describe(const cg_batch *b, char *buf, size_t len);
 /* statistics: number of transfers, destination points and algorithmic
    bytes (destination + unique source footprint per time level) */

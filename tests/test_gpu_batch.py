@@ -1,6 +1,9 @@
-"""Batched level step on the GPU: persistent launches over the descriptor
-list, the plain per-region launches and hipGraph replays all equal the CPU
-oracle's replay of the same transfers, bit for bit."""
+"""Batched level step on the GPU: batched launches over the descriptor list
+(one workgroup per item, and persistent grids with per-XCD work counters),
+autotuned launches, the plain per-region launches and hipGraph replays all
+equal the CPU oracle's replay of the same transfers, bit for bit."""
+import re
+
 import numpy as np
 import pytest
 
@@ -37,7 +40,7 @@ def _poison_outputs(w, dev, stream):
 
 @pytest.mark.parametrize("make", [lambda: config1(), lambda: config4(n_regions=45, n_gf=2, edge=(4, 40))],
                          ids=["config1", "config4-small"])
-def test_batch_modes_match_oracle(make):
+def test_batch_modes_match_oracle(make, tmp_path):
     w = make()
     print(f"workload {w.name}: {len(w.steps)} transfers", flush=True)
     ref = run_oracle(w)
@@ -46,10 +49,31 @@ def test_batch_modes_match_oracle(make):
     b = w.batch(dev)
     b.finalize()
     print("stats", b.stats(), flush=True)
-    # persistent batched launches
+    print(b.describe(), flush=True)
+    # batched launches, one workgroup per item
     b.run(st)
     st.synchronize()
     _compare(_outputs(w, dev), ref, "batched")
+    if w.name != "config1":
+        # persistent grids (a tuner cache naming every class with wpc 0 and
+        # 2), then the tuner's own search; each time from poisoned outputs
+        keys = re.findall(r"\[([^\]]+)\]", b.describe())
+        for wpc, zc, pt in ((0, 8, 1024), (2, 0, 65536)):
+            cache = tmp_path / f"tune_{wpc}.txt"
+            cache.write_text("".join(f"{k}|{zc} {pt} {wpc}\n" for k in keys))
+            b.autotune(st, reps=1, cache_path=str(cache))
+            assert f"wpc {wpc}" in b.describe()
+            _poison_outputs(w, dev, st)
+            b.run(st)
+            st.synchronize()
+            _compare(_outputs(w, dev), ref, f"persistent wpc {wpc}")
+        cache = tmp_path / "tune_search.txt"
+        b.autotune(st, reps=1, cache_path=str(cache))
+        print("tuned:\n" + cache.read_text() + b.describe(), flush=True)
+        _poison_outputs(w, dev, st)
+        b.run(st)
+        st.synchronize()
+        _compare(_outputs(w, dev), ref, "autotuned")
     # plain: one launch per transfer
     _poison_outputs(w, dev, st)
     if w.name == "config1":
