@@ -6,9 +6,10 @@ at t = 1, 0, -1, prolongated by prolongate_3d_rf2<5> fused with
 interpolate_3d_3tl onto the fine region [2,513]^3 at t = 1/3 and 2/3.
 One step = all 48 (grid function, target time) transfers of the level step.
 
-The batched path (one persistent launch over the device descriptor list,
-replayed as a captured hipGraph) gives `value`; the plain path (one launch
-per transfer, the reference's per-region call shape) is timed beside it.
+The batched path (one launch over the device descriptor list, a workgroup
+per tile column, replayed as a captured hipGraph) gives `value`; the plain
+path (one launch per transfer, the reference's per-region call shape) is
+timed beside it.
 Kernel time is measured with HIP events on the stream the work runs on.
 `cpu_baseline` times the CPU oracle (the reference's algorithm restated,
 prolongating every time level into a temporary and interpolating in time,
@@ -218,7 +219,7 @@ def main(argv=None):
                 "data": "synthetic",
                 "config": {"workload": f"batched uneven level step: {args.n_regions} regions x {n_gf} grid "
                                        "functions (edges U[8,96]; copy sync, O=5 prolongation + 3 time levels, "
-                                       "rf2 restriction), batched persistent launches + hipGraph",
+                                       "rf2 restriction), one batched launch per phase + hipGraph",
                            "transfers": stats["transfers"], "dst_points_per_step": pts,
                            "algorithmic_bytes_per_step": alg},
                 "gbytes_per_s": round(alg / (ms_b * 1e-3) / 1e9, 1),
@@ -258,7 +259,8 @@ def main(argv=None):
                                "(+3 ghosts, 3 time levels t=1,0,-1) -> fine 512^3, prolongate_3d_rf2<5> fused "
                                "with interpolate_3d_3tl at t=1/3 and 2/3",
                    "grid_functions": cfg.n_gf, "fine_points_per_step": pts,
-                   "algorithmic_bytes_per_step": alg, "path": "batched persistent launch, hipGraph replay"},
+                   "algorithmic_bytes_per_step": alg,
+                   "path": "one batched launch (a workgroup per 32x8-base tile column), hipGraph replay"},
         "gbytes_per_s": round(gbs, 1),
         "plain": {"ms_per_step": round(ms_plain, 4), "gpoints_per_s": round(pts / (ms_plain * 1e-3) / 1e9, 3),
                   "launches_per_step": stats["transfers"]},
@@ -266,7 +268,7 @@ def main(argv=None):
         "wall_s_timed": round(wall, 3),
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs / HBM_PEAK_GBS, 4),
-                     "traffic": _profile_traffic("k_persist_tensor_vlag5_3tl_config2")},
+                     "traffic": _profile_traffic("k_grid_tensor_vlag5_3tl_config2")},
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.cpu_seconds)

@@ -176,7 +176,7 @@ extern "C" int cg_batch_autotune(cg_batch *b, cg_stream_t stream, int reps,
     // item size first (one workgroup per item), then persistent grids
     std::vector<Tune> cands;
     if (l.is_tensor)
-      for (int z : {4, 8, 16, 32, 64, 0})
+      for (int z : {8, 16, 32, 64, 128, 0})
         cands.push_back({z, l.pt, -1});
     else
       for (int p : {2048, 4096, 16384, 65536})

@@ -31,6 +31,20 @@ struct BatchLaunch {
   std::string key; // tuner cache key (class + box-size histogram bucket)
 };
 
+// one work item: a tile (tensor) or a run of points (pointwise) of one
+// transfer's descriptor
+struct Item {
+  unsigned desc;
+  unsigned tile;
+};
+
+// batched tensor launches per line kind (LK_*); defined in
+// cg_batch_tensor.hpp, instantiated in cg_batch_{vlag,clag,eno}.hip
+template <int KIND>
+int batch_tensor_kind(const BatchLaunch &l, const TensorArgs *td, const Item *items,
+                      const int *seg, int *ctr, hipStream_t st);
+template <int KIND> const void *batch_tensor_fn_kind(const BatchLaunch &l, int *threads);
+
 // tuned parameters of one launch class; wpc 0 = resident capacity
 struct Tune {
   int zchunk, pt, wpc;

@@ -1,0 +1,8 @@
+// cg_batch_eno.hip -- batched tensor launches for LK_ENO line operators.
+#include "cg_batch_tensor.hpp"
+
+namespace cg {
+template int batch_tensor_kind<LK_ENO>(const BatchLaunch &, const TensorArgs *, const Item *,
+                                        const int *, int *, hipStream_t);
+template const void *batch_tensor_fn_kind<LK_ENO>(const BatchLaunch &, int *);
+} // namespace cg

@@ -24,8 +24,7 @@
 #include <cstring>
 #include <vector>
 
-#include "cg_batch.hpp"
-#include "cg_tiles.hpp"
+#include "cg_batch_dev.hpp"
 
 namespace cg {
 
@@ -34,56 +33,6 @@ namespace cg {
 constexpr int PT_DEFAULT = 4096;
 constexpr int ZCHUNK_DEFAULT = 16;
 
-struct Item {
-  unsigned desc;
-  unsigned tile;
-};
-
-// Work items are split into 8 contiguous, cost-balanced segments, one per
-// XCD (workgroups are dispatched round-robin over the XCDs, so workgroup b
-// runs on XCD b % 8): neighbouring tiles -- which share stencil halos --
-// are then processed at the same time on the same L2.  A workgroup takes
-// items from its own segment's counter and, once that is exhausted, from
-// the other segments', so the load still balances.  Returns -1 when every
-// segment is exhausted; every workgroup gets there, so the grid drains.
-__device__ __forceinline__ int dequeue(int *ctr8, const int *seg, int *s_item) {
-  if (threadIdx.x == 0) {
-    int it = -1;
-    const int home = blockIdx.x & 7;
-    for (int k = 0; k < 8 && it < 0; ++k) {
-      const int sg = (home + k) & 7;
-      const int n = seg[sg + 1] - seg[sg];
-      if (n <= 0 || __hip_atomic_load(&ctr8[sg], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= n)
-        continue;
-      const int i = atomicAdd(&ctr8[sg], 1);
-      if (i < n)
-        it = seg[sg] + i;
-    }
-    *s_item = it;
-  }
-  __syncthreads();
-  // wave-uniform: the item's descriptor is then read with scalar loads and
-  // held in SGPRs, as a kernel argument would be
-  const int it = __builtin_amdgcn_readfirstlane(*s_item);
-  __syncthreads();
-  return it;
-}
-
-template <class L, int NTL, int CX, int CY, int IPT, typename TS>
-__global__ __launch_bounds__(2 * CX * CY / IPT) void k_persist_tensor(
-    const TensorArgs *__restrict__ descs, const Item *__restrict__ items,
-    const int *__restrict__ seg, int *counter) {
-  __shared__ double smem[TensorLds<L, NTL, CX, CY>::TOTAL];
-  __shared__ int s_item;
-  for (;;) {
-    const int it = dequeue(counter, seg, &s_item);
-    if (it < 0)
-      return;
-    const Item w = items[it];
-    const TensorArgs a = descs[w.desc];
-    tensor_tile<L, NTL, CX, CY, IPT, TS>(a, (int)w.tile, smem);
-  }
-}
 
 constexpr int PU = 8; // points per thread and round of the pointwise kernel
 
@@ -134,27 +83,6 @@ __global__ __launch_bounds__(256) void k_persist_point(
   }
 }
 
-// Grid mode: one workgroup per item, no loop and no counters.  Workgroup b
-// runs on XCD b % 8 and takes slot b / 8 of that XCD's segment; segments
-// shorter than the longest leave their last workgroups without an item.
-__device__ __forceinline__ int grid_item(const int *__restrict__ seg) {
-  const int x = blockIdx.x & 7, slot = blockIdx.x >> 3;
-  const int i = seg[x] + slot;
-  return i < seg[x + 1] ? i : -1;
-}
-
-template <class L, int NTL, int CX, int CY, int IPT, typename TS>
-__global__ __launch_bounds__(2 * CX * CY / IPT) void k_grid_tensor(
-    const TensorArgs *__restrict__ descs, const Item *__restrict__ items,
-    const int *__restrict__ seg) {
-  __shared__ double smem[TensorLds<L, NTL, CX, CY>::TOTAL];
-  const int it = grid_item(seg);
-  if (it < 0)
-    return;
-  const Item w = items[it];
-  tensor_tile<L, NTL, CX, CY, IPT, TS>(descs[w.desc], (int)w.tile, smem);
-}
-
 template <int OP>
 __global__ __launch_bounds__(256) void k_grid_point(const PointArgs *__restrict__ descs,
                                                     const Item *__restrict__ items,
@@ -184,13 +112,6 @@ struct Group {
   }
 };
 
-template <class L, int NTL> const void *persist_tensor_fn(int prec) {
-  using T = TileShape<NTL>;
-  if (prec)
-    return reinterpret_cast<const void *>(&k_persist_tensor<L, NTL, T::CX, T::CY, T::IPT, float>);
-  return reinterpret_cast<const void *>(&k_persist_tensor<L, NTL, T::CX, T::CY, T::IPT, double>);
-}
-
 const void *persist_point_fn(int op) {
   switch (op) {
   case PO_COPY: return reinterpret_cast<const void *>(&k_persist_point<PO_COPY>);
@@ -207,17 +128,12 @@ const void *launch_fn(const BatchLaunch &l, int *threads) {
     *threads = 256;
     return persist_point_fn(l.op);
   }
-  const void *fn = nullptr;
-  with_line(l.kind, l.order, [&](auto tag) -> int {
-    using L = decltype(tag);
-    switch (l.ntl) {
-    case 1: fn = persist_tensor_fn<L, 1>(l.prec); *threads = tile_threads<1>(); break;
-    case 2: fn = persist_tensor_fn<L, 2>(l.prec); *threads = tile_threads<2>(); break;
-    case 3: fn = persist_tensor_fn<L, 3>(l.prec); *threads = tile_threads<3>(); break;
-    }
-    return 0;
-  });
-  return fn;
+  switch (l.kind) {
+  case LK_VLAG: return batch_tensor_fn_kind<LK_VLAG>(l, threads);
+  case LK_CLAG: return batch_tensor_fn_kind<LK_CLAG>(l, threads);
+  case LK_ENO: return batch_tensor_fn_kind<LK_ENO>(l, threads);
+  }
+  return nullptr;
 }
 
 } // namespace
@@ -373,11 +289,12 @@ int batch_device_build(BatchDevice &dev, const std::vector<Transfer> &tr,
       TileCfg cfg;
       int tthreads = 0;
       tile_shape(g.ntl, &cfg.cx, &cfg.cy, &tthreads);
-      // z chunk: whole columns (no halo planes) when that already gives the
-      // grid a few items per workgroup, else the longest chunk that does
+      // z chunk: the longest of 128..8 planes that still gives the grid a few
+      // items per resident workgroup (tiles that march further apart in z
+      // stop sharing their halo rows through L2)
       if (tile_cfg < 0 && !(tuned && tuned->count(l.key))) {
-        l.zchunk = 0;
-        for (int zc : {0, 64, 32, 16, 8}) {
+        l.zchunk = 128;
+        for (int zc : {128, 64, 32, 16, 8}) {
           l.zchunk = zc;
           long long tiles = 0;
           for (int m : members) {
@@ -542,32 +459,13 @@ static int run_launch(BatchDevice &dev, const BatchLaunch &l, hipStream_t st) {
   const bool grid_mode = l.wpc < 0;
   int rc = 0;
   if (l.is_tensor) {
-    rc = with_line(l.kind, l.order, [&](auto tag) -> int {
-      using L = decltype(tag);
-      auto go = [&](auto ntlc, auto tsc) {
-        constexpr int NTL = decltype(ntlc)::value;
-        using TS = decltype(tsc);
-        using T = TileShape<NTL>;
-        const TensorArgs *td = static_cast<const TensorArgs *>(descs);
-        const dim3 block(2 * T::CX * T::CY / T::IPT);
-        if (grid_mode)
-          hipLaunchKernelGGL((k_grid_tensor<L, NTL, T::CX, T::CY, T::IPT, TS>), dim3(l.grid), block,
-                             0, st, td, items, seg);
-        else
-          hipLaunchKernelGGL((k_persist_tensor<L, NTL, T::CX, T::CY, T::IPT, TS>), dim3(l.grid),
-                             block, 0, st, td, items, seg, ctr);
-        return 0;
-      };
-      auto by_prec = [&](auto ntlc) {
-        return l.prec ? go(ntlc, float{}) : go(ntlc, double{});
-      };
-      switch (l.ntl) {
-      case 1: return by_prec(std::integral_constant<int, 1>{});
-      case 2: return by_prec(std::integral_constant<int, 2>{});
-      case 3: return by_prec(std::integral_constant<int, 3>{});
-      }
-      return -1;
-    });
+    const TensorArgs *td = static_cast<const TensorArgs *>(descs);
+    switch (l.kind) {
+    case LK_VLAG: rc = batch_tensor_kind<LK_VLAG>(l, td, items, seg, ctr, st); break;
+    case LK_CLAG: rc = batch_tensor_kind<LK_CLAG>(l, td, items, seg, ctr, st); break;
+    case LK_ENO: rc = batch_tensor_kind<LK_ENO>(l, td, items, seg, ctr, st); break;
+    default: rc = -1;
+    }
   } else {
     const PointArgs *pd = static_cast<const PointArgs *>(descs);
     auto go = [&](auto opc) {

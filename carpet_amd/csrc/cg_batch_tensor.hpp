@@ -1,0 +1,98 @@
+// cg_batch_tensor.hpp -- batched tensor prolongation kernels (grid and
+// persistent modes), one launch template per line kind; instantiated in
+// cg_batch_{vlag,clag,eno}.hip so the kinds compile in parallel.
+#pragma once
+
+#include "cg_batch_dev.hpp"
+
+namespace cg {
+
+template <class L, int NTL, int CX, int CY, int IPT, int XQ, typename TS>
+__global__ __launch_bounds__(2 * CX * CY / IPT) void k_persist_tensor(
+    const TensorArgs *__restrict__ descs, const Item *__restrict__ items,
+    const int *__restrict__ seg, int *counter) {
+  __shared__ __align__(16) double smem[TensorLds<L, NTL, CX, CY, XQ>::TOTAL];
+  __shared__ int s_item;
+  for (;;) {
+    const int it = dequeue(counter, seg, &s_item);
+    if (it < 0)
+      return;
+    const Item w = items[it];
+    const TensorArgs a = descs[w.desc];
+    tensor_tile<L, NTL, CX, CY, IPT, TS, 1, XQ>(a, (int)w.tile, smem);
+  }
+}
+
+template <class L, int NTL, int CX, int CY, int IPT, int XQ, typename TS>
+__global__ __launch_bounds__(2 * CX * CY / IPT) void k_grid_tensor(
+    const TensorArgs *__restrict__ descs, const Item *__restrict__ items,
+    const int *__restrict__ seg) {
+  __shared__ __align__(16) double smem[TensorLds<L, NTL, CX, CY, XQ>::TOTAL];
+  const int it = grid_item(seg);
+  if (it < 0)
+    return;
+  const Item w = items[it];
+  tensor_tile<L, NTL, CX, CY, IPT, TS, 1, XQ>(descs[w.desc], (int)w.tile, smem);
+}
+
+template <class L, int NTL, typename TS>
+int batch_tensor_go(const BatchLaunch &l, const TensorArgs *td, const Item *items,
+                    const int *seg, int *ctr, hipStream_t st) {
+  using T = TileShape<NTL>;
+  const dim3 block(tile_threads<NTL>());
+  if (l.wpc < 0)
+    hipLaunchKernelGGL((k_grid_tensor<L, NTL, T::CX, T::CY, T::IPT, T::XQ, TS>), dim3(l.grid), block, 0,
+                       st, td, items, seg);
+  else
+    hipLaunchKernelGGL((k_persist_tensor<L, NTL, T::CX, T::CY, T::IPT, T::XQ, TS>), dim3(l.grid), block,
+                       0, st, td, items, seg, ctr);
+  return 0;
+}
+
+template <int KIND>
+int batch_tensor_kind(const BatchLaunch &l, const TensorArgs *td, const Item *items,
+                      const int *seg, int *ctr, hipStream_t st) {
+  return with_order<KIND>(l.order, [&](auto tag) -> int {
+    using L = decltype(tag);
+    if (l.prec == 0) {
+      switch (l.ntl) {
+      case 1: return batch_tensor_go<L, 1, double>(l, td, items, seg, ctr, st);
+      case 2: return batch_tensor_go<L, 2, double>(l, td, items, seg, ctr, st);
+      case 3: return batch_tensor_go<L, 3, double>(l, td, items, seg, ctr, st);
+      }
+    } else if constexpr (KIND != LK_ENO) {
+      switch (l.ntl) {
+      case 1: return batch_tensor_go<L, 1, float>(l, td, items, seg, ctr, st);
+      case 2: return batch_tensor_go<L, 2, float>(l, td, items, seg, ctr, st);
+      case 3: return batch_tensor_go<L, 3, float>(l, td, items, seg, ctr, st);
+      }
+    }
+    return -1;
+  });
+}
+
+// persistent kernel of a launch class (for the resident-capacity query)
+template <int KIND> const void *batch_tensor_fn_kind(const BatchLaunch &l, int *threads) {
+  const void *fn = nullptr;
+  with_order<KIND>(l.order, [&](auto tag) -> int {
+    using L = decltype(tag);
+    auto pick = [&](auto ntlc) {
+      constexpr int NTL = decltype(ntlc)::value;
+      using T = TileShape<NTL>;
+      *threads = tile_threads<NTL>();
+      if (l.prec == 0)
+        fn = reinterpret_cast<const void *>(&k_persist_tensor<L, NTL, T::CX, T::CY, T::IPT, T::XQ, double>);
+      else if constexpr (KIND != LK_ENO)
+        fn = reinterpret_cast<const void *>(&k_persist_tensor<L, NTL, T::CX, T::CY, T::IPT, T::XQ, float>);
+    };
+    switch (l.ntl) {
+    case 1: pick(std::integral_constant<int, 1>{}); break;
+    case 2: pick(std::integral_constant<int, 2>{}); break;
+    case 3: pick(std::integral_constant<int, 3>{}); break;
+    }
+    return 0;
+  });
+  return fn;
+}
+
+} // namespace cg

@@ -1,0 +1,8 @@
+// cg_batch_vlag.hip -- batched tensor launches for LK_VLAG line operators.
+#include "cg_batch_tensor.hpp"
+
+namespace cg {
+template int batch_tensor_kind<LK_VLAG>(const BatchLaunch &, const TensorArgs *, const Item *,
+                                        const int *, int *, hipStream_t);
+template const void *batch_tensor_fn_kind<LK_VLAG>(const BatchLaunch &, int *);
+} // namespace cg

@@ -1,17 +1,12 @@
-// cg_kernels.hip -- per-call launches of the transfer kernels (bodies in
-// cg_tiles.hpp).
+// cg_kernels.hip -- per-call launches of the pointwise kernels, the tensor
+// launch dispatch (instances in cg_tensor_{vlag,clag,eno}.hip) and the
+// synthetic-data fills (bodies in cg_tiles.hpp).
 
 #include <hip/hip_runtime.h>
 
 #include "cg_tiles.hpp"
 
 namespace cg {
-
-template <class L, int NTL, int CX, int CY, int IPT, typename TS>
-__global__ __launch_bounds__(2 * CX * CY / IPT) void k_tensor(TensorArgs a) {
-  __shared__ double smem[TensorLds<L, NTL, CX, CY>::TOTAL];
-  tensor_tile<L, NTL, CX, CY, IPT, TS>(a, xcd_remap(blockIdx.x, gridDim.x), smem);
-}
 
 long long tensor_tiles(TensorArgs &a, const TileCfg &cfg) {
   a.ntiles[0] = (a.nq[0] + cfg.cx - 1) / cfg.cx;
@@ -21,43 +16,16 @@ long long tensor_tiles(TensorArgs &a, const TileCfg &cfg) {
   return (long long)a.ntiles[0] * a.ntiles[1] * a.ntiles[2];
 }
 
-template <class L, int NTL, typename TS>
-static int launch_tensor_t(const TensorArgs &a0, const TileCfg &cfg0, hipStream_t s) {
-  using T = TileShape<NTL>;
-  TensorArgs a = a0;
-  TileCfg cfg = cfg0;
-  cfg.cx = T::CX;
-  cfg.cy = T::CY;
-  const long long nblk = tensor_tiles(a, cfg);
-  if (nblk <= 0 || nblk > 0x7fffffffLL)
-    return -1;
-  constexpr int NT = 2 * T::CX * T::CY / T::IPT;
-  hipLaunchKernelGGL((k_tensor<L, NTL, T::CX, T::CY, T::IPT, TS>), dim3((unsigned)nblk),
-                     dim3(NT), 0, s, a);
-  return hipGetLastError() == hipSuccess ? 0 : -2;
-}
-
 int launch_tensor(int kind, int order, int ntl, int prec, const TensorArgs &a,
                   const TileCfg &cfg, hipStream_t s) {
   if (prec != 0 && (prec != 1 || kind == LK_ENO))
     return -1;
-  return with_line(kind, order, [&](auto tag) -> int {
-    using L = decltype(tag);
-    if (prec == 0) {
-      switch (ntl) {
-      case 1: return launch_tensor_t<L, 1, double>(a, cfg, s);
-      case 2: return launch_tensor_t<L, 2, double>(a, cfg, s);
-      case 3: return launch_tensor_t<L, 3, double>(a, cfg, s);
-      }
-    } else {
-      switch (ntl) {
-      case 1: return launch_tensor_t<L, 1, float>(a, cfg, s);
-      case 2: return launch_tensor_t<L, 2, float>(a, cfg, s);
-      case 3: return launch_tensor_t<L, 3, float>(a, cfg, s);
-      }
-    }
-    return -1;
-  });
+  switch (kind) {
+  case LK_VLAG: return launch_tensor_kind<LK_VLAG>(order, ntl, prec, a, cfg, s);
+  case LK_CLAG: return launch_tensor_kind<LK_CLAG>(order, ntl, prec, a, cfg, s);
+  case LK_ENO: return launch_tensor_kind<LK_ENO>(order, ntl, prec, a, cfg, s);
+  }
+  return -1;
 }
 
 template <int OP>

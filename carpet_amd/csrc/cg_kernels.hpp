@@ -73,6 +73,11 @@ struct TileCfg {
 // ---- launchers (defined in cg_kernels.hip) --------------------------------
 // kind: LK_VLAG/LK_CLAG/LK_ENO; prec 0: double sources/destination,
 // 1: float sources/destination with compensated accumulation.
+// per line kind (LK_*); defined in cg_tensor_plain.hpp, instantiated in
+// cg_tensor_{vlag,clag,eno}.hip
+template <int KIND>
+int launch_tensor_kind(int order, int ntl, int prec, const TensorArgs &a, const TileCfg &cfg,
+                       hipStream_t s);
 int launch_tensor(int kind, int order, int ntl, int prec, const TensorArgs &a,
                   const TileCfg &cfg, hipStream_t s);
 // computes ntiles/zchunk of a for a tile configuration; returns block count

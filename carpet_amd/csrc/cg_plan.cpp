@@ -268,15 +268,16 @@ int launch_prolong(const ProlongPlan &pl, int prec, hipStream_t st) {
     r = launch_point(PO_TVD, pl.p, st);
   } else {
     TileCfg cfg{0, 0, 0};
-    // z chunks: enough threads to fill the chip (2^19: 8 resident waves per
-    // SIMD), each chunk long enough to amortise the stencil's halo planes
+    // z chunks: at most 128 planes (tiles that march much further apart in
+    // z stop sharing their halo rows through L2), at least 2^18 threads (a
+    // full round of resident workgroups), at least 16 planes per chunk
     TensorArgs a = pl.t;
     int threads = 0;
     tile_shape(pl.ntl, &cfg.cx, &cfg.cy, &threads);
     const long long cols = (long long)((a.nq[0] + cfg.cx - 1) / cfg.cx) *
                            ((a.nq[1] + cfg.cy - 1) / cfg.cy);
-    int zc = a.nq[2];
-    while (zc > 16 && cols * ((a.nq[2] + zc - 1) / zc) * threads < (1LL << 19))
+    int zc = std::min(a.nq[2], 128);
+    while (zc > 16 && cols * ((a.nq[2] + zc - 1) / zc) * threads < (1LL << 18))
       zc = (zc + 1) / 2;
     cfg.zchunk = zc;
     r = launch_tensor(pl.kind, pl.order, pl.ntl, prec, a, cfg, st);

@@ -54,15 +54,21 @@ template <typename TS> __device__ __forceinline__ double ld(const void *p, long 
   return (double)(static_cast<const TS *>(p)[o]);
 }
 
-// LDS footprint (doubles) of one tensor tile
-template <class L, int NTL, int CX, int CY> struct TensorLds {
+// LDS footprint (doubles) of one tensor tile; XQ = bases per thread in the
+// x-pass (2: rows padded to an even stride for 16-byte LDS accesses)
+template <class L, int NTL, int CX, int CY, int XQ = 1> struct TensorLds {
   static constexpr int SPAN = L::TMAXALL - L::TMINALL;
   static constexpr int WX = CX + SPAN;
+  static constexpr int WXS = XQ == 1 ? WX : (WX + 1) / 2 * 2;
   static constexpr int WY = CY + SPAN;
-  static constexpr int A = NTL * WY * WX;
+  static constexpr int A = NTL * WY * WXS;
   static constexpr int X = NTL * WY * 2 * CX;
   static constexpr int TOTAL = A + X;
 };
+
+template <typename TS> struct Pair;
+template <> struct Pair<double> { using T = double2; };
+template <> struct Pair<float> { using T = float2; };
 
 // One z-marching tile; tile b indexes (x tile, y tile, z chunk) of a.
 // line operator in the storage precision: FP64 (the drop-in) or FP32 with
@@ -86,23 +92,28 @@ template <int I, int N, class F> __device__ __forceinline__ bool unroll_while(F 
 }
 
 // PF: planes loaded ahead of their use (register buffers per plane)
-template <class L, int NTL, int CX, int CY, int IPT, typename TS = double, int PF = 1>
+// XQ: coarse bases per thread in the x-pass (1, or 2 with paired LDS
+// accesses); smem must be 16-byte aligned
+template <class L, int NTL, int CX, int CY, int IPT, typename TS = double, int PF = 1,
+          int XQ = 1>
 __device__ __forceinline__ void tensor_tile(const TensorArgs &a, int b, double *smem) {
   constexpr int FX = 2 * CX;
   constexpr int NT = FX * CY / IPT;
   constexpr int NQY = NT / FX;
   static_assert(NQY * IPT == CY, "tile rows must divide among threads");
-  using LD = TensorLds<L, NTL, CX, CY>;
+  static_assert(XQ == 1 || (XQ == 2 && CX % 2 == 0), "x-pass bases per thread: 1 or 2");
+  using LD = TensorLds<L, NTL, CX, CY, XQ>;
   constexpr int WX = LD::WX;
+  constexpr int WXS = LD::WXS; // LDS row stride of the window
   constexpr int WY = LD::WY;
   constexpr int R = L::R;
-  constexpr int NLD = (WY * WX + NT - 1) / NT; // window elements per thread
+  constexpr int NLD = (WY * WXS + NT - 1) / NT; // window elements per thread
   // planes per unrolled round: ring slot and prefetch slot are both
   // compile-time constants within a round
   constexpr int U = (R % PF == 0) ? R : R * PF;
   static_assert(U % PF == 0 && U % R == 0, "unroll must cover ring and prefetch slots");
 
-  TS(*A)[WY][WX] = reinterpret_cast<TS(*)[WY][WX]>(smem);
+  TS(*A)[WY][WXS] = reinterpret_cast<TS(*)[WY][WXS]>(smem);
   TS(*X)[WY][FX] = reinterpret_cast<TS(*)[WY][FX]>(reinterpret_cast<TS *>(smem) + LD::A);
 
   const int tx = b % a.ntiles[0];
@@ -132,9 +143,9 @@ __device__ __forceinline__ void tensor_tile(const TensorArgs &a, int b, double *
 #pragma unroll
   for (int k = 0; k < NLD; ++k) {
     const int e = tid + k * NT;
-    const int r = e / WX, x = e - r * WX;
+    const int r = e / WXS, x = e - r * WXS;
     const int gy = wy0 + r, gx = wx0 + x;
-    woff[k] = (e < WY * WX && gy >= ly0 && gy <= ly1 && gx >= lx0 && gx <= lx1)
+    woff[k] = (e < WY * WXS && x < WX && gy >= ly0 && gy <= ly1 && gx >= lx0 && gx <= lx1)
                   ? gx + a.sj * gy
                   : -1;
   }
@@ -187,15 +198,43 @@ __device__ __forceinline__ void tensor_tile(const TensorArgs &a, int b, double *
       fetch(c + PF, std::integral_constant<int, SL>{});
     __syncthreads();
     // 2. x-pass: both children of each base
-    for (int e = tid; e < WY * CX; e += NT) {
-      const int r = e / CX, q = e - r * CX;
+    if constexpr (XQ == 1) {
+      for (int e = tid; e < WY * CX; e += NT) {
+        const int r = e / CX, q = e - r * CX;
 #pragma unroll
-      for (int tl = 0; tl < NTL; ++tl) {
-        const TS *w = &A[tl][r][q - L::TMINALL];
-        const TS v0 = line_apply<L, 0, TS>([&](int t) { return w[L::TMIN[0] + t]; });
-        const TS v1 = line_apply<L, 1, TS>([&](int t) { return w[L::TMIN[1] + t]; });
-        X[tl][r][2 * q] = v0;
-        X[tl][r][2 * q + 1] = v1;
+        for (int tl = 0; tl < NTL; ++tl) {
+          const TS *w = &A[tl][r][q - L::TMINALL];
+          const TS v0 = line_apply<L, 0, TS>([&](int t) { return w[L::TMIN[0] + t]; });
+          const TS v1 = line_apply<L, 1, TS>([&](int t) { return w[L::TMIN[1] + t]; });
+          X[tl][r][2 * q] = v0;
+          X[tl][r][2 * q + 1] = v1;
+        }
+      }
+    } else {
+      // bases q0, q0+1 from the window columns q0 .. q0+SPAN+1 (window
+      // column 0 is offset TMINALL of base 0), read and written in pairs
+      using P2 = typename Pair<TS>::T;
+      constexpr int NP = (LD::SPAN + 3) / 2;
+      for (int e = tid; e < WY * (CX / 2); e += NT) {
+        const int r = e / (CX / 2), q0 = 2 * (e - r * (CX / 2));
+#pragma unroll
+        for (int tl = 0; tl < NTL; ++tl) {
+          TS v[2 * NP];
+#pragma unroll
+          for (int m = 0; m < NP; ++m) {
+            const P2 p = *reinterpret_cast<const P2 *>(&A[tl][r][q0 + 2 * m]);
+            v[2 * m] = p.x;
+            v[2 * m + 1] = p.y;
+          }
+          constexpr int O0 = L::TMIN[0] - L::TMINALL, O1 = L::TMIN[1] - L::TMINALL;
+          P2 lo, hi;
+          lo.x = line_apply<L, 0, TS>([&](int t) { return v[O0 + t]; });
+          lo.y = line_apply<L, 1, TS>([&](int t) { return v[O1 + t]; });
+          hi.x = line_apply<L, 0, TS>([&](int t) { return v[1 + O0 + t]; });
+          hi.y = line_apply<L, 1, TS>([&](int t) { return v[1 + O1 + t]; });
+          *reinterpret_cast<P2 *>(&X[tl][r][2 * q0]) = lo;
+          *reinterpret_cast<P2 *>(&X[tl][r][2 * q0 + 2]) = hi;
+        }
       }
     }
     __syncthreads();
@@ -319,15 +358,17 @@ __device__ __forceinline__ void point_elem(const PointArgs &a, int i, int j, int
 // threads.  With 2-3 time levels the register ring is 2-3x larger, so each
 // thread keeps one base row (IPT 1, 512 threads); 8 base rows per tile cut
 // the y halo of the x-pass and of the window loads to (8+span)/8.
+// XQ = 2: the x-pass reads and writes LDS in 16-byte pairs (two bases per
+// thread), a third fewer LDS reads than one base per thread.
 template <int NTL> struct TileShape;
 template <> struct TileShape<1> {
-  static constexpr int CX = 32, CY = 8, IPT = 2;
+  static constexpr int CX = 32, CY = 8, IPT = 2, XQ = 2;
 };
 template <> struct TileShape<2> {
-  static constexpr int CX = 32, CY = 8, IPT = 1;
+  static constexpr int CX = 32, CY = 8, IPT = 1, XQ = 2;
 };
 template <> struct TileShape<3> {
-  static constexpr int CX = 32, CY = 8, IPT = 1;
+  static constexpr int CX = 32, CY = 8, IPT = 1, XQ = 2;
 };
 template <int NTL> constexpr int tile_threads() {
   return 2 * TileShape<NTL>::CX * TileShape<NTL>::CY / TileShape<NTL>::IPT;
@@ -344,11 +385,10 @@ inline void tile_shape(int ntl, int *cx, int *cy, int *threads) {
 } // namespace cg
 
 namespace cg {
-// Calls fn(Line<KIND,O>{}) for a runtime (kind, order); returns -1 if there
-// is no such line operator.
-template <class Fn> int with_line(int kind, int order, Fn &&fn) {
-  switch (kind) {
-  case LK_VLAG:
+// Calls fn(Line<KIND,O>{}) for a runtime order; returns -1 if there is no
+// such line operator.
+template <int KIND, class Fn> int with_order(int order, Fn &&fn) {
+  if constexpr (KIND == LK_VLAG) {
     switch (order) {
     case 1: return fn(Line<LK_VLAG, 1>{});
     case 3: return fn(Line<LK_VLAG, 3>{});
@@ -357,8 +397,7 @@ template <class Fn> int with_line(int kind, int order, Fn &&fn) {
     case 9: return fn(Line<LK_VLAG, 9>{});
     case 11: return fn(Line<LK_VLAG, 11>{});
     }
-    break;
-  case LK_CLAG:
+  } else if constexpr (KIND == LK_CLAG) {
     switch (order) {
     case 0: return fn(Line<LK_CLAG, 0>{});
     case 1: return fn(Line<LK_CLAG, 1>{});
@@ -367,13 +406,22 @@ template <class Fn> int with_line(int kind, int order, Fn &&fn) {
     case 4: return fn(Line<LK_CLAG, 4>{});
     case 5: return fn(Line<LK_CLAG, 5>{});
     }
-    break;
-  case LK_ENO:
+  } else if constexpr (KIND == LK_ENO) {
     switch (order) {
     case 2: return fn(Line<LK_ENO, 2>{});
     case 3: return fn(Line<LK_ENO, 3>{});
     }
-    break;
+  }
+  return -1;
+}
+
+// Calls fn(Line<KIND,O>{}) for a runtime (kind, order); returns -1 if there
+// is no such line operator.
+template <class Fn> int with_line(int kind, int order, Fn &&fn) {
+  switch (kind) {
+  case LK_VLAG: return with_order<LK_VLAG>(order, fn);
+  case LK_CLAG: return with_order<LK_CLAG>(order, fn);
+  case LK_ENO: return with_order<LK_ENO>(order, fn);
   }
   return -1;
 }

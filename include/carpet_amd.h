@@ -320,7 +320,7 @@ int cg_batch_run_plain(cg_batch *b, cg_stream_t stream);
 int cg_batch_capture_graph(cg_batch *b, cg_stream_t stream);
 int cg_batch_launch_graph(cg_batch *b, cg_stream_t stream);
 /* Autotuner: times each launch of the finalized batch under candidate
-   work-item sizes (tensor tiles: z chunk of 4/8/16/32/64/all coarse bases;
+   work-item sizes (tensor tiles: z chunk of 8/16/32/64/128/all coarse bases;
    pointwise items: 2048..65536 points), one workgroup per item, then the
    best size on persistent grids of resident capacity and of 2 workgroups
    per CU, with HIP events, and keeps the fastest per launch class.
