@@ -146,7 +146,10 @@ def run_config4(n_regions: int, n_gf: int, steps: int, warmup: int, plain_steps:
                 tune_cache=None):
     """Batched uneven hierarchy: n_regions regions (copy sync / O=5 prolongation
     with 3 time levels / rf2 restriction, edges uniform in [8,96]) per grid
-    function; batched graph replay vs one launch per transfer."""
+    function; batched graph replay vs one launch per transfer, and the two
+    paths' outputs compared bit for bit on a sample of destinations."""
+    import numpy as np
+
     import carpet_amd as cg
     from carpet_amd.workloads import config4
 
@@ -158,8 +161,9 @@ def run_config4(n_regions: int, n_gf: int, steps: int, warmup: int, plain_steps:
     b.finalize()
     if tune_cache:
         b.autotune(st, reps=3, cache_path=tune_cache)
+    desc = b.describe()
     print(f"# config4: {len(w.steps)} transfers, {len(w.arrays)} arrays, set up in "
-          f"{time.perf_counter() - t0:.1f} s\n" + b.describe(), file=sys.stderr, flush=True)
+          f"{time.perf_counter() - t0:.1f} s\n" + desc, file=sys.stderr, flush=True)
     stats = b.stats()
     b.capture_graph(st)
     for _ in range(warmup):
@@ -172,8 +176,16 @@ def run_config4(n_regions: int, n_gf: int, steps: int, warmup: int, plain_steps:
     e1.record(st)
     st.synchronize()
     ms_b = e0.elapsed_ms(e1) / steps
+    sample = w.outputs[:: max(1, len(w.outputs) // 64)]
+    got_b = {k: dev[k].download() for k in sample}
+    for k in sample:  # back to the initial state (NaN, then the seeded values)
+        dev[k].fill(np.nan)
+        if w.arrays[k].seed is not None:
+            dev[k].fill_hash(w.arrays[k].seed)
+    st.synchronize()
     b.run_plain(st)
     st.synchronize()
+    same = all(np.array_equal(dev[k].download(), got_b[k], equal_nan=True) for k in sample)
     e0.record(st)
     for _ in range(plain_steps):
         b.run_plain(st)
@@ -182,7 +194,138 @@ def run_config4(n_regions: int, n_gf: int, steps: int, warmup: int, plain_steps:
     ms_p = e0.elapsed_ms(e1) / plain_steps
     b.close()
     w.free(dev)
-    return stats, ms_b, ms_p
+    pool = getattr(w, "pool_bytes", None)
+    desc_bytes = int(desc.split("device descriptor bytes ")[1].split()[0]) if "descriptor bytes" in desc else None
+    pts, alg = stats["dst_points"], stats["alg_bytes"]
+    return {"regions": n_regions, "grid_functions": n_gf, "transfers": stats["transfers"],
+            "dst_points_per_step": pts, "algorithmic_bytes_per_step": alg,
+            "resident_array_bytes": pool, "device_descriptor_bytes": desc_bytes,
+            "batched_ms": round(ms_b, 4), "batched_gpoints_per_s": round(pts / ms_b / 1e6, 3),
+            "batched_gbytes_per_s": round(alg / ms_b / 1e6, 1),
+            "batched_roofline_frac": round(alg / ms_b / 1e6 / HBM_PEAK_GBS, 4),
+            "plain_ms": round(ms_p, 4), "plain_gpoints_per_s": round(pts / ms_p / 1e6, 3),
+            "plain_launches_per_step": stats["transfers"], "batched_vs_plain": round(ms_p / ms_b, 3),
+            "batched_equals_plain_on_sample": bool(same), "sample_destinations": len(sample)}
+
+
+def _time_graph(b, st, steps: int, warmup: int) -> float:
+    """ms per replay of b's captured graph (HIP events on st)."""
+    import carpet_amd as cg
+
+    b.capture_graph(st)
+    for _ in range(warmup):
+        b.launch_graph(st)
+    st.synchronize()
+    e0, e1 = cg.Event(), cg.Event()
+    e0.record(st)
+    for _ in range(steps):
+        b.launch_graph(st)
+    e1.record(st)
+    st.synchronize()
+    return e0.elapsed_ms(e1) / steps
+
+
+def run_config3(steps: int, warmup: int, n_var: int):
+    """Cell-centred conservative set: per operator (cc TVD, cc ENO<3>,
+    cc ENO<2>) the batched prolongation of n_var variables 192^3 -> 384^3,
+    then restrict_3d_cc_rf2 back; conservation of every coarse cell."""
+    import numpy as np
+
+    import carpet_amd as cg
+    from carpet_amd.workloads import Config3
+
+    cfg = Config3(n_var=n_var)
+    cbox, owned, fbox, reg = cfg.boxes()
+    st = cg.Stream()
+    coarse, fine, back = cfg.allocate(stream=st)
+    st.synchronize()
+    out = {}
+    for name, op, order_space in (("cc_tvd", cg.OP_TVD, 1), ("cc_eno3", cg.OP_ENO, 5), ("cc_eno2", cg.OP_ENO, 3)):
+        bp = cfg.prolong_batch(coarse, fine, op, order_space)
+        bp.finalize()
+        br = cfg.restrict_batch(fine, back)
+        br.finalize()
+        ms_p = _time_graph(bp, st, steps, warmup)
+        ms_r = _time_graph(br, st, steps, warmup)
+        sp, sr = bp.stats(), br.stats()
+        worst = 0.0
+        for v in range(n_var):
+            o = [(l - dl) // s for l, dl, s in zip(owned.lower, cbox.lower, cbox.stride)]
+            e = owned.sizes()
+            sl = np.s_[o[2]:o[2] + e[2], o[1]:o[1] + e[1], o[0]:o[0] + e[0]]
+            r = back[v].download()[sl]
+            orig = cfg.coarse_data(v)[sl]
+            worst = max(worst, float(np.max(np.abs(r - orig)) / np.max(np.abs(orig))))
+        out[name] = {"prolong_ms": round(ms_p, 4), "prolong_gpoints_per_s": round(sp["dst_points"] / ms_p / 1e6, 3),
+                     "prolong_gbytes_per_s": round(sp["alg_bytes"] / ms_p / 1e6, 1),
+                     "restrict_ms": round(ms_r, 4), "restrict_gbytes_per_s": round(sr["alg_bytes"] / ms_r / 1e6, 1),
+                     "step_ms": round(ms_p + ms_r, 4),
+                     "step_gpoints_per_s": round((sp["dst_points"] + sr["dst_points"]) / (ms_p + ms_r) / 1e6, 3),
+                     "step_gbytes_per_s": round((sp["alg_bytes"] + sr["alg_bytes"]) / (ms_p + ms_r) / 1e6, 1),
+                     "conservation_max_rel": worst}
+        print(f"# config3 {name}: {out[name]}", file=sys.stderr, flush=True)
+        bp.close()
+        br.close()
+    for a in coarse + fine + back:
+        a.free()
+    return out
+
+
+def run_config5(steps: int, warmup: int):
+    """Mixed precision vs batched FP64 on the config-1 and config-3 shapes
+    (vertex order 5 + 3 time levels), for both mixed modes (FP64 arithmetic
+    on FP32 storage; compensated FP32 sums): times, speedups, and the
+    normwise max / RMS relative error of the mixed result against FP64 (a) on
+    the original FP64 sources (storage + arithmetic) and (b) on the
+    FP32-rounded sources (arithmetic and the final rounding only)."""
+    import numpy as np
+
+    import carpet_amd as cg
+    from carpet_amd.workloads import Config5
+
+    out = {}
+    for shape in ("cfg1", "cfg3"):
+        cfg = Config5(shape=shape)
+        _, _, fbox, reg = cfg.boxes()
+        st = cg.Stream()
+        src64, src32, src64r, dst64, dst32 = cfg.allocate(stream=st)
+        st.synchronize()
+        o = [(l - dl) // s for l, dl, s in zip(reg.lower, fbox.lower, fbox.stride)]
+        e = reg.sizes()
+        sl = np.s_[o[2]:o[2] + e[2], o[1]:o[1] + e[1], o[0]:o[0] + e[0]]
+        b64 = cfg.batch(src64, dst64, cg.PREC_F64)
+        b64.finalize()
+        ms64 = _time_graph(b64, st, steps, warmup)
+        s64 = b64.stats()
+        ref = [dst64[v].download()[sl] for v in range(cfg.n_var)]
+        br = cfg.batch(src64r, dst64, cg.PREC_F64)
+        br.finalize()
+        br.run(st)
+        st.synchronize()
+        ref_r = [dst64[v].download()[sl] for v in range(cfg.n_var)]
+        res = {"fine_points": s64["dst_points"], "fp64_ms": round(ms64, 4),
+               "fp64_gbytes_per_s": round(s64["alg_bytes"] / ms64 / 1e6, 1)}
+        for name, prec in (("mixed_fp64_arith", cg.PREC_MIXED), ("mixed_dot2", cg.PREC_MIXED_DOT2)):
+            bm = cfg.batch(src32, dst32, prec)
+            bm.finalize()
+            ms = _time_graph(bm, st, steps, warmup)
+            sm = bm.stats()
+            got = [dst32[v].download()[sl] for v in range(cfg.n_var)]
+            full = [cg.relative_errors(g, r) for g, r in zip(got, ref)]
+            arith = [cg.relative_errors(g, r) for g, r in zip(got, ref_r)]
+            res[name] = {"ms": round(ms, 4), "speedup_vs_fp64": round(ms64 / ms, 3),
+                         "gbytes_per_s": round(sm["alg_bytes"] / ms / 1e6, 1),
+                         "vs_fp64_inputs": {"max_rel": max(x[0] for x in full), "rms_rel": max(x[1] for x in full)},
+                         "vs_fp64_on_fp32_inputs": {"max_rel": max(x[0] for x in arith),
+                                                    "rms_rel": max(x[1] for x in arith)}}
+            bm.close()
+        out[shape] = res
+        print(f"# config5 {shape}: {res}", file=sys.stderr, flush=True)
+        for b in (b64, br):
+            b.close()
+        for a in [x for v in src64 + src32 + src64r for x in v] + dst64 + dst32:
+            a.free()
+    return out
 
 
 def main(argv=None):
@@ -193,9 +336,13 @@ def main(argv=None):
     ap.add_argument("--n-gf", type=int, default=24)
     ap.add_argument("--plain-steps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-config4", action="store_true",
+                    help="skip the 2000-region level step reported beside the headline line")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--config", type=int, default=2, choices=(2, 4),
-                    help="2: single large patch (default, the headline line); 4: batched uneven hierarchy")
+    ap.add_argument("--config", type=int, default=2, choices=(2, 3, 4, 5),
+                    help="2: single large patch (default, the headline line); 3: cell-centred conservative "
+                         "set; 4: batched uneven hierarchy; 5: mixed precision vs FP64")
+    ap.add_argument("--n-var", type=int, default=8, help="config 3 variables")
     ap.add_argument("--n-regions", type=int, default=2000)
     ap.add_argument("--tune-cache", default=None,
                     help="batch tuner cache file: classes it names take its parameters, the others are "
@@ -208,27 +355,61 @@ def main(argv=None):
               "MI355X here (sharding across GPUs is a design note, not a measured path); "
               "run with --gpus 1", file=sys.stderr)
         return 2
-    if args.config == 4:
-        n_gf = args.n_gf
-        stats, ms_b, ms_p = run_config4(args.n_regions, n_gf, args.steps, args.warmup, args.plain_steps,
-                                        args.tune_cache)
-        pts, alg = stats["dst_points"], stats["alg_bytes"]
-        line = {"metric": METRIC, "value": round(pts / (ms_b * 1e-3) / 1e9, 3), "unit": "Gpoints/s",
-                "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_b, 4),
+    if args.config == 3:
+        res = run_config3(args.steps, args.warmup, args.n_var)
+        tvd = res["cc_tvd"]
+        line = {"metric": METRIC, "value": tvd["step_gpoints_per_s"], "unit": "Gpoints/s", "n_gpus": 1,
+                "steps": args.steps, "warmup": args.warmup, "ms_per_step": tvd["step_ms"],
                 "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
                 "data": "synthetic",
-                "config": {"workload": f"batched uneven level step: {args.n_regions} regions x {n_gf} grid "
-                                       "functions (edges U[8,96]; copy sync, O=5 prolongation + 3 time levels, "
-                                       "rf2 restriction), one batched launch per phase + hipGraph",
-                           "transfers": stats["transfers"], "dst_points_per_step": pts,
-                           "algorithmic_bytes_per_step": alg},
-                "gbytes_per_s": round(alg / (ms_b * 1e-3) / 1e9, 1),
-                "plain": {"ms_per_step": round(ms_p, 4), "gpoints_per_s": round(pts / (ms_p * 1e-3) / 1e9, 3),
-                          "launches_per_step": stats["transfers"]},
-                "batched_vs_plain": round(ms_p / ms_b, 3),
-                "roofline": {"bound": "hbm", "achieved": round(alg / (ms_b * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
-                             "unit": "GB/s", "frac": round(alg / (ms_b * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                "config": {"workload": f"cell-centred conservative set: {args.n_var} FP64 variables, coarse 192^3 "
+                                       "(+3 ghosts) -> fine 384^3 by prolongate_3d_cc_tvd / cc_eno, then "
+                                       "restrict_3d_cc_rf2 back; value = TVD prolong + restrict step",
+                           "operators": res},
+                "gbytes_per_s": tvd["step_gbytes_per_s"],
+                "roofline": {"bound": "hbm", "achieved": tvd["step_gbytes_per_s"], "peak": HBM_PEAK_GBS,
+                             "unit": "GB/s", "frac": round(tvd["step_gbytes_per_s"] / HBM_PEAK_GBS, 4),
                              "traffic": None}}
+        print(json.dumps(line))
+        return 0
+    if args.config == 5:
+        res = run_config5(args.steps, args.warmup)
+        c3 = res["cfg3"]
+        mx = c3["mixed_fp64_arith"]
+        pts = c3["fine_points"]
+        line = {"metric": METRIC, "value": round(pts / mx["ms"] / 1e6, 3), "unit": "Gpoints/s",
+                "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": mx["ms"],
+                "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+                "data": "synthetic",
+                "config": {"workload": "mixed precision: vertex prolongate_3d_rf2<5> + interpolate_3d_3tl at "
+                                       "t=1/3, batched; sources and destination stored in FP32, time "
+                                       "interpolation in FP64; stencil sums in FP64 (mixed_fp64_arith, the "
+                                       "value) or compensated FP32 (mixed_dot2); config-1 shape (33^3 -> "
+                                       "64^3, 1 variable) and config-3 shape (193^3 -> 384^3, 8 variables); "
+                                       "value = config-3 shape",
+                           "destination_precision": "fp32", "shapes": res},
+                "speedup_vs_batched_fp64": mx["speedup_vs_fp64"],
+                "roofline": {"bound": "hbm", "achieved": mx["gbytes_per_s"], "peak": HBM_PEAK_GBS,
+                             "unit": "GB/s", "frac": round(mx["gbytes_per_s"] / HBM_PEAK_GBS, 4),
+                             "traffic": None}}
+        print(json.dumps(line))
+        return 0
+    if args.config == 4:
+        r = run_config4(args.n_regions, args.n_gf, args.steps, args.warmup, args.plain_steps,
+                        args.tune_cache)
+        line = {"metric": METRIC, "value": r["batched_gpoints_per_s"], "unit": "Gpoints/s",
+                "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": r["batched_ms"],
+                "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+                "data": "synthetic",
+                "config": {"workload": f"batched uneven level step: {args.n_regions} regions x {args.n_gf} grid "
+                                       "functions (edges U[8,96]; copy sync, O=5 prolongation + 3 time levels, "
+                                       "rf2 restriction), one batched launch per phase + hipGraph", **r},
+                "gbytes_per_s": r["batched_gbytes_per_s"],
+                "plain": {"ms_per_step": r["plain_ms"], "gpoints_per_s": r["plain_gpoints_per_s"],
+                          "launches_per_step": r["plain_launches_per_step"]},
+                "batched_vs_plain": r["batched_vs_plain"],
+                "roofline": {"bound": "hbm", "achieved": r["batched_gbytes_per_s"], "peak": HBM_PEAK_GBS,
+                             "unit": "GB/s", "frac": r["batched_roofline_frac"], "traffic": None}}
         print(json.dumps(line))
         return 0
     if args.calib_copy:
@@ -270,6 +451,10 @@ def main(argv=None):
                      "frac": round(gbs / HBM_PEAK_GBS, 4),
                      "traffic": _profile_traffic("k_grid_tensor_vlag5_3tl_config2")},
     }
+    if not args.no_config4:
+        # the 2000-region level step (batched vs plain) beside the headline
+        print("# config4 beside the headline line", file=sys.stderr, flush=True)
+        line["level_step_2000_regions"] = run_config4(2000, args.n_gf, 5, 2, 1)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     print(json.dumps(line))

@@ -16,6 +16,7 @@ from .core import (  # noqa: F401
     OP_TVD,
     PREC_F64,
     PREC_MIXED,
+    PREC_MIXED_DOT2,
     VERTEX_CENTERED,
     Batch,
     Box,

@@ -21,7 +21,7 @@ from ._lib import GData, I3, I9, Islab, check, lib
 # CarpetLib enums (operators.hh:8-25, defs.hh:134)
 OP_ERROR, OP_NONE, OP_SYNC, OP_RESTRICT, OP_COPY, OP_LAGRANGE, OP_ENO, OP_WENO, OP_TVD = range(9)
 VERTEX_CENTERED, CELL_CENTERED = 1, 2
-PREC_F64, PREC_MIXED = 0, 1
+PREC_F64, PREC_MIXED, PREC_MIXED_DOT2 = 0, 1, 2
 
 
 @dataclass(frozen=True)

@@ -159,7 +159,7 @@ int resolve_transfer(const cg_gdata *dst, const cg_gdata *const *srcs,
 
 int run_transfer(const Transfer &tr, int prec, hipStream_t st) {
   if (tr.is_prolong)
-    return launch_prolong(tr.pr, tr.pr.es == 4 ? 1 : prec, st);
+    return launch_prolong(tr.pr, tr.pr.es == 4 && prec == CG_PREC_F64 ? CG_PREC_MIXED : prec, st);
   const int r = launch_point(tr.pt.op, tr.pt.a, st);
   if (r == -1)
     return set_error(CG_ERR_ARG, "region too large for the pointwise kernel");

@@ -67,12 +67,12 @@ int cg_batch_add(cg_batch *b, int phase, const cg_gdata *dst,
     return set_error(CG_ERR_STATE, "batch already finalized");
   if (phase < 0 || phase > 15)
     return set_error(CG_ERR_ARG, "phase must be in [0,15]");
-  if (prec != CG_PREC_F64 && prec != CG_PREC_MIXED)
+  if (prec != CG_PREC_F64 && prec != CG_PREC_MIXED && prec != CG_PREC_MIXED_DOT2)
     return set_error(CG_ERR_ARG, "unknown precision %d", prec);
   Transfer tr;
   int e = resolve_transfer(dst, srcs, times, ntimes, dstregbbox, srcregbbox, slab,
                            time, transport_operator, centering, order_space,
-                           order_time, tr, prec == CG_PREC_MIXED ? 4 : 8);
+                           order_time, tr, prec == CG_PREC_F64 ? 8 : 4);
   if (e)
     return e;
   b->transfers.push_back(tr);
@@ -173,14 +173,14 @@ extern "C" int cg_batch_autotune(cg_batch *b, cg_stream_t stream, int reps,
     const BatchLaunch l = b->dev.launches[i];
     if (b->tuned.count(l.key))
       continue;
-    // item size first (one workgroup per item), then persistent grids
+    // item size first (in the class's default mode), then the other modes
     std::vector<Tune> cands;
     if (l.is_tensor)
       for (int z : {8, 16, 32, 64, 128, 0})
-        cands.push_back({z, l.pt, -1});
+        cands.push_back({z, l.pt, l.wpc});
     else
       for (int p : {2048, 4096, 16384, 65536})
-        cands.push_back({l.zchunk, p, -1});
+        cands.push_back({l.zchunk, p, l.wpc});
     float best = 1e30f;
     Tune bestc = cands[0];
     auto trial_run = [&](const Tune &c) -> int {
@@ -202,9 +202,10 @@ extern "C" int cg_batch_autotune(cg_batch *b, cg_stream_t stream, int reps,
       if (int e = trial_run(c))
         return e;
     const Tune base = bestc;
-    for (int w : {0, 2})
-      if (int e = trial_run(Tune{base.zchunk, base.pt, w}))
-        return e;
+    for (int w : {-1, 0, 2})
+      if (w != base.wpc)
+        if (int e = trial_run(Tune{base.zchunk, base.pt, w}))
+          return e;
     b->tuned[l.key] = bestc;
   }
   int e = batch_device_build(b->dev, b->transfers, b->phase, b->prec, b->tile_cfg, &b->tuned);
@@ -217,6 +218,11 @@ extern "C" int cg_batch_describe(const cg_batch *b, char *buf, size_t len) {
   if (!b || !buf || !len)
     return set_error(CG_ERR_ARG, "null argument");
   std::ostringstream o;
+  long long items = 0;
+  for (const BatchLaunch &l : b->dev.launches)
+    items += l.nitems;
+  o << "transfers " << b->transfers.size() << " launches " << b->dev.launches.size()
+    << " work items " << items << " device descriptor bytes " << b->dev.blob_bytes << "\n";
   for (const BatchLaunch &l : b->dev.launches)
     o << "phase " << l.phase << " [" << l.key << "] items " << l.nitems << " grid " << l.grid
       << " zchunk " << l.zchunk << " pt " << l.pt << " wpc " << l.wpc << "\n";
@@ -304,7 +310,7 @@ int cg_batch_stats(const cg_batch *b, long long *ntransfers, long long *dst_poin
   long long n = 0, pts = 0, bytes = 0;
   for (size_t i = 0; i < b->transfers.size(); ++i) {
     const Transfer &t = b->transfers[i];
-    const long long es = b->prec[i] == CG_PREC_MIXED ? 4 : 8;
+    const long long es = b->prec[i] == CG_PREC_F64 ? 8 : 4;
     ++n;
     if (t.is_prolong) {
       pts += t.pr.dst_points;

@@ -5,14 +5,15 @@
 // by decreasing size, each transfer's tiles in spatial order -- arranged in
 // 8 segments, one per XCD, every transfer striped over all 8 in slabs of
 // equal cost.  Two launch modes:
-//  * grid mode (default): one workgroup per item; workgroup b runs on XCD
-//    b % 8 (the hardware dispatches workgroups round-robin over the XCDs)
-//    and takes slot b / 8 of that XCD's segment, so neighbouring tiles,
-//    which share stencil halos, meet in the same L2;
-//  * persistent mode: resident-capacity workgroups take items from per-XCD
-//    counters, their own segment first, then the others' (work stealing);
-//    the loop ends for every workgroup once all counters pass their
-//    segment ends, so the grid always drains.
+//  * persistent mode (default for tensor tiles): resident-capacity
+//    workgroups take items from per-XCD counters, their own segment first,
+//    then the others' (work stealing); items are taken in order, so the
+//    tiles in flight on an XCD are neighbours that march through z together
+//    and share halo rows in its L2; the loop ends for every workgroup once
+//    all counters pass their segment ends, so the grid always drains;
+//  * grid mode (default for pointwise runs): one workgroup per item;
+//    workgroup b runs on XCD b % 8 (the hardware dispatches workgroups
+//    round-robin over the XCDs) and takes slot b / 8 of that XCD's segment.
 // Phases are separate launches in stream order (a kernel boundary makes one
 // phase's results visible to the next across all XCD L2s), and the counters
 // are cleared by a small kernel at the start of the step, so the whole step
@@ -36,36 +37,56 @@ constexpr int ZCHUNK_DEFAULT = 16;
 
 constexpr int PU = 8; // points per thread and round of the pointwise kernel
 
-// points [tile*pt, tile*pt + pt) of a region, x fastest
+// points [tile*pt, tile*pt + pt) of a region, x fastest; for TVD the
+// items count the region's coarse parent cells (fx/fxy divide by their
+// extents), each with all its children
 template <int OP>
 __device__ __forceinline__ void point_item(const PointArgs &a, unsigned tile, int pt) {
-  const unsigned nx = (unsigned)a.n[0];
-  const unsigned nxy = nx * (unsigned)a.n[1];
-  const unsigned tot = nxy * (unsigned)a.n[2]; // < 2^31 (checked on the host)
-  const unsigned e0 = tile * (unsigned)pt;
-  const unsigned e1 = min(tot, e0 + (unsigned)pt);
-  // PU points per thread per round: all loads first, then all stores, so
-  // that PU loads per thread are in flight at once
-  for (unsigned eb = e0; eb < e1; eb += 256u * PU) {
-    double v[PU];
-    long long o[PU];
-#pragma unroll
-    for (int u = 0; u < PU; ++u) {
-      const unsigned e = eb + u * 256u + threadIdx.x;
-      o[u] = -1;
-      if (e < e1) {
-        const unsigned k = fastdiv(e, a.fxy);
-        const unsigned r = e - k * nxy;
-        const unsigned j = fastdiv(r, a.fx);
-        const unsigned i = r - j * nx;
-        v[u] = point_value<OP>(a, (int)i, (int)j, (int)k);
-        o[u] = (long long)i + a.dj * j + a.dk * k;
-      }
+  if constexpr (OP == PO_TVD) {
+    int qx0, qy0, qz0, nqx, nqy, nqz;
+    tvd_range(a, 0, &qx0, &nqx);
+    tvd_range(a, 1, &qy0, &nqy);
+    tvd_range(a, 2, &qz0, &nqz);
+    const unsigned nxy = (unsigned)nqx * (unsigned)nqy;
+    const unsigned tot = nxy * (unsigned)nqz; // < 2^31 (checked on the host)
+    const unsigned e0 = tile * (unsigned)pt;
+    const unsigned e1 = min(tot, e0 + (unsigned)pt);
+    for (unsigned e = e0 + threadIdx.x; e < e1; e += 256u) {
+      const unsigned k = fastdiv(e, a.fxy);
+      const unsigned r = e - k * nxy;
+      const unsigned j = fastdiv(r, a.fx);
+      const unsigned i = r - j * (unsigned)nqx;
+      tvd_cell(a, qx0 + (int)i, qy0 + (int)j, qz0 + (int)k);
     }
+  } else {
+    const unsigned nx = (unsigned)a.n[0];
+    const unsigned nxy = nx * (unsigned)a.n[1];
+    const unsigned tot = nxy * (unsigned)a.n[2]; // < 2^31 (checked on the host)
+    const unsigned e0 = tile * (unsigned)pt;
+    const unsigned e1 = min(tot, e0 + (unsigned)pt);
+    // PU points per thread per round: all loads first, then all stores, so
+    // that PU loads per thread are in flight at once
+    for (unsigned eb = e0; eb < e1; eb += 256u * PU) {
+      double v[PU];
+      long long o[PU];
 #pragma unroll
-    for (int u = 0; u < PU; ++u)
-      if (o[u] >= 0)
-        __builtin_nontemporal_store(v[u], a.dst + o[u]);
+      for (int u = 0; u < PU; ++u) {
+        const unsigned e = eb + u * 256u + threadIdx.x;
+        o[u] = -1;
+        if (e < e1) {
+          const unsigned k = fastdiv(e, a.fxy);
+          const unsigned r = e - k * nxy;
+          const unsigned j = fastdiv(r, a.fx);
+          const unsigned i = r - j * nx;
+          v[u] = point_value<OP>(a, (int)i, (int)j, (int)k);
+          o[u] = (long long)i + a.dj * j + a.dk * k;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < PU; ++u)
+        if (o[u] >= 0)
+          __builtin_nontemporal_store(v[u], a.dst + o[u]);
+    }
   }
 }
 
@@ -259,7 +280,10 @@ int batch_device_build(BatchDevice &dev, const std::vector<Transfer> &tr,
     }
     l.zchunk = tile_cfg < 0 ? ZCHUNK_DEFAULT : tile_cfg;
     l.pt = PT_DEFAULT;
-    l.wpc = -1;
+    // tensor tiles: persistent grid (its workgroups take the tiles in order,
+    // so neighbouring tiles march through z together and share halo rows in
+    // L2); pointwise runs: one workgroup per item
+    l.wpc = g.is_tensor ? 0 : -1;
     if (tuned) {
       auto it = tuned->find(l.key);
       if (it != tuned->end()) {
@@ -328,11 +352,15 @@ int batch_device_build(BatchDevice &dev, const std::vector<Transfer> &tr,
       std::vector<PointArgs> descs;
       for (int m : members) {
         PointArgs a = tr[m].is_prolong ? tr[m].pr.p : tr[m].pt.a;
-        const long long tot = (long long)a.n[0] * a.n[1] * a.n[2];
+        // work units: destination points, or TVD's coarse parent cells
+        long long u[3];
+        for (int d = 0; d < 3; ++d)
+          u[d] = g.op == PO_TVD ? ((a.off[d] + a.n[d] - 1) >> 1) - (a.off[d] >> 1) + 1 : a.n[d];
+        const long long tot = u[0] * u[1] * u[2];
         if (tot >= (1LL << 31) - l.pt)
           return set_error(CG_ERR_ARG, "region too large for the batch");
-        a.fx = make_fastdiv((unsigned)a.n[0]);
-        a.fxy = make_fastdiv((unsigned)a.n[0] * (unsigned)a.n[1]);
+        a.fx = make_fastdiv((unsigned)u[0]);
+        a.fxy = make_fastdiv((unsigned)(u[0] * u[1]));
         const unsigned d = (unsigned)descs.size();
         descs.push_back(a);
         first.push_back((int)items.size());

@@ -7,7 +7,7 @@
 
 namespace cg {
 
-template <class L, int NTL, int CX, int CY, int IPT, int XQ, typename TS>
+template <class L, int NTL, int CX, int CY, int IPT, int XQ, int PR>
 __global__ __launch_bounds__(2 * CX * CY / IPT) void k_persist_tensor(
     const TensorArgs *__restrict__ descs, const Item *__restrict__ items,
     const int *__restrict__ seg, int *counter) {
@@ -19,11 +19,11 @@ __global__ __launch_bounds__(2 * CX * CY / IPT) void k_persist_tensor(
       return;
     const Item w = items[it];
     const TensorArgs a = descs[w.desc];
-    tensor_tile<L, NTL, CX, CY, IPT, TS, 1, XQ>(a, (int)w.tile, smem);
+    tensor_tile<L, NTL, CX, CY, IPT, PR, 1, XQ>(a, (int)w.tile, smem);
   }
 }
 
-template <class L, int NTL, int CX, int CY, int IPT, int XQ, typename TS>
+template <class L, int NTL, int CX, int CY, int IPT, int XQ, int PR>
 __global__ __launch_bounds__(2 * CX * CY / IPT) void k_grid_tensor(
     const TensorArgs *__restrict__ descs, const Item *__restrict__ items,
     const int *__restrict__ seg) {
@@ -32,19 +32,19 @@ __global__ __launch_bounds__(2 * CX * CY / IPT) void k_grid_tensor(
   if (it < 0)
     return;
   const Item w = items[it];
-  tensor_tile<L, NTL, CX, CY, IPT, TS, 1, XQ>(descs[w.desc], (int)w.tile, smem);
+  tensor_tile<L, NTL, CX, CY, IPT, PR, 1, XQ>(descs[w.desc], (int)w.tile, smem);
 }
 
-template <class L, int NTL, typename TS>
+template <class L, int NTL, int PR>
 int batch_tensor_go(const BatchLaunch &l, const TensorArgs *td, const Item *items,
                     const int *seg, int *ctr, hipStream_t st) {
   using T = TileShape<NTL>;
   const dim3 block(tile_threads<NTL>());
   if (l.wpc < 0)
-    hipLaunchKernelGGL((k_grid_tensor<L, NTL, T::CX, T::CY, T::IPT, T::XQ, TS>), dim3(l.grid), block, 0,
+    hipLaunchKernelGGL((k_grid_tensor<L, NTL, T::CX, T::CY, T::IPT, T::XQ, PR>), dim3(l.grid), block, 0,
                        st, td, items, seg);
   else
-    hipLaunchKernelGGL((k_persist_tensor<L, NTL, T::CX, T::CY, T::IPT, T::XQ, TS>), dim3(l.grid), block,
+    hipLaunchKernelGGL((k_persist_tensor<L, NTL, T::CX, T::CY, T::IPT, T::XQ, PR>), dim3(l.grid), block,
                        0, st, td, items, seg, ctr);
   return 0;
 }
@@ -54,18 +54,22 @@ int batch_tensor_kind(const BatchLaunch &l, const TensorArgs *td, const Item *it
                       const int *seg, int *ctr, hipStream_t st) {
   return with_order<KIND>(l.order, [&](auto tag) -> int {
     using L = decltype(tag);
-    if (l.prec == 0) {
+    auto by_ntl = [&](auto prc) -> int {
+      constexpr int PR = decltype(prc)::value;
       switch (l.ntl) {
-      case 1: return batch_tensor_go<L, 1, double>(l, td, items, seg, ctr, st);
-      case 2: return batch_tensor_go<L, 2, double>(l, td, items, seg, ctr, st);
-      case 3: return batch_tensor_go<L, 3, double>(l, td, items, seg, ctr, st);
+      case 1: return batch_tensor_go<L, 1, PR>(l, td, items, seg, ctr, st);
+      case 2: return batch_tensor_go<L, 2, PR>(l, td, items, seg, ctr, st);
+      case 3: return batch_tensor_go<L, 3, PR>(l, td, items, seg, ctr, st);
       }
-    } else if constexpr (KIND != LK_ENO) {
-      switch (l.ntl) {
-      case 1: return batch_tensor_go<L, 1, float>(l, td, items, seg, ctr, st);
-      case 2: return batch_tensor_go<L, 2, float>(l, td, items, seg, ctr, st);
-      case 3: return batch_tensor_go<L, 3, float>(l, td, items, seg, ctr, st);
-      }
+      return -1;
+    };
+    if (l.prec == 0)
+      return by_ntl(std::integral_constant<int, 0>{});
+    if constexpr (KIND != LK_ENO) {
+      if (l.prec == 1)
+        return by_ntl(std::integral_constant<int, 1>{});
+      if (l.prec == 2)
+        return by_ntl(std::integral_constant<int, 2>{});
     }
     return -1;
   });
@@ -76,19 +80,27 @@ template <int KIND> const void *batch_tensor_fn_kind(const BatchLaunch &l, int *
   const void *fn = nullptr;
   with_order<KIND>(l.order, [&](auto tag) -> int {
     using L = decltype(tag);
-    auto pick = [&](auto ntlc) {
+    auto pick = [&](auto ntlc, auto prc) {
       constexpr int NTL = decltype(ntlc)::value;
+      constexpr int PR = decltype(prc)::value;
       using T = TileShape<NTL>;
       *threads = tile_threads<NTL>();
+      fn = reinterpret_cast<const void *>(&k_persist_tensor<L, NTL, T::CX, T::CY, T::IPT, T::XQ, PR>);
+    };
+    auto by_prec = [&](auto ntlc) {
       if (l.prec == 0)
-        fn = reinterpret_cast<const void *>(&k_persist_tensor<L, NTL, T::CX, T::CY, T::IPT, T::XQ, double>);
-      else if constexpr (KIND != LK_ENO)
-        fn = reinterpret_cast<const void *>(&k_persist_tensor<L, NTL, T::CX, T::CY, T::IPT, T::XQ, float>);
+        pick(ntlc, std::integral_constant<int, 0>{});
+      if constexpr (KIND != LK_ENO) {
+        if (l.prec == 1)
+          pick(ntlc, std::integral_constant<int, 1>{});
+        if (l.prec == 2)
+          pick(ntlc, std::integral_constant<int, 2>{});
+      }
     };
     switch (l.ntl) {
-    case 1: pick(std::integral_constant<int, 1>{}); break;
-    case 2: pick(std::integral_constant<int, 2>{}); break;
-    case 3: pick(std::integral_constant<int, 3>{}); break;
+    case 1: by_prec(std::integral_constant<int, 1>{}); break;
+    case 2: by_prec(std::integral_constant<int, 2>{}); break;
+    case 3: by_prec(std::integral_constant<int, 3>{}); break;
     }
     return 0;
   });

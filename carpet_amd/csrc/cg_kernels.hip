@@ -18,7 +18,7 @@ long long tensor_tiles(TensorArgs &a, const TileCfg &cfg) {
 
 int launch_tensor(int kind, int order, int ntl, int prec, const TensorArgs &a,
                   const TileCfg &cfg, hipStream_t s) {
-  if (prec != 0 && (prec != 1 || kind == LK_ENO))
+  if (prec < 0 || prec > 2 || (prec != 0 && kind == LK_ENO))
     return -1;
   switch (kind) {
   case LK_VLAG: return launch_tensor_kind<LK_VLAG>(order, ntl, prec, a, cfg, s);
@@ -41,7 +41,35 @@ __global__ __launch_bounds__(256) void k_point(PointArgs a) {
     point_elem<OP>(a, i, j, k);
 }
 
+// TVD: one thread per coarse cell of the region's parents, x-y plane
+// flattened, z strided over blockIdx.y
+__global__ __launch_bounds__(256) void k_tvd(PointArgs a) {
+  int qx0, qy0, qz0, nqx, nqy, nqz;
+  tvd_range(a, 0, &qx0, &nqx);
+  tvd_range(a, 1, &qy0, &nqy);
+  tvd_range(a, 2, &qz0, &nqz);
+  const unsigned e = blockIdx.x * 256u + threadIdx.x;
+  if (e >= (unsigned)nqx * (unsigned)nqy)
+    return;
+  const int cy = (int)(e / (unsigned)nqx);
+  const int cx = (int)(e - (unsigned)cy * (unsigned)nqx);
+  for (int cz = blockIdx.y; cz < nqz; cz += gridDim.y)
+    tvd_cell(a, qx0 + cx, qy0 + cy, qz0 + cz);
+}
+
 int launch_point(int op, const PointArgs &a, hipStream_t s) {
+  if (op == PO_TVD) {
+    // coarse parents per axis: ((off + n - 1) >> 1) - (off >> 1) + 1
+    long long nq[3];
+    for (int d = 0; d < 3; ++d)
+      nq[d] = ((a.off[d] + a.n[d] - 1) >> 1) - (a.off[d] >> 1) + 1;
+    const long long nxy = nq[0] * nq[1];
+    if (nxy <= 0 || nq[2] <= 0 || nxy >= (1LL << 31))
+      return -1;
+    dim3 grid((unsigned)((nxy + 255) / 256), (unsigned)(nq[2] < 65535 ? nq[2] : 65535));
+    hipLaunchKernelGGL(k_tvd, grid, dim3(256), 0, s, a);
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+  }
   const long long nxy = (long long)a.n[0] * a.n[1];
   if (nxy <= 0 || a.n[2] <= 0 || nxy >= (1LL << 32) / 2)
     return -1;
@@ -53,7 +81,6 @@ int launch_point(int op, const PointArgs &a, hipStream_t s) {
   case PO_RESTRICT_VC: hipLaunchKernelGGL(k_point<PO_RESTRICT_VC>, grid, block, 0, s, a); break;
   case PO_RESTRICT_CC: hipLaunchKernelGGL(k_point<PO_RESTRICT_CC>, grid, block, 0, s, a); break;
   case PO_INTERP: hipLaunchKernelGGL(k_point<PO_INTERP>, grid, block, 0, s, a); break;
-  case PO_TVD: hipLaunchKernelGGL(k_point<PO_TVD>, grid, block, 0, s, a); break;
   default: return -1;
   }
   return hipGetLastError() == hipSuccess ? 0 : -2;

@@ -9,13 +9,13 @@
 
 namespace cg {
 
-template <class L, int NTL, int CX, int CY, int IPT, int XQ, typename TS>
+template <class L, int NTL, int CX, int CY, int IPT, int XQ, int PR>
 __global__ __launch_bounds__(2 * CX * CY / IPT) void k_tensor(TensorArgs a) {
   __shared__ __align__(16) double smem[TensorLds<L, NTL, CX, CY, XQ>::TOTAL];
-  tensor_tile<L, NTL, CX, CY, IPT, TS, 1, XQ>(a, xcd_remap(blockIdx.x, gridDim.x), smem);
+  tensor_tile<L, NTL, CX, CY, IPT, PR, 1, XQ>(a, xcd_remap(blockIdx.x, gridDim.x), smem);
 }
 
-template <class L, int NTL, typename TS>
+template <class L, int NTL, int PR>
 int launch_tensor_t(const TensorArgs &a0, const TileCfg &cfg0, hipStream_t s) {
   using T = TileShape<NTL>;
   TensorArgs a = a0;
@@ -25,29 +25,34 @@ int launch_tensor_t(const TensorArgs &a0, const TileCfg &cfg0, hipStream_t s) {
   const long long nblk = tensor_tiles(a, cfg);
   if (nblk <= 0 || nblk > 0x7fffffffLL)
     return -1;
-  hipLaunchKernelGGL((k_tensor<L, NTL, T::CX, T::CY, T::IPT, T::XQ, TS>), dim3((unsigned)nblk),
+  hipLaunchKernelGGL((k_tensor<L, NTL, T::CX, T::CY, T::IPT, T::XQ, PR>), dim3((unsigned)nblk),
                      dim3(tile_threads<NTL>()), 0, s, a);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
-// prec 0: FP64; prec 1: FP32 storage with compensated sums (Lagrange only)
+// prec: precision mode (Prec in cg_tiles.hpp); the FP32-storage modes
+// (1, 2) exist for the Lagrange kinds only
 template <int KIND>
 int launch_tensor_kind(int order, int ntl, int prec, const TensorArgs &a, const TileCfg &cfg,
                        hipStream_t s) {
   return with_order<KIND>(order, [&](auto tag) -> int {
     using L = decltype(tag);
-    if (prec == 0) {
+    auto by_ntl = [&](auto prc) -> int {
+      constexpr int PR = decltype(prc)::value;
       switch (ntl) {
-      case 1: return launch_tensor_t<L, 1, double>(a, cfg, s);
-      case 2: return launch_tensor_t<L, 2, double>(a, cfg, s);
-      case 3: return launch_tensor_t<L, 3, double>(a, cfg, s);
+      case 1: return launch_tensor_t<L, 1, PR>(a, cfg, s);
+      case 2: return launch_tensor_t<L, 2, PR>(a, cfg, s);
+      case 3: return launch_tensor_t<L, 3, PR>(a, cfg, s);
       }
-    } else if constexpr (KIND != LK_ENO) {
-      switch (ntl) {
-      case 1: return launch_tensor_t<L, 1, float>(a, cfg, s);
-      case 2: return launch_tensor_t<L, 2, float>(a, cfg, s);
-      case 3: return launch_tensor_t<L, 3, float>(a, cfg, s);
-      }
+      return -1;
+    };
+    if (prec == 0)
+      return by_ntl(std::integral_constant<int, 0>{});
+    if constexpr (KIND != LK_ENO) {
+      if (prec == 1)
+        return by_ntl(std::integral_constant<int, 1>{});
+      if (prec == 2)
+        return by_ntl(std::integral_constant<int, 2>{});
     }
     return -1;
   });

@@ -70,12 +70,21 @@ template <typename TS> struct Pair;
 template <> struct Pair<double> { using T = double2; };
 template <> struct Pair<float> { using T = float2; };
 
-// One z-marching tile; tile b indexes (x tile, y tile, z chunk) of a.
-// line operator in the storage precision: FP64 (the drop-in) or FP32 with
-// compensated accumulation (mixed)
-template <class L, int P, typename TS, class V> __device__ __forceinline__ TS line_apply(V v) {
-  if constexpr (sizeof(TS) == 8)
-    return L::template apply<P>(v);
+// Precision modes: storage type S of the arrays, arithmetic type C of the
+// passes.  0: FP64 (the drop-in).  1: FP32 storage, FP64 arithmetic -- the
+// product of a dyadic weight (at most 9 significant bits) and an FP32 value
+// is exact in FP64 and the sums carry 29 guard bits, an error-free-product
+// accumulation.  2: FP32 storage, compensated FP32 sums (Dot2 with FMA
+// TwoProduct and TwoSum).  Time interpolation is FP64 in every mode.
+template <int PR> struct Prec;
+template <> struct Prec<0> { using S = double; using C = double; };
+template <> struct Prec<1> { using S = float; using C = double; };
+template <> struct Prec<2> { using S = float; using C = float; };
+
+// line operator in the arithmetic type C (values converted from storage)
+template <class L, int P, typename C, class V> __device__ __forceinline__ C line_apply(V v) {
+  if constexpr (sizeof(C) == 8)
+    return L::template apply<P>([&](int t) { return (double)v(t); });
   else
     return L::template apply_f<P>(v);
 }
@@ -91,12 +100,14 @@ template <int I, int N, class F> __device__ __forceinline__ bool unroll_while(F 
   return true;
 }
 
-// PF: planes loaded ahead of their use (register buffers per plane)
-// XQ: coarse bases per thread in the x-pass (1, or 2 with paired LDS
-// accesses); smem must be 16-byte aligned
-template <class L, int NTL, int CX, int CY, int IPT, typename TS = double, int PF = 1,
-          int XQ = 1>
+// One z-marching tile; tile b indexes (x tile, y tile, z chunk) of a.
+// PR: precision mode (Prec); PF: planes loaded ahead of their use (register
+// buffers per plane); XQ: coarse bases per thread in the x-pass (1, or 2 with
+// paired LDS accesses); smem must be 16-byte aligned
+template <class L, int NTL, int CX, int CY, int IPT, int PR = 0, int PF = 1, int XQ = 1>
 __device__ __forceinline__ void tensor_tile(const TensorArgs &a, int b, double *smem) {
+  using TS = typename Prec<PR>::S; // storage
+  using CT = typename Prec<PR>::C; // arithmetic of the passes
   constexpr int FX = 2 * CX;
   constexpr int NT = FX * CY / IPT;
   constexpr int NQY = NT / FX;
@@ -114,7 +125,8 @@ __device__ __forceinline__ void tensor_tile(const TensorArgs &a, int b, double *
   static_assert(U % PF == 0 && U % R == 0, "unroll must cover ring and prefetch slots");
 
   TS(*A)[WY][WXS] = reinterpret_cast<TS(*)[WY][WXS]>(smem);
-  TS(*X)[WY][FX] = reinterpret_cast<TS(*)[WY][FX]>(reinterpret_cast<TS *>(smem) + LD::A);
+  constexpr int XOFF = (LD::A * (int)sizeof(TS) + 15) / 16 * 16; // bytes, 16-aligned
+  CT(*X)[WY][FX] = reinterpret_cast<CT(*)[WY][FX]>(reinterpret_cast<char *>(smem) + XOFF);
 
   const int tx = b % a.ntiles[0];
   b /= a.ntiles[0];
@@ -163,14 +175,14 @@ __device__ __forceinline__ void tensor_tile(const TensorArgs &a, int b, double *
   };
 
   // ring of y-pass results: slot j holds the planes c with (c - c0) % R == j
-  TS ring[NTL][R][IPT][2];
+  CT ring[NTL][R][IPT][2];
 #pragma unroll
   for (int tl = 0; tl < NTL; ++tl)
 #pragma unroll
     for (int i = 0; i < R; ++i)
 #pragma unroll
       for (int m = 0; m < IPT; ++m)
-        ring[tl][i][m][0] = ring[tl][i][m][1] = TS(0);
+        ring[tl][i][m][0] = ring[tl][i][m][1] = CT(0);
 
   auto prologue = [&](auto Sc) -> bool {
     constexpr int S = decltype(Sc)::value;
@@ -204,8 +216,8 @@ __device__ __forceinline__ void tensor_tile(const TensorArgs &a, int b, double *
 #pragma unroll
         for (int tl = 0; tl < NTL; ++tl) {
           const TS *w = &A[tl][r][q - L::TMINALL];
-          const TS v0 = line_apply<L, 0, TS>([&](int t) { return w[L::TMIN[0] + t]; });
-          const TS v1 = line_apply<L, 1, TS>([&](int t) { return w[L::TMIN[1] + t]; });
+          const CT v0 = line_apply<L, 0, CT>([&](int t) { return w[L::TMIN[0] + t]; });
+          const CT v1 = line_apply<L, 1, CT>([&](int t) { return w[L::TMIN[1] + t]; });
           X[tl][r][2 * q] = v0;
           X[tl][r][2 * q + 1] = v1;
         }
@@ -214,6 +226,7 @@ __device__ __forceinline__ void tensor_tile(const TensorArgs &a, int b, double *
       // bases q0, q0+1 from the window columns q0 .. q0+SPAN+1 (window
       // column 0 is offset TMINALL of base 0), read and written in pairs
       using P2 = typename Pair<TS>::T;
+      using C2 = typename Pair<CT>::T;
       constexpr int NP = (LD::SPAN + 3) / 2;
       for (int e = tid; e < WY * (CX / 2); e += NT) {
         const int r = e / (CX / 2), q0 = 2 * (e - r * (CX / 2));
@@ -227,13 +240,13 @@ __device__ __forceinline__ void tensor_tile(const TensorArgs &a, int b, double *
             v[2 * m + 1] = p.y;
           }
           constexpr int O0 = L::TMIN[0] - L::TMINALL, O1 = L::TMIN[1] - L::TMINALL;
-          P2 lo, hi;
-          lo.x = line_apply<L, 0, TS>([&](int t) { return v[O0 + t]; });
-          lo.y = line_apply<L, 1, TS>([&](int t) { return v[O1 + t]; });
-          hi.x = line_apply<L, 0, TS>([&](int t) { return v[1 + O0 + t]; });
-          hi.y = line_apply<L, 1, TS>([&](int t) { return v[1 + O1 + t]; });
-          *reinterpret_cast<P2 *>(&X[tl][r][2 * q0]) = lo;
-          *reinterpret_cast<P2 *>(&X[tl][r][2 * q0 + 2]) = hi;
+          C2 lo, hi;
+          lo.x = line_apply<L, 0, CT>([&](int t) { return v[O0 + t]; });
+          lo.y = line_apply<L, 1, CT>([&](int t) { return v[O1 + t]; });
+          hi.x = line_apply<L, 0, CT>([&](int t) { return v[1 + O0 + t]; });
+          hi.y = line_apply<L, 1, CT>([&](int t) { return v[1 + O1 + t]; });
+          *reinterpret_cast<C2 *>(&X[tl][r][2 * q0]) = lo;
+          *reinterpret_cast<C2 *>(&X[tl][r][2 * q0 + 2]) = hi;
         }
       }
     }
@@ -244,9 +257,9 @@ __device__ __forceinline__ void tensor_tile(const TensorArgs &a, int b, double *
       const int qy = qyl + NQY * m;
 #pragma unroll
       for (int tl = 0; tl < NTL; ++tl) {
-        const TS *col = &X[tl][qy - L::TMINALL][fx];
-        ring[tl][J][m][0] = line_apply<L, 0, TS>([&](int t) { return col[(L::TMIN[0] + t) * FX]; });
-        ring[tl][J][m][1] = line_apply<L, 1, TS>([&](int t) { return col[(L::TMIN[1] + t) * FX]; });
+        const CT *col = &X[tl][qy - L::TMINALL][fx];
+        ring[tl][J][m][0] = line_apply<L, 0, CT>([&](int t) { return col[(L::TMIN[0] + t) * FX]; });
+        ring[tl][J][m][1] = line_apply<L, 1, CT>([&](int t) { return col[(L::TMIN[1] + t) * FX]; });
       }
     }
     // 4. z-emission of the fine planes whose last coarse plane is c; their
@@ -269,7 +282,7 @@ __device__ __forceinline__ void tensor_tile(const TensorArgs &a, int b, double *
           double v[NTL];
 #pragma unroll
           for (int tl = 0; tl < NTL; ++tl)
-            v[tl] = (double)line_apply<L, PZ, TS>(
+            v[tl] = (double)line_apply<L, PZ, CT>(
                 [&](int t) { return ring[tl][(J - NTP + 1 + t + 2 * R) % R][m][py]; });
           // time interpolation in FP64 in both precisions
           double out;
@@ -306,6 +319,7 @@ __device__ __forceinline__ double minmod(double a, double b) {
 // so that a caller can issue the loads of several points before storing)
 template <int OP>
 __device__ __forceinline__ double point_value(const PointArgs &a, int i, int j, int k) {
+  static_assert(OP != PO_TVD, "TVD runs per coarse cell (tvd_cell)");
   double out;
   if constexpr (OP == PO_COPY) {
     out = a.src[0][i + a.sj * j + a.sk * k];
@@ -332,19 +346,6 @@ __device__ __forceinline__ double point_value(const PointArgs &a, int i, int j, 
     if (a.ntl == 3)
       r = fmad(a.s[2], a.src[2][o], r);
     out = r;
-  } else { // PO_TVD, prolongate_3d_cc_real8_tvd.F90:183-225
-    const int gi = a.off[0] + i, gj = a.off[1] + j, gk = a.off[2] + k;
-    const int i0 = gi >> 1, j0 = gj >> 1, k0 = gk >> 1;
-    const double sx0 = double(2 * (gi & 1) - 1) * (1.0 / 4);
-    const double sy0 = double(2 * (gj & 1) - 1) * (1.0 / 4);
-    const double sz0 = double(2 * (gk & 1) - 1) * (1.0 / 4);
-    const double *p = a.src[0] + i0 + a.sj * j0 + a.sk * k0;
-    const long long J = a.sj, K = a.sk;
-    const double c = p[0];
-    const double sx = sx0 * minmod(c - p[-1], p[1] - c);
-    const double sy = sy0 * minmod(c - p[-J], p[J] - c);
-    const double sz = sz0 * minmod(c - p[-K], p[K] - c);
-    out = c + sx + sy + sz;
   }
   return out;
 }
@@ -352,6 +353,48 @@ __device__ __forceinline__ double point_value(const PointArgs &a, int i, int j, 
 template <int OP>
 __device__ __forceinline__ void point_elem(const PointArgs &a, int i, int j, int k) {
   a.dst[i + a.dj * j + a.dk * k] = point_value<OP>(a, i, j, k);
+}
+
+// Cell-centred TVD prolongation (prolongate_3d_cc_real8_tvd.F90:183-225)
+// of all children of one coarse cell.  Fine point f of axis d has parent
+// (off+f)>>1 and side (off+f)&1, so coarse cell q owns the children
+// f = 2q - off + s, s = 0, 1, those inside [0, n).  Per axis the slope is
+// minmod(c - c[-1], c[+1] - c), signed by the side and scaled by 1/4 (both
+// exact); every child is ((c + sx) + sy) + sz, the reference's order.  The
+// seven loads serve up to eight children.
+__device__ __forceinline__ void tvd_range(const PointArgs &a, int d, int *q0, int *nq) {
+  *q0 = a.off[d] >> 1;
+  *nq = ((a.off[d] + a.n[d] - 1) >> 1) - *q0 + 1;
+}
+
+__device__ __forceinline__ void tvd_cell(const PointArgs &a, int qx, int qy, int qz) {
+  const long long J = a.sj, K = a.sk;
+  const double *p = a.src[0] + qx + J * qy + K * qz;
+  const double c = p[0];
+  const double mx = minmod(c - p[-1], p[1] - c);
+  const double my = minmod(c - p[-J], p[J] - c);
+  const double mz = minmod(c - p[-K], p[K] - c);
+  const double q = 1.0 / 4;
+  const double sx[2] = {-q * mx, q * mx}, sy[2] = {-q * my, q * my}, sz[2] = {-q * mz, q * mz};
+  const int fx0 = 2 * qx - a.off[0], fy0 = 2 * qy - a.off[1], fz0 = 2 * qz - a.off[2];
+#pragma unroll
+  for (int z = 0; z < 2; ++z) {
+    const int fz = fz0 + z;
+    if (fz < 0 || fz >= a.n[2])
+      continue;
+#pragma unroll
+    for (int y = 0; y < 2; ++y) {
+      const int fy = fy0 + y;
+      if (fy < 0 || fy >= a.n[1])
+        continue;
+#pragma unroll
+      for (int x = 0; x < 2; ++x) {
+        const int fx = fx0 + x;
+        if (fx >= 0 && fx < a.n[0])
+          __builtin_nontemporal_store(c + sx[x] + sy[y] + sz[z], &a.dst[fx + a.dj * fy + a.dk * fz]);
+      }
+    }
+  }
 }
 
 // tile shapes: one z-marching column of 2CX x 2CY fine points, 2CX*CY/IPT

@@ -16,7 +16,7 @@ from typing import Dict, List
 import numpy as np
 
 from .core import (CELL_CENTERED, OP_ENO, OP_LAGRANGE, OP_RESTRICT, OP_SYNC, OP_TVD, PREC_F64,
-                   VERTEX_CENTERED, Batch, Box, DeviceArray, pad_shape, transfer_from)
+                   PREC_MIXED, VERTEX_CENTERED, Batch, Box, DeviceArray, pad_shape, transfer_from)
 from ._lib import check, lib
 
 GOLDEN = 0x9E3779B97F4A7C15
@@ -100,6 +100,7 @@ class Workload:
             base = C.c_void_p()
             check(lib.cg_dev_malloc(C.byref(base), total), "cg_dev_malloc")
             self._pool = base.value
+            self.pool_bytes = total
             for k, spec, pad, start in layout:
                 dev[k] = DeviceArray.view(spec.box, self._pool + start, pad)
         else:
@@ -386,3 +387,115 @@ class Config3:
         a = np.sin(2 * np.pi * k * x + 0.3 * var)
         b = np.cos(2 * np.pi * (k + 1) * x - 0.1 * var)
         return 1.0 + 0.2 * a[:, None, None] * b[None, :, None] * a[None, None, :]
+
+
+    def allocate(self, stream=None):
+        """Device arrays: coarse sources (uploaded), fine destinations and
+        restriction targets (NaN), per variable."""
+        cbox, owned, fbox, reg = self.boxes()
+        coarse, fine, back = [], [], []
+        for var in range(self.n_var):
+            c = DeviceArray(cbox, owned=owned)
+            c.upload(self.coarse_data(var))
+            coarse.append(c)
+            f = DeviceArray(fbox, owned=reg)
+            f.fill(np.nan, stream)
+            fine.append(f)
+            r = DeviceArray(cbox, owned=owned)
+            r.fill(np.nan, stream)
+            back.append(r)
+        return coarse, fine, back
+
+    def prolong_batch(self, coarse, fine, op: int, order_space: int) -> Batch:
+        _, _, _, reg = self.boxes()
+        b = Batch()
+        for c, f in zip(coarse, fine):
+            b.add(0, f, [c], [0.0], reg, reg, 0.0, op, CELL_CENTERED, order_space, 0)
+        return b
+
+    def restrict_batch(self, fine, back) -> Batch:
+        _, owned, _, _ = self.boxes()
+        b = Batch()
+        for f, r in zip(fine, back):
+            b.add(0, r, [f], [0.0], owned, owned, 0.0, OP_TVD, CELL_CENTERED, 1, 0)
+        return b
+
+
+@dataclass
+class Config5:
+    """Mixed precision on the config-1 and config-3 shapes: n_var vertex-
+    centred grid functions on a coarse n^3 box (+3 ghosts, stride 2), 3 time
+    levels at t = 1, 0, -1, prolongated by prolongate_3d_rf2<5> fused with
+    interpolate_3d_3tl at t = 1/3 onto the fine region [2, 2n-1]^3 (bases
+    1..n-1, order-5 taps -2..+3 inside the ghosts).  Shape "cfg1": n = 33
+    (fine 64^3), 1 variable; shape "cfg3": n = 193 (fine 384^3), 8 variables.
+    Data: sums of 8 Fourier modes as in the single-patch configuration.  The
+    same values feed the FP64 path (float64 arrays) and the mixed path
+    (float32 storage of sources and destination; compensated FP32 sums; FP64
+    time interpolation)."""
+
+    shape: str = "cfg3"
+    order: int = 5
+    target: float = 1.0 / 3.0
+    times: tuple = (1.0, 0.0, -1.0)
+    ghosts: int = 3
+
+    @property
+    def n(self) -> int:
+        return 33 if self.shape == "cfg1" else 193
+
+    @property
+    def n_var(self) -> int:
+        return 1 if self.shape == "cfg1" else 8
+
+    def boxes(self):
+        g, n = self.ghosts, self.n
+        c = Box((-2 * g,) * 3, (2 * (n - 1 + g),) * 3, (2, 2, 2))
+        owned = Box((0,) * 3, (2 * (n - 1),) * 3, (2, 2, 2))
+        reg = Box((2,) * 3, (2 * n - 1,) * 3, (1, 1, 1))
+        fine = reg.expand((g, g, g), (g, g, g))
+        return c, owned, fine, reg
+
+    def allocate(self, stream=None):
+        """Per precision: sources [var][tl] and destinations [var].  The FP64
+        sources hold the FP32-rounded values when rounded=True (isolates the
+        arithmetic), else the original FP64 values."""
+        cbox, owned, fbox, reg = self.boxes()
+        src64, src32, src64r, dst64, dst32 = [], [], [], [], []
+        for var in range(self.n_var):
+            rng = np.random.default_rng(5000 + var)
+            kvec = rng.integers(1, 9, size=(8, 3))
+            phase = rng.uniform(0, 2 * np.pi, size=8)
+            s64, s32, s64r = [], [], []
+            for t in self.times:
+                a = DeviceArray(cbox, owned=owned)
+                ph = phase + 0.1 * t * np.linalg.norm(kvec, axis=1)
+                a.fill_fourier(1.0 / (2 * (self.n - 1)), 1.0 / 8, kvec.tolist(), ph.tolist(), stream)
+                if stream is not None:
+                    stream.synchronize()
+                h = a.download()
+                b32 = DeviceArray(cbox, owned=owned, dtype=np.float32)
+                b32.upload(h.astype(np.float32))
+                b64 = DeviceArray(cbox, owned=owned)
+                b64.upload(h.astype(np.float32).astype(np.float64))
+                s64.append(a)
+                s32.append(b32)
+                s64r.append(b64)
+            src64.append(s64)
+            src32.append(s32)
+            src64r.append(s64r)
+            d64 = DeviceArray(fbox, owned=reg)
+            d64.fill(np.nan, stream)
+            dst64.append(d64)
+            d32 = DeviceArray(fbox, owned=reg, dtype=np.float32)
+            d32.upload(np.full(d32.padded_shape_zyx(), np.nan, np.float32))
+            dst32.append(d32)
+        return src64, src32, src64r, dst64, dst32
+
+    def batch(self, srcs, dsts, prec) -> Batch:
+        _, _, _, reg = self.boxes()
+        b = Batch()
+        for ss, d in zip(srcs, dsts):
+            b.add(0, d, ss, list(self.times), reg, reg, self.target, OP_LAGRANGE, VERTEX_CENTERED,
+                  self.order, 2, prec=prec)
+        return b

@@ -82,10 +82,14 @@ enum {
    (CarpetLib/src/defs.hh:134) */
 enum { CG_ERROR_CENTERED = 0, CG_VERTEX_CENTERED = 1, CG_CELL_CENTERED = 2 };
 
-/* precision of prolongation sources/destinations in the batch and fused
-   paths: CG_PREC_F64 is the drop-in; CG_PREC_MIXED stores prolongation
-   sources and destinations in FP32 with compensated FP32 accumulation */
-enum { CG_PREC_F64 = 0, CG_PREC_MIXED = 1 };
+/* precision of Lagrange prolongation in the batch and mixed paths:
+   CG_PREC_F64 is the drop-in.  The mixed modes store prolongation sources
+   and destinations in FP32 and keep the time interpolation in FP64:
+   CG_PREC_MIXED accumulates each stencil in FP64 (the product of a dyadic
+   weight and an FP32 value is exact in FP64, so the sum is an error-free-
+   product accumulation with 29 guard bits); CG_PREC_MIXED_DOT2 uses
+   compensated FP32 sums (Dot2: FMA TwoProduct + TwoSum per tap) */
+enum { CG_PREC_F64 = 0, CG_PREC_MIXED = 1, CG_PREC_MIXED_DOT2 = 2 };
 
 /* Slab description for copies: dst[ipos] = src[ipos + offset*stride].
    Layout of CarpetLib's slab<int,3> (CarpetLib/src/gdata.hh:25-31). */
@@ -300,8 +304,8 @@ typedef struct cg_batch cg_batch;
 int cg_batch_create(cg_batch **out);
 int cg_batch_destroy(cg_batch *b);
 /* same arguments as cg_transfer_from plus the phase; returns the transfer
-   index (>= 0) or an error code.  prec selects CG_PREC_F64 or
-   CG_PREC_MIXED (mixed only for Lagrange prolongation; dst/src storage
+   index (>= 0) or an error code.  prec selects CG_PREC_F64, CG_PREC_MIXED
+   or CG_PREC_MIXED_DOT2 (mixed only for Lagrange prolongation; dst/src storage
    then holds float). */
 int cg_batch_add(cg_batch *b, int phase, const cg_gdata *dst,
                  const cg_gdata *const *srcs, const double *times, int ntimes,
@@ -321,9 +325,11 @@ int cg_batch_capture_graph(cg_batch *b, cg_stream_t stream);
 int cg_batch_launch_graph(cg_batch *b, cg_stream_t stream);
 /* Autotuner: times each launch of the finalized batch under candidate
    work-item sizes (tensor tiles: z chunk of 8/16/32/64/128/all coarse bases;
-   pointwise items: 2048..65536 points), one workgroup per item, then the
-   best size on persistent grids of resident capacity and of 2 workgroups
-   per CU, with HIP events, and keeps the fastest per launch class.
+   pointwise items: 2048..65536 points) in the class's default mode
+   (persistent grid for tensor tiles, one workgroup per item for pointwise
+   runs), then the best size in the other modes (one workgroup per item,
+   persistent grids of resident capacity and of 2 workgroups per CU), with
+   HIP events, and keeps the fastest per launch class.
    Choices are keyed on (kernel class, order, time levels, precision, log2
    of the median region size, log2 of the region count) and kept in the
    text file cache_path (may be NULL; lines "<key>|<zchunk> <points>
@@ -334,8 +340,10 @@ int cg_batch_launch_graph(cg_batch *b, cg_stream_t stream);
    results are kept, or on scratch data.  Drops a captured graph. */
 int cg_batch_autotune(cg_batch *b, cg_stream_t stream, int reps,
                       const char *cache_path);
-/* human-readable summary of the launches (phase, class key, work items,
-   grid, tile choice, grid mode); returns the full length */
+/* human-readable summary: a totals line (transfers, launches, work items,
+   bytes of the device descriptor blob), then one line per launch (phase,
+   class key, work items, grid, tile choice, grid mode); returns the full
+   length */
 int cg_batch_describe(const cg_batch *b, char *buf, size_t len);
 /* statistics: number of transfers, destination points and algorithmic
    bytes (destination + unique source footprint per time level) */

@@ -1,5 +1,5 @@
 """Batched level step on the GPU: batched launches over the descriptor list
-(one workgroup per item, and persistent grids with per-XCD work counters),
+(persistent grids with per-XCD work counters, and one workgroup per item),
 autotuned launches, the plain per-region launches and hipGraph replays all
 equal the CPU oracle's replay of the same transfers, bit for bit."""
 import re
@@ -55,10 +55,11 @@ def test_batch_modes_match_oracle(make, tmp_path):
     st.synchronize()
     _compare(_outputs(w, dev), ref, "batched")
     if w.name != "config1":
-        # persistent grids (a tuner cache naming every class with wpc 0 and
-        # 2), then the tuner's own search; each time from poisoned outputs
+        # every launch mode (a tuner cache naming every class with one
+        # workgroup per item, wpc -1, and persistent grids, wpc 0 and 2),
+        # then the tuner's own search; each time from poisoned outputs
         keys = re.findall(r"\[([^\]]+)\]", b.describe())
-        for wpc, zc, pt in ((0, 8, 1024), (2, 0, 65536)):
+        for wpc, zc, pt in ((-1, 16, 4096), (0, 8, 1024), (2, 0, 65536)):
             cache = tmp_path / f"tune_{wpc}.txt"
             cache.write_text("".join(f"{k}|{zc} {pt} {wpc}\n" for k in keys))
             b.autotune(st, reps=1, cache_path=str(cache))

@@ -1,8 +1,8 @@
-"""Mixed precision (FP32 storage, compensated FP32 sums, FP64 time
-interpolation): the error against the FP64 oracle on the same (FP32-
-representable) inputs stays below 1e-6 normwise and is reported per
+"""Mixed precision (FP32 storage; FP64 arithmetic, or compensated FP32 sums;
+FP64 time interpolation): the error against the FP64 oracle on the same
+(FP32-representable) inputs stays below 1e-6 normwise and is reported per
 operator; inputs are rounded to FP32 first so the comparison isolates the
-arithmetic and the one final rounding."""
+arithmetic and the final rounding."""
 import numpy as np
 import pytest
 
@@ -70,6 +70,46 @@ def test_mixed_transfer_with_time_interpolation(kind, order):
     m = ~np.isnan(exp.view())
     emax, erms = cg.relative_errors(got[m], exp.view()[m])
     print(f"{kind} o{order} + 3tl: max rel {emax:.3e} rms rel {erms:.3e}", flush=True)
+    assert emax <= TOL_MAX
+
+
+@pytest.mark.parametrize("kind,order", [("vlag", 5), ("vlag", 3), ("clag", 5), ("clag", 3)])
+def test_mixed_batch_modes(kind, order):
+    """Both mixed modes through the batch: CG_PREC_MIXED (FP64 arithmetic on
+    FP32 storage) equals the FP64 oracle on the same FP32-representable
+    inputs rounded once to FP32, bit for bit; CG_PREC_MIXED_DOT2 (compensated
+    FP32 sums) stays within 1e-6 normwise."""
+    rng = np.random.default_rng(11 + order)
+    sb, db, rb = prolong_case(rng, kind, order, n=(40, 36, 34))
+    pad = sizes(sb)
+    srcs = []
+    for _ in range(3):
+        h = host_array(sb, pad)
+        h.view()[...] = field(rng, sizes(sb), "smooth").astype(np.float32).astype(np.float64)
+        srcs.append(h)
+    times = [1.0, 0.0, -1.0]
+    exp = host_array(db, sizes(db))
+    okind = orc.KIND_VLAG if kind == "vlag" else orc.KIND_CLAG
+    assert orc.prolongate_time(okind, order, srcs, times, exp, 1 / 3, rb, 2) == 0
+    m = ~np.isnan(exp.view())
+    ds = [_dev32(h) for h in srcs]
+    cent = cg.VERTEX_CENTERED if kind == "vlag" else cg.CELL_CENTERED
+    out = {}
+    for prec in (cg.PREC_MIXED, cg.PREC_MIXED_DOT2):
+        dd = cg.DeviceArray(cg.Box(*db), dtype=np.float32)
+        dd.upload(np.full(exp.view().shape, np.nan, np.float32))
+        b = cg.Batch()
+        b.add(0, dd, ds, times, cg.Box(*rb), cg.Box(*rb), 1 / 3, cg.OP_LAGRANGE, cent, order, 2, prec=prec)
+        b.finalize()
+        b.run()
+        cg.synchronize()
+        out[prec] = dd.download()
+        assert np.array_equal(m, ~np.isnan(out[prec])), "mixed kernel wrote outside the region"
+        b.close()
+    exp32 = exp.view()[m].astype(np.float32)
+    assert np.array_equal(out[cg.PREC_MIXED][m], exp32), "FP64-arithmetic mixed mode is not the rounded oracle"
+    emax, erms = cg.relative_errors(out[cg.PREC_MIXED_DOT2][m].astype(np.float64), exp.view()[m])
+    print(f"{kind} o{order} + 3tl Dot2: max rel {emax:.3e} rms rel {erms:.3e}", flush=True)
     assert emax <= TOL_MAX
 
 
