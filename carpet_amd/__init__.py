@@ -26,6 +26,7 @@ from .core import (  # noqa: F401
     copy_3d,
     interpolate_3d_2tl,
     interpolate_3d_3tl,
+    interpolate_eno_3d_3tl,
     pad_shape,
     prolongate_3d_cc_eno_rf2,
     prolongate_3d_cc_rf2,

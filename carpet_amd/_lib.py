@@ -78,6 +78,7 @@ _decl("cg_restrict_3d_rf2_f64", i, *_OPARGS, vp)
 _decl("cg_restrict_3d_cc_rf2_f64", i, *_OPARGS, vp)
 _decl("cg_interpolate_3d_2tl_f64", i, vp, dp, vp, dp, ip, ip, vp, dp, ip, ip, ip, ip, ip, ip, vp)
 _decl("cg_interpolate_3d_3tl_f64", i, vp, dp, vp, dp, vp, dp, ip, ip, vp, dp, ip, ip, ip, ip, ip, ip, vp)
+_decl("cg_interpolate_eno_3d_3tl_f64", i, vp, dp, vp, dp, vp, dp, ip, ip, vp, dp, ip, ip, ip, ip, ip, ip, vp)
 _GP = C.POINTER(GData)
 _decl("cg_transfer_from", i, _GP, C.POINTER(_GP), C.POINTER(C.c_double), i, ip, ip,
       C.POINTER(Islab), dp, i, i, i, i, vp)

@@ -325,6 +325,16 @@ def interpolate_3d_3tl(src1: DeviceArray, t1: float, src2: DeviceArray, t2: floa
           "interpolate_3d_3tl")
 
 
+def interpolate_eno_3d_3tl(src1: DeviceArray, t1: float, src2: DeviceArray, t2: float,
+                           src3: DeviceArray, t3: float, dst: DeviceArray, t: float,
+                           dstregbbox: Box, stream=None):
+    check(lib.cg_interpolate_eno_3d_3tl_f64(src1.ptr, t1, src2.ptr, t2, src3.ptr, t3, I3(*src1.padext),
+                                            I3(*src1.ext), dst.ptr, t, I3(*dst.padext), I3(*dst.ext),
+                                            src1.box.as_c(), dst.box.as_c(), dstregbbox.as_c(),
+                                            dstregbbox.as_c(), _sh(stream)),
+          "interpolate_eno_3d_3tl")
+
+
 def _gptrs(srcs: Sequence[DeviceArray]):
     arr = (C.POINTER(GData) * len(srcs))(*[C.pointer(s.g) for s in srcs])
     return arr

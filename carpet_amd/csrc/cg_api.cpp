@@ -75,10 +75,11 @@ int resolve_transfer(const cg_gdata *dst, const cg_gdata *const *srcs,
       return set_error(CG_ERR_OPERATOR, "slabbed copy with time interpolation");
     if (ntl > 3)
       return set_error(CG_ERR_ORDER, "time interpolation order %d not supported", order_time);
-    if (ntl == 3 && op != CG_OP_LAGRANGE && op != CG_OP_COPY)
-      return set_error(CG_ERR_ORDER, "interpolate_eno_3d_3tl is not provided");
+    // ENO and TVD groups interpolate 3 levels with interpolate_eno_3d_3tl
+    // (data.cc:1262-1292)
     tr.is_prolong = false;
-    return plan_interp(ntl, sp, times, time, s0, d, dreg, tr.pt);
+    return plan_interp(ntl, sp, times, time, s0, d, dreg, tr.pt,
+                       op == CG_OP_ENO || op == CG_OP_TVD);
   }
   if (coarser) {
     if (op == CG_OP_SYNC || op == CG_OP_RESTRICT)
@@ -124,18 +125,26 @@ int resolve_transfer(const cg_gdata *dst, const cg_gdata *const *srcs,
       return set_error(CG_ERR_OPERATOR, "prolongation operator %d is not provided", op);
     }
     double sf[3] = {1.0, 0.0, 0.0};
+    EnoTime ew;
+    const bool eno_time = ntl == 3 && (op == CG_OP_ENO || op == CG_OP_TVD);
     if (ntl > 1) {
       if (ntl > 3)
         return set_error(CG_ERR_ORDER, "time interpolation order %d not supported", order_time);
-      if (ntl == 3 && (op == CG_OP_ENO || op == CG_OP_TVD))
-        return set_error(CG_ERR_ORDER, "interpolate_eno_3d_3tl is not provided");
-      if ((e = time_factors(ntl, times, time, sf)))
+      // each level prolongated, then combined in time (data.cc:497-521):
+      // ENO/TVD groups with 3 levels by interpolate_eno_3d_3tl, the others
+      // by interpolate_3d_2tl/3tl (data.cc:1182-1299)
+      if (eno_time) {
+        if ((e = eno_time_factors(times, time, ew)))
+          return e;
+        for (int l = 0; l < 3; ++l)
+          sf[l] = ew.s3[l];
+      } else if ((e = time_factors(ntl, times, time, sf))) {
         return e;
-      if (kind == PK_TVD)
-        return set_error(CG_ERR_ORDER, "TVD with time interpolation is not provided");
+      }
     }
     tr.is_prolong = true;
-    return plan_prolong(kind, order, s0, sp, ntl, sf, d, dreg, tr.pr, es);
+    return plan_prolong(kind, order, s0, sp, ntl, sf, d, dreg, tr.pr, es,
+                        eno_time ? &ew : nullptr);
   }
   if (finer) {
     if (op == CG_OP_SYNC)
@@ -557,7 +566,7 @@ static int interp_call(int ntl, const double *const *srcs, const double *times,
                        const int *srcpadext, const int *srcext, double *dst,
                        double t, const int *dstpadext, const int *dstext,
                        const int *srcbbox, const int *dstbbox,
-                       const int *dstregbbox, cg_stream_t stream) {
+                       const int *dstregbbox, cg_stream_t stream, bool eno = false) {
   if (!srcpadext || !srcext || !dstpadext || !dstext || !srcbbox || !dstbbox || !dstregbbox)
     return set_error(CG_ERR_ARG, "null argument");
   Transfer tr;
@@ -565,7 +574,7 @@ static int interp_call(int ntl, const double *const *srcs, const double *times,
   const void *sp[3] = {srcs[0], srcs[1], ntl > 2 ? srcs[2] : nullptr};
   int e = plan_interp(ntl, sp, times, t, arr_from(srcs[0], srcpadext, srcext, srcbbox),
                       arr_from(dst, dstpadext, dstext, dstbbox),
-                      Box::from(dstregbbox), tr.pt);
+                      Box::from(dstregbbox), tr.pt, eno);
   if (e)
     return e;
   return run_transfer(tr, 0, S(stream));
@@ -596,6 +605,20 @@ int cg_interpolate_3d_3tl_f64(const double *src1, double t1, const double *src2,
   const double tt[3] = {t1, t2, t3};
   return interp_call(3, srcs, tt, srcpadext, srcext, dst, t, dstpadext, dstext,
                      srcbbox, dstbbox, dstregbbox, stream);
+}
+
+int cg_interpolate_eno_3d_3tl_f64(const double *src1, double t1, const double *src2,
+                                  double t2, const double *src3, double t3,
+                                  const int srcpadext[3], const int srcext[3],
+                                  double *dst, double t, const int dstpadext[3],
+                                  const int dstext[3], const int srcbbox[9],
+                                  const int dstbbox[9], const int srcregbbox[9],
+                                  const int dstregbbox[9], cg_stream_t stream) {
+  (void)srcregbbox;
+  const double *srcs[3] = {src1, src2, src3};
+  const double tt[3] = {t1, t2, t3};
+  return interp_call(3, srcs, tt, srcpadext, srcext, dst, t, dstpadext, dstext,
+                     srcbbox, dstbbox, dstregbbox, stream, true);
 }
 
 static int prolong_mixed_call(int kind, int order, const float *src, const int *srcpadext,

@@ -37,27 +37,58 @@ constexpr int ZCHUNK_DEFAULT = 16;
 
 constexpr int PU = 8; // points per thread and round of the pointwise kernel
 
+// coarse cells [tile*pt, tile*pt + pt) of a TVD region (fx/fxy divide by
+// the parent-cell extents), each with all its children, NTL time levels
+template <int NTL>
+__device__ __forceinline__ void tvd_item(const PointArgs &a, unsigned tile, int pt) {
+  int qx0, qy0, qz0, nqx, nqy, nqz;
+  tvd_range(a, 0, &qx0, &nqx);
+  tvd_range(a, 1, &qy0, &nqy);
+  tvd_range(a, 2, &qz0, &nqz);
+  const unsigned nxy = (unsigned)nqx * (unsigned)nqy;
+  const unsigned tot = nxy * (unsigned)nqz; // < 2^31 (checked on the host)
+  const unsigned e0 = tile * (unsigned)pt;
+  const unsigned e1 = min(tot, e0 + (unsigned)pt);
+  // TU cells per thread per round: every cell's loads before any store
+  constexpr int TU = NTL == 1 ? 4 : 2;
+  for (unsigned eb = e0; eb < e1; eb += 256u * TU) {
+    TvdCell v[TU][NTL];
+    int q[TU][3];
+#pragma unroll
+    for (int u = 0; u < TU; ++u) {
+      const unsigned e = eb + u * 256u + threadIdx.x;
+      q[u][0] = -1;
+      if (e < e1) {
+        const unsigned k = fastdiv(e, a.fxy);
+        const unsigned r = e - k * nxy;
+        const unsigned j = fastdiv(r, a.fx);
+        const unsigned i = r - j * (unsigned)nqx;
+        q[u][0] = qx0 + (int)i;
+        q[u][1] = qy0 + (int)j;
+        q[u][2] = qz0 + (int)k;
+#pragma unroll
+        for (int tl = 0; tl < NTL; ++tl)
+          v[u][tl] = tvd_load(a, tl, q[u][0], q[u][1], q[u][2]);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < TU; ++u)
+      if (q[u][0] >= 0)
+        tvd_store<NTL>(a, q[u][0], q[u][1], q[u][2], v[u]);
+  }
+}
+
 // points [tile*pt, tile*pt + pt) of a region, x fastest; for TVD the
-// items count the region's coarse parent cells (fx/fxy divide by their
-// extents), each with all its children
+// items count the region's coarse parent cells
 template <int OP>
 __device__ __forceinline__ void point_item(const PointArgs &a, unsigned tile, int pt) {
   if constexpr (OP == PO_TVD) {
-    int qx0, qy0, qz0, nqx, nqy, nqz;
-    tvd_range(a, 0, &qx0, &nqx);
-    tvd_range(a, 1, &qy0, &nqy);
-    tvd_range(a, 2, &qz0, &nqz);
-    const unsigned nxy = (unsigned)nqx * (unsigned)nqy;
-    const unsigned tot = nxy * (unsigned)nqz; // < 2^31 (checked on the host)
-    const unsigned e0 = tile * (unsigned)pt;
-    const unsigned e1 = min(tot, e0 + (unsigned)pt);
-    for (unsigned e = e0 + threadIdx.x; e < e1; e += 256u) {
-      const unsigned k = fastdiv(e, a.fxy);
-      const unsigned r = e - k * nxy;
-      const unsigned j = fastdiv(r, a.fx);
-      const unsigned i = r - j * (unsigned)nqx;
-      tvd_cell(a, qx0 + (int)i, qy0 + (int)j, qz0 + (int)k);
-    }
+    if (a.ntl == 1)
+      tvd_item<1>(a, tile, pt);
+    else if (a.ntl == 2)
+      tvd_item<2>(a, tile, pt);
+    else
+      tvd_item<3>(a, tile, pt);
   } else {
     const unsigned nx = (unsigned)a.n[0];
     const unsigned nxy = nx * (unsigned)a.n[1];

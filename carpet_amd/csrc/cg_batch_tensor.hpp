@@ -11,7 +11,7 @@ template <class L, int NTL, int CX, int CY, int IPT, int XQ, int PR>
 __global__ __launch_bounds__(2 * CX * CY / IPT) void k_persist_tensor(
     const TensorArgs *__restrict__ descs, const Item *__restrict__ items,
     const int *__restrict__ seg, int *counter) {
-  __shared__ __align__(16) double smem[TensorLds<L, NTL, CX, CY, XQ>::TOTAL];
+  __shared__ __align__(16) double smem[tile_lds<L, NTL, CX, CY, XQ>()];
   __shared__ int s_item;
   for (;;) {
     const int it = dequeue(counter, seg, &s_item);
@@ -19,7 +19,7 @@ __global__ __launch_bounds__(2 * CX * CY / IPT) void k_persist_tensor(
       return;
     const Item w = items[it];
     const TensorArgs a = descs[w.desc];
-    tensor_tile<L, NTL, CX, CY, IPT, PR, 1, XQ>(a, (int)w.tile, smem);
+    tensor_tile<L, NTL, CX, CY, IPT, PR, 1, XQ, pipe_fits<L, NTL, CX, CY, XQ>()>(a, (int)w.tile, smem);
   }
 }
 
@@ -27,12 +27,12 @@ template <class L, int NTL, int CX, int CY, int IPT, int XQ, int PR>
 __global__ __launch_bounds__(2 * CX * CY / IPT) void k_grid_tensor(
     const TensorArgs *__restrict__ descs, const Item *__restrict__ items,
     const int *__restrict__ seg) {
-  __shared__ __align__(16) double smem[TensorLds<L, NTL, CX, CY, XQ>::TOTAL];
+  __shared__ __align__(16) double smem[tile_lds<L, NTL, CX, CY, XQ>()];
   const int it = grid_item(seg);
   if (it < 0)
     return;
   const Item w = items[it];
-  tensor_tile<L, NTL, CX, CY, IPT, PR, 1, XQ>(descs[w.desc], (int)w.tile, smem);
+  tensor_tile<L, NTL, CX, CY, IPT, PR, 1, XQ, pipe_fits<L, NTL, CX, CY, XQ>()>(descs[w.desc], (int)w.tile, smem);
 }
 
 template <class L, int NTL, int PR>

@@ -80,9 +80,19 @@ struct ProlongPlan {
 // validates and builds a prolongation plan (srcs[0..ntl-1] share src's
 // layout); s[] are the time weights
 // es: element size of the arrays (8 = FP64 drop-in, 4 = mixed precision)
+// interpolate_eno_3d_3tl's weights (interpolate_eno_3d_3tl.cc:104-137): the
+// 3-level weights, the fallback pair's weights and its first level
+struct EnoTime {
+  double s3[3];
+  double s2[2];
+  int l0;
+};
+int eno_time_factors(const double *tt, double t, EnoTime &w);
+
+// eno: the time levels combine with interpolate_eno_3d_3tl's rule (3 levels)
 int plan_prolong(int kind, int order, const Arr &src, const void *const *srcs,
                  int ntl, const double *s, const Arr &dst, const Box &reg,
-                 ProlongPlan &plan, int es = 8);
+                 ProlongPlan &plan, int es = 8, const EnoTime *eno = nullptr);
 int launch_prolong(const ProlongPlan &plan, int prec, hipStream_t st);
 
 struct PointPlan {
@@ -97,7 +107,7 @@ int plan_restrict(bool cell, const Arr &src, const Arr &dst, const Box &reg,
                   PointPlan &plan);
 int plan_interp(int ntl, const void *const *srcs, const double *times,
                 double t, const Arr &src, const Arr &dst, const Box &reg,
-                PointPlan &plan);
+                PointPlan &plan, bool eno = false);
 
 // time weights of interpolate_3d_2tl/3tl (interpolate_3d_2tl.cc:97-107,
 // interpolate_3d_3tl.cc:98-115)

@@ -43,6 +43,7 @@ __global__ __launch_bounds__(256) void k_point(PointArgs a) {
 
 // TVD: one thread per coarse cell of the region's parents, x-y plane
 // flattened, z strided over blockIdx.y
+template <int NTL>
 __global__ __launch_bounds__(256) void k_tvd(PointArgs a) {
   int qx0, qy0, qz0, nqx, nqy, nqz;
   tvd_range(a, 0, &qx0, &nqx);
@@ -54,7 +55,7 @@ __global__ __launch_bounds__(256) void k_tvd(PointArgs a) {
   const int cy = (int)(e / (unsigned)nqx);
   const int cx = (int)(e - (unsigned)cy * (unsigned)nqx);
   for (int cz = blockIdx.y; cz < nqz; cz += gridDim.y)
-    tvd_cell(a, qx0 + cx, qy0 + cy, qz0 + cz);
+    tvd_cell<NTL>(a, qx0 + cx, qy0 + cy, qz0 + cz);
 }
 
 int launch_point(int op, const PointArgs &a, hipStream_t s) {
@@ -67,7 +68,12 @@ int launch_point(int op, const PointArgs &a, hipStream_t s) {
     if (nxy <= 0 || nq[2] <= 0 || nxy >= (1LL << 31))
       return -1;
     dim3 grid((unsigned)((nxy + 255) / 256), (unsigned)(nq[2] < 65535 ? nq[2] : 65535));
-    hipLaunchKernelGGL(k_tvd, grid, dim3(256), 0, s, a);
+    switch (a.ntl) {
+    case 1: hipLaunchKernelGGL(k_tvd<1>, grid, dim3(256), 0, s, a); break;
+    case 2: hipLaunchKernelGGL(k_tvd<2>, grid, dim3(256), 0, s, a); break;
+    case 3: hipLaunchKernelGGL(k_tvd<3>, grid, dim3(256), 0, s, a); break;
+    default: return -1;
+    }
     return hipGetLastError() == hipSuccess ? 0 : -2;
   }
   const long long nxy = (long long)a.n[0] * a.n[1];

@@ -24,6 +24,9 @@ struct TensorArgs {
   int ntiles[3];      // tile counts: x, y (in CX/CY bases), z (chunks)
   int zchunk;         // coarse bases per z chunk
   double s[3];        // time weights
+  double s2[2];       // ENO time rule: weights of the fallback pair
+  int l0;             //   first level of the fallback pair (0: 1-2, 1: 2-3)
+  int teno;           //   1: interpolate_eno_3d_3tl's rule for 3 levels
 };
 
 // Division by a fixed d >= 1 for dividends n < 2^31 as a multiply and a
@@ -58,8 +61,10 @@ struct PointArgs {
   double *dst;          // destination element at the region origin
   long long dj, dk;
   int n[3];             // region extents (destination points)
-  int ntl;              // time levels (interp)
-  double s[3];          // time weights (interp)
+  int ntl;              // time levels (interp, TVD)
+  double s[3];          // time weights (interp, TVD)
+  double s2[2];         // ENO time rule: fallback pair weights
+  int l0, teno;         //   first level of the pair; 1 = rule on (3 levels)
   int off[3];           // TVD: fine offsets (off+f)/2 = parent, (off+f)%2 = side
   FastDiv fx, fxy;      // division by n[0] and n[0]*n[1] (batch work items)
 };

@@ -107,7 +107,7 @@ static LineInfo line_info_rt(int kind, int order, bool &ok) {
 // prolongate_3d_cc_real8_tvd.F90:94-179 (cell TVD)
 int plan_prolong(int kind, int order, const Arr &src, const void *const *srcs,
                  int ntl, const double *s, const Arr &dst, const Box &reg,
-                 ProlongPlan &pl, int es) {
+                 ProlongPlan &pl, int es, const EnoTime *eno) {
   if (es != 8 && !(es == 4 && (kind == PK_VLAG || kind == PK_CLAG)))
     return set_error(CG_ERR_OPERATOR, "mixed precision applies to Lagrange prolongation only");
   int e;
@@ -173,10 +173,18 @@ int plan_prolong(int kind, int order, const Arr &src, const void *const *srcs,
       p.n[d] = n[d];
       p.off[d] = off[d];
     }
-    p.ntl = 1;
-    if (ntl != 1)
-      return set_error(CG_ERR_ORDER, "TVD prolongation is fused with time "
-                                     "interpolation only through the dispatch");
+    // time levels: each prolongated, then combined (data.cc:497-521)
+    p.ntl = ntl;
+    for (int l = 0; l < ntl; ++l) {
+      p.src[l] = static_cast<const double *>(srcs[l]);
+      p.s[l] = s[l];
+    }
+    if (eno && ntl == 3) {
+      p.teno = 1;
+      p.l0 = eno->l0;
+      p.s2[0] = eno->s2[0];
+      p.s2[1] = eno->s2[1];
+    }
     pl.dst_points = (long long)n[0] * n[1] * n[2];
     pl.src_points = srcpts;
     return CG_OK;
@@ -256,6 +264,12 @@ int plan_prolong(int kind, int order, const Arr &src, const void *const *srcs,
   t.dk = dst.sk();
   for (int l = 0; l < 3; ++l)
     t.s[l] = l < ntl ? s[l] : 0.0;
+  if (eno && ntl == 3) {
+    t.teno = 1;
+    t.l0 = eno->l0;
+    t.s2[0] = eno->s2[0];
+    t.s2[1] = eno->s2[1];
+  }
   pl.dst_points = (long long)t.n[0] * t.n[1] * t.n[2];
   pl.src_points = srcpts;
   pl.es = es;
@@ -423,8 +437,36 @@ int time_factors(int ntl, const double *tt, double t, double *s) {
 }
 
 // interpolate_3d_2tl.cc:40-107, interpolate_3d_3tl.cc:41-115
+int eno_time_factors(const double *tt, double t, EnoTime &w) {
+  const double t1 = tt[0], t2 = tt[1], t3 = tt[2];
+  const double tmin = std::fmin(std::fmin(t1, t2), t3);
+  const double tmax = std::fmax(std::fmax(t1, t2), t3);
+  const double eps = 1.0e-12 * (tmax - tmin);
+  if (std::fabs(t1 - t2) < eps || std::fabs(t1 - t3) < eps || std::fabs(t2 - t3) < eps)
+    return set_error(CG_ERR_TIME_SAME, "Internal error: arrays have same time");
+  if (t < tmin - eps || t > tmax + eps)
+    return set_error(CG_ERR_TIME_EXTRAP, "Internal error: extrapolation in time");
+  w.s3[0] = (t - t2) * (t - t3) / ((t1 - t2) * (t1 - t3));
+  w.s3[1] = (t - t1) * (t - t3) / ((t2 - t1) * (t2 - t3));
+  w.s3[2] = (t - t1) * (t - t2) / ((t3 - t1) * (t3 - t2));
+  const bool use_12 = t >= std::min(t1, t2) - eps && t <= std::max(t1, t2) + eps;
+  const bool use_23 = t >= std::min(t2, t3) - eps && t <= std::max(t2, t3) + eps;
+  if (!use_12 && !use_23)
+    return set_error(CG_ERR_TIME_EXTRAP, "Internal error: no time interval holds t");
+  if (use_12) {
+    w.l0 = 0;
+    w.s2[0] = (t - t2) / (t1 - t2);
+    w.s2[1] = (t - t1) / (t2 - t1);
+  } else {
+    w.l0 = 1;
+    w.s2[0] = (t - t3) / (t2 - t3);
+    w.s2[1] = (t - t2) / (t3 - t2);
+  }
+  return CG_OK;
+}
+
 int plan_interp(int ntl, const void *const *srcs, const double *times, double t,
-                const Arr &src, const Arr &dst, const Box &reg, PointPlan &pl) {
+                const Arr &src, const Arr &dst, const Box &reg, PointPlan &pl, bool eno) {
   int e;
   if ((e = check_array(src, "source")) || (e = check_array(dst, "destination")))
     return e;
@@ -442,8 +484,19 @@ int plan_interp(int ntl, const void *const *srcs, const double *times, double t,
   if (!reg.contained_in(sb) || !reg.contained_in(db))
     return set_error(CG_ERR_CONTAIN, "Internal error: region extent is not contained in array extent");
   std::memset(&pl, 0, sizeof pl);
-  if ((e = time_factors(ntl, times, t, pl.a.s)))
+  if (eno && ntl == 3) {
+    EnoTime w;
+    if ((e = eno_time_factors(times, t, w)))
+      return e;
+    for (int l = 0; l < 3; ++l)
+      pl.a.s[l] = w.s3[l];
+    pl.a.s2[0] = w.s2[0];
+    pl.a.s2[1] = w.s2[1];
+    pl.a.l0 = w.l0;
+    pl.a.teno = 1;
+  } else if ((e = time_factors(ntl, times, t, pl.a.s))) {
     return e;
+  }
   if (ntl < 2)
     return set_error(CG_ERR_ORDER, "time interpolation needs 2 or 3 time levels");
   int so[3], dof[3];

@@ -11,8 +11,8 @@ namespace cg {
 
 template <class L, int NTL, int CX, int CY, int IPT, int XQ, int PR>
 __global__ __launch_bounds__(2 * CX * CY / IPT) void k_tensor(TensorArgs a) {
-  __shared__ __align__(16) double smem[TensorLds<L, NTL, CX, CY, XQ>::TOTAL];
-  tensor_tile<L, NTL, CX, CY, IPT, PR, 1, XQ>(a, xcd_remap(blockIdx.x, gridDim.x), smem);
+  __shared__ __align__(16) double smem[tile_lds<L, NTL, CX, CY, XQ>()];
+  tensor_tile<L, NTL, CX, CY, IPT, PR, 1, XQ, pipe_fits<L, NTL, CX, CY, XQ>()>(a, xcd_remap(blockIdx.x, gridDim.x), smem);
 }
 
 template <class L, int NTL, int PR>

@@ -55,16 +55,39 @@ template <typename TS> __device__ __forceinline__ double ld(const void *p, long 
 }
 
 // LDS footprint (doubles) of one tensor tile; XQ = bases per thread in the
-// x-pass (2: rows padded to an even stride for 16-byte LDS accesses)
-template <class L, int NTL, int CX, int CY, int XQ = 1> struct TensorLds {
+// x-pass (2: rows padded to an even stride for 16-byte LDS accesses); NB =
+// buffers of the window and of the x-pass results (2 for the pipelined
+// schedule)
+template <class L, int NTL, int CX, int CY, int XQ = 1, int NB = 1> struct TensorLds {
   static constexpr int SPAN = L::TMAXALL - L::TMINALL;
   static constexpr int WX = CX + SPAN;
   static constexpr int WXS = XQ == 1 ? WX : (WX + 1) / 2 * 2;
   static constexpr int WY = CY + SPAN;
-  static constexpr int A = NTL * WY * WXS;
-  static constexpr int X = NTL * WY * 2 * CX;
-  static constexpr int TOTAL = A + X;
+  static constexpr int A = NTL * WY * WXS; // one window buffer
+  static constexpr int X = NTL * WY * 2 * CX; // one x-pass buffer
+  static constexpr int TOTAL = NB * (A + X);  // NB (1 or 2) buffers of each
 };
+
+// Combination of one point's values on ntl time levels (canonical order:
+// s0*v0, then fma(s1, v1, .), fma(s2, v2, .)); with the ENO rule
+// (interpolate_eno_3d_3tl.cc:146-159) a 3-level result outside the range of
+// the three values falls back to the pair l0, l0+1.
+template <int NTL>
+__device__ __forceinline__ double time_combine(const double *v, const double *s, const double *s2,
+                                               int l0, int teno) {
+  if constexpr (NTL == 1) {
+    return v[0];
+  } else if constexpr (NTL == 2) {
+    return fmad(s[1], v[1], s[0] * v[0]);
+  } else {
+    double d = fmad(s[2], v[2], fmad(s[1], v[1], s[0] * v[0]));
+    if (teno && (d > fmax(fmax(v[0], v[1]), v[2]) || d < fmin(fmin(v[0], v[1]), v[2]))) {
+      const double u = l0 ? v[1] : v[0], w = l0 ? v[2] : v[1];
+      d = fmad(s2[1], w, s2[0] * u);
+    }
+    return d;
+  }
+}
 
 template <typename TS> struct Pair;
 template <> struct Pair<double> { using T = double2; };
@@ -103,8 +126,13 @@ template <int I, int N, class F> __device__ __forceinline__ bool unroll_while(F 
 // One z-marching tile; tile b indexes (x tile, y tile, z chunk) of a.
 // PR: precision mode (Prec); PF: planes loaded ahead of their use (register
 // buffers per plane); XQ: coarse bases per thread in the x-pass (1, or 2 with
-// paired LDS accesses); smem must be 16-byte aligned
-template <class L, int NTL, int CX, int CY, int IPT, int PR = 0, int PF = 1, int XQ = 1>
+// paired LDS accesses); PIPE: schedule of the plane loop (0: window store,
+// barrier, x-pass, barrier, y-pass per plane; 1: software pipeline -- the
+// window of plane c, the x-pass of plane c-1 and the y-pass of plane c-2
+// between two barriers, with double-buffered window and x-pass results);
+// smem must be 16-byte aligned and hold TensorLds<..., XQ, PIPE ? 2 : 1>
+template <class L, int NTL, int CX, int CY, int IPT, int PR = 0, int PF = 1, int XQ = 1,
+          int PIPE = 0>
 __device__ __forceinline__ void tensor_tile(const TensorArgs &a, int b, double *smem) {
   using TS = typename Prec<PR>::S; // storage
   using CT = typename Prec<PR>::C; // arithmetic of the passes
@@ -113,20 +141,29 @@ __device__ __forceinline__ void tensor_tile(const TensorArgs &a, int b, double *
   constexpr int NQY = NT / FX;
   static_assert(NQY * IPT == CY, "tile rows must divide among threads");
   static_assert(XQ == 1 || (XQ == 2 && CX % 2 == 0), "x-pass bases per thread: 1 or 2");
-  using LD = TensorLds<L, NTL, CX, CY, XQ>;
+  constexpr int NB = PIPE ? 2 : 1;
+  static_assert(!PIPE || PF == 1, "the pipelined schedule loads one plane ahead");
+  using LD = TensorLds<L, NTL, CX, CY, XQ, NB>;
   constexpr int WX = LD::WX;
   constexpr int WXS = LD::WXS; // LDS row stride of the window
   constexpr int WY = LD::WY;
   constexpr int R = L::R;
   constexpr int NLD = (WY * WXS + NT - 1) / NT; // window elements per thread
-  // planes per unrolled round: ring slot and prefetch slot are both
-  // compile-time constants within a round
-  constexpr int U = (R % PF == 0) ? R : R * PF;
-  static_assert(U % PF == 0 && U % R == 0, "unroll must cover ring and prefetch slots");
+  // planes per unrolled round: ring slot, prefetch slot and buffer parity
+  // are all compile-time constants within a round
+  constexpr int U0 = (R % PF == 0) ? R : R * PF;
+  constexpr int U = (PIPE && U0 % 2) ? 2 * U0 : U0;
+  static_assert(U % PF == 0 && U % R == 0 && U % NB == 0, "unroll must cover the slots");
 
-  TS(*A)[WY][WXS] = reinterpret_cast<TS(*)[WY][WXS]>(smem);
-  constexpr int XOFF = (LD::A * (int)sizeof(TS) + 15) / 16 * 16; // bytes, 16-aligned
-  CT(*X)[WY][FX] = reinterpret_cast<CT(*)[WY][FX]>(reinterpret_cast<char *>(smem) + XOFF);
+  // buffer B of the window (storage type) and of the x-pass (arithmetic type)
+  constexpr int ABYTES = (LD::A * (int)sizeof(TS) + 15) / 16 * 16;
+  constexpr int XBYTES = (LD::X * (int)sizeof(CT) + 15) / 16 * 16;
+  auto Abuf = [&](int B) {
+    return reinterpret_cast<TS(*)[WY][WXS]>(reinterpret_cast<char *>(smem) + B * ABYTES);
+  };
+  auto Xbuf = [&](int B) {
+    return reinterpret_cast<CT(*)[WY][FX]>(reinterpret_cast<char *>(smem) + NB * ABYTES + B * XBYTES);
+  };
 
   const int tx = b % a.ntiles[0];
   b /= a.ntiles[0];
@@ -191,13 +228,11 @@ __device__ __forceinline__ void tensor_tile(const TensorArgs &a, int b, double *
     return true;
   };
   unroll_while<0, PF>(prologue);
-  int c = c0;
-  auto plane = [&](auto Uc) -> bool {
-    constexpr int J = decltype(Uc)::value % R;  // ring slot of plane c
-    constexpr int SL = decltype(Uc)::value % PF; // prefetch buffer of plane c
-    if (c > c1)
-      return false;
-    // 1. window of plane c -> LDS; start loading plane c+PF into its buffer
+
+  // window registers pre[SL] -> LDS window buffer B
+  auto store_window = [&](auto SLc, auto Bc) {
+    constexpr int SL = decltype(SLc)::value;
+    auto A = Abuf(decltype(Bc)::value);
 #pragma unroll
     for (int k = 0; k < NLD; ++k)
       if (woff[k] >= 0) {
@@ -206,10 +241,11 @@ __device__ __forceinline__ void tensor_tile(const TensorArgs &a, int b, double *
         for (int tl = 0; tl < NTL; ++tl)
           (&A[tl][0][0])[e] = pre[SL][tl][k];
       }
-    if (c + PF <= c1)
-      fetch(c + PF, std::integral_constant<int, SL>{});
-    __syncthreads();
-    // 2. x-pass: both children of each base
+  };
+  // x-pass: both children of each base, window buffer B -> x buffer B
+  auto xpass = [&](auto Bc) {
+    auto A = Abuf(decltype(Bc)::value);
+    auto X = Xbuf(decltype(Bc)::value);
     if constexpr (XQ == 1) {
       for (int e = tid; e < WY * CX; e += NT) {
         const int r = e / CX, q = e - r * CX;
@@ -250,8 +286,13 @@ __device__ __forceinline__ void tensor_tile(const TensorArgs &a, int b, double *
         }
       }
     }
-    __syncthreads();
-    // 3. y-pass into ring slot J
+  };
+  // y-pass of plane p (x buffer B) into ring slot J, then the z-emission of
+  // the fine planes whose last coarse plane is p: their planes p-NTP+1..p
+  // sit in slots (J-NTP+1+t) mod R
+  auto ypass_emit = [&](auto Bc, auto Jc, int p) {
+    constexpr int J = decltype(Jc)::value;
+    auto X = Xbuf(decltype(Bc)::value);
 #pragma unroll
     for (int m = 0; m < IPT; ++m) {
       const int qy = qyl + NQY * m;
@@ -262,13 +303,11 @@ __device__ __forceinline__ void tensor_tile(const TensorArgs &a, int b, double *
         ring[tl][J][m][1] = line_apply<L, 1, CT>([&](int t) { return col[(L::TMIN[1] + t) * FX]; });
       }
     }
-    // 4. z-emission of the fine planes whose last coarse plane is c; their
-    //    planes c-NTP+1..c sit in slots (J-NTP+1+t) mod R
     auto emit = [&](auto PZc) {
       constexpr int PZ = decltype(PZc)::value;
       constexpr int NTP = L::NTAP[PZ];
       constexpr int TMAX = L::TMIN[PZ] + NTP - 1;
-      const int base = c - TMAX;
+      const int base = p - TMAX;
       if (base < qz0 || base > qz1)
         return;
       const int fz = 2 * base + PZ - a.off[2];
@@ -284,14 +323,8 @@ __device__ __forceinline__ void tensor_tile(const TensorArgs &a, int b, double *
           for (int tl = 0; tl < NTL; ++tl)
             v[tl] = (double)line_apply<L, PZ, CT>(
                 [&](int t) { return ring[tl][(J - NTP + 1 + t + 2 * R) % R][m][py]; });
-          // time interpolation in FP64 in both precisions
-          double out;
-          if constexpr (NTL == 1)
-            out = v[0];
-          else if constexpr (NTL == 2)
-            out = fmad(a.s[1], v[1], a.s[0] * v[0]);
-          else
-            out = fmad(a.s[2], v[2], fmad(a.s[1], v[1], a.s[0] * v[0]));
+          // time interpolation in FP64 in every precision mode
+          const double out = time_combine<NTL>(v, a.s, a.s2, a.l0, a.teno);
           if (xin && fy >= 0 && fy < a.n[1])
             __builtin_nontemporal_store((TS)out, &static_cast<TS *>(a.dst)[gfx + a.dj * fy + a.dk * fz]);
         }
@@ -299,10 +332,54 @@ __device__ __forceinline__ void tensor_tile(const TensorArgs &a, int b, double *
     };
     emit(std::integral_constant<int, 0>{});
     emit(std::integral_constant<int, 1>{});
-    ++c;
-    return true;
   };
-  while (unroll_while<0, U>(plane)) {
+
+  int c = c0;
+  if constexpr (PIPE == 0) {
+    auto plane = [&](auto Uc) -> bool {
+      constexpr int J = decltype(Uc)::value % R;  // ring slot of plane c
+      constexpr int SL = decltype(Uc)::value % PF; // prefetch buffer of plane c
+      if (c > c1)
+        return false;
+      store_window(std::integral_constant<int, SL>{}, std::integral_constant<int, 0>{});
+      if (c + PF <= c1)
+        fetch(c + PF, std::integral_constant<int, SL>{});
+      __syncthreads();
+      xpass(std::integral_constant<int, 0>{});
+      __syncthreads();
+      ypass_emit(std::integral_constant<int, 0>{}, std::integral_constant<int, J>{}, c);
+      ++c;
+      return true;
+    };
+    while (unroll_while<0, U>(plane)) {
+    }
+  } else {
+    // iteration c: window of plane c (buffer c&1), x-pass of plane c-1
+    // (buffer (c-1)&1), y-pass of plane c-2 (buffer c&1), one barrier.
+    // Buffer B is written by the window store of iteration c and read by
+    // the x-pass of iteration c+1; its x buffer is written there and read
+    // by the y-pass of iteration c+2: every hazard crosses a barrier.
+    auto iter = [&](auto Uc) -> bool {
+      constexpr int u = decltype(Uc)::value;
+      constexpr int B0 = u & 1, B1 = (u + 1) & 1;
+      constexpr int J2 = ((u - 2) % R + R) % R; // ring slot of plane c-2
+      if (c > c1 + 2)
+        return false;
+      if (c <= c1) {
+        store_window(std::integral_constant<int, 0>{}, std::integral_constant<int, B0>{});
+        if (c + 1 <= c1)
+          fetch(c + 1, std::integral_constant<int, 0>{});
+      }
+      if (c - 1 >= c0 && c - 1 <= c1)
+        xpass(std::integral_constant<int, B1>{});
+      if (c - 2 >= c0)
+        ypass_emit(std::integral_constant<int, B0>{}, std::integral_constant<int, J2>{}, c - 2);
+      __syncthreads();
+      ++c;
+      return true;
+    };
+    while (unroll_while<0, U>(iter)) {
+    }
   }
   // the next tile of a persistent workgroup overwrites the LDS window
   __syncthreads();
@@ -341,11 +418,13 @@ __device__ __forceinline__ double point_value(const PointArgs &a, int i, int j, 
     out = r;
   } else if constexpr (OP == PO_INTERP) {
     const long long o = i + a.sj * j + a.sk * k;
-    double r = a.s[0] * a.src[0][o];
-    r = fmad(a.s[1], a.src[1][o], r);
-    if (a.ntl == 3)
-      r = fmad(a.s[2], a.src[2][o], r);
-    out = r;
+    if (a.ntl == 3) {
+      const double v[3] = {a.src[0][o], a.src[1][o], a.src[2][o]};
+      out = time_combine<3>(v, a.s, a.s2, a.l0, a.teno);
+    } else {
+      const double v[2] = {a.src[0][o], a.src[1][o]};
+      out = time_combine<2>(v, a.s, a.s2, a.l0, a.teno);
+    }
   }
   return out;
 }
@@ -367,15 +446,25 @@ __device__ __forceinline__ void tvd_range(const PointArgs &a, int d, int *q0, in
   *nq = ((a.off[d] + a.n[d] - 1) >> 1) - *q0 + 1;
 }
 
-__device__ __forceinline__ void tvd_cell(const PointArgs &a, int qx, int qy, int qz) {
+// parent value and the three minmod slopes of coarse cell (qx,qy,qz) on
+// time level tl
+struct TvdCell {
+  double c, mx, my, mz;
+};
+__device__ __forceinline__ TvdCell tvd_load(const PointArgs &a, int tl, int qx, int qy, int qz) {
   const long long J = a.sj, K = a.sk;
-  const double *p = a.src[0] + qx + J * qy + K * qz;
+  const double *p = a.src[tl] + qx + J * qy + K * qz;
   const double c = p[0];
-  const double mx = minmod(c - p[-1], p[1] - c);
-  const double my = minmod(c - p[-J], p[J] - c);
-  const double mz = minmod(c - p[-K], p[K] - c);
+  return TvdCell{c, minmod(c - p[-1], p[1] - c), minmod(c - p[-J], p[J] - c),
+                 minmod(c - p[-K], p[K] - c)};
+}
+
+// the children of one cell on NTL time levels (data<T>::transfer_time: each
+// level prolongated, then the levels combined in time)
+template <int NTL>
+__device__ __forceinline__ void tvd_store(const PointArgs &a, int qx, int qy, int qz,
+                                          const TvdCell *v) {
   const double q = 1.0 / 4;
-  const double sx[2] = {-q * mx, q * mx}, sy[2] = {-q * my, q * my}, sz[2] = {-q * mz, q * mz};
   const int fx0 = 2 * qx - a.off[0], fy0 = 2 * qy - a.off[1], fz0 = 2 * qz - a.off[2];
 #pragma unroll
   for (int z = 0; z < 2; ++z) {
@@ -390,11 +479,29 @@ __device__ __forceinline__ void tvd_cell(const PointArgs &a, int qx, int qy, int
 #pragma unroll
       for (int x = 0; x < 2; ++x) {
         const int fx = fx0 + x;
-        if (fx >= 0 && fx < a.n[0])
-          __builtin_nontemporal_store(c + sx[x] + sy[y] + sz[z], &a.dst[fx + a.dj * fy + a.dk * fz]);
+        if (fx < 0 || fx >= a.n[0])
+          continue;
+        double w[NTL];
+#pragma unroll
+        for (int tl = 0; tl < NTL; ++tl) {
+          const double sx = (x ? q : -q) * v[tl].mx, sy = (y ? q : -q) * v[tl].my,
+                       sz = (z ? q : -q) * v[tl].mz;
+          w[tl] = v[tl].c + sx + sy + sz;
+        }
+        __builtin_nontemporal_store(time_combine<NTL>(w, a.s, a.s2, a.l0, a.teno),
+                                    &a.dst[fx + a.dj * fy + a.dk * fz]);
       }
     }
   }
+}
+
+template <int NTL>
+__device__ __forceinline__ void tvd_cell(const PointArgs &a, int qx, int qy, int qz) {
+  TvdCell v[NTL];
+#pragma unroll
+  for (int tl = 0; tl < NTL; ++tl)
+    v[tl] = tvd_load(a, tl, qx, qy, qz);
+  tvd_store<NTL>(a, qx, qy, qz, v);
 }
 
 // tile shapes: one z-marching column of 2CX x 2CY fine points, 2CX*CY/IPT
@@ -413,6 +520,15 @@ template <> struct TileShape<2> {
 template <> struct TileShape<3> {
   static constexpr int CX = 32, CY = 8, IPT = 1, XQ = 2;
 };
+// the pipelined schedule (double-buffered window and x-pass results) where
+// its LDS fits the 64 KB of a static allocation, else two barriers per plane
+template <class L, int NTL, int CX, int CY, int XQ> constexpr int pipe_fits() {
+  return TensorLds<L, NTL, CX, CY, XQ, 2>::TOTAL * 8 <= 65536 ? 1 : 0;
+}
+template <class L, int NTL, int CX, int CY, int XQ> constexpr int tile_lds() {
+  return TensorLds<L, NTL, CX, CY, XQ, pipe_fits<L, NTL, CX, CY, XQ>() ? 2 : 1>::TOTAL;
+}
+
 template <int NTL> constexpr int tile_threads() {
   return 2 * TileShape<NTL>::CX * TileShape<NTL>::CY / TileShape<NTL>::IPT;
 }

@@ -249,6 +249,19 @@ int cg_interpolate_3d_3tl_f64(const double *src1, double t1,
                               const int dstbbox[9], const int srcregbbox[9],
                               const int dstregbbox[9], cg_stream_t stream);
 
+/* interpolate_eno_3d_3tl<double>: CarpetLib/src/interpolate_eno_3d_3tl.cc:
+   88-164 -- quadratic in time; a point where that makes a new extremum
+   falls back to linear interpolation between the two levels whose interval
+   holds t (the time interpolation of ENO/TVD groups, data.cc:1282-1292) */
+int cg_interpolate_eno_3d_3tl_f64(const double *src1, double t1,
+                                  const double *src2, double t2,
+                                  const double *src3, double t3,
+                                  const int srcpadext[3], const int srcext[3],
+                                  double *dst, double t, const int dstpadext[3],
+                                  const int dstext[3], const int srcbbox[9],
+                                  const int dstbbox[9], const int srcregbbox[9],
+                                  const int dstregbbox[9], cg_stream_t stream);
+
 /* ---- mixed precision (no reference counterpart: the FP32 form beside the
    FP64 drop-in).  Sources and destination hold float; every 1-D sum runs in
    FP32 with compensated (TwoProduct/TwoSum, Dot2) accumulation; the fused

@@ -909,6 +909,93 @@ int orc_interpolate_3d(int ntl, const double *const *srcs, const double *tt,
   return CG_OK;
 }
 
+// interpolate_eno_3d_3tl, interpolate_eno_3d_3tl.cc:24-164: quadratic
+// interpolation through three time levels; where it makes a new extremum
+// (above the largest or below the smallest of the three values) the point
+// falls back to linear interpolation between the two levels whose interval
+// holds t (levels 1-2 if t lies in theirs, else 2-3).  eps is relative to
+// the time span (:107), unlike interpolate_3d_3tl's absolute 1e-12.
+// Weights: the reference's expressions (:119-129); sums in canonical order.
+int orc_eno_time_factors(const double *tt, double t, double *s3, double *s2, int *l0) {
+  const double t1 = tt[0], t2 = tt[1], t3 = tt[2];
+  const double tmin = std::fmin(std::fmin(t1, t2), t3);
+  const double tmax = std::fmax(std::fmax(t1, t2), t3);
+  const double eps = 1.0e-12 * (tmax - tmin);
+  if (std::fabs(t1 - t2) < eps || std::fabs(t1 - t3) < eps || std::fabs(t2 - t3) < eps)
+    return CG_ERR_TIME_SAME;
+  if (t < tmin - eps || t > tmax + eps)
+    return CG_ERR_TIME_EXTRAP;
+  s3[0] = (t - t2) * (t - t3) / ((t1 - t2) * (t1 - t3));
+  s3[1] = (t - t1) * (t - t3) / ((t2 - t1) * (t2 - t3));
+  s3[2] = (t - t1) * (t - t2) / ((t3 - t1) * (t3 - t2));
+  const bool use_12 = t >= std::min(t1, t2) - eps && t <= std::max(t1, t2) + eps;
+  const bool use_23 = t >= std::min(t2, t3) - eps && t <= std::max(t2, t3) + eps;
+  if (!use_12 && !use_23)
+    return CG_ERR_TIME_EXTRAP;
+  if (use_12) {
+    *l0 = 0;
+    s2[0] = (t - t2) / (t1 - t2);
+    s2[1] = (t - t1) / (t2 - t1);
+  } else {
+    *l0 = 1;
+    s2[0] = (t - t3) / (t2 - t3);
+    s2[1] = (t - t2) / (t3 - t2);
+  }
+  return CG_OK;
+}
+
+static inline double eno_time_point(double a, double b, double c, const double *s3,
+                                    const double *s2, int l0) {
+  double d = std::fma(s3[2], c, std::fma(s3[1], b, s3[0] * a));
+  if (d > std::max(std::max(a, b), c) || d < std::min(std::min(a, b), c)) {
+    const double u = l0 == 0 ? a : b, v = l0 == 0 ? b : c;
+    d = std::fma(s2[1], v, s2[0] * u);
+  }
+  return d;
+}
+
+int orc_interpolate_eno_3d_3tl(const double *const *srcs, const double *tt, const int *spad,
+                               const int *sext, double *dst, double t, const int *dpad,
+                               const int *dext, const int *sb, const int *db, const int *srb,
+                               const int *drb) {
+  (void)srb;
+  const box3 s = mkbox(sb), dbx = mkbox(db), r = mkbox(drb);
+  int e;
+  if ((e = check_array(spad, sext, s)) || (e = check_array(dpad, dext, dbx)))
+    return e;
+  for (int d = 0; d < 3; ++d) {
+    if (r.str[d] <= 0)
+      return CG_ERR_ARG;
+    if (s.str[d] != r.str[d] || dbx.str[d] != r.str[d])
+      return CG_ERR_STRIDE;
+  }
+  if (empty(r))
+    return CG_ERR_EMPTY;
+  if (!contained(r, s) || !contained(r, dbx))
+    return CG_ERR_CONTAIN;
+  double s3[3], s2[2];
+  int l0;
+  if ((e = orc_eno_time_factors(tt, t, s3, s2, &l0)))
+    return e;
+  int n[3], so[3], dof[3];
+  for (int d = 0; d < 3; ++d) {
+    if ((r.lo[d] - s.lo[d]) % s.str[d] || (r.lo[d] - dbx.lo[d]) % dbx.str[d])
+      return CG_ERR_ALIGN;
+    n[d] = sizes(r, d);
+    so[d] = (r.lo[d] - s.lo[d]) / s.str[d];
+    dof[d] = (r.lo[d] - dbx.lo[d]) / dbx.str[d];
+  }
+#pragma omp parallel for collapse(2) if (g_use_openmp)
+  for (int k = 0; k < n[2]; ++k)
+    for (int j = 0; j < n[1]; ++j)
+      for (int i = 0; i < n[0]; ++i) {
+        size_t si = idx3(so[0] + i, so[1] + j, so[2] + k, spad);
+        dst[idx3(dof[0] + i, dof[1] + j, dof[2] + k, dpad)] =
+            eno_time_point(srcs[0][si], srcs[1][si], srcs[2][si], s3, s2, l0);
+      }
+  return CG_OK;
+}
+
 // gdata::find_source_timelevel, gdata.cc:288-349.  op uses CG_OP_*.
 int orc_find_source_timelevel(const double *times, int ntimes, double time,
                               int order_time, int op, int *timelevel0,
@@ -978,9 +1065,7 @@ int orc_prolongate_time(int kind, int order, const double *const *srcs,
   };
   if (ntl == 1)
     return prolong(srcs[tl0], dst, dpad, dext, db);
-  // ENO/TVD with 3 time levels use interpolate_eno_3d_3tl (data.cc:1282-1292),
-  // which is not restated here
-  if (ntl == 3 && (kind == 2 || kind == 3))
+  if (ntl > 3)
     return CG_ERR_ORDER;
   // temporaries with extent = region, unpadded (data.cc:507-509)
   const box3 r = mkbox(drb);
@@ -994,8 +1079,11 @@ int orc_prolongate_time(int kind, int order, const double *const *srcs,
       return e;
   const double *tp[3] = {tmp[0].data(), ntl > 1 ? tmp[1].data() : nullptr,
                          ntl > 2 ? tmp[2].data() : nullptr};
-  if (ntl > 3)
-    return CG_ERR_ORDER;
+  // ENO and TVD groups interpolate 3 time levels with interpolate_eno_3d_3tl
+  // (data.cc:1282-1292), Lagrange with interpolate_3d_2tl/3tl
+  if (ntl == 3 && (kind == 2 || kind == 3))
+    return orc_interpolate_eno_3d_3tl(tp, times, text, text, dst, time, dpad, dext, drb, db,
+                                      drb, drb);
   return orc_interpolate_3d(ntl, tp, times, text, text, dst, time, dpad, dext,
                             drb, db, drb, drb);
 }

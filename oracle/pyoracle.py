@@ -57,6 +57,9 @@ _decl("orc_restrict_3d_cc_rf2", C.c_int, *_OP)
 _decl("orc_time_factors", C.c_int, C.c_int, _dp, C.c_double, _dp)
 _decl("orc_interpolate_3d", C.c_int, C.c_int, C.POINTER(_dp), _dp, _ip, _ip, _dp, C.c_double,
       _ip, _ip, _ip, _ip, _ip, _ip)
+_decl("orc_eno_time_factors", C.c_int, _dp, C.c_double, _dp, _dp, _ip)
+_decl("orc_interpolate_eno_3d_3tl", C.c_int, C.POINTER(_dp), _dp, _ip, _ip, _dp, C.c_double,
+      _ip, _ip, _ip, _ip, _ip, _ip)
 _decl("orc_find_source_timelevel", C.c_int, _dp, C.c_int, C.c_double, C.c_int, C.c_int, _ip, _ip)
 _decl("orc_prolongate_time", C.c_int, C.c_int, C.c_int, C.POINTER(_dp), _dp, C.c_int, _ip, _ip, _dp,
       C.c_double, _ip, _ip, _ip, _ip, _ip, C.c_int)
@@ -157,6 +160,24 @@ def interpolate_3d(srcs, times, dst, t, dstreg) -> int:
     return _lib.orc_interpolate_3d(n, ptrs, tt, I3(*s0.padext), I3(*s0.ext), _p(dst.data), t,
                                    I3(*dst.padext), I3(*dst.ext), s0.c_box(), dst.c_box(),
                                    _b(dstreg), _b(dstreg))
+
+
+def eno_time_factors(times, t):
+    """(rc, 3-point weights, 2-point weights, first level of the 2-point pair)"""
+    tt = (C.c_double * 3)(*times)
+    s3, s2, l0 = (C.c_double * 3)(), (C.c_double * 2)(), C.c_int()
+    rc = _lib.orc_eno_time_factors(tt, t, s3, s2, C.byref(l0))
+    return rc, list(s3), list(s2), l0.value
+
+
+def interpolate_eno_3d_3tl(srcs, times, dst, t, dstreg) -> int:
+    assert len(srcs) == 3
+    ptrs = (_dp * 3)(*[_p(s.data) for s in srcs])
+    tt = (C.c_double * 3)(*times)
+    s0 = srcs[0]
+    return _lib.orc_interpolate_eno_3d_3tl(ptrs, tt, I3(*s0.padext), I3(*s0.ext), _p(dst.data), t,
+                                           I3(*dst.padext), I3(*dst.ext), s0.c_box(), dst.c_box(),
+                                           _b(dstreg), _b(dstreg))
 
 
 def find_source_timelevel(times, time, order_time, op):

@@ -189,15 +189,47 @@ def test_interpolate_parity(ntl):
 
 KIND_IDS = {"vlag": (orc.KIND_VLAG, cg.OP_LAGRANGE, cg.VERTEX_CENTERED),
             "clag": (orc.KIND_CLAG, cg.OP_LAGRANGE, cg.CELL_CENTERED),
-            "eno": (orc.KIND_ENO, cg.OP_ENO, cg.CELL_CENTERED)}
+            "eno": (orc.KIND_ENO, cg.OP_ENO, cg.CELL_CENTERED),
+            "tvd": (orc.KIND_TVD, cg.OP_TVD, cg.CELL_CENTERED)}
+
+
+def test_interpolate_eno_3d_3tl():
+    """interpolate_eno_3d_3tl against the oracle, bit for bit, on independent
+    random levels (the quadratic overshoots often, so both branches run) and
+    on times in either order."""
+    rng = np.random.default_rng(77)
+    for c, (times, frac) in enumerate((([0.5, 0.25, 0.0], 0.3), ([0.0, 0.25, 0.5], 0.8),
+                                       ([0.5, 0.25, 0.0], 0.9), ([0.2, 0.0, 0.1], 0.55))):
+        print(f"interp_eno case {c}", flush=True)
+        n = tuple(int(v) for v in rng.integers(5, 60, 3))
+        box = ((0, 0, 0), tuple(v - 1 for v in n), (1, 1, 1))
+        sz = sizes(box)
+        rb = (tuple(int(rng.integers(0, 3)) for _ in range(3)),
+              tuple(v - 1 - int(rng.integers(0, 3)) for v in sz), (1, 1, 1))
+        pad = pad_of(sz, rng)
+        srcs = []
+        for _ in range(3):
+            h = host_array(box, pad)
+            h.view()[...] = field(rng, sz, "random")
+            srcs.append(h)
+        t = min(times) + frac * (max(times) - min(times))
+        dst = host_array(box, pad_of(sz, rng))
+        exp = orc.HostArr(box, dst.data.copy())
+        assert orc.interpolate_eno_3d_3tl(srcs, times, exp, t, rb) == 0
+        ds = [_dev(h) for h in srcs]
+        ddst = _dev(dst)
+        cg.interpolate_eno_3d_3tl(ds[0], times[0], ds[1], times[1], ds[2], times[2], ddst, t, cg.Box(*rb))
+        _check(exp, ddst, f"interp_eno case {c}")
 
 
 @pytest.mark.parametrize("kind,order,ntl", [("vlag", 5, 3), ("vlag", 3, 2), ("vlag", 1, 3),
-                                            ("clag", 3, 3), ("eno", 3, 2), ("eno", 2, 2)])
+                                            ("clag", 3, 3), ("eno", 3, 2), ("eno", 2, 2),
+                                            ("eno", 3, 3), ("eno", 2, 3), ("tvd", 1, 2), ("tvd", 1, 3)])
 def test_transfer_time_interpolation(kind, order, ntl):
     """cg_transfer_from with time interpolation (one fused kernel) against the
     reference's shape (prolongate every level into a temporary, then
-    interpolate_3d_2tl/3tl), data.cc:497-521."""
+    interpolate_3d_2tl/3tl, or interpolate_eno_3d_3tl for ENO/TVD groups with
+    3 levels), data.cc:497-521, 1182-1299; also through a batch."""
     rng = np.random.default_rng(31 * order + ntl)
     okind, op, cent = KIND_IDS[kind]
     # ENO order_space 5 -> ENO<3>, 2..4 -> ENO<2> (data.cc:697-704)
@@ -220,3 +252,12 @@ def test_transfer_time_interpolation(kind, order, ntl):
         ddst = _dev(dst)
         cg.transfer_from(ddst, ds, times, cg.Box(*rb), cg.Box(*rb), t, op, cent, order_space, ntl - 1)
         _check(exp, ddst, f"transfer {kind} case {c}")
+        if c == 1:
+            bdst = _dev(dst)
+            b = cg.Batch()
+            b.add(0, bdst, ds, times, cg.Box(*rb), cg.Box(*rb), t, op, cent, order_space, ntl - 1)
+            b.finalize()
+            b.run()
+            cg.synchronize()
+            _check(exp, bdst, f"batched transfer {kind} case {c}")
+            b.close()

@@ -443,3 +443,104 @@ def test_prolongate_time_matches_temporaries_shape():
     d4 = host_array(db, sizes(db))
     assert orc.prolongate_3d_rf2(5, srcs[1], d4, rb) == 0
     assert np.array_equal(d3.data, d4.data, equal_nan=True)
+
+
+# ---- ENO time interpolation (interpolate_eno_3d_3tl) -------------------------
+def test_eno_time_factors():
+    # 3-point weights are the Lagrange ones; the 2-point pair is the interval
+    # holding t (1-2 if possible, else 2-3); eps is relative to the time span
+    for times in ([1.0, 0.0, -1.0], [0.0, 1.0, 2.0], [0.3, 0.2, 0.1]):
+        for t in np.linspace(min(times), max(times), 9):
+            rc, s3, s2, l0 = orc.eno_time_factors(times, float(t))
+            assert rc == 0
+            rc3, s = orc.time_factors(times, float(t))
+            assert s3 == s
+            t1, t2, t3 = times
+            in12 = min(t1, t2) <= t <= max(t1, t2)
+            assert l0 == (0 if in12 else 1)
+            a, b = (t1, t2) if l0 == 0 else (t2, t3)
+            assert sum(s2) == pytest.approx(1.0, abs=1e-15)
+            assert s2[0] * a + s2[1] * b == pytest.approx(t, abs=1e-15)
+    # relative eps: distinct times 1e-13 apart are fine when the span is 2e-13
+    assert orc.eno_time_factors([2e-13, 1e-13, 0.0], 1.5e-13)[0] == 0
+    assert orc.eno_time_factors([1.0, 1.0, 0.0], 0.5)[0] == -8
+    assert orc.eno_time_factors([1.0, 0.0, -1.0], 1.5)[0] == -9
+
+
+def _eno_time_case(rng, n=(9, 8, 7)):
+    box = ((0, 0, 0), tuple(v - 1 for v in n), (1, 1, 1))
+    srcs = []
+    for _ in range(3):
+        h = host_array(box, n)
+        h.view()[...] = rng.uniform(-1, 1, size=(n[2], n[1], n[0]))
+        srcs.append(h)
+    return box, srcs
+
+
+def test_interpolate_eno_3d_3tl_bounded_and_quadratic_inside():
+    """Every point lies within the three time levels' range; where the
+    quadratic interpolant does, the result is that interpolant bit for bit
+    (the same sum as interpolate_3d_3tl), else the linear interpolant of the
+    pair whose interval holds t."""
+    rng = np.random.default_rng(31)
+    box, srcs = _eno_time_case(rng)
+    for times, t in (([1.0, 0.0, -1.0], 1 / 3), ([1.0, 0.0, -1.0], -0.7), ([0.0, 1.0, 2.0], 1.5)):
+        got = host_array(box, (9, 8, 7))
+        assert orc.interpolate_eno_3d_3tl(srcs, times, got, t, box) == 0
+        quad = host_array(box, (9, 8, 7))
+        assert orc.interpolate_3d(srcs, times, quad, t, box) == 0
+        a, b, c = (s.view() for s in srcs)
+        lo = np.minimum(np.minimum(a, b), c)
+        hi = np.maximum(np.maximum(a, b), c)
+        g = got.view()
+        assert np.all(g >= lo) and np.all(g <= hi)
+        inside = (quad.view() >= lo) & (quad.view() <= hi)
+        assert np.array_equal(g[inside], quad.view()[inside])
+        assert (~inside).any() and inside.any()
+        rc, s3, s2, l0 = orc.eno_time_factors(times, t)
+        u, v = (a, b) if l0 == 0 else (b, c)
+        lin = s2[0] * u + s2[1] * v
+        assert np.allclose(g[~inside], lin[~inside], rtol=0, atol=1e-15)
+
+
+def test_interpolate_eno_3d_3tl_known_overshoot():
+    # values 0, 1, 1 at t = 0, 1, 2: the parabola reaches 1.125 at t = 1.5,
+    # above the data, so the point falls back to levels 2-3: exactly 1
+    box = ((0, 0, 0), (0, 0, 0), (1, 1, 1))
+    srcs = [host_array(box, (1, 1, 1)) for _ in range(3)]
+    for h, v in zip(srcs, (0.0, 1.0, 1.0)):
+        h.view()[...] = v
+    out = host_array(box, (1, 1, 1))
+    assert orc.interpolate_eno_3d_3tl(srcs, [0.0, 1.0, 2.0], out, 1.5, box) == 0
+    assert out.view()[0, 0, 0] == 1.0
+    assert orc.interpolate_3d(srcs, [0.0, 1.0, 2.0], out, 1.5, box) == 0
+    assert out.view()[0, 0, 0] == 1.125
+
+
+@pytest.mark.parametrize("kind,order", [(orc.KIND_TVD, 1), (orc.KIND_ENO, 3), (orc.KIND_ENO, 2)])
+def test_prolongate_time_eno_groups_three_levels(kind, order):
+    """ENO and TVD groups with 3 time levels: each level prolongated into a
+    temporary, then interpolate_eno_3d_3tl (data.cc:1282-1292)."""
+    rng = np.random.default_rng(40 + order)
+    sb, db, rb = prolong_case(rng, "eno" if kind == orc.KIND_ENO else "tvd", order, maxn=10)
+    pad = pad_of(sizes(sb), rng)
+    srcs = []
+    for _ in range(3):
+        h = host_array(sb, pad)
+        h.view()[...] = field(rng, sizes(sb), "smooth")
+        srcs.append(h)
+    times = [1.0, 0.0, -1.0]
+    dst = host_array(db, sizes(db))
+    assert orc.prolongate_time(kind, order, srcs, times, dst, 0.4, rb, 2) == 0
+    e = tuple(sizes(rb))
+    tmps = []
+    for s in srcs:
+        tp = host_array(rb, e)
+        if kind == orc.KIND_TVD:
+            assert orc.prolongate_3d_cc_tvd(s, tp, rb) == 0
+        else:
+            assert orc.prolongate_3d_cc_eno_rf2(order, s, tp, rb) == 0
+        tmps.append(tp)
+    exp = host_array(db, sizes(db))
+    assert orc.interpolate_eno_3d_3tl(tmps, times, exp, 0.4, rb) == 0
+    assert np.array_equal(dst.data, exp.data, equal_nan=True)
