@@ -355,6 +355,9 @@ def main(argv=None):
               "MI355X here (sharding across GPUs is a design note, not a measured path); "
               "run with --gpus 1", file=sys.stderr)
         return 2
+    if args.calib_copy:
+        nbytes = calib_copy()
+        print(f"# calibration copy: {nbytes} bytes read and written", file=sys.stderr, flush=True)
     if args.config == 3:
         res = run_config3(args.steps, args.warmup, args.n_var)
         tvd = res["cc_tvd"]
@@ -412,9 +415,6 @@ def main(argv=None):
                              "unit": "GB/s", "frac": r["batched_roofline_frac"], "traffic": None}}
         print(json.dumps(line))
         return 0
-    if args.calib_copy:
-        nbytes = calib_copy()
-        print(f"# calibration copy: {nbytes} bytes read and written", file=sys.stderr, flush=True)
     print(f"# config2: {args.n_gf} grid functions, {args.steps} steps", file=sys.stderr, flush=True)
     cfg, stats, ms_step, wall, ms_plain = run_config2(args.steps, args.warmup, args.n_gf, args.plain_steps,
                                                       args.tune_cache)

@@ -36,8 +36,11 @@ from .core import (  # noqa: F401
     prolongate_3d_cc_rf2_mixed,
     relative_errors,
     transfer_from_mixed,
+    restrict_3d_cc_o3_rf2,
+    restrict_3d_cc_o5_rf2,
     restrict_3d_cc_rf2,
     restrict_3d_rf2,
+    set_restriction,
     synchronize,
     transfer_from,
 )

@@ -307,6 +307,34 @@ def restrict_3d_cc_rf2(src: DeviceArray, dst: DeviceArray, dstregbbox: Box,
           "restrict_3d_cc_rf2")
 
 
+def restrict_3d_cc_o3_rf2(src: DeviceArray, dst: DeviceArray, dstregbbox: Box,
+                          srcregbbox: Optional[Box] = None, stream=None):
+    """Cubic cell-centred restriction (CarpetLib/src/restrict_3d_cc_o3_rf2.cc);
+    the source must hold one fine cell beyond the region's footprint."""
+    check(lib.cg_restrict_3d_cc_o3_rf2_f64(*_arr(src), *_arr(dst), src.box.as_c(), dst.box.as_c(),
+                                           (srcregbbox or dstregbbox).as_c(), dstregbbox.as_c(),
+                                           _sh(stream)),
+          "restrict_3d_cc_o3_rf2")
+
+
+def restrict_3d_cc_o5_rf2(src: DeviceArray, dst: DeviceArray, dstregbbox: Box,
+                          srcregbbox: Optional[Box] = None, stream=None):
+    """Quintic cell-centred restriction (CarpetLib/src/restrict_3d_cc_o5_rf2.cc);
+    the source must hold two fine cells beyond the region's footprint."""
+    check(lib.cg_restrict_3d_cc_o5_rf2_f64(*_arr(src), *_arr(dst), src.box.as_c(), dst.box.as_c(),
+                                           (srcregbbox or dstregbbox).as_c(), dstregbbox.as_c(),
+                                           _sh(stream)),
+          "restrict_3d_cc_o5_rf2")
+
+
+def set_restriction(use_higher_order_restriction: bool, restriction_order_space: int = 3):
+    """CarpetLib's use_higher_order_restriction / restriction_order_space
+    parameters (CarpetLib/param.ccl) as seen by transfer_from: cell-centred
+    restrictions of non-ENO/WENO groups then use the o3 or o5 stencil."""
+    check(lib.cg_set_restriction(int(bool(use_higher_order_restriction)), int(restriction_order_space)),
+          "set_restriction")
+
+
 def interpolate_3d_2tl(src1: DeviceArray, t1: float, src2: DeviceArray, t2: float,
                        dst: DeviceArray, t: float, dstregbbox: Box, stream=None):
     check(lib.cg_interpolate_3d_2tl_f64(src1.ptr, t1, src2.ptr, t2, I3(*src1.padext), I3(*src1.ext),

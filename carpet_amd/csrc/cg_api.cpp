@@ -160,8 +160,13 @@ int resolve_transfer(const cg_gdata *dst, const cg_gdata *const *srcs,
     if (slab && !(slab->is_centered[0] == 1 && slab->is_centered[1] == 1 &&
                   slab->is_centered[2] == 1))
       return set_error(CG_ERR_OPERATOR, "restrict_3d_vc_rf2 is not provided");
+    // data.cc:1064-1094: higher-order restriction when the parameter asks
+    // for it, except for ENO and WENO groups
+    int use_ho = 0, rorder = 1;
+    restriction_setting(&use_ho, &rorder);
     tr.is_prolong = false;
-    return plan_restrict(true, s0, d, dreg, tr.pt);
+    return plan_restrict(true, s0, d, dreg, tr.pt,
+                         use_ho && op != CG_OP_ENO && op != CG_OP_WENO ? rorder : 1);
   }
   return set_error(CG_ERR_STRIDE, "source and destination strides are not ordered");
 }
@@ -528,14 +533,14 @@ static int restrict_call(bool cell, const double *src, const int *srcpadext,
                          const int *srcext, double *dst, const int *dstpadext,
                          const int *dstext, const int *srcbbox,
                          const int *dstbbox, const int *dstregbbox,
-                         cg_stream_t stream) {
+                         cg_stream_t stream, int order = 1) {
   if (!srcpadext || !srcext || !dstpadext || !dstext || !srcbbox || !dstbbox || !dstregbbox)
     return set_error(CG_ERR_ARG, "null argument");
   Transfer tr;
   tr.is_prolong = false;
   int e = plan_restrict(cell, arr_from(src, srcpadext, srcext, srcbbox),
                         arr_from(dst, dstpadext, dstext, dstbbox),
-                        Box::from(dstregbbox), tr.pt);
+                        Box::from(dstregbbox), tr.pt, order);
   if (e)
     return e;
   return run_transfer(tr, 0, S(stream));
@@ -560,6 +565,28 @@ int cg_restrict_3d_cc_rf2_f64(const double *src, const int srcpadext[3],
   (void)srcregbbox;
   return restrict_call(true, src, srcpadext, srcext, dst, dstpadext, dstext,
                        srcbbox, dstbbox, dstregbbox, stream);
+}
+
+int cg_restrict_3d_cc_o3_rf2_f64(const double *src, const int srcpadext[3],
+                                 const int srcext[3], double *dst,
+                                 const int dstpadext[3], const int dstext[3],
+                                 const int srcbbox[9], const int dstbbox[9],
+                                 const int srcregbbox[9], const int dstregbbox[9],
+                                 cg_stream_t stream) {
+  (void)srcregbbox;
+  return restrict_call(true, src, srcpadext, srcext, dst, dstpadext, dstext,
+                       srcbbox, dstbbox, dstregbbox, stream, 3);
+}
+
+int cg_restrict_3d_cc_o5_rf2_f64(const double *src, const int srcpadext[3],
+                                 const int srcext[3], double *dst,
+                                 const int dstpadext[3], const int dstext[3],
+                                 const int srcbbox[9], const int dstbbox[9],
+                                 const int srcregbbox[9], const int dstregbbox[9],
+                                 cg_stream_t stream) {
+  (void)srcregbbox;
+  return restrict_call(true, src, srcpadext, srcext, dst, dstpadext, dstext,
+                       srcbbox, dstbbox, dstregbbox, stream, 5);
 }
 
 static int interp_call(int ntl, const double *const *srcs, const double *times,

@@ -113,9 +113,9 @@ static void load_cache(const char *path, TuneMap &m) {
     const size_t bar = line.rfind('|');
     if (line.empty() || line[0] == '#' || bar == std::string::npos)
       continue;
-    int z = 0, p = 0, w = 0;
-    if (sscanf(line.c_str() + bar + 1, "%d %d %d", &z, &p, &w) >= 2)
-      m[line.substr(0, bar)] = Tune{z, p, w};
+    int z = 0, p = 0, w = 0, sh = -1;
+    if (sscanf(line.c_str() + bar + 1, "%d %d %d %d", &z, &p, &w, &sh) >= 2)
+      m[line.substr(0, bar)] = Tune{z, p, w, sh};
   }
 }
 
@@ -126,10 +126,11 @@ static int save_cache(const char *path, const TuneMap &m) {
   if (!f)
     return set_error(CG_ERR_ARG, "cannot write tuner cache %s", path);
   f << "# carpet_amd batch tuner cache: <class key>|<zchunk> <points per item> "
-       "<workgroups per CU: 0 resident capacity, -1 one per item>\n";
+       "<workgroups per CU: 0 resident capacity, -1 one per item> "
+       "<tile shape: 0 32x8, 1 16x16, -1 by region size>\n";
   for (const auto &kv : m)
-    f << kv.first << "|" << kv.second.zchunk << " " << kv.second.pt << " " << kv.second.wpc
-      << "\n";
+    f << kv.first << "|" << kv.second.zchunk << " " << kv.second.pt << " " << kv.second.wpc << " "
+      << kv.second.shape << "\n";
   return CG_OK;
 }
 
@@ -177,10 +178,10 @@ extern "C" int cg_batch_autotune(cg_batch *b, cg_stream_t stream, int reps,
     std::vector<Tune> cands;
     if (l.is_tensor)
       for (int z : {8, 16, 32, 64, 128, 0})
-        cands.push_back({z, l.pt, l.wpc});
+        cands.push_back({z, l.pt, l.wpc, l.shape});
     else
       for (int p : {2048, 4096, 16384, 65536})
-        cands.push_back({l.zchunk, p, l.wpc});
+        cands.push_back({l.zchunk, p, l.wpc, -1});
     float best = 1e30f;
     Tune bestc = cands[0];
     auto trial_run = [&](const Tune &c) -> int {
@@ -204,8 +205,11 @@ extern "C" int cg_batch_autotune(cg_batch *b, cg_stream_t stream, int reps,
     const Tune base = bestc;
     for (int w : {-1, 0, 2})
       if (w != base.wpc)
-        if (int e = trial_run(Tune{base.zchunk, base.pt, w}))
+        if (int e = trial_run(Tune{base.zchunk, base.pt, w, base.shape}))
           return e;
+    if (l.is_tensor) // the other tile shape, in the best mode
+      if (int e = trial_run(Tune{bestc.zchunk, bestc.pt, bestc.wpc, 1 - l.shape}))
+        return e;
     b->tuned[l.key] = bestc;
   }
   int e = batch_device_build(b->dev, b->transfers, b->phase, b->prec, b->tile_cfg, &b->tuned);
@@ -225,7 +229,8 @@ extern "C" int cg_batch_describe(const cg_batch *b, char *buf, size_t len) {
     << " work items " << items << " device descriptor bytes " << b->dev.blob_bytes << "\n";
   for (const BatchLaunch &l : b->dev.launches)
     o << "phase " << l.phase << " [" << l.key << "] items " << l.nitems << " grid " << l.grid
-      << " zchunk " << l.zchunk << " pt " << l.pt << " wpc " << l.wpc << "\n";
+      << " zchunk " << l.zchunk << " pt " << l.pt << " wpc " << l.wpc << " shape " << l.shape
+      << "\n";
   const std::string s = o.str();
   const size_t n = std::min(len - 1, s.size());
   std::memcpy(buf, s.data(), n);

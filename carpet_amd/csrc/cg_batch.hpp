@@ -28,6 +28,7 @@ struct BatchLaunch {
   int zchunk;      // tensor work item: coarse z bases per chunk (0 = all)
   int pt;          // pointwise work item: destination points
   int wpc;         // workgroups per CU of the grid (-1 = one per item)
+  int shape;       // tensor tile shape (0: 32 x 8 bases, 1: 16 x 16)
   std::string key; // tuner cache key (class + box-size histogram bucket)
 };
 
@@ -38,16 +39,17 @@ struct Item {
   unsigned tile;
 };
 
-// batched tensor launches per line kind (LK_*); defined in
-// cg_batch_tensor.hpp, instantiated in cg_batch_{vlag,clag,eno}.hip
-template <int KIND>
+// batched tensor launches per line kind (LK_*) and tile shape; defined in
+// cg_batch_tensor.hpp, instantiated in cg_batch_{vlag,clag,eno}_{w,n}.hip
+template <int KIND, int SH>
 int batch_tensor_kind(const BatchLaunch &l, const TensorArgs *td, const Item *items,
                       const int *seg, int *ctr, hipStream_t st);
-template <int KIND> const void *batch_tensor_fn_kind(const BatchLaunch &l, int *threads);
+template <int KIND, int SH> const void *batch_tensor_fn_kind(const BatchLaunch &l, int *threads);
 
-// tuned parameters of one launch class; wpc 0 = resident capacity
+// tuned parameters of one launch class; wpc 0 = resident capacity;
+// shape -1 = chosen from the region sizes
 struct Tune {
-  int zchunk, pt, wpc;
+  int zchunk, pt, wpc, shape = -1;
 };
 using TuneMap = std::map<std::string, Tune>;
 

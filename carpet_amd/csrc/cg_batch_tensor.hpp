@@ -1,14 +1,22 @@
 // cg_batch_tensor.hpp -- batched tensor prolongation kernels (grid and
-// persistent modes), one launch template per line kind; instantiated in
-// cg_batch_{vlag,clag,eno}.hip so the kinds compile in parallel.
+// persistent modes), one launch template per line kind and tile shape;
+// instantiated in cg_batch_{vlag,clag,eno}_{w,n}.hip so they compile in
+// parallel.
 #pragma once
 
 #include "cg_batch_dev.hpp"
 
 namespace cg {
 
+// 512-thread workgroups of orders up to 5 are held to 128 VGPRs: two
+// workgroups (4 waves per SIMD) per CU instead of one
+template <class L, int NTL, int CX, int CY, int IPT> constexpr int persist_waves() {
+  return (2 * CX * CY / IPT >= 512 && L::R <= 6) ? 4 : 1;
+}
+
 template <class L, int NTL, int CX, int CY, int IPT, int XQ, int PR>
-__global__ __launch_bounds__(2 * CX * CY / IPT) void k_persist_tensor(
+__global__ __launch_bounds__(2 * CX * CY / IPT)
+__attribute__((amdgpu_waves_per_eu(persist_waves<L, NTL, CX, CY, IPT>()))) void k_persist_tensor(
     const TensorArgs *__restrict__ descs, const Item *__restrict__ items,
     const int *__restrict__ seg, int *counter) {
   __shared__ __align__(16) double smem[tile_lds<L, NTL, CX, CY, XQ>()];
@@ -35,11 +43,11 @@ __global__ __launch_bounds__(2 * CX * CY / IPT) void k_grid_tensor(
   tensor_tile<L, NTL, CX, CY, IPT, PR, 1, XQ, pipe_fits<L, NTL, CX, CY, XQ>()>(descs[w.desc], (int)w.tile, smem);
 }
 
-template <class L, int NTL, int PR>
+template <class L, int NTL, int PR, int SH>
 int batch_tensor_go(const BatchLaunch &l, const TensorArgs *td, const Item *items,
                     const int *seg, int *ctr, hipStream_t st) {
-  using T = TileShape<NTL>;
-  const dim3 block(tile_threads<NTL>());
+  using T = TileShape<NTL, SH>;
+  const dim3 block(tile_threads<NTL, SH>());
   if (l.wpc < 0)
     hipLaunchKernelGGL((k_grid_tensor<L, NTL, T::CX, T::CY, T::IPT, T::XQ, PR>), dim3(l.grid), block, 0,
                        st, td, items, seg);
@@ -49,7 +57,7 @@ int batch_tensor_go(const BatchLaunch &l, const TensorArgs *td, const Item *item
   return 0;
 }
 
-template <int KIND>
+template <int KIND, int SH>
 int batch_tensor_kind(const BatchLaunch &l, const TensorArgs *td, const Item *items,
                       const int *seg, int *ctr, hipStream_t st) {
   return with_order<KIND>(l.order, [&](auto tag) -> int {
@@ -57,9 +65,9 @@ int batch_tensor_kind(const BatchLaunch &l, const TensorArgs *td, const Item *it
     auto by_ntl = [&](auto prc) -> int {
       constexpr int PR = decltype(prc)::value;
       switch (l.ntl) {
-      case 1: return batch_tensor_go<L, 1, PR>(l, td, items, seg, ctr, st);
-      case 2: return batch_tensor_go<L, 2, PR>(l, td, items, seg, ctr, st);
-      case 3: return batch_tensor_go<L, 3, PR>(l, td, items, seg, ctr, st);
+      case 1: return batch_tensor_go<L, 1, PR, SH>(l, td, items, seg, ctr, st);
+      case 2: return batch_tensor_go<L, 2, PR, SH>(l, td, items, seg, ctr, st);
+      case 3: return batch_tensor_go<L, 3, PR, SH>(l, td, items, seg, ctr, st);
       }
       return -1;
     };
@@ -76,15 +84,15 @@ int batch_tensor_kind(const BatchLaunch &l, const TensorArgs *td, const Item *it
 }
 
 // persistent kernel of a launch class (for the resident-capacity query)
-template <int KIND> const void *batch_tensor_fn_kind(const BatchLaunch &l, int *threads) {
+template <int KIND, int SH> const void *batch_tensor_fn_kind(const BatchLaunch &l, int *threads) {
   const void *fn = nullptr;
   with_order<KIND>(l.order, [&](auto tag) -> int {
     using L = decltype(tag);
     auto pick = [&](auto ntlc, auto prc) {
       constexpr int NTL = decltype(ntlc)::value;
       constexpr int PR = decltype(prc)::value;
-      using T = TileShape<NTL>;
-      *threads = tile_threads<NTL>();
+      using T = TileShape<NTL, SH>;
+      *threads = tile_threads<NTL, SH>();
       fn = reinterpret_cast<const void *>(&k_persist_tensor<L, NTL, T::CX, T::CY, T::IPT, T::XQ, PR>);
     };
     auto by_prec = [&](auto ntlc) {

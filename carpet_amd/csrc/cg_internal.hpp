@@ -103,8 +103,13 @@ struct PointPlan {
 };
 int plan_copy(const Arr &src, const Arr &dst, const Box &srcreg1,
               const Box &dreg, const cg_islab *slab, PointPlan &plan);
+// order: 1 (restrict_3d_rf2 / restrict_3d_cc_rf2), 3 or 5 (cell-centred
+// restrict_3d_cc_o3_rf2 / restrict_3d_cc_o5_rf2)
 int plan_restrict(bool cell, const Arr &src, const Arr &dst, const Box &reg,
-                  PointPlan &plan);
+                  PointPlan &plan, int order = 1);
+// CarpetLib's use_higher_order_restriction / restriction_order_space
+// parameters, as seen by the dispatch (process-wide, default off / 1)
+void restriction_setting(int *use_higher_order, int *order_space);
 int plan_interp(int ntl, const void *const *srcs, const double *times,
                 double t, const Arr &src, const Arr &dst, const Box &reg,
                 PointPlan &plan, bool eno = false);

@@ -8,6 +8,7 @@
 // outside the arrays it was given.
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 
@@ -352,8 +353,13 @@ int plan_copy(const Arr &src, const Arr &dst, const Box &srcreg1,
 
 // restrict_3d_rf2.cc:38-68 (vertex), restrict_3d_cc_rf2.cc:38-73 (cell)
 int plan_restrict(bool cell, const Arr &src, const Arr &dst, const Box &reg,
-                  PointPlan &pl) {
+                  PointPlan &pl, int order) {
   int e;
+  if (order != 1 && !(cell && (order == 3 || order == 5)))
+    return set_error(CG_ERR_ORDER, "There is no restriction stencil with restriction_order_space==%d",
+                     order);
+  // fine cells beyond the children that the stencil reads, per side
+  const int halo = order == 3 ? 1 : order == 5 ? 2 : 0;
   if ((e = check_array(src, "source")) || (e = check_array(dst, "destination")))
     return e;
   if ((e = check_region_stride(reg)))
@@ -379,8 +385,9 @@ int plan_restrict(bool cell, const Arr &src, const Arr &dst, const Box &reg,
       if (num % sb.str[d])
         return set_error(CG_ERR_ALIGN, "region is not aligned with the source cells (dir %d)", d);
       so[d] = num / sb.str[d];
-      if (so[d] < 0 || so[d] + 2 * pl.a.n[d] > src.ext[d])
-        return set_error(CG_ERR_CONTAIN, "Internal error: region extent is not contained in array extent");
+      if (so[d] - halo < 0 || so[d] + 2 * pl.a.n[d] + halo > src.ext[d])
+        return set_error(CG_ERR_CONTAIN, "Internal error: region extent is not contained in array "
+                                         "extent (restriction footprint, dir %d)", d);
     } else {
       if ((reg.lo[d] - sb.lo[d]) % sb.str[d])
         return set_error(CG_ERR_ALIGN, "region is not aligned with the source (dir %d)", d);
@@ -392,7 +399,10 @@ int plan_restrict(bool cell, const Arr &src, const Arr &dst, const Box &reg,
   }
   if ((!cell && !reg.contained_in(sb)) || !reg.contained_in(db))
     return set_error(CG_ERR_CONTAIN, "Internal error: region extent is not contained in array extent");
-  pl.op = cell ? PO_RESTRICT_CC : PO_RESTRICT_VC;
+  pl.op = !cell ? PO_RESTRICT_VC
+          : order == 3 ? PO_RESTRICT_CC_O3
+          : order == 5 ? PO_RESTRICT_CC_O5
+                       : PO_RESTRICT_CC;
   pl.a.src[0] = static_cast<const double *>(src.p) + src.at(so[0], so[1], so[2]);
   pl.a.sj = src.sj();
   pl.a.sk = src.sk();
@@ -401,6 +411,22 @@ int plan_restrict(bool cell, const Arr &src, const Arr &dst, const Box &reg,
   pl.a.dk = dst.sk();
   pl.dst_points = (long long)pl.a.n[0] * pl.a.n[1] * pl.a.n[2];
   pl.src_points = pl.dst_points * (cell ? 8 : 1);
+  return CG_OK;
+}
+
+static std::atomic<int> g_ho_restrict{0}, g_restrict_order{3};
+
+void restriction_setting(int *use_higher_order, int *order_space) {
+  *use_higher_order = g_ho_restrict.load(std::memory_order_relaxed);
+  *order_space = g_restrict_order.load(std::memory_order_relaxed);
+}
+
+extern "C" int cg_set_restriction(int use_higher_order_restriction, int restriction_order_space) {
+  if (restriction_order_space != 1 && restriction_order_space != 3 && restriction_order_space != 5)
+    return set_error(CG_ERR_ORDER, "There is no restriction stencil with restriction_order_space==%d",
+                     restriction_order_space);
+  g_ho_restrict.store(use_higher_order_restriction ? 1 : 0, std::memory_order_relaxed);
+  g_restrict_order.store(restriction_order_space, std::memory_order_relaxed);
   return CG_OK;
 }
 

@@ -142,7 +142,6 @@ __device__ __forceinline__ void tensor_tile(const TensorArgs &a, int b, double *
   static_assert(NQY * IPT == CY, "tile rows must divide among threads");
   static_assert(XQ == 1 || (XQ == 2 && CX % 2 == 0), "x-pass bases per thread: 1 or 2");
   constexpr int NB = PIPE ? 2 : 1;
-  static_assert(!PIPE || PF == 1, "the pipelined schedule loads one plane ahead");
   using LD = TensorLds<L, NTL, CX, CY, XQ, NB>;
   constexpr int WX = LD::WX;
   constexpr int WXS = LD::WXS; // LDS row stride of the window
@@ -152,7 +151,8 @@ __device__ __forceinline__ void tensor_tile(const TensorArgs &a, int b, double *
   // planes per unrolled round: ring slot, prefetch slot and buffer parity
   // are all compile-time constants within a round
   constexpr int U0 = (R % PF == 0) ? R : R * PF;
-  constexpr int U = (PIPE && U0 % 2) ? 2 * U0 : U0;
+  constexpr int U1 = (PIPE && U0 % 2) ? 2 * U0 : U0;
+  constexpr int U = (U1 % PF == 0) ? U1 : U1 * PF;
   static_assert(U % PF == 0 && U % R == 0 && U % NB == 0, "unroll must cover the slots");
 
   // buffer B of the window (storage type) and of the x-pass (arithmetic type)
@@ -246,17 +246,16 @@ __device__ __forceinline__ void tensor_tile(const TensorArgs &a, int b, double *
   auto xpass = [&](auto Bc) {
     auto A = Abuf(decltype(Bc)::value);
     auto X = Xbuf(decltype(Bc)::value);
+    // work items (time level, row, base group) spread over all threads
     if constexpr (XQ == 1) {
-      for (int e = tid; e < WY * CX; e += NT) {
-        const int r = e / CX, q = e - r * CX;
-#pragma unroll
-        for (int tl = 0; tl < NTL; ++tl) {
-          const TS *w = &A[tl][r][q - L::TMINALL];
-          const CT v0 = line_apply<L, 0, CT>([&](int t) { return w[L::TMIN[0] + t]; });
-          const CT v1 = line_apply<L, 1, CT>([&](int t) { return w[L::TMIN[1] + t]; });
-          X[tl][r][2 * q] = v0;
-          X[tl][r][2 * q + 1] = v1;
-        }
+      for (int e = tid; e < NTL * WY * CX; e += NT) {
+        const int tl = e / (WY * CX), e2 = e - tl * (WY * CX);
+        const int r = e2 / CX, q = e2 - r * CX;
+        const TS *w = &A[tl][r][q - L::TMINALL];
+        const CT v0 = line_apply<L, 0, CT>([&](int t) { return w[L::TMIN[0] + t]; });
+        const CT v1 = line_apply<L, 1, CT>([&](int t) { return w[L::TMIN[1] + t]; });
+        X[tl][r][2 * q] = v0;
+        X[tl][r][2 * q + 1] = v1;
       }
     } else {
       // bases q0, q0+1 from the window columns q0 .. q0+SPAN+1 (window
@@ -264,26 +263,25 @@ __device__ __forceinline__ void tensor_tile(const TensorArgs &a, int b, double *
       using P2 = typename Pair<TS>::T;
       using C2 = typename Pair<CT>::T;
       constexpr int NP = (LD::SPAN + 3) / 2;
-      for (int e = tid; e < WY * (CX / 2); e += NT) {
-        const int r = e / (CX / 2), q0 = 2 * (e - r * (CX / 2));
+      constexpr int NQ2 = WY * (CX / 2);
+      for (int e = tid; e < NTL * NQ2; e += NT) {
+        const int tl = e / NQ2, e2 = e - tl * NQ2;
+        const int r = e2 / (CX / 2), q0 = 2 * (e2 - r * (CX / 2));
+        TS v[2 * NP];
 #pragma unroll
-        for (int tl = 0; tl < NTL; ++tl) {
-          TS v[2 * NP];
-#pragma unroll
-          for (int m = 0; m < NP; ++m) {
-            const P2 p = *reinterpret_cast<const P2 *>(&A[tl][r][q0 + 2 * m]);
-            v[2 * m] = p.x;
-            v[2 * m + 1] = p.y;
-          }
-          constexpr int O0 = L::TMIN[0] - L::TMINALL, O1 = L::TMIN[1] - L::TMINALL;
-          C2 lo, hi;
-          lo.x = line_apply<L, 0, CT>([&](int t) { return v[O0 + t]; });
-          lo.y = line_apply<L, 1, CT>([&](int t) { return v[O1 + t]; });
-          hi.x = line_apply<L, 0, CT>([&](int t) { return v[1 + O0 + t]; });
-          hi.y = line_apply<L, 1, CT>([&](int t) { return v[1 + O1 + t]; });
-          *reinterpret_cast<C2 *>(&X[tl][r][2 * q0]) = lo;
-          *reinterpret_cast<C2 *>(&X[tl][r][2 * q0 + 2]) = hi;
+        for (int m = 0; m < NP; ++m) {
+          const P2 p = *reinterpret_cast<const P2 *>(&A[tl][r][q0 + 2 * m]);
+          v[2 * m] = p.x;
+          v[2 * m + 1] = p.y;
         }
+        constexpr int O0 = L::TMIN[0] - L::TMINALL, O1 = L::TMIN[1] - L::TMINALL;
+        C2 lo, hi;
+        lo.x = line_apply<L, 0, CT>([&](int t) { return v[O0 + t]; });
+        lo.y = line_apply<L, 1, CT>([&](int t) { return v[O1 + t]; });
+        hi.x = line_apply<L, 0, CT>([&](int t) { return v[1 + O0 + t]; });
+        hi.y = line_apply<L, 1, CT>([&](int t) { return v[1 + O1 + t]; });
+        *reinterpret_cast<C2 *>(&X[tl][r][2 * q0]) = lo;
+        *reinterpret_cast<C2 *>(&X[tl][r][2 * q0 + 2]) = hi;
       }
     }
   };
@@ -363,12 +361,13 @@ __device__ __forceinline__ void tensor_tile(const TensorArgs &a, int b, double *
       constexpr int u = decltype(Uc)::value;
       constexpr int B0 = u & 1, B1 = (u + 1) & 1;
       constexpr int J2 = ((u - 2) % R + R) % R; // ring slot of plane c-2
+      constexpr int SL = u % PF;                 // prefetch buffer of plane c
       if (c > c1 + 2)
         return false;
       if (c <= c1) {
-        store_window(std::integral_constant<int, 0>{}, std::integral_constant<int, B0>{});
-        if (c + 1 <= c1)
-          fetch(c + 1, std::integral_constant<int, 0>{});
+        store_window(std::integral_constant<int, SL>{}, std::integral_constant<int, B0>{});
+        if (c + PF <= c1)
+          fetch(c + PF, std::integral_constant<int, SL>{});
       }
       if (c - 1 >= c0 && c - 1 <= c1)
         xpass(std::integral_constant<int, B1>{});
@@ -416,6 +415,44 @@ __device__ __forceinline__ double point_value(const PointArgs &a, int i, int j, 
     r = r + w * p[J + K];
     r = r + w * p[1 + J + K];
     out = r;
+  } else if constexpr (OP == PO_RESTRICT_CC_O3) {
+    // restrict_3d_cc_o3_rf2.cc:146-211: 64 terms, (w_a w_b w_c)/4096 with
+    // w = {-1, 9, 9, -1}, a fastest, then b, then c; fine cells 2i-1..2i+2
+    const double *p = a.src[0] + 2 * i + a.sj * (2 * j) + a.sk * (2 * k);
+    constexpr double w[4] = {-1, 9, 9, -1};
+    double r = 0.0;
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+#pragma unroll
+        for (int aa = 0; aa < 4; ++aa) {
+          const double cf = w[aa] * w[b] * w[c] / 4096;
+          const double x = p[(aa - 1) + a.sj * (b - 1) + a.sk * (c - 1)];
+          r = (aa | b | c) == 0 ? cf * x : fmad(cf, x, r);
+        }
+    out = r;
+  } else if constexpr (OP == PO_RESTRICT_CC_O5) {
+    // restrict_3d_cc_o5_rf2.cc:36-74: 6-point sums, x inside y inside z,
+    // weights {3, -25, 150, 150, -25, 3}/256; fine cells 2i-2..2i+3
+    const double *p = a.src[0] + 2 * i + a.sj * (2 * j) + a.sk * (2 * k);
+    constexpr double w[6] = {3.0 / 256, -25.0 / 256, 150.0 / 256, 150.0 / 256, -25.0 / 256, 3.0 / 256};
+    double r2 = 0.0;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      double r1 = 0.0;
+#pragma unroll
+      for (int b = 0; b < 6; ++b) {
+        const double *q = p + a.sj * (b - 2) + a.sk * (c - 2);
+        double x = w[0] * q[-2];
+#pragma unroll
+        for (int aa = 1; aa < 6; ++aa)
+          x = fmad(w[aa], q[aa - 2], x);
+        r1 = b == 0 ? w[0] * x : fmad(w[b], x, r1);
+      }
+      r2 = c == 0 ? w[0] * r1 : fmad(w[c], r1, r2);
+    }
+    out = r2;
   } else if constexpr (OP == PO_INTERP) {
     const long long o = i + a.sj * j + a.sk * k;
     if (a.ntl == 3) {
@@ -510,15 +547,27 @@ __device__ __forceinline__ void tvd_cell(const PointArgs &a, int qx, int qy, int
 // the y halo of the x-pass and of the window loads to (8+span)/8.
 // XQ = 2: the x-pass reads and writes LDS in 16-byte pairs (two bases per
 // thread), a third fewer LDS reads than one base per thread.
-template <int NTL> struct TileShape;
-template <> struct TileShape<1> {
+// SH = 0 ("wide", 32 x 8 bases) suits large regions; SH = 1 ("narrow",
+// 16 x 16 bases) wastes fewer lanes on regions of a few dozen points per
+// axis and has a smaller x halo per fine point.
+template <int NTL, int SH = 0> struct TileShape;
+template <> struct TileShape<1, 0> {
   static constexpr int CX = 32, CY = 8, IPT = 2, XQ = 2;
 };
-template <> struct TileShape<2> {
+template <> struct TileShape<2, 0> {
   static constexpr int CX = 32, CY = 8, IPT = 1, XQ = 2;
 };
-template <> struct TileShape<3> {
+template <> struct TileShape<3, 0> {
   static constexpr int CX = 32, CY = 8, IPT = 1, XQ = 2;
+};
+template <> struct TileShape<1, 1> {
+  static constexpr int CX = 16, CY = 16, IPT = 2, XQ = 2;
+};
+template <> struct TileShape<2, 1> {
+  static constexpr int CX = 16, CY = 16, IPT = 1, XQ = 2;
+};
+template <> struct TileShape<3, 1> {
+  static constexpr int CX = 16, CY = 16, IPT = 1, XQ = 2;
 };
 // the pipelined schedule (double-buffered window and x-pass results) where
 // its LDS fits the 64 KB of a static allocation, else two barriers per plane
@@ -529,15 +578,27 @@ template <class L, int NTL, int CX, int CY, int XQ> constexpr int tile_lds() {
   return TensorLds<L, NTL, CX, CY, XQ, pipe_fits<L, NTL, CX, CY, XQ>() ? 2 : 1>::TOTAL;
 }
 
-template <int NTL> constexpr int tile_threads() {
-  return 2 * TileShape<NTL>::CX * TileShape<NTL>::CY / TileShape<NTL>::IPT;
+template <int NTL, int SH = 0> constexpr int tile_threads() {
+  using T = TileShape<NTL, SH>;
+  return 2 * T::CX * T::CY / T::IPT;
 }
 // host view of the tile shape for ntl time levels
-inline void tile_shape(int ntl, int *cx, int *cy, int *threads) {
-  switch (ntl) {
-  case 1: *cx = TileShape<1>::CX; *cy = TileShape<1>::CY; *threads = tile_threads<1>(); break;
-  case 2: *cx = TileShape<2>::CX; *cy = TileShape<2>::CY; *threads = tile_threads<2>(); break;
-  default: *cx = TileShape<3>::CX; *cy = TileShape<3>::CY; *threads = tile_threads<3>(); break;
+inline void tile_shape(int ntl, int *cx, int *cy, int *threads, int sh = 0) {
+  auto get = [&](auto ntlc, auto shc) {
+    constexpr int N = decltype(ntlc)::value, S = decltype(shc)::value;
+    *cx = TileShape<N, S>::CX;
+    *cy = TileShape<N, S>::CY;
+    *threads = tile_threads<N, S>();
+  };
+  using I1 = std::integral_constant<int, 1>;
+  using I2 = std::integral_constant<int, 2>;
+  using I3 = std::integral_constant<int, 3>;
+  using S0 = std::integral_constant<int, 0>;
+  using S1 = std::integral_constant<int, 1>;
+  if (sh == 0) {
+    if (ntl == 1) get(I1{}, S0{}); else if (ntl == 2) get(I2{}, S0{}); else get(I3{}, S0{});
+  } else {
+    if (ntl == 1) get(I1{}, S1{}); else if (ntl == 2) get(I2{}, S1{}); else get(I3{}, S1{});
   }
 }
 

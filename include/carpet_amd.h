@@ -249,6 +249,32 @@ int cg_interpolate_3d_3tl_f64(const double *src1, double t1,
                               const int dstbbox[9], const int srcregbbox[9],
                               const int dstregbbox[9], cg_stream_t stream);
 
+/* restrict_3d_cc_o3_rf2<double> / restrict_3d_cc_o5_rf2<double>:
+   CarpetLib/src/restrict_3d_cc_o3_rf2.cc, restrict_3d_cc_o5_rf2.cc -- the
+   cubic (quintic) through 4 (6) fine cells per axis evaluated at the coarse
+   cell centre (not conservative); the footprint, fine cells 2i-1..2i+2
+   (2i-2..2i+3) per axis, must lie in the source array or CG_ERR_CONTAIN
+   is returned */
+int cg_restrict_3d_cc_o3_rf2_f64(const double *src, const int srcpadext[3],
+                                 const int srcext[3], double *dst,
+                                 const int dstpadext[3], const int dstext[3],
+                                 const int srcbbox[9], const int dstbbox[9],
+                                 const int srcregbbox[9], const int dstregbbox[9],
+                                 cg_stream_t stream);
+int cg_restrict_3d_cc_o5_rf2_f64(const double *src, const int srcpadext[3],
+                                 const int srcext[3], double *dst,
+                                 const int dstpadext[3], const int dstext[3],
+                                 const int srcbbox[9], const int dstbbox[9],
+                                 const int srcregbbox[9], const int dstregbbox[9],
+                                 cg_stream_t stream);
+/* CarpetLib's parameters use_higher_order_restriction and
+   restriction_order_space (CarpetLib/param.ccl) as the transfer dispatch
+   sees them: cell-centred restrictions of groups whose operator is not
+   ENO/WENO use the o3 or o5 stencil when enabled (CarpetLib/src/data.cc,
+   transfer_restrict).  Process-wide; defaults as param.ccl: off, order 3;
+   an order other than 1, 3, 5 returns CG_ERR_ORDER */
+int cg_set_restriction(int use_higher_order_restriction, int restriction_order_space);
+
 /* interpolate_eno_3d_3tl<double>: CarpetLib/src/interpolate_eno_3d_3tl.cc:
    88-164 -- quadratic in time; a point where that makes a new extremum
    falls back to linear interpolation between the two levels whose interval
@@ -295,7 +321,7 @@ int cg_transfer_from_mixed(const cg_gdata *dst, const cg_gdata *const *srcs,
    times[] their times.  Where the reference prolongates every time level
    into a temporary and then interpolates in time, this call runs one fused
    kernel that keeps the prolongated values in registers; the arithmetic is
-   the same.  use_higher_order_restriction is not supported (CG_ERR_ORDER). */
+   the same.  Cell-centred restrictions follow cg_set_restriction. */
 int cg_transfer_from(const cg_gdata *dst, const cg_gdata *const *srcs,
                      const double *times, int ntimes,
                      const int dstregbbox[9], const int srcregbbox[9],
@@ -341,13 +367,15 @@ int cg_batch_launch_graph(cg_batch *b, cg_stream_t stream);
    pointwise items: 2048..65536 points) in the class's default mode
    (persistent grid for tensor tiles, one workgroup per item for pointwise
    runs), then the best size in the other modes (one workgroup per item,
-   persistent grids of resident capacity and of 2 workgroups per CU), with
-   HIP events, and keeps the fastest per launch class.
+   persistent grids of resident capacity and of 2 workgroups per CU) and,
+   for tensor tiles, the other tile shape (32x8 or 16x16 coarse bases),
+   with HIP events, and keeps the fastest per launch class.
    Choices are keyed on (kernel class, order, time levels, precision, log2
    of the median region size, log2 of the region count) and kept in the
    text file cache_path (may be NULL; lines "<key>|<zchunk> <points>
-   <workgroups per CU: -1 one per item, 0 resident capacity>"): classes the
-   file names take its parameters, the others are searched.  The
+   <workgroups per CU: -1 one per item, 0 resident capacity> <tile shape:
+   0 32x8, 1 16x16, -1 by region size>"): classes the file names take its
+   parameters, the others are searched.  The
    search executes each launch (one phase's class of transfers) reps times
    on its own, outside phase order, so call it before the step whose
    results are kept, or on scratch data.  Drops a captured graph. */

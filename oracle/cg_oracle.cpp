@@ -830,6 +830,98 @@ int orc_restrict_3d_cc_rf2(const double *src, const int *spad, const int *sext,
   return CG_OK;
 }
 
+// Higher-order cell-centred restriction (use_higher_order_restriction,
+// data.cc:1064-1094): restrict_3d_cc_o3_rf2.cc:33-212 evaluates at the coarse
+// cell centre the cubic through 4 fine cells per axis, 64 terms with weights
+// (w_i w_j w_k)/4096, w = {-1, 9, 9, -1}, listed with i fastest, then j,
+// then k (:146-211); restrict_3d_cc_o5_rf2.cc:36-74 nests 6-point quintic
+// sums, x inside y inside z, weights {3, -25, 150, 150, -25, 3}/256.  Both
+// fit polynomials through the fine cell centres, so they are not
+// conservative.  Canonical order: res = c0 * x0, then fma(c, x, res) term by
+// term (o3 in the reference's listing order, o5 in its nesting).  The
+// footprint -- fine cells 2i-1..2i+2 (o3) or 2i-2..2i+3 (o5) -- must lie in
+// the source array (the reference's expand(1,1) check is one cell short of
+// the o5 stencil).
+int orc_restrict_3d_cc_ho_rf2(int order, const double *src, const int *spad, const int *sext,
+                              double *dst, const int *dpad, const int *dext, const int *sb,
+                              const int *db, const int *srb, const int *drb) {
+  (void)srb;
+  if (order != 3 && order != 5)
+    return CG_ERR_ORDER;
+  const box3 s = mkbox(sb), dbx = mkbox(db), r = mkbox(drb);
+  int e;
+  if ((e = check_array(spad, sext, s)) || (e = check_array(dpad, dext, dbx)))
+    return e;
+  for (int d = 0; d < 3; ++d) {
+    if (r.str[d] <= 0)
+      return CG_ERR_ARG;
+    if (s.str[d] >= r.str[d] || dbx.str[d] != r.str[d])
+      return CG_ERR_STRIDE;
+    if (2 * s.str[d] != dbx.str[d])
+      return CG_ERR_REFFACT;
+    if (s.str[d] % 2)
+      return CG_ERR_ALIGN;
+  }
+  if (empty(r))
+    return CG_ERR_EMPTY;
+  const int h = order == 3 ? 1 : 2;
+  int n[3], so[3], dof[3];
+  for (int d = 0; d < 3; ++d) {
+    if ((r.lo[d] - s.lo[d] - s.str[d] / 2) % s.str[d] ||
+        (r.lo[d] - dbx.lo[d]) % dbx.str[d])
+      return CG_ERR_ALIGN;
+    n[d] = shape(r, d) / r.str[d];
+    so[d] = (r.lo[d] - s.lo[d] - s.str[d] / 2) / s.str[d];
+    dof[d] = (r.lo[d] - dbx.lo[d]) / dbx.str[d];
+    if (so[d] - h < 0 || so[d] + 2 * n[d] - 1 + h >= sext[d])
+      return CG_ERR_CONTAIN;
+  }
+  if (!contained(r, dbx))
+    return CG_ERR_CONTAIN;
+#pragma omp parallel for collapse(2) if (g_use_openmp)
+  for (int k = 0; k < n[2]; ++k)
+    for (int j = 0; j < n[1]; ++j)
+      for (int i = 0; i < n[0]; ++i) {
+        auto S = [&](int a, int b, int c) {
+          return src[idx3(so[0] + 2 * i + a, so[1] + 2 * j + b, so[2] + 2 * k + c, spad)];
+        };
+        double res = 0;
+        if (order == 3) {
+          const double w[4] = {-1, 9, 9, -1};
+          bool first = true;
+          for (int c = 0; c < 4; ++c)
+            for (int b = 0; b < 4; ++b)
+              for (int a = 0; a < 4; ++a) {
+                const double cf = w[a] * w[b] * w[c] / 4096;
+                const double x = S(a - 1, b - 1, c - 1);
+                res = first ? cf * x : std::fma(cf, x, res);
+                first = false;
+              }
+        } else {
+          const double w[6] = {3.0 / 256, -25.0 / 256, 150.0 / 256, 150.0 / 256, -25.0 / 256, 3.0 / 256};
+          double r2[6];
+          for (int c = 0; c < 6; ++c) {
+            double r1[6];
+            for (int b = 0; b < 6; ++b) {
+              double x = w[0] * S(-2, b - 2, c - 2);
+              for (int a = 1; a < 6; ++a)
+                x = std::fma(w[a], S(a - 2, b - 2, c - 2), x);
+              r1[b] = x;
+            }
+            double y = w[0] * r1[0];
+            for (int b = 1; b < 6; ++b)
+              y = std::fma(w[b], r1[b], y);
+            r2[c] = y;
+          }
+          res = w[0] * r2[0];
+          for (int c = 1; c < 6; ++c)
+            res = std::fma(w[c], r2[c], res);
+        }
+        dst[idx3(dof[0] + i, dof[1] + j, dof[2] + k, dpad)] = res;
+      }
+  return CG_OK;
+}
+
 // Time-interpolation weights, interpolate_3d_2tl.cc:97-107 and
 // interpolate_3d_3tl.cc:98-115; ntl in {1,2,3}.  Returns an error code for
 // equal times or extrapolation, exactly as the reference checks.

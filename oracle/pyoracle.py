@@ -54,6 +54,7 @@ _decl("orc_prolongate_3d_cc_eno_rf2", C.c_int, C.c_int, *_OP)
 _decl("orc_prolongate_3d_cc_tvd", C.c_int, *_OP)
 _decl("orc_restrict_3d_rf2", C.c_int, *_OP)
 _decl("orc_restrict_3d_cc_rf2", C.c_int, *_OP)
+_decl("orc_restrict_3d_cc_ho_rf2", C.c_int, C.c_int, *_OP)
 _decl("orc_time_factors", C.c_int, C.c_int, _dp, C.c_double, _dp)
 _decl("orc_interpolate_3d", C.c_int, C.c_int, C.POINTER(_dp), _dp, _ip, _ip, _dp, C.c_double,
       _ip, _ip, _ip, _ip, _ip, _ip)
@@ -142,6 +143,11 @@ def restrict_3d_rf2(src, dst, dstreg) -> int:
 
 def restrict_3d_cc_rf2(src, dst, dstreg) -> int:
     return _lib.orc_restrict_3d_cc_rf2(*_args(src, dst, dstreg, dstreg))
+
+
+def restrict_3d_cc_ho_rf2(order, src, dst, dstreg) -> int:
+    """restrict_3d_cc_o3_rf2 (order 3) / restrict_3d_cc_o5_rf2 (order 5)"""
+    return _lib.orc_restrict_3d_cc_ho_rf2(order, *_args(src, dst, dstreg, dstreg))
 
 
 def time_factors(times, t):

@@ -261,3 +261,100 @@ def test_transfer_time_interpolation(kind, order, ntl):
             cg.synchronize()
             _check(exp, bdst, f"batched transfer {kind} case {c}")
             b.close()
+
+
+def _cc_restrict_case(rng, halo, n):
+    """Cell-centred restriction geometry: the source holds the region's fine
+    footprint plus `halo` cells per side (and 0..2 more)."""
+    sf, sc = 2, 4
+    rlo, rup, slo, sup, dlo, dup = [], [], [], [], [], []
+    for d in range(3):
+        C0 = int(rng.integers(-4, 5))
+        rlo.append(sc // 2 + C0 * sc)
+        rup.append(sc // 2 + (C0 + n[d] - 1) * sc)
+        F0 = 2 * C0 - halo - int(rng.integers(0, 3))
+        slo.append(sf // 2 + F0 * sf)
+        sup.append(sf // 2 + (2 * (C0 + n[d]) - 1 + halo + int(rng.integers(0, 3))) * sf)
+        dlo.append(rlo[-1] - int(rng.integers(0, 2)) * sc)
+        dup.append(rup[-1] + int(rng.integers(0, 2)) * sc)
+    return ((tuple(slo), tuple(sup), (sf,) * 3), (tuple(dlo), tuple(dup), (sc,) * 3),
+            (tuple(rlo), tuple(rup), (sc,) * 3))
+
+
+@pytest.mark.parametrize("order", [3, 5])
+def test_restrict_cc_ho_parity(order):
+    """restrict_3d_cc_o3_rf2 / restrict_3d_cc_o5_rf2 (bit-exact)."""
+    rng = np.random.default_rng(70 + order)
+    halo = (order - 1) // 2
+    fn = cg.restrict_3d_cc_o3_rf2 if order == 3 else cg.restrict_3d_cc_o5_rf2
+    for c in range(6):
+        print(f"restrict_cc_o{order} #{c}", flush=True)
+        n = (1, 1, 1) if c == 0 else (70, 3, 41) if c == 1 else tuple(
+            int(v) for v in rng.integers(1, 40, size=3))
+        sb, db, rb = _cc_restrict_case(rng, halo, n)
+        src = _src_host(rng, sb, ("random", "smooth", "mixed")[c % 3])
+        dst = host_array(db, pad_of(sizes(db), rng))
+        exp = orc.HostArr(db, dst.data.copy())
+        assert orc.restrict_3d_cc_ho_rf2(order, src, exp, rb) == 0
+        dsrc, ddst = _dev(src), _dev(dst)
+        fn(dsrc, ddst, cg.Box(*rb))
+        _check(exp, ddst, f"restrict_cc_o{order} case {c} reg={rb}")
+
+
+@pytest.mark.parametrize("order", [3, 5])
+def test_restrict_cc_ho_needs_footprint(order):
+    """A source without the stencil's extra fine cells is refused before any
+    launch and the destination is left as it was."""
+    rng = np.random.default_rng(80 + order)
+    fn = cg.restrict_3d_cc_o3_rf2 if order == 3 else cg.restrict_3d_cc_o5_rf2
+    sb, db, rb = _cc_restrict_case(rng, 0, (5, 4, 3))
+    lo, up, st = sb
+    # trim the source to exactly the order-1 footprint on the lower x side
+    sb = ((rb[0][0] - 1, lo[1], lo[2]), up, st)
+    src = _src_host(rng, sb, "random")
+    dst = host_array(db, pad_of(sizes(db), rng))
+    dsrc, ddst = _dev(src), _dev(dst)
+    with pytest.raises(cg.CarpetError) as e:
+        fn(dsrc, ddst, cg.Box(*rb))
+    assert e.value.code == -6
+    _check(orc.HostArr(db, dst.data.copy()), ddst, "untouched destination")
+
+
+@pytest.mark.parametrize("order", [3, 5])
+def test_transfer_higher_order_restriction(order):
+    """cg_transfer_from on a finer cell-centred source follows
+    use_higher_order_restriction / restriction_order_space for Lagrange groups
+    and keeps the conservative stencil for ENO groups (data.cc transfer_restrict);
+    also through a batch."""
+    rng = np.random.default_rng(90 + order)
+    _, op_lag, cent = KIND_IDS["clag"]
+    op_eno = KIND_IDS["eno"][1]
+    halo = (order - 1) // 2
+    cg.set_restriction(True, order)
+    try:
+        for c, op in enumerate((op_lag, op_eno, op_lag)):
+            print(f"transfer restrict o{order} op{op} #{c}", flush=True)
+            sb, db, rb = _cc_restrict_case(rng, halo, tuple(int(v) for v in rng.integers(2, 30, size=3)))
+            src = _src_host(rng, sb, "random")
+            dst = host_array(db, pad_of(sizes(db), rng))
+            exp = orc.HostArr(db, dst.data.copy())
+            if op == op_lag:
+                assert orc.restrict_3d_cc_ho_rf2(order, src, exp, rb) == 0
+            else:
+                assert orc.restrict_3d_cc_rf2(src, exp, rb) == 0
+            dsrc, ddst = _dev(src), _dev(dst)
+            cg.transfer_from(ddst, [dsrc], [0.0], cg.Box(*rb), cg.Box(*rb), 0.0, op, cent, 3, 0)
+            _check(exp, ddst, f"transfer restrict o{order} op{op} case {c}")
+            if c == 2:
+                bdst = _dev(dst)
+                b = cg.Batch()
+                b.add(0, bdst, [dsrc], [0.0], cg.Box(*rb), cg.Box(*rb), 0.0, op, cent, 3, 0)
+                b.finalize()
+                b.run()
+                cg.synchronize()
+                _check(exp, bdst, f"batched transfer restrict o{order}")
+                b.close()
+    finally:
+        cg.set_restriction(False, 3)
+    with pytest.raises(cg.CarpetError):
+        cg.set_restriction(True, 4)

@@ -544,3 +544,88 @@ def test_prolongate_time_eno_groups_three_levels(kind, order):
     exp = host_array(db, sizes(db))
     assert orc.interpolate_eno_3d_3tl(tmps, times, exp, 0.4, rb) == 0
     assert np.array_equal(dst.data, exp.data, equal_nan=True)
+
+
+# ---- higher-order cell-centred restriction (restrict_3d_cc_o3/o5_rf2) ------
+def _cc_restrict_case(rng, order, n, data_fn):
+    """Fine cells of width 2 (centres at odd integers), coarse cells of
+    width 4; region n coarse cells with the o3/o5 halo inside the source."""
+    h = 1 if order == 3 else 2
+    C0 = [int(v) for v in rng.integers(-3, 4, 3)]
+    rlo = tuple(2 + 4 * c for c in C0)
+    rup = tuple(l + 4 * (k - 1) for l, k in zip(rlo, n))
+    # fine cells 2i-h .. 2i+1+h around coarse cell i; fine cell f centred at 1 + 2 f
+    slo = tuple(l - 1 - 2 * h - 2 * int(rng.integers(0, 2)) for l in rlo)
+    sup = tuple(u + 1 + 2 * h + 2 * int(rng.integers(0, 2)) for u in rup)
+    sb = (slo, sup, (2, 2, 2))
+    rb = (rlo, rup, (4, 4, 4))
+    db = (tuple(l - 4 for l in rlo), tuple(u + 4 for u in rup), (4, 4, 4))
+    src = host_array(sb, sizes(sb))
+    ext = sizes(sb)
+    xs = [slo[d] + 2 * np.arange(ext[d]) for d in range(3)]
+    X, Y, Z = np.meshgrid(xs[0], xs[1], xs[2], indexing="ij")
+    src.view()[...] = np.transpose(data_fn(X, Y, Z), (2, 1, 0))
+    return sb, db, rb, src
+
+
+@pytest.mark.parametrize("order", [3, 5])
+def test_cc_restrict_ho_reproduces_polynomials(order):
+    """The o3 (o5) restriction evaluates the tensor-product polynomial of
+    degree 3 (5) per axis through the fine cell centres at the coarse
+    centre: monomials up to that degree per variable come out exactly (to
+    rounding); one degree higher does not."""
+    rng = np.random.default_rng(60 + order)
+    for deg in ((0, 0, 0), (1, 0, 0), (0, 2, 1), (order, 1, 0), (order, order, order)):
+        sc = 1.0 / 64
+        fn = lambda X, Y, Z: (X * sc) ** deg[0] * (Y * sc) ** deg[1] * (Z * sc) ** deg[2]
+        sb, db, rb, src = _cc_restrict_case(rng, order, (3, 4, 2), fn)
+        dst = host_array(db, sizes(db))
+        assert orc.restrict_3d_cc_ho_rf2(order, src, dst, rb) == 0
+        e = sizes(rb)
+        cs = [rb[0][d] + 4 * np.arange(e[d]) for d in range(3)]
+        CX, CY, CZ = np.meshgrid(cs[0], cs[1], cs[2], indexing="ij")
+        exp = np.transpose(fn(CX, CY, CZ), (2, 1, 0))
+        o = [(rb[0][d] - db[0][d]) // 4 for d in range(3)]
+        got = dst.view()[o[2]:o[2] + e[2], o[1]:o[1] + e[1], o[0]:o[0] + e[0]]
+        assert np.allclose(got, exp, rtol=0, atol=1e-13), deg
+    # degree order+1 along x is not reproduced
+    fn = lambda X, Y, Z: (X * 0.01) ** (order + 1)
+    sb, db, rb, src = _cc_restrict_case(rng, order, (3, 2, 2), fn)
+    dst = host_array(db, sizes(db))
+    assert orc.restrict_3d_cc_ho_rf2(order, src, dst, rb) == 0
+    e = sizes(rb)
+    o = [(rb[0][d] - db[0][d]) // 4 for d in range(3)]
+    got = dst.view()[o[2]:o[2] + e[2], o[1]:o[1] + e[1], o[0]:o[0] + e[0]]
+    cs = rb[0][0] + 4 * np.arange(e[0])
+    assert not np.allclose(got[0, 0, :], (cs * 0.01) ** (order + 1), rtol=0, atol=1e-15)
+
+
+@pytest.mark.parametrize("order", [3, 5])
+def test_cc_restrict_ho_brute_force_and_footprint(order):
+    """Against the tensor product of the 1-D weights summed in any order
+    (to 1e-15 relative), and the exact footprint: one fine cell short on any
+    side is CG_ERR_CONTAIN."""
+    rng = np.random.default_rng(70 + order)
+    sb, db, rb, src = _cc_restrict_case(rng, order, (4, 3, 5), lambda X, Y, Z: np.sin(0.1 * X + 0.2 * Y - 0.3 * Z))
+    dst = host_array(db, sizes(db))
+    assert orc.restrict_3d_cc_ho_rf2(order, src, dst, rb) == 0
+    w = np.array([-1, 9, 9, -1]) / 16 if order == 3 else np.array([3, -25, 150, 150, -25, 3]) / 256
+    h = 1 if order == 3 else 2
+    e = sizes(rb)
+    f = src.view()
+    so = [(rb[0][d] - 1 - sb[0][d]) // 2 for d in range(3)]  # first child of coarse cell 0
+    for (i, j, k) in ((0, 0, 0), (e[0] - 1, e[1] - 1, e[2] - 1), (1, 2, 3)):
+        blk = f[so[2] + 2 * k - h:so[2] + 2 * k + 2 + h, so[1] + 2 * j - h:so[1] + 2 * j + 2 + h,
+                so[0] + 2 * i - h:so[0] + 2 * i + 2 + h]
+        exp = np.einsum("k,j,i,kji->", w, w, w, blk)
+        o = [(rb[0][d] - db[0][d]) // 4 for d in range(3)]
+        assert dst.view()[o[2] + k, o[1] + j, o[0] + i] == pytest.approx(exp, rel=1e-14, abs=1e-15)
+    # the exact footprint suffices; one fine cell less on the low x side fails
+    for cut, rc_exp in ((0, 0), (1, -6)):
+        lo = (rb[0][0] - 1 - 2 * h + 2 * cut,) + tuple(rb[0][d] - 1 - 2 * h for d in (1, 2))
+        up = tuple(rb[1][d] + 1 + 2 * h for d in range(3))
+        tight = (lo, up, (2, 2, 2))
+        src2 = host_array(tight, sizes(tight))
+        src2.view()[...] = 0.0
+        dst2 = host_array(db, sizes(db))
+        assert orc.restrict_3d_cc_ho_rf2(order, src2, dst2, rb) == rc_exp
