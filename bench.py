@@ -271,7 +271,7 @@ def run_config3(steps: int, warmup: int, n_var: int):
     return out
 
 
-def run_config5(steps: int, warmup: int):
+def run_config5(steps: int, warmup: int, tune_cache=None):
     """Mixed precision vs batched FP64 on the config-1 and config-3 shapes
     (vertex order 5 + 3 time levels), for both mixed modes (FP64 arithmetic
     on FP32 storage; compensated FP32 sums): times, speedups, and the
@@ -295,6 +295,9 @@ def run_config5(steps: int, warmup: int):
         sl = np.s_[o[2]:o[2] + e[2], o[1]:o[1] + e[1], o[0]:o[0] + e[0]]
         b64 = cfg.batch(src64, dst64, cg.PREC_F64)
         b64.finalize()
+        if tune_cache:
+            b64.autotune(st, reps=3, cache_path=tune_cache)
+            print(b64.describe(), file=sys.stderr, flush=True)
         ms64 = _time_graph(b64, st, steps, warmup)
         s64 = b64.stats()
         ref = [dst64[v].download()[sl] for v in range(cfg.n_var)]
@@ -308,6 +311,9 @@ def run_config5(steps: int, warmup: int):
         for name, prec in (("mixed_fp64_arith", cg.PREC_MIXED), ("mixed_dot2", cg.PREC_MIXED_DOT2)):
             bm = cfg.batch(src32, dst32, prec)
             bm.finalize()
+            if tune_cache:
+                bm.autotune(st, reps=3, cache_path=tune_cache)
+            print(bm.describe(), file=sys.stderr, flush=True)
             ms = _time_graph(bm, st, steps, warmup)
             sm = bm.stats()
             got = [dst32[v].download()[sl] for v in range(cfg.n_var)]
@@ -376,7 +382,7 @@ def main(argv=None):
         print(json.dumps(line))
         return 0
     if args.config == 5:
-        res = run_config5(args.steps, args.warmup)
+        res = run_config5(args.steps, args.warmup, tune_cache=args.tune_cache)
         c3 = res["cfg3"]
         mx = c3["mixed_fp64_arith"]
         pts = c3["fine_points"]
@@ -386,8 +392,9 @@ def main(argv=None):
                 "data": "synthetic",
                 "config": {"workload": "mixed precision: vertex prolongate_3d_rf2<5> + interpolate_3d_3tl at "
                                        "t=1/3, batched; sources and destination stored in FP32, time "
-                                       "interpolation in FP64; stencil sums in FP64 (mixed_fp64_arith, the "
-                                       "value) or compensated FP32 (mixed_dot2); config-1 shape (33^3 -> "
+                                       "interpolation in FP64; stencil sums in FP64 with the time levels "
+                                       "combined before the spatial passes (mixed_fp64_arith, the value) or "
+                                       "compensated FP32 per level (mixed_dot2); config-1 shape (33^3 -> "
                                        "64^3, 1 variable) and config-3 shape (193^3 -> 384^3, 8 variables); "
                                        "value = config-3 shape",
                            "destination_precision": "fp32", "shapes": res},

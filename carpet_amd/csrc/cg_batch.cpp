@@ -127,7 +127,8 @@ static int save_cache(const char *path, const TuneMap &m) {
     return set_error(CG_ERR_ARG, "cannot write tuner cache %s", path);
   f << "# carpet_amd batch tuner cache: <class key>|<zchunk> <points per item> "
        "<workgroups per CU: 0 resident capacity, -1 one per item> "
-       "<tile shape: 0 32x8, 1 16x16, -1 by region size>\n";
+       "<tile shape: 0 wide (32x8 coarse bases, 64x8 in the FP32-storage FP64 mode), 1 16x16, "
+       "-1 by region size>\n";
   for (const auto &kv : m)
     f << kv.first << "|" << kv.second.zchunk << " " << kv.second.pt << " " << kv.second.wpc << " "
       << kv.second.shape << "\n";
@@ -180,7 +181,7 @@ extern "C" int cg_batch_autotune(cg_batch *b, cg_stream_t stream, int reps,
       for (int z : {8, 16, 32, 64, 128, 0})
         cands.push_back({z, l.pt, l.wpc, l.shape});
     else
-      for (int p : {2048, 4096, 16384, 65536})
+      for (int p : {256, 512, 1024, 2048, 4096, 16384})
         cands.push_back({l.zchunk, p, l.wpc, -1});
     float best = 1e30f;
     Tune bestc = cands[0];

@@ -35,7 +35,12 @@ constexpr int PT_DEFAULT = 4096;
 constexpr int ZCHUNK_DEFAULT = 16;
 
 
-constexpr int PU = 8; // points per thread and round of the pointwise kernel
+// points per thread and round of the pointwise kernel: items of 256, 512,
+// 1024 points take one round of 1, 2, 4 points per thread, larger items
+// rounds of 8 (a plain copy streams fastest with one point per thread and
+// many small workgroups; stencils with several loads per point prefer more
+// independent points per thread)
+constexpr int pu_of(int pt) { return pt >= 2048 ? 8 : pt >= 1024 ? 4 : pt >= 512 ? 2 : 1; }
 
 // coarse cells [tile*pt, tile*pt + pt) of a TVD region (fx/fxy divide by
 // the parent-cell extents), each with all its children, NTL time levels
@@ -80,7 +85,7 @@ __device__ __forceinline__ void tvd_item(const PointArgs &a, unsigned tile, int 
 
 // points [tile*pt, tile*pt + pt) of a region, x fastest; for TVD the
 // items count the region's coarse parent cells
-template <int OP>
+template <int OP, int PU>
 __device__ __forceinline__ void point_item(const PointArgs &a, unsigned tile, int pt) {
   if constexpr (OP == PO_TVD) {
     if (a.ntl == 1)
@@ -121,7 +126,7 @@ __device__ __forceinline__ void point_item(const PointArgs &a, unsigned tile, in
   }
 }
 
-template <int OP>
+template <int OP, int PU>
 __global__ __launch_bounds__(256) void k_persist_point(
     const PointArgs *__restrict__ descs, const Item *__restrict__ items,
     const int *__restrict__ seg, int *counter, int pt) {
@@ -131,11 +136,11 @@ __global__ __launch_bounds__(256) void k_persist_point(
     if (it < 0)
       return;
     const Item w = items[it];
-    point_item<OP>(descs[w.desc], w.tile, pt);
+    point_item<OP, PU>(descs[w.desc], w.tile, pt);
   }
 }
 
-template <int OP>
+template <int OP, int PU>
 __global__ __launch_bounds__(256) void k_grid_point(const PointArgs *__restrict__ descs,
                                                     const Item *__restrict__ items,
                                                     const int *__restrict__ seg, int pt) {
@@ -143,8 +148,11 @@ __global__ __launch_bounds__(256) void k_grid_point(const PointArgs *__restrict_
   if (it < 0)
     return;
   const Item w = items[it];
-  point_item<OP>(descs[w.desc], w.tile, pt);
+  point_item<OP, PU>(descs[w.desc], w.tile, pt);
 }
+
+// TVD items run per coarse cell with their own unrolling: one instance
+template <int OP> constexpr int pu_inst(int pu) { return OP == PO_TVD ? 8 : pu; }
 
 __global__ void k_zero_counters(int *c, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -164,15 +172,25 @@ struct Group {
   }
 };
 
-const void *persist_point_fn(int op) {
+template <int OP> const void *persist_point_pu(int pu) {
+  switch (pu_inst<OP>(pu)) {
+  case 1: return reinterpret_cast<const void *>(&k_persist_point<OP, 1>);
+  case 2: return reinterpret_cast<const void *>(&k_persist_point<OP, 2>);
+  case 4: return reinterpret_cast<const void *>(&k_persist_point<OP, 4>);
+  }
+  return reinterpret_cast<const void *>(&k_persist_point<OP, 8>);
+}
+
+const void *persist_point_fn(int op, int pt) {
+  const int pu = pu_of(pt);
   switch (op) {
-  case PO_COPY: return reinterpret_cast<const void *>(&k_persist_point<PO_COPY>);
-  case PO_RESTRICT_VC: return reinterpret_cast<const void *>(&k_persist_point<PO_RESTRICT_VC>);
-  case PO_RESTRICT_CC: return reinterpret_cast<const void *>(&k_persist_point<PO_RESTRICT_CC>);
-  case PO_INTERP: return reinterpret_cast<const void *>(&k_persist_point<PO_INTERP>);
-  case PO_TVD: return reinterpret_cast<const void *>(&k_persist_point<PO_TVD>);
-  case PO_RESTRICT_CC_O3: return reinterpret_cast<const void *>(&k_persist_point<PO_RESTRICT_CC_O3>);
-  case PO_RESTRICT_CC_O5: return reinterpret_cast<const void *>(&k_persist_point<PO_RESTRICT_CC_O5>);
+  case PO_COPY: return persist_point_pu<PO_COPY>(pu);
+  case PO_RESTRICT_VC: return persist_point_pu<PO_RESTRICT_VC>(pu);
+  case PO_RESTRICT_CC: return persist_point_pu<PO_RESTRICT_CC>(pu);
+  case PO_INTERP: return persist_point_pu<PO_INTERP>(pu);
+  case PO_TVD: return persist_point_pu<PO_TVD>(pu);
+  case PO_RESTRICT_CC_O3: return persist_point_pu<PO_RESTRICT_CC_O3>(pu);
+  case PO_RESTRICT_CC_O5: return persist_point_pu<PO_RESTRICT_CC_O5>(pu);
   }
   return nullptr;
 }
@@ -180,7 +198,7 @@ const void *persist_point_fn(int op) {
 const void *launch_fn(const BatchLaunch &l, int *threads) {
   if (!l.is_tensor) {
     *threads = 256;
-    return persist_point_fn(l.op);
+    return persist_point_fn(l.op, l.pt);
   }
   const bool n = l.shape == 1;
   switch (l.kind) {
@@ -334,7 +352,7 @@ int batch_device_build(BatchDevice &dev, const std::vector<Transfer> &tr,
       double busy[2] = {0, 0};
       for (int sh = 0; sh < 2; ++sh) {
         int cx, cy, th;
-        tile_shape(g.ntl, &cx, &cy, &th, sh);
+        tile_shape(g.ntl, g.prec, &cx, &cy, &th, sh);
         for (int m : g.members) {
           const TensorArgs &a = tr[m].pr.t;
           busy[sh] += (double)((a.nq[0] + cx - 1) / cx) * ((a.nq[1] + cy - 1) / cy) *
@@ -373,7 +391,7 @@ int batch_device_build(BatchDevice &dev, const std::vector<Transfer> &tr,
     if (g.is_tensor) {
       TileCfg cfg;
       int tthreads = 0;
-      tile_shape(g.ntl, &cfg.cx, &cfg.cy, &tthreads, l.shape);
+      tile_shape(g.ntl, g.prec, &cfg.cx, &cfg.cy, &tthreads, l.shape);
       // z chunk: the longest of 128..8 planes that still gives the grid a few
       // items per resident workgroup (tiles that march further apart in z
       // stop sharing their halo rows through L2)
@@ -567,14 +585,24 @@ static int run_launch(BatchDevice &dev, const BatchLaunch &l, hipStream_t st) {
     }
   } else {
     const PointArgs *pd = static_cast<const PointArgs *>(descs);
+    auto go_pu = [&](auto opc, auto puc) {
+      constexpr int OP = decltype(opc)::value;
+      constexpr int PU = decltype(puc)::value;
+      if (grid_mode)
+        hipLaunchKernelGGL((k_grid_point<OP, PU>), dim3(l.grid), dim3(256), 0, st, pd, items, seg, l.pt);
+      else
+        hipLaunchKernelGGL((k_persist_point<OP, PU>), dim3(l.grid), dim3(256), 0, st, pd, items, seg,
+                           ctr, l.pt);
+      return 0;
+    };
     auto go = [&](auto opc) {
       constexpr int OP = decltype(opc)::value;
-      if (grid_mode)
-        hipLaunchKernelGGL(k_grid_point<OP>, dim3(l.grid), dim3(256), 0, st, pd, items, seg, l.pt);
-      else
-        hipLaunchKernelGGL(k_persist_point<OP>, dim3(l.grid), dim3(256), 0, st, pd, items, seg, ctr,
-                           l.pt);
-      return 0;
+      switch (pu_inst<OP>(pu_of(l.pt))) {
+      case 1: return go_pu(opc, std::integral_constant<int, pu_inst<OP>(1)>{});
+      case 2: return go_pu(opc, std::integral_constant<int, pu_inst<OP>(2)>{});
+      case 4: return go_pu(opc, std::integral_constant<int, pu_inst<OP>(4)>{});
+      }
+      return go_pu(opc, std::integral_constant<int, 8>{});
     };
     switch (l.op) {
     case PO_COPY: rc = go(std::integral_constant<int, PO_COPY>{}); break;

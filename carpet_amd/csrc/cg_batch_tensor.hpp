@@ -14,7 +14,9 @@ template <class L, int NTL, int CX, int CY, int IPT> constexpr int persist_waves
   return (2 * CX * CY / IPT >= 512 && L::R <= 6) ? 4 : 1;
 }
 
-template <class L, int NTL, int CX, int CY, int IPT, int XQ, int PR>
+// NTL: levels of the passes; NSRC: source levels (NSRC > NTL: combined at
+// the window store)
+template <class L, int NTL, int CX, int CY, int IPT, int XQ, int PR, int NSRC>
 __global__ __launch_bounds__(2 * CX * CY / IPT)
 __attribute__((amdgpu_waves_per_eu(persist_waves<L, NTL, CX, CY, IPT>()))) void k_persist_tensor(
     const TensorArgs *__restrict__ descs, const Item *__restrict__ items,
@@ -27,11 +29,11 @@ __attribute__((amdgpu_waves_per_eu(persist_waves<L, NTL, CX, CY, IPT>()))) void 
       return;
     const Item w = items[it];
     const TensorArgs a = descs[w.desc];
-    tensor_tile<L, NTL, CX, CY, IPT, PR, 1, XQ, pipe_fits<L, NTL, CX, CY, XQ>()>(a, (int)w.tile, smem);
+    tensor_tile<L, NTL, CX, CY, IPT, PR, 1, XQ, pipe_fits<L, NTL, CX, CY, XQ>(), NSRC>(a, (int)w.tile, smem);
   }
 }
 
-template <class L, int NTL, int CX, int CY, int IPT, int XQ, int PR>
+template <class L, int NTL, int CX, int CY, int IPT, int XQ, int PR, int NSRC>
 __global__ __launch_bounds__(2 * CX * CY / IPT) void k_grid_tensor(
     const TensorArgs *__restrict__ descs, const Item *__restrict__ items,
     const int *__restrict__ seg) {
@@ -40,20 +42,22 @@ __global__ __launch_bounds__(2 * CX * CY / IPT) void k_grid_tensor(
   if (it < 0)
     return;
   const Item w = items[it];
-  tensor_tile<L, NTL, CX, CY, IPT, PR, 1, XQ, pipe_fits<L, NTL, CX, CY, XQ>()>(descs[w.desc], (int)w.tile, smem);
+  tensor_tile<L, NTL, CX, CY, IPT, PR, 1, XQ, pipe_fits<L, NTL, CX, CY, XQ>(), NSRC>(descs[w.desc], (int)w.tile,
+                                                                                   smem);
 }
 
-template <class L, int NTL, int PR, int SH>
+template <class L, int NSRC, int PR, int SH>
 int batch_tensor_go(const BatchLaunch &l, const TensorArgs *td, const Item *items,
                     const int *seg, int *ctr, hipStream_t st) {
-  using T = TileShape<NTL, SH>;
-  const dim3 block(tile_threads<NTL, SH>());
+  constexpr int NTL = pass_levels(NSRC, PR);
+  using T = TileShape<NTL, SH, PR>;
+  const dim3 block(tile_threads<NTL, SH, PR>());
   if (l.wpc < 0)
-    hipLaunchKernelGGL((k_grid_tensor<L, NTL, T::CX, T::CY, T::IPT, T::XQ, PR>), dim3(l.grid), block, 0,
-                       st, td, items, seg);
+    hipLaunchKernelGGL((k_grid_tensor<L, NTL, T::CX, T::CY, T::IPT, T::XQ, PR, NSRC>), dim3(l.grid), block,
+                       0, st, td, items, seg);
   else
-    hipLaunchKernelGGL((k_persist_tensor<L, NTL, T::CX, T::CY, T::IPT, T::XQ, PR>), dim3(l.grid), block,
-                       0, st, td, items, seg, ctr);
+    hipLaunchKernelGGL((k_persist_tensor<L, NTL, T::CX, T::CY, T::IPT, T::XQ, PR, NSRC>), dim3(l.grid),
+                       block, 0, st, td, items, seg, ctr);
   return 0;
 }
 
@@ -89,11 +93,12 @@ template <int KIND, int SH> const void *batch_tensor_fn_kind(const BatchLaunch &
   with_order<KIND>(l.order, [&](auto tag) -> int {
     using L = decltype(tag);
     auto pick = [&](auto ntlc, auto prc) {
-      constexpr int NTL = decltype(ntlc)::value;
+      constexpr int NSRC = decltype(ntlc)::value;
       constexpr int PR = decltype(prc)::value;
-      using T = TileShape<NTL, SH>;
-      *threads = tile_threads<NTL, SH>();
-      fn = reinterpret_cast<const void *>(&k_persist_tensor<L, NTL, T::CX, T::CY, T::IPT, T::XQ, PR>);
+      constexpr int NTL = pass_levels(NSRC, PR);
+      using T = TileShape<NTL, SH, PR>;
+      *threads = tile_threads<NTL, SH, PR>();
+      fn = reinterpret_cast<const void *>(&k_persist_tensor<L, NTL, T::CX, T::CY, T::IPT, T::XQ, PR, NSRC>);
     };
     auto by_prec = [&](auto ntlc) {
       if (l.prec == 0)

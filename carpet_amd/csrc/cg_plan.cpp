@@ -288,7 +288,7 @@ int launch_prolong(const ProlongPlan &pl, int prec, hipStream_t st) {
     // full round of resident workgroups), at least 16 planes per chunk
     TensorArgs a = pl.t;
     int threads = 0;
-    tile_shape(pl.ntl, &cfg.cx, &cfg.cy, &threads);
+    tile_shape(pl.ntl, prec, &cfg.cx, &cfg.cy, &threads);
     const long long cols = (long long)((a.nq[0] + cfg.cx - 1) / cfg.cx) *
                            ((a.nq[1] + cfg.cy - 1) / cfg.cy);
     int zc = std::min(a.nq[2], 128);

@@ -9,15 +9,19 @@
 
 namespace cg {
 
-template <class L, int NTL, int CX, int CY, int IPT, int XQ, int PR>
+// NTL: levels of the passes; NSRC: source levels (NSRC > NTL: combined at
+// the window store)
+template <class L, int NTL, int CX, int CY, int IPT, int XQ, int PR, int NSRC>
 __global__ __launch_bounds__(2 * CX * CY / IPT) void k_tensor(TensorArgs a) {
   __shared__ __align__(16) double smem[tile_lds<L, NTL, CX, CY, XQ>()];
-  tensor_tile<L, NTL, CX, CY, IPT, PR, 1, XQ, pipe_fits<L, NTL, CX, CY, XQ>()>(a, xcd_remap(blockIdx.x, gridDim.x), smem);
+  tensor_tile<L, NTL, CX, CY, IPT, PR, 1, XQ, pipe_fits<L, NTL, CX, CY, XQ>(), NSRC>(
+      a, xcd_remap(blockIdx.x, gridDim.x), smem);
 }
 
-template <class L, int NTL, int PR>
+template <class L, int NSRC, int PR>
 int launch_tensor_t(const TensorArgs &a0, const TileCfg &cfg0, hipStream_t s) {
-  using T = TileShape<NTL>;
+  constexpr int NTL = pass_levels(NSRC, PR);
+  using T = TileShape<NTL, 0, PR>;
   TensorArgs a = a0;
   TileCfg cfg = cfg0;
   cfg.cx = T::CX;
@@ -25,8 +29,8 @@ int launch_tensor_t(const TensorArgs &a0, const TileCfg &cfg0, hipStream_t s) {
   const long long nblk = tensor_tiles(a, cfg);
   if (nblk <= 0 || nblk > 0x7fffffffLL)
     return -1;
-  hipLaunchKernelGGL((k_tensor<L, NTL, T::CX, T::CY, T::IPT, T::XQ, PR>), dim3((unsigned)nblk),
-                     dim3(tile_threads<NTL>()), 0, s, a);
+  hipLaunchKernelGGL((k_tensor<L, NTL, T::CX, T::CY, T::IPT, T::XQ, PR, NSRC>), dim3((unsigned)nblk),
+                     dim3(tile_threads<NTL, 0, PR>()), 0, s, a);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

@@ -32,6 +32,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "cg_kernels.hpp"
 #include "cg_stencil.hpp"
 
@@ -97,8 +99,9 @@ template <> struct Pair<float> { using T = float2; };
 // passes.  0: FP64 (the drop-in).  1: FP32 storage, FP64 arithmetic -- the
 // product of a dyadic weight (at most 9 significant bits) and an FP32 value
 // is exact in FP64 and the sums carry 29 guard bits, an error-free-product
-// accumulation.  2: FP32 storage, compensated FP32 sums (Dot2 with FMA
-// TwoProduct and TwoSum).  Time interpolation is FP64 in every mode.
+// accumulation; the time levels are combined (in FP64) before the spatial
+// passes (pass_levels).  2: FP32 storage, compensated FP32 sums (Dot2 with
+// FMA TwoProduct and TwoSum).  Time interpolation is FP64 in every mode.
 template <int PR> struct Prec;
 template <> struct Prec<0> { using S = double; using C = double; };
 template <> struct Prec<1> { using S = float; using C = double; };
@@ -130,12 +133,19 @@ template <int I, int N, class F> __device__ __forceinline__ bool unroll_while(F 
 // barrier, x-pass, barrier, y-pass per plane; 1: software pipeline -- the
 // window of plane c, the x-pass of plane c-1 and the y-pass of plane c-2
 // between two barriers, with double-buffered window and x-pass results);
+// NSRC: source time levels; NSRC > NTL (= 1) combines the levels in FP64 as
+// the window is stored (time interpolation before the spatial stencil, exact
+// by linearity up to rounding; the FP32-storage, FP64-arithmetic mode), so
+// the passes run once instead of once per level.
 // smem must be 16-byte aligned and hold TensorLds<..., XQ, PIPE ? 2 : 1>
 template <class L, int NTL, int CX, int CY, int IPT, int PR = 0, int PF = 1, int XQ = 1,
-          int PIPE = 0>
+          int PIPE = 0, int NSRC = NTL>
 __device__ __forceinline__ void tensor_tile(const TensorArgs &a, int b, double *smem) {
   using TS = typename Prec<PR>::S; // storage
   using CT = typename Prec<PR>::C; // arithmetic of the passes
+  constexpr bool TF = NSRC > NTL;  // time levels combined at the window store
+  static_assert(!TF || (NTL == 1 && sizeof(CT) == 8), "time-first needs one pass level in FP64");
+  using WT = std::conditional_t<TF, CT, TS>; // LDS window type
   constexpr int FX = 2 * CX;
   constexpr int NT = FX * CY / IPT;
   constexpr int NQY = NT / FX;
@@ -156,10 +166,10 @@ __device__ __forceinline__ void tensor_tile(const TensorArgs &a, int b, double *
   static_assert(U % PF == 0 && U % R == 0 && U % NB == 0, "unroll must cover the slots");
 
   // buffer B of the window (storage type) and of the x-pass (arithmetic type)
-  constexpr int ABYTES = (LD::A * (int)sizeof(TS) + 15) / 16 * 16;
+  constexpr int ABYTES = (LD::A * (int)sizeof(WT) + 15) / 16 * 16;
   constexpr int XBYTES = (LD::X * (int)sizeof(CT) + 15) / 16 * 16;
   auto Abuf = [&](int B) {
-    return reinterpret_cast<TS(*)[WY][WXS]>(reinterpret_cast<char *>(smem) + B * ABYTES);
+    return reinterpret_cast<WT(*)[WY][WXS]>(reinterpret_cast<char *>(smem) + B * ABYTES);
   };
   auto Xbuf = [&](int B) {
     return reinterpret_cast<CT(*)[WY][FX]>(reinterpret_cast<char *>(smem) + NB * ABYTES + B * XBYTES);
@@ -200,14 +210,14 @@ __device__ __forceinline__ void tensor_tile(const TensorArgs &a, int b, double *
   }
   // windows of the next PF planes, loaded PF planes ahead of their use;
   // plane c sits in buffer (c - c0) % PF
-  TS pre[PF][NTL][NLD];
+  TS pre[PF][NSRC][NLD];
   auto fetch = [&](int c, auto slotc) {
     constexpr int SL = decltype(slotc)::value;
 #pragma unroll
     for (int k = 0; k < NLD; ++k)
       if (woff[k] >= 0)
 #pragma unroll
-        for (int tl = 0; tl < NTL; ++tl)
+        for (int tl = 0; tl < NSRC; ++tl)
           pre[SL][tl][k] = static_cast<const TS *>(a.src[tl])[woff[k] + a.sk * c];
   };
 
@@ -237,9 +247,18 @@ __device__ __forceinline__ void tensor_tile(const TensorArgs &a, int b, double *
     for (int k = 0; k < NLD; ++k)
       if (woff[k] >= 0) {
         const int e = tid + k * NT;
+        if constexpr (TF) {
+          // Lagrange time levels only (no ENO rule): canonical FP64 order
+          double v[NSRC];
 #pragma unroll
-        for (int tl = 0; tl < NTL; ++tl)
-          (&A[tl][0][0])[e] = pre[SL][tl][k];
+          for (int tl = 0; tl < NSRC; ++tl)
+            v[tl] = (double)pre[SL][tl][k];
+          (&A[0][0][0])[e] = time_combine<NSRC>(v, a.s, a.s2, a.l0, 0);
+        } else {
+#pragma unroll
+          for (int tl = 0; tl < NTL; ++tl)
+            (&A[tl][0][0])[e] = pre[SL][tl][k];
+        }
       }
   };
   // x-pass: both children of each base, window buffer B -> x buffer B
@@ -251,7 +270,7 @@ __device__ __forceinline__ void tensor_tile(const TensorArgs &a, int b, double *
       for (int e = tid; e < NTL * WY * CX; e += NT) {
         const int tl = e / (WY * CX), e2 = e - tl * (WY * CX);
         const int r = e2 / CX, q = e2 - r * CX;
-        const TS *w = &A[tl][r][q - L::TMINALL];
+        const WT *w = &A[tl][r][q - L::TMINALL];
         const CT v0 = line_apply<L, 0, CT>([&](int t) { return w[L::TMIN[0] + t]; });
         const CT v1 = line_apply<L, 1, CT>([&](int t) { return w[L::TMIN[1] + t]; });
         X[tl][r][2 * q] = v0;
@@ -260,14 +279,14 @@ __device__ __forceinline__ void tensor_tile(const TensorArgs &a, int b, double *
     } else {
       // bases q0, q0+1 from the window columns q0 .. q0+SPAN+1 (window
       // column 0 is offset TMINALL of base 0), read and written in pairs
-      using P2 = typename Pair<TS>::T;
+      using P2 = typename Pair<WT>::T;
       using C2 = typename Pair<CT>::T;
       constexpr int NP = (LD::SPAN + 3) / 2;
       constexpr int NQ2 = WY * (CX / 2);
       for (int e = tid; e < NTL * NQ2; e += NT) {
         const int tl = e / NQ2, e2 = e - tl * NQ2;
         const int r = e2 / (CX / 2), q0 = 2 * (e2 - r * (CX / 2));
-        TS v[2 * NP];
+        WT v[2 * NP];
 #pragma unroll
         for (int m = 0; m < NP; ++m) {
           const P2 p = *reinterpret_cast<const P2 *>(&A[tl][r][q0 + 2 * m]);
@@ -485,6 +504,8 @@ __device__ __forceinline__ void tvd_range(const PointArgs &a, int d, int *q0, in
 
 // parent value and the three minmod slopes of coarse cell (qx,qy,qz) on
 // time level tl
+typedef double dbl2 __attribute__((ext_vector_type(2)));
+
 struct TvdCell {
   double c, mx, my, mz;
 };
@@ -513,11 +534,9 @@ __device__ __forceinline__ void tvd_store(const PointArgs &a, int qx, int qy, in
       const int fy = fy0 + y;
       if (fy < 0 || fy >= a.n[1])
         continue;
+      double out[2];
 #pragma unroll
       for (int x = 0; x < 2; ++x) {
-        const int fx = fx0 + x;
-        if (fx < 0 || fx >= a.n[0])
-          continue;
         double w[NTL];
 #pragma unroll
         for (int tl = 0; tl < NTL; ++tl) {
@@ -525,8 +544,20 @@ __device__ __forceinline__ void tvd_store(const PointArgs &a, int qx, int qy, in
                        sz = (z ? q : -q) * v[tl].mz;
           w[tl] = v[tl].c + sx + sy + sz;
         }
-        __builtin_nontemporal_store(time_combine<NTL>(w, a.s, a.s2, a.l0, a.teno),
-                                    &a.dst[fx + a.dj * fy + a.dk * fz]);
+        out[x] = time_combine<NTL>(w, a.s, a.s2, a.l0, a.teno);
+      }
+      double *p = &a.dst[fx0 + a.dj * fy + a.dk * fz];
+      const bool in0 = fx0 >= 0 && fx0 < a.n[0], in1 = fx0 + 1 >= 0 && fx0 + 1 < a.n[0];
+      // both x children in one 16-byte store where the pair is aligned, so a
+      // wavefront writes whole contiguous lines (two 8-byte stores per lane
+      // at a 16-byte stride leave every line half written per instruction)
+      if (in0 && in1 && ((reinterpret_cast<uintptr_t>(p) & 15) == 0)) {
+        __builtin_nontemporal_store(dbl2{out[0], out[1]}, reinterpret_cast<dbl2 *>(p));
+      } else {
+        if (in0)
+          __builtin_nontemporal_store(out[0], p);
+        if (in1)
+          __builtin_nontemporal_store(out[1], p + 1);
       }
     }
   }
@@ -549,25 +580,31 @@ __device__ __forceinline__ void tvd_cell(const PointArgs &a, int qx, int qy, int
 // thread), a third fewer LDS reads than one base per thread.
 // SH = 0 ("wide", 32 x 8 bases) suits large regions; SH = 1 ("narrow",
 // 16 x 16 bases) wastes fewer lanes on regions of a few dozen points per
-// axis and has a smaller x halo per fine point.
-template <int NTL, int SH = 0> struct TileShape;
-template <> struct TileShape<1, 0> {
+// axis and has a smaller x halo per fine point.  PR: precision mode; the
+// FP32-storage, FP64-arithmetic mode (one pass level) runs its wide tiles
+// 64 bases across, so a fine row is 512 bytes of FP32 and the x halo is
+// half as large per point.
+template <int NTL, int SH = 0, int PR = 0> struct TileShape : TileShape<NTL, SH, 0> {};
+template <> struct TileShape<1, 0, 0> {
   static constexpr int CX = 32, CY = 8, IPT = 2, XQ = 2;
 };
-template <> struct TileShape<2, 0> {
+template <> struct TileShape<2, 0, 0> {
   static constexpr int CX = 32, CY = 8, IPT = 1, XQ = 2;
 };
-template <> struct TileShape<3, 0> {
+template <> struct TileShape<3, 0, 0> {
   static constexpr int CX = 32, CY = 8, IPT = 1, XQ = 2;
 };
-template <> struct TileShape<1, 1> {
+template <> struct TileShape<1, 1, 0> {
   static constexpr int CX = 16, CY = 16, IPT = 2, XQ = 2;
 };
-template <> struct TileShape<2, 1> {
+template <> struct TileShape<2, 1, 0> {
   static constexpr int CX = 16, CY = 16, IPT = 1, XQ = 2;
 };
-template <> struct TileShape<3, 1> {
+template <> struct TileShape<3, 1, 0> {
   static constexpr int CX = 16, CY = 16, IPT = 1, XQ = 2;
+};
+template <> struct TileShape<1, 0, 1> {
+  static constexpr int CX = 64, CY = 8, IPT = 2, XQ = 2;
 };
 // the pipelined schedule (double-buffered window and x-pass results) where
 // its LDS fits the 64 KB of a static allocation, else two barriers per plane
@@ -578,28 +615,44 @@ template <class L, int NTL, int CX, int CY, int XQ> constexpr int tile_lds() {
   return TensorLds<L, NTL, CX, CY, XQ, pipe_fits<L, NTL, CX, CY, XQ>() ? 2 : 1>::TOTAL;
 }
 
-template <int NTL, int SH = 0> constexpr int tile_threads() {
-  using T = TileShape<NTL, SH>;
+// time levels the tile passes carry: the FP32-storage, FP64-arithmetic mode
+// (1) combines the source levels as the window is stored
+constexpr int pass_levels(int ntl, int prec) { return prec == 1 ? 1 : ntl; }
+
+template <int NTL, int SH = 0, int PR = 0> constexpr int tile_threads() {
+  using T = TileShape<NTL, SH, PR>;
   return 2 * T::CX * T::CY / T::IPT;
 }
-// host view of the tile shape for ntl time levels
-inline void tile_shape(int ntl, int *cx, int *cy, int *threads, int sh = 0) {
-  auto get = [&](auto ntlc, auto shc) {
-    constexpr int N = decltype(ntlc)::value, S = decltype(shc)::value;
-    *cx = TileShape<N, S>::CX;
-    *cy = TileShape<N, S>::CY;
-    *threads = tile_threads<N, S>();
+// host view of the tile shape of a transfer class: ntl source levels,
+// precision mode prec, shape sh
+inline void tile_shape(int ntl, int prec, int *cx, int *cy, int *threads, int sh = 0) {
+  auto get = [&](auto ntlc, auto shc, auto prc) {
+    constexpr int N = decltype(ntlc)::value, S = decltype(shc)::value, P = decltype(prc)::value;
+    *cx = TileShape<N, S, P>::CX;
+    *cy = TileShape<N, S, P>::CY;
+    *threads = tile_threads<N, S, P>();
   };
-  using I1 = std::integral_constant<int, 1>;
-  using I2 = std::integral_constant<int, 2>;
-  using I3 = std::integral_constant<int, 3>;
-  using S0 = std::integral_constant<int, 0>;
-  using S1 = std::integral_constant<int, 1>;
-  if (sh == 0) {
-    if (ntl == 1) get(I1{}, S0{}); else if (ntl == 2) get(I2{}, S0{}); else get(I3{}, S0{});
-  } else {
-    if (ntl == 1) get(I1{}, S1{}); else if (ntl == 2) get(I2{}, S1{}); else get(I3{}, S1{});
-  }
+  auto by_sh = [&](auto ntlc, auto prc) {
+    if (sh == 0)
+      get(ntlc, std::integral_constant<int, 0>{}, prc);
+    else
+      get(ntlc, std::integral_constant<int, 1>{}, prc);
+  };
+  auto by_prec = [&](auto ntlc) {
+    if (prec == 1)
+      by_sh(ntlc, std::integral_constant<int, 1>{});
+    else if (prec == 2)
+      by_sh(ntlc, std::integral_constant<int, 2>{});
+    else
+      by_sh(ntlc, std::integral_constant<int, 0>{});
+  };
+  const int np = pass_levels(ntl, prec);
+  if (np == 1)
+    by_prec(std::integral_constant<int, 1>{});
+  else if (np == 2)
+    by_prec(std::integral_constant<int, 2>{});
+  else
+    by_prec(std::integral_constant<int, 3>{});
 }
 
 } // namespace cg

@@ -85,9 +85,12 @@ enum { CG_ERROR_CENTERED = 0, CG_VERTEX_CENTERED = 1, CG_CELL_CENTERED = 2 };
 /* precision of Lagrange prolongation in the batch and mixed paths:
    CG_PREC_F64 is the drop-in.  The mixed modes store prolongation sources
    and destinations in FP32 and keep the time interpolation in FP64:
-   CG_PREC_MIXED accumulates each stencil in FP64 (the product of a dyadic
-   weight and an FP32 value is exact in FP64, so the sum is an error-free-
-   product accumulation with 29 guard bits); CG_PREC_MIXED_DOT2 uses
+   CG_PREC_MIXED combines the time levels in FP64 as each source window is
+   loaded and then runs the stencil once in FP64 (the product of a dyadic
+   weight and an FP32 value is exact in FP64 and the sums carry 29 guard
+   bits; the result is the FP64 operator on the FP32 values rounded once
+   to FP32, to within one FP32 ulp, the difference being the order of the
+   time and space sums); CG_PREC_MIXED_DOT2 keeps the levels apart and uses
    compensated FP32 sums (Dot2: FMA TwoProduct + TwoSum per tap) */
 enum { CG_PREC_F64 = 0, CG_PREC_MIXED = 1, CG_PREC_MIXED_DOT2 = 2 };
 
@@ -368,13 +371,14 @@ int cg_batch_launch_graph(cg_batch *b, cg_stream_t stream);
    (persistent grid for tensor tiles, one workgroup per item for pointwise
    runs), then the best size in the other modes (one workgroup per item,
    persistent grids of resident capacity and of 2 workgroups per CU) and,
-   for tensor tiles, the other tile shape (32x8 or 16x16 coarse bases),
+   for tensor tiles, the other tile shape (wide: 32x8 coarse bases, 64x8
+   for CG_PREC_MIXED; narrow: 16x16),
    with HIP events, and keeps the fastest per launch class.
    Choices are keyed on (kernel class, order, time levels, precision, log2
    of the median region size, log2 of the region count) and kept in the
    text file cache_path (may be NULL; lines "<key>|<zchunk> <points>
    <workgroups per CU: -1 one per item, 0 resident capacity> <tile shape:
-   0 32x8, 1 16x16, -1 by region size>"): classes the file names take its
+   0 wide, 1 narrow, -1 by region size>"): classes the file names take its
    parameters, the others are searched.  The
    search executes each launch (one phase's class of transfers) reps times
    on its own, outside phase order, so call it before the step whose

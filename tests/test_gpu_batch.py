@@ -59,7 +59,7 @@ def test_batch_modes_match_oracle(make, tmp_path):
         # workgroup per item, wpc -1, and persistent grids, wpc 0 and 2),
         # then the tuner's own search; each time from poisoned outputs
         keys = re.findall(r"\[([^\]]+)\]", b.describe())
-        for wpc, zc, pt in ((-1, 16, 4096), (0, 8, 1024), (2, 0, 65536)):
+        for wpc, zc, pt in ((-1, 16, 256), (0, 8, 512), (2, 0, 65536)):
             cache = tmp_path / f"tune_{wpc}.txt"
             cache.write_text("".join(f"{k}|{zc} {pt} {wpc}\n" for k in keys))
             b.autotune(st, reps=1, cache_path=str(cache))

@@ -73,12 +73,23 @@ def test_mixed_transfer_with_time_interpolation(kind, order):
     assert emax <= TOL_MAX
 
 
+def _ulp_distance(a, b):
+    """FP32 units in the last place between a and b (same shape)."""
+    def ordered(x):
+        i = np.ascontiguousarray(x, dtype=np.float32).view(np.int32).astype(np.int64)
+        return np.where(i < 0, -(i & 0x7FFFFFFF), i)
+    return np.abs(ordered(a) - ordered(b))
+
+
 @pytest.mark.parametrize("kind,order", [("vlag", 5), ("vlag", 3), ("clag", 5), ("clag", 3)])
 def test_mixed_batch_modes(kind, order):
     """Both mixed modes through the batch: CG_PREC_MIXED (FP64 arithmetic on
-    FP32 storage) equals the FP64 oracle on the same FP32-representable
-    inputs rounded once to FP32, bit for bit; CG_PREC_MIXED_DOT2 (compensated
-    FP32 sums) stays within 1e-6 normwise."""
+    FP32 storage, the three time levels combined before the spatial stencil)
+    is within one FP32 ulp of the FP64 oracle on the same FP32-representable
+    inputs rounded once to FP32 (the two FP64 results differ only by
+    reassociation, ~1e-16 relative, so they round to the same FP32 value
+    except next to a rounding boundary); CG_PREC_MIXED_DOT2 (compensated FP32
+    sums) stays within 1e-6 normwise."""
     rng = np.random.default_rng(11 + order)
     sb, db, rb = prolong_case(rng, kind, order, n=(40, 36, 34))
     pad = sizes(sb)
@@ -107,7 +118,10 @@ def test_mixed_batch_modes(kind, order):
         assert np.array_equal(m, ~np.isnan(out[prec])), "mixed kernel wrote outside the region"
         b.close()
     exp32 = exp.view()[m].astype(np.float32)
-    assert np.array_equal(out[cg.PREC_MIXED][m], exp32), "FP64-arithmetic mixed mode is not the rounded oracle"
+    ulps = _ulp_distance(out[cg.PREC_MIXED][m], exp32)
+    print(f"{kind} o{order} + 3tl FP64 arithmetic: {np.count_nonzero(ulps)} of {ulps.size} points "
+          f"one ulp from the rounded oracle", flush=True)
+    assert ulps.max() <= 1, "FP64-arithmetic mixed mode is not the rounded oracle to one ulp"
     emax, erms = cg.relative_errors(out[cg.PREC_MIXED_DOT2][m].astype(np.float64), exp.view()[m])
     print(f"{kind} o{order} + 3tl Dot2: max rel {emax:.3e} rms rel {erms:.3e}", flush=True)
     assert emax <= TOL_MAX
