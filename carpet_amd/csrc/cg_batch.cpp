@@ -204,8 +204,8 @@ extern "C" int cg_batch_autotune(cg_batch *b, cg_stream_t stream, int reps,
       if (int e = trial_run(c))
         return e;
     const Tune base = bestc;
-    for (int w : {-1, 0, 2})
-      if (w != base.wpc)
+    for (int w : {-1, 0, 2, -2})
+      if (w != base.wpc && (w != -2 || l.is_tensor))
         if (int e = trial_run(Tune{base.zchunk, base.pt, w, base.shape}))
           return e;
     if (l.is_tensor) // the other tile shape, in the best mode

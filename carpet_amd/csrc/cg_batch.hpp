@@ -27,7 +27,10 @@ struct BatchLaunch {
   int grid;        // persistent workgroups
   int zchunk;      // tensor work item: coarse z bases per chunk (0 = all)
   int pt;          // pointwise work item: destination points
-  int wpc;         // workgroups per CU of the grid (-1 = one per item)
+  int wpc;         // workgroups per CU of a persistent grid taking items
+                   // from counters (0 = resident capacity); -1 one
+                   // workgroup per item; -2 resident capacity taking items
+                   // in fixed rounds (tensor tiles)
   int shape;       // tensor tile shape (0: 32 x 8 bases, 1: 16 x 16)
   std::string key; // tuner cache key (class + box-size histogram bucket)
 };

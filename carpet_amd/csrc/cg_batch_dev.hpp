@@ -39,6 +39,21 @@ __device__ __forceinline__ int dequeue(int *ctr8, const int *seg, int *s_item) {
   return it;
 }
 
+// Static mode: a resident grid (a multiple of 8 workgroups) in which the
+// workgroups of XCD x take the items of segment x in a fixed round-robin:
+// round k of workgroup b is slot b/8 + k * (grid/8).  Neighbouring tiles
+// then start their z-march together in every round, as the tiles of one
+// launch per transfer do, and share their halo rows in L2 (a workgroup
+// taking whichever item is next when it finishes starts neighbours at
+// staggered times, and their halo rows leave L2 in between).  Returns -1
+// after the workgroup's last round.
+__device__ __forceinline__ int static_item(const int *__restrict__ seg, int k) {
+  const int x = blockIdx.x & 7;
+  const int slot = (int)(blockIdx.x >> 3) + k * (int)(gridDim.x >> 3);
+  const int i = seg[x] + slot;
+  return i < seg[x + 1] ? i : -1;
+}
+
 // Grid mode: one workgroup per item, no loop and no counters.  Workgroup b
 // runs on XCD b % 8 and takes slot b / 8 of that XCD's segment; segments
 // shorter than the longest leave their last workgroups without an item.

@@ -367,6 +367,8 @@ int batch_device_build(BatchDevice &dev, const std::vector<Transfer> &tr,
         l.zchunk = it->second.zchunk;
         l.pt = it->second.pt;
         l.wpc = it->second.wpc;
+        if (!g.is_tensor && l.wpc == -2) // static rounds: tensor tiles only
+          l.wpc = -1;
         if (g.is_tensor && it->second.shape >= 0)
           l.shape = it->second.shape;
       }
@@ -498,7 +500,7 @@ int batch_device_build(BatchDevice &dev, const std::vector<Transfer> &tr,
     }
     l.item_off = append(items.data(), items.size() * sizeof(Item));
     l.counter = (int)dev.launches.size();
-    if (l.wpc < 0) {
+    if (l.wpc == -1) {
       // grid mode: 8 x the longest segment
       long long longest = 1;
       for (int x = 0; x < 8; ++x)
@@ -508,6 +510,7 @@ int batch_device_build(BatchDevice &dev, const std::vector<Transfer> &tr,
       l.grid = (int)(8 * longest);
     } else {
       const long long all = std::max(8, (l.nitems + 7) / 8 * 8);
+      // persistent (counters or static rounds): a multiple of 8 workgroups
       const long long grid =
           l.wpc > 0 ? std::max(8LL, (long long)l.wpc * ncu / 8 * 8) : full_grid;
       l.grid = (int)std::min(grid, all);
@@ -563,7 +566,7 @@ static int run_launch(BatchDevice &dev, const BatchLaunch &l, hipStream_t st) {
   const Item *items = reinterpret_cast<const Item *>(dev.blob + l.item_off);
   const int *seg = reinterpret_cast<const int *>(dev.blob + l.seg_off);
   int *ctr = dev.counters + 8 * l.counter;
-  const bool grid_mode = l.wpc < 0;
+  const bool grid_mode = l.wpc == -1;
   int rc = 0;
   if (l.is_tensor) {
     const TensorArgs *td = static_cast<const TensorArgs *>(descs);

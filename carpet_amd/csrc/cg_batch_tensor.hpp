@@ -20,11 +20,12 @@ template <class L, int NTL, int CX, int CY, int IPT, int XQ, int PR, int NSRC>
 __global__ __launch_bounds__(2 * CX * CY / IPT)
 __attribute__((amdgpu_waves_per_eu(persist_waves<L, NTL, CX, CY, IPT>()))) void k_persist_tensor(
     const TensorArgs *__restrict__ descs, const Item *__restrict__ items,
-    const int *__restrict__ seg, int *counter) {
+    const int *__restrict__ seg, int *counter, int stat) {
   __shared__ __align__(16) double smem[tile_lds<L, NTL, CX, CY, XQ>()];
   __shared__ int s_item;
-  for (;;) {
-    const int it = dequeue(counter, seg, &s_item);
+  for (int k = 0;; ++k) {
+    // stat: fixed rounds over the XCD's segment, else the work counters
+    const int it = stat ? static_item(seg, k) : dequeue(counter, seg, &s_item);
     if (it < 0)
       return;
     const Item w = items[it];
@@ -52,12 +53,12 @@ int batch_tensor_go(const BatchLaunch &l, const TensorArgs *td, const Item *item
   constexpr int NTL = pass_levels(NSRC, PR);
   using T = TileShape<NTL, SH, PR>;
   const dim3 block(tile_threads<NTL, SH, PR>());
-  if (l.wpc < 0)
+  if (l.wpc == -1)
     hipLaunchKernelGGL((k_grid_tensor<L, NTL, T::CX, T::CY, T::IPT, T::XQ, PR, NSRC>), dim3(l.grid), block,
                        0, st, td, items, seg);
   else
     hipLaunchKernelGGL((k_persist_tensor<L, NTL, T::CX, T::CY, T::IPT, T::XQ, PR, NSRC>), dim3(l.grid),
-                       block, 0, st, td, items, seg, ctr);
+                       block, 0, st, td, items, seg, ctr, l.wpc == -2 ? 1 : 0);
   return 0;
 }
 

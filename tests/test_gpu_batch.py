@@ -56,10 +56,12 @@ def test_batch_modes_match_oracle(make, tmp_path):
     _compare(_outputs(w, dev), ref, "batched")
     if w.name != "config1":
         # every launch mode (a tuner cache naming every class with one
-        # workgroup per item, wpc -1, and persistent grids, wpc 0 and 2),
+        # workgroup per item, wpc -1, persistent grids taking items from
+        # counters, wpc 0 and 2, and in fixed rounds, wpc -2, which point
+        # classes run as wpc -1),
         # then the tuner's own search; each time from poisoned outputs
         keys = re.findall(r"\[([^\]]+)\]", b.describe())
-        for wpc, zc, pt in ((-1, 16, 256), (0, 8, 512), (2, 0, 65536)):
+        for wpc, zc, pt in ((-1, 16, 256), (0, 8, 512), (2, 0, 65536), (-2, 32, 1024)):
             cache = tmp_path / f"tune_{wpc}.txt"
             cache.write_text("".join(f"{k}|{zc} {pt} {wpc}\n" for k in keys))
             b.autotune(st, reps=1, cache_path=str(cache))
