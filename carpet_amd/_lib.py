@@ -117,3 +117,17 @@ _decl("cg_prolongate_3d_cc_rf2_mixed", i, i, *_OPARGS, vp)
 _decl("cg_transfer_from_mixed", i, _GP, C.POINTER(_GP), C.POINTER(C.c_double), i, ip, ip, dp, i, i, i, i, vp)
 _decl("cg_batch_autotune", i, vp, vp, i, C.c_char_p)
 _decl("cg_batch_describe", i, vp, C.c_char_p, C.c_size_t)
+
+
+class LevelSpec(C.Structure):
+    _fields_ = [("baseextent", C.c_int * 9), ("ncomp", C.c_int), ("interior", C.POINTER(C.c_int)),
+                ("reffact", C.c_int * 3), ("ghost_width", C.c_int * 6), ("buffer_width", C.c_int * 6),
+                ("overlap_width", C.c_int * 6), ("prolongation_order", C.c_int)]
+
+
+_decl("cg_schedule_build", i, i, ip, i, C.POINTER(LevelSpec), i, i, C.POINTER(vp))
+_decl("cg_schedule_free", None, vp)
+_decl("cg_schedule_count", i, vp, i, i)
+_decl("cg_schedule_region", i, vp, i, i, i, ip, ip, ip, ip)
+_decl("cg_schedule_component", i, vp, i, i, i, ip)
+_decl("cg_schedule_set", i, vp, i, i, i, i, ip)

@@ -208,6 +208,7 @@ const char *cg_error_string(int code) {
   case CG_ERR_NOMEM: return "out of memory";
   case CG_ERR_STATE: return "batch used in the wrong state";
   case CG_ERR_DEVICE: return "device-side check failed";
+  case CG_ERR_SCHEDULE: return "schedule consistency check failed";
   }
   return "unknown error";
 }

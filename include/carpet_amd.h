@@ -60,7 +60,8 @@ enum {
   CG_ERR_HIP = -12,         /* HIP runtime error */
   CG_ERR_NOMEM = -13,       /* allocation failed */
   CG_ERR_STATE = -14,       /* batch object used in the wrong state */
-  CG_ERR_DEVICE = -15       /* device kernel reported a failed check */
+  CG_ERR_DEVICE = -15,      /* device kernel reported a failed check */
+  CG_ERR_SCHEDULE = -16     /* a dh region check failed (schedule build) */
 };
 
 /* transport operators: same numbering as CarpetLib's enum operator_type
@@ -394,6 +395,65 @@ int cg_batch_describe(const cg_batch *b, char *buf, size_t len);
    bytes (destination + unique source footprint per time level) */
 int cg_batch_stats(const cg_batch *b, long long *ntransfers,
                    long long *dst_points, long long *alg_bytes);
+
+/* ---- communication schedule ------------------------------------------
+   dh::regrid's region description and communication lists for one
+   multigrid level (CarpetLib/src/dh.cc:211-1080): per level the components'
+   exterior / owned / interior / communicated boxes, the buffer / overlap /
+   active split, and the lists
+     CG_SCHED_REF_PROL      refinement prolongation, fast_ref_prol_sendrecv
+                            (dh.cc:766-830): level rl-1 -> active + overlaps
+     CG_SCHED_SYNC          synchronisation, fast_sync_sendrecv
+                            (dh.cc:834-890): interiors -> ghost zones
+     CG_SCHED_REF_BND_PROL  boundary prolongation, fast_ref_bnd_prol_sendrecv
+                            (dh.cc:894-975): level rl-1 -> unsynchronised
+                            ghosts + buffer zones
+     CG_SCHED_REF_REST      refinement restriction, fast_ref_rest_sendrecv
+                            (dh.cc:981-1080), stored on the coarse level like
+                            the reference's: level rl+1 -> level rl
+   Each entry is (source component, send box, destination component,
+   receive box): one cg_transfer_from (or cg_batch_add) with srcregbbox =
+   send and dstregbbox = recv.  Region sets may be split into boxes
+   differently from the reference's bboxset normal form; each list's union
+   of receive boxes per (source, destination) is the same, and so is every
+   transferred value.  Refluxing and multigrid lists are not built.
+   The reference's consistency assertions return CG_ERR_SCHEDULE with the
+   reference's message in cg_last_error_message. */
+typedef struct cg_schedule cg_schedule;
+typedef struct cg_level_spec {
+  int baseextent[9];        /* h.baseextent(ml, rl): the level's domain exterior */
+  int ncomp;                /* components of the level */
+  const int *interior;      /* ncomp x 9: h.extent(ml, rl, c), the regridder's boxes */
+  int reffact[3];           /* h.reffacts[rl] (level 0 = 1) */
+  int ghost_width[6];       /* lower x y z, upper x y z */
+  int buffer_width[6];
+  int overlap_width[6];
+  int prolongation_order;   /* prolongation_orders_space[rl]; stencil = order / 2 */
+} cg_level_spec;
+enum { CG_SCHED_REF_PROL = 0, CG_SCHED_SYNC = 1, CG_SCHED_REF_BND_PROL = 2, CG_SCHED_REF_REST = 3,
+       CG_SCHED_NLISTS = 4 };
+/* boxes of a component (cg_schedule_component) */
+enum { CG_SCHED_EXTERIOR = 0, CG_SCHED_INTERIOR = 1, CG_SCHED_OWNED = 2, CG_SCHED_COMMUNICATED = 3 };
+/* region sets of a component (cg_schedule_set) */
+enum { CG_SCHED_GHOSTS = 10, CG_SCHED_BUFFERS = 11, CG_SCHED_OVERLAPS = 12, CG_SCHED_ACTIVE = 13,
+       CG_SCHED_SYNCED = 14, CG_SCHED_BNDREF = 15, CG_SCHED_BOUNDARIES = 16 };
+/* centering: CG_VERTEX_CENTERED or CG_CELL_CENTERED (h.refcent);
+   boundary_width: h.boundary_width (lower x y z, upper x y z);
+   use_higher_order_restriction / restriction_order_space: CarpetLib's
+   parameters (restriction into interiors only, and for cell centring a
+   source shrunk by order/2, dh.cc:1019-1050) */
+int cg_schedule_build(int centering, const int boundary_width[6], int nlevels,
+                      const cg_level_spec *levels, int use_higher_order_restriction,
+                      int restriction_order_space, cg_schedule **out);
+void cg_schedule_free(cg_schedule *s);
+/* entries of list on level rl, or a negative CG_ERR_* */
+int cg_schedule_count(const cg_schedule *s, int rl, int list);
+int cg_schedule_region(const cg_schedule *s, int rl, int list, int i, int *src_comp, int send[9],
+                       int *dst_comp, int recv[9]);
+int cg_schedule_component(const cg_schedule *s, int rl, int c, int which, int bbox[9]);
+/* boxes of a region set: writes min(n, cap) boxes (9 ints each) when bboxes
+   is not NULL and returns n */
+int cg_schedule_set(const cg_schedule *s, int rl, int c, int which, int cap, int *bboxes);
 
 #ifdef __cplusplus
 }
