@@ -499,3 +499,93 @@ class Config5:
             b.add(0, d, ss, list(self.times), reg, reg, self.target, OP_LAGRANGE, VERTEX_CENTERED,
                   self.order, 2, prec=prec)
         return b
+
+
+def nested_levels(ncoarse: int = 32, nlevels: int = 3, split=(2, 1, 1), centering: int = VERTEX_CENTERED,
+                  ghost: int = 3, buffer: int = 6, order: int = 5, fraction: float = 0.5):
+    """A nested hierarchy as CarpetLib's gh would hold it (schedule.LevelDesc
+    per level): level 0 covers the domain of ncoarse coarse points per axis,
+    every finer level (refinement factor 2) the central `fraction` of the
+    previous level's region; each level's region is tiled into
+    split[0] x split[1] x split[2] components.  Vertex centring puts level rl
+    on stride 2^(nlevels-1-rl); cell centring on stride 2^(nlevels-rl) with
+    cell centres at odd multiples of half the stride."""
+    from .schedule import LevelDesc
+    ns = 2 ** (nlevels - 1) if centering == VERTEX_CENTERED else 2 ** nlevels
+    levels = []
+    plo, phi = None, None
+    for rl in range(nlevels):
+        s = ns // 2 ** rl
+        off = 0 if centering == VERTEX_CENTERED else s // 2
+        npts = ncoarse * 2 ** rl + (1 if centering == VERTEX_CENTERED else 0)
+        dom = ((off,) * 3, (off + s * (npts - 1),) * 3, (s,) * 3)
+        if rl == 0:
+            lo, hi = [0] * 3, [npts - 1] * 3
+        else:
+            lo, hi = [], []
+            for d in range(3):
+                a, b = 2 * plo[d], 2 * phi[d] + (0 if centering == VERTEX_CENTERED else 1)
+                n = b - a + 1
+                keep = max(8, int(round(n * fraction)))
+                c0 = a + (n - keep) // 2
+                lo.append(c0)
+                hi.append(c0 + keep - 1)
+        comps = []
+        for ix in range(split[0]):
+            for iy in range(split[1]):
+                for iz in range(split[2]):
+                    blo, bhi = [], []
+                    for d, (k, nk) in enumerate(zip((ix, iy, iz), split)):
+                        n = hi[d] - lo[d] + 1
+                        a = lo[d] + n * k // nk
+                        b = lo[d] + n * (k + 1) // nk - 1
+                        blo.append(off + s * a)
+                        bhi.append(off + s * b)
+                    comps.append((tuple(blo), tuple(bhi), (s,) * 3))
+        g = ((ghost,) * 3, (ghost,) * 3)
+        bw = ((buffer,) * 3, (buffer,) * 3) if rl > 0 else ((0,) * 3, (0,) * 3)
+        levels.append(LevelDesc(dom, comps, (2 ** rl,) * 3, g, bw, ((0,) * 3, (0,) * 3), order))
+        plo, phi = lo, hi
+    return levels
+
+
+def hierarchy_step(levels, centering: int = VERTEX_CENTERED, n_gf: int = 2, seed: int = 6,
+                   boundary=((0, 0, 0), (0, 0, 0)), op: int = OP_LAGRANGE) -> Workload:
+    """One level step of a real hierarchy, from its communication schedule
+    (schedule.Schedule, dh's lists): for each level rl in turn, restriction
+    from rl+1 into rl (phase 3rl), boundary prolongation from rl-1 with
+    3-level time interpolation to the intermediate fine time (phase 3rl+1),
+    then synchronisation (phase 3rl+2).  Arrays: every (level, component,
+    grid function, time level) on the component's exterior, seeded; the
+    outputs are the current time level of every component."""
+    from .schedule import REF_BND_PROL, REF_REST, SYNC, Schedule
+    sch = Schedule(centering, boundary, levels)
+    w = Workload(f"hierarchy[{len(levels)} levels, {[len(L.components) for L in levels]} components x {n_gf}]")
+    times = [0.0, -1.0, -2.0]
+    for rl, L in enumerate(levels):
+        for c in range(len(L.components)):
+            ext = Box(*sch.component(rl, c, 0))
+            own = Box(*sch.component(rl, c, 2))
+            for g in range(n_gf):
+                for tl in range(3):
+                    nm = f"l{rl}c{c}g{g}t{tl}"
+                    w.arrays[nm] = ArraySpec(ext, owned=own, seed=seed * 7919 + hash(nm) % 1000003)
+                w.outputs.append(f"l{rl}c{c}g{g}t0")
+    order = {rl: L.prolongation_order for rl, L in enumerate(levels)}
+    for rl in range(len(levels)):
+        for g in range(n_gf):
+            for r in sch.lists[rl][REF_REST]:  # fine rl+1 -> coarse rl
+                w.steps.append(Transfer(f"rest l{rl + 1}c{r.src}->l{rl}c{r.dst} g{g}", f"l{rl}c{r.dst}g{g}t0",
+                                        [f"l{rl + 1}c{r.src}g{g}t0"], [0.0], Box(*r.recv), Box(*r.send), 0.0, op,
+                                        centering, order[rl + 1], 0, phase=3 * rl))
+            for r in sch.lists[rl][REF_BND_PROL]:  # coarse rl-1 -> fine rl
+                w.steps.append(Transfer(f"bnd l{rl - 1}c{r.src}->l{rl}c{r.dst} g{g}", f"l{rl}c{r.dst}g{g}t0",
+                                        [f"l{rl - 1}c{r.src}g{g}t{tl}" for tl in range(3)], times,
+                                        Box(*r.recv), Box(*r.send), -0.5, op, centering, order[rl], 2,
+                                        phase=3 * rl + 1))
+            for r in sch.lists[rl][SYNC]:
+                w.steps.append(Transfer(f"sync l{rl}c{r.src}->c{r.dst} g{g}", f"l{rl}c{r.dst}g{g}t0",
+                                        [f"l{rl}c{r.src}g{g}t0"], [0.0], Box(*r.recv), Box(*r.send), 0.0, op,
+                                        centering, order[rl], 0, phase=3 * rl + 2))
+    w.schedule = sch
+    return w

@@ -1,0 +1,43 @@
+"""A level step of a real nested hierarchy, generated from its
+communication schedule (dh's lists: restriction, boundary prolongation with
+3-level time interpolation, sync; workloads.hierarchy_step), executed by the
+batch (one launch per phase and kernel class) and by one cg_transfer_from
+per region, both equal to the CPU oracle's replay bit for bit."""
+import pytest
+
+from helpers import first_diff, same
+from workload_check import run_oracle
+
+cg = pytest.importorskip("carpet_amd")
+from carpet_amd.workloads import hierarchy_step, nested_levels  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("cent,order", [(cg.VERTEX_CENTERED, 5), (cg.CELL_CENTERED, 3)], ids=["vertex-o5", "cell-o3"])
+def test_hierarchy_step_matches_oracle(cent, order):
+    lv = nested_levels(ncoarse=32, nlevels=3, split=(2, 1, 1), centering=cent, buffer=6, order=order)
+    w = hierarchy_step(lv, cent, n_gf=2)
+    print(f"{w.name}: {len(w.steps)} transfers", flush=True)
+    ref = run_oracle(w)
+    st = cg.Stream()
+    dev = w.allocate()
+    b = w.batch(dev)
+    b.finalize()
+    print(b.describe(), flush=True)
+    b.run(st)
+    st.synchronize()
+    for k in ref:
+        got = dev[k].download()
+        if not same(got, ref[k]):
+            raise AssertionError(f"batched: {k} differs {first_diff(got, ref[k])}")
+    b.close()
+    w.free(dev)
+    dev = w.allocate()
+    w.run_steps(dev)
+    cg.synchronize()
+    for k in ref:
+        got = dev[k].download()
+        if not same(got, ref[k]):
+            raise AssertionError(f"per region: {k} differs {first_diff(got, ref[k])}")
+    w.free(dev)

@@ -290,3 +290,31 @@ def test_schedule_rejects_what_dh_asserts():
     with pytest.raises(cg.CarpetError) as e:
         S.Schedule(V, (Z3, Z3), descs)
     assert e.value.code == -16
+
+
+@pytest.mark.parametrize("cent", [V, CC])
+def test_hierarchy_step_transfers_are_consistent(cent):
+    """The level step built from a schedule (workloads.hierarchy_step): every
+    destination and source region lies in its array, and within one phase
+    no two transfers write the same point of one array (dh's lists are
+    disjoint per destination)."""
+    from carpet_amd.workloads import hierarchy_step, nested_levels
+    lv = nested_levels(ncoarse=32, nlevels=3, split=(2, 1, 1), centering=cent, buffer=6, order=5 if cent == V else 3)
+    w = hierarchy_step(lv, cent, n_gf=1)
+    written = {}
+    for t in w.steps:
+        dbox = w.arrays[t.dst].box
+        assert all(dl <= rl and ru <= du for dl, rl, ru, du in zip(dbox.lower, t.dstreg.lower, t.dstreg.upper, dbox.upper))
+        sbox = w.arrays[t.srcs[0]].box
+        assert all(sl <= rl and ru <= su for sl, rl, ru, su in zip(sbox.lower, t.srcreg.lower, t.srcreg.upper, sbox.upper))
+        key = (t.phase, t.dst)
+        pts = set(itertools_product(t.dstreg))
+        assert not (written.get(key, set()) & pts), f"overlapping writes in phase {t.phase} to {t.dst}"
+        written.setdefault(key, set()).update(pts)
+    assert {t.phase for t in w.steps} >= {0, 2, 4, 5, 7, 8}
+
+
+def itertools_product(b):
+    import itertools
+    rng = [range(lo, up + 1, st) for lo, up, st in zip(b.lower, b.upper, b.stride)]
+    return itertools.product(*rng)
