@@ -208,6 +208,70 @@ def run_config4(n_regions: int, n_gf: int, steps: int, warmup: int, plain_steps:
             "batched_equals_plain_on_sample": bool(same), "sample_destinations": len(sample)}
 
 
+def run_config6(n_gf: int, steps: int, warmup: int, plain_steps: int, ncoarse: int = 128, tune_cache=None):
+    """Level step of a nested 3-level hierarchy (ncoarse^3 coarse points,
+    each finer level the central half of the previous, 8 components per
+    level, ghost 3, buffer 6, order-5 vertex Lagrange with 3-level time
+    interpolation) built from its communication schedule: restriction,
+    boundary prolongation and sync lists of dh, batched (one launch per
+    phase and class, hipGraph) vs one cg_transfer_from per region."""
+    import numpy as np
+
+    import carpet_amd as cg
+    from carpet_amd.workloads import hierarchy_step, nested_levels
+
+    t0 = time.perf_counter()
+    lv = nested_levels(ncoarse=ncoarse, nlevels=3, split=(2, 2, 2), centering=cg.VERTEX_CENTERED, buffer=6,
+                       order=5)
+    w = hierarchy_step(lv, cg.VERTEX_CENTERED, n_gf=n_gf)
+    dev = w.allocate(pool=True)
+    st = cg.Stream()
+    b = w.batch(dev)
+    b.finalize()
+    if tune_cache:
+        b.autotune(st, reps=3, cache_path=tune_cache)
+    desc = b.describe()
+    print(f"# config6: {w.name}: {len(w.steps)} transfers, {len(w.arrays)} arrays, set up in "
+          f"{time.perf_counter() - t0:.1f} s\n" + desc, file=sys.stderr, flush=True)
+    stats = b.stats()
+    ms_b = _time_graph(b, st, steps, warmup)
+
+    def reseed():
+        # later phases read what earlier ones write (restricted coarse points
+        # feed the next level's prolongation), so each comparison starts
+        # from the seeded state and runs one step
+        for k in w.outputs:
+            dev[k].fill(np.nan, stream=st)
+            dev[k].fill_hash(w.arrays[k].seed, stream=st)
+
+    sample = w.outputs[:: max(1, len(w.outputs) // 48)]
+    reseed()
+    b.run(st)
+    st.synchronize()
+    got_b = {k: dev[k].download() for k in sample}
+    reseed()
+    w.run_steps(dev, st)
+    st.synchronize()
+    same = all(np.array_equal(dev[k].download(), got_b[k], equal_nan=True) for k in sample)
+    e0, e1 = cg.Event(), cg.Event()
+    e0.record(st)
+    for _ in range(plain_steps):
+        w.run_steps(dev, st)
+    e1.record(st)
+    st.synchronize()
+    ms_p = e0.elapsed_ms(e1) / plain_steps
+    b.close()
+    w.free(dev)
+    pts, alg = stats["dst_points"], stats["alg_bytes"]
+    return {"transfers": stats["transfers"], "dst_points_per_step": pts, "algorithmic_bytes_per_step": alg,
+            "batched_ms": round(ms_b, 4), "batched_gpoints_per_s": round(pts / ms_b / 1e6, 3),
+            "batched_gbytes_per_s": round(alg / ms_b / 1e6, 1),
+            "batched_roofline_frac": round(alg / ms_b / 1e6 / HBM_PEAK_GBS, 4),
+            "plain_ms": round(ms_p, 4), "plain_gpoints_per_s": round(pts / ms_p / 1e6, 3),
+            "batched_vs_plain": round(ms_p / ms_b, 3), "batched_equals_plain_on_sample": bool(same),
+            "sample_destinations": len(sample)}
+
+
 def _time_graph(b, st, steps: int, warmup: int) -> float:
     """ms per replay of b's captured graph (HIP events on st)."""
     import carpet_amd as cg
@@ -345,9 +409,9 @@ def main(argv=None):
     ap.add_argument("--no-config4", action="store_true",
                     help="skip the 2000-region level step reported beside the headline line")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--config", type=int, default=2, choices=(2, 3, 4, 5),
+    ap.add_argument("--config", type=int, default=2, choices=(2, 3, 4, 5, 6),
                     help="2: single large patch (default, the headline line); 3: cell-centred conservative "
-                         "set; 4: batched uneven hierarchy; 5: mixed precision vs FP64")
+                         "set; 4: batched uneven hierarchy; 5: mixed precision vs FP64; 6: level step of a nested hierarchy from its communication schedule")
     ap.add_argument("--n-var", type=int, default=8, help="config 3 variables")
     ap.add_argument("--n-regions", type=int, default=2000)
     ap.add_argument("--tune-cache", default=None,
@@ -403,6 +467,20 @@ def main(argv=None):
                 "roofline": {"bound": "hbm", "achieved": mx["gbytes_per_s"], "peak": HBM_PEAK_GBS,
                              "unit": "GB/s", "frac": round(mx["gbytes_per_s"] / HBM_PEAK_GBS, 4),
                              "traffic": _profile_traffic("k_persist_tensor_vlag5_mixed_config3_shape")}}
+        print(json.dumps(line))
+        return 0
+    if args.config == 6:
+        r = run_config6(args.n_gf, args.steps, args.warmup, args.plain_steps, tune_cache=args.tune_cache)
+        line = {"metric": METRIC, "value": r["batched_gpoints_per_s"], "unit": "Gpoints/s",
+                "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": r["batched_ms"],
+                "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+                "data": "synthetic",
+                "config": {"workload": f"nested 3-level vertex hierarchy, 128^3 coarse points, 8 components per "
+                                       f"level, {args.n_gf} grid functions x 3 time levels; level step from the "
+                                       "communication schedule (restriction, boundary prolongation O5 + "
+                                       "3-level time interpolation, sync)", **r},
+                "roofline": {"bound": "hbm", "achieved": r["batched_gbytes_per_s"], "peak": HBM_PEAK_GBS,
+                             "unit": "GB/s", "frac": r["batched_roofline_frac"], "traffic": None}}
         print(json.dumps(line))
         return 0
     if args.config == 4:
