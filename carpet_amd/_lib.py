@@ -131,3 +131,4 @@ _decl("cg_schedule_count", i, vp, i, i)
 _decl("cg_schedule_region", i, vp, i, i, i, ip, ip, ip, ip)
 _decl("cg_schedule_component", i, vp, i, i, i, ip)
 _decl("cg_schedule_set", i, vp, i, i, i, i, ip)
+_decl("cg_schedule_regrid", i, vp, vp)

@@ -418,7 +418,7 @@ Set expanded_oactive(const Comp &obox, const Comp &box, const int *reffact, int 
 // prolongation from level orl into needrecv of component c (dh.cc:776-826
 // and 926-970)
 int prolong_list(cg_schedule &S, int rl, int c, Set needrecv, std::vector<SR> &out, Set *filled,
-                 const char *what) {
+                 const char *what, Set *left = nullptr) {
   Level &L = S.levels[rl];
   const Level &O = S.levels[rl - 1];
   const Comp &box = L.comps[c];
@@ -436,6 +436,10 @@ int prolong_list(cg_schedule &S, int rl, int c, Set needrecv, std::vector<SR> &o
     needrecv = needrecv - ovlp;
     if (filled)
       *filled = *filled | ovlp;
+  }
+  if (left) {
+    *left = needrecv;
+    return CG_OK;
   }
   SCHED_CHECK(needrecv.empty(), "%s: All points must have been received (level %d, component %d)", what, rl, c);
   return CG_OK;
@@ -675,4 +679,54 @@ extern "C" int cg_schedule_set(const cg_schedule *s, int rl, int c, int which, i
     for (int i = 0; i < std::min(n, cap); ++i)
       st->b[i].to(bboxes + 9 * i);
   return n;
+}
+
+// Regridding schedule (dh.cc:1712-1850): the active and overlap points of
+// every new component, first from the owned regions of the same level's old
+// components (fast_old2new_sync_sendrecv), the rest by prolongation from
+// the new coarser level (fast_old2new_ref_prol_sendrecv).
+extern "C" int cg_schedule_regrid(cg_schedule *s, const cg_schedule *old) {
+  if (!s)
+    return set_error(CG_ERR_ARG, "null schedule");
+  if (old && old->centering != s->centering)
+    return set_error(CG_ERR_ARG, "old and new hierarchies differ in centering");
+  const bool have_old = old && !old->levels.empty();
+  if (old)
+    for (size_t rl = 0; rl < std::min(s->levels.size(), old->levels.size()); ++rl)
+      for (int d = 0; d < 3; ++d)
+        if (old->levels[rl].domain.st[d] != s->levels[rl].domain.st[d])
+          return set_error(CG_ERR_STRIDE, "old and new hierarchies differ in the stride of level %zu", rl);
+  for (size_t rl = 0; rl < s->levels.size(); ++rl) {
+    Level &L = s->levels[rl];
+    L.lists[CG_SCHED_OLD2NEW_SYNC].clear();
+    L.lists[CG_SCHED_OLD2NEW_PROL].clear();
+  }
+  for (size_t rl = 0; rl < s->levels.size(); ++rl) {
+    Level &L = s->levels[rl];
+    for (size_t c = 0; c < L.comps.size(); ++c) {
+      const Comp &box = L.comps[c];
+      Set needrecv = box.active | box.overlaps;
+      if (old && rl < old->levels.size()) {
+        const Level &OL = old->levels[rl];
+        for (size_t cc = 0; cc < OL.comps.size() && !needrecv.empty(); ++cc) {
+          const Set ovlp = needrecv & OL.comps[cc].owned;
+          for (const IB &recv : ovlp.b)
+            L.lists[CG_SCHED_OLD2NEW_SYNC].push_back(SR{recv, (int)cc, recv, (int)c});
+          needrecv = needrecv - ovlp;
+        }
+      }
+      if (rl > 0 && !needrecv.empty()) {
+        Set left;
+        if (int e = prolong_list(*s, (int)rl, (int)c, needrecv, L.lists[CG_SCHED_OLD2NEW_PROL], nullptr,
+                                 "Regridding prolongation", &left))
+          return e;
+        needrecv = left;
+      }
+      if (have_old)
+        SCHED_CHECK(needrecv.empty(),
+                    "Regridding prolongation: All points must have been received (level %zu, component %zu)",
+                    rl, c);
+    }
+  }
+  return CG_OK;
 }

@@ -18,8 +18,9 @@ from typing import Dict, List, Sequence, Tuple
 from ._lib import I9, LevelSpec, check, lib
 from .core import Box
 
-REF_PROL, SYNC, REF_BND_PROL, REF_REST = range(4)
-LIST_NAMES = {REF_PROL: "ref_prol", SYNC: "sync", REF_BND_PROL: "ref_bnd_prol", REF_REST: "ref_rest"}
+REF_PROL, SYNC, REF_BND_PROL, REF_REST, OLD2NEW_SYNC, OLD2NEW_PROL = range(6)
+LIST_NAMES = {REF_PROL: "ref_prol", SYNC: "sync", REF_BND_PROL: "ref_bnd_prol", REF_REST: "ref_rest",
+              OLD2NEW_SYNC: "old2new_sync", OLD2NEW_PROL: "old2new_prol"}
 EXTERIOR, INTERIOR, OWNED, COMMUNICATED = range(4)
 GHOSTS, BUFFERS, OVERLAPS, ACTIVE, SYNCED, BNDREF, BOUNDARIES = range(10, 17)
 
@@ -92,8 +93,12 @@ class Schedule:
         check(lib.cg_schedule_build(centering, bw, len(levels), specs, int(bool(use_higher_order_restriction)),
                                     restriction_order_space, C.byref(h)), "cg_schedule_build")
         self._h = h
+        self._read_lists()
+
+    def _read_lists(self):
+        h = self._h
         self.lists: List[Dict[int, List[Region]]] = []
-        for rl in range(len(levels)):
+        for rl in range(len(self.levels)):
             per = {}
             for kind in LIST_NAMES:
                 n = check(lib.cg_schedule_count(h, rl, kind), "cg_schedule_count")
@@ -106,6 +111,13 @@ class Schedule:
                     out.append(Region(s.value, _from9(sb), d.value, _from9(rb)))
                 per[kind] = out
             self.lists.append(per)
+
+    def regrid(self, old: "Schedule | None"):
+        """The regridding lists of this hierarchy after a regrid from old
+        (cg_schedule_regrid): OLD2NEW_SYNC from old's components,
+        OLD2NEW_PROL from this hierarchy's next coarser level."""
+        check(lib.cg_schedule_regrid(self._h, old._h if old is not None else None), "cg_schedule_regrid")
+        self._read_lists()
 
     def component(self, rl: int, c: int, which: int = EXTERIOR) -> tuple:
         b = I9()

@@ -502,16 +502,20 @@ class Config5:
 
 
 def nested_levels(ncoarse: int = 32, nlevels: int = 3, split=(2, 1, 1), centering: int = VERTEX_CENTERED,
-                  ghost: int = 3, buffer: int = 6, order: int = 5, fraction: float = 0.5):
+                  ghost: int = 3, buffer: int = 6, order: int = 5, fraction: float = 0.5, maxlevels: int = 0):
     """A nested hierarchy as CarpetLib's gh would hold it (schedule.LevelDesc
     per level): level 0 covers the domain of ncoarse coarse points per axis,
     every finer level (refinement factor 2) the central `fraction` of the
     previous level's region; each level's region is tiled into
     split[0] x split[1] x split[2] components.  Vertex centring puts level rl
     on stride 2^(nlevels-1-rl); cell centring on stride 2^(nlevels-rl) with
-    cell centres at odd multiples of half the stride."""
+    cell centres at odd multiples of half the stride.  maxlevels (default
+    nlevels) fixes the stride of the finest possible level, as Carpet's
+    maximum number of refinement levels does, so hierarchies with different
+    numbers of levels share one index space."""
     from .schedule import LevelDesc
-    ns = 2 ** (nlevels - 1) if centering == VERTEX_CENTERED else 2 ** nlevels
+    top = max(maxlevels, nlevels)
+    ns = 2 ** (top - 1) if centering == VERTEX_CENTERED else 2 ** top
     levels = []
     plo, phi = None, None
     for rl in range(nlevels):
@@ -588,4 +592,49 @@ def hierarchy_step(levels, centering: int = VERTEX_CENTERED, n_gf: int = 2, seed
                                         [f"l{rl}c{r.src}g{g}t0"], [0.0], Box(*r.recv), Box(*r.send), 0.0, op,
                                         centering, order[rl], 0, phase=3 * rl + 2))
     w.schedule = sch
+    return w
+
+
+def recompose_step(old_levels, new_levels, centering: int = VERTEX_CENTERED, n_gf: int = 2, seed: int = 8,
+                   boundary=((0, 0, 0), (0, 0, 0)), op: int = OP_LAGRANGE) -> Workload:
+    """The fill of a new hierarchy's storage after a regrid
+    (ggf::recompose_fill, CarpetLib/src/ggf.cc:175-215) from the regridding
+    lists (schedule.Schedule.regrid): for every new level in turn and each of
+    its 3 time levels, the copies from the old components of that level
+    (phase 2rl), then prolongation from the new coarser level, whose 3 time
+    levels at t = 0, -1, -2 are interpolated to the fine time level's time
+    0, -1/2, -1 (phase 2rl+1).  Old and new arrays are seeded (new ones stand
+    for freshly allocated storage); the outputs are the new arrays."""
+    from .schedule import OLD2NEW_PROL, OLD2NEW_SYNC, Schedule
+    s_old = Schedule(centering, boundary, old_levels)
+    s_new = Schedule(centering, boundary, new_levels)
+    s_new.regrid(s_old)
+    w = Workload(f"recompose[{len(old_levels)} -> {len(new_levels)} levels x {n_gf}]")
+    for tag, sch, levels in (("o", s_old, old_levels), ("n", s_new, new_levels)):
+        for rl, L in enumerate(levels):
+            for c in range(len(L.components)):
+                ext = Box(*sch.component(rl, c, 0))
+                own = Box(*sch.component(rl, c, 2))
+                for g in range(n_gf):
+                    for tl in range(3):
+                        nm = f"{tag}l{rl}c{c}g{g}t{tl}"
+                        w.arrays[nm] = ArraySpec(ext, owned=own, seed=seed * 7919 + hash(nm) % 1000003)
+                        if tag == "n":
+                            w.outputs.append(nm)
+    ctimes = [0.0, -1.0, -2.0]
+    for rl, L in enumerate(new_levels):
+        for g in range(n_gf):
+            for tl in range(3):
+                for r in s_new.lists[rl][OLD2NEW_SYNC]:
+                    w.steps.append(Transfer(f"old2new l{rl} c{r.src}->c{r.dst} g{g} t{tl}",
+                                            f"nl{rl}c{r.dst}g{g}t{tl}", [f"ol{rl}c{r.src}g{g}t{tl}"], [0.0],
+                                            Box(*r.recv), Box(*r.send), 0.0, op, centering, L.prolongation_order,
+                                            0, phase=2 * rl))
+                for r in s_new.lists[rl][OLD2NEW_PROL]:
+                    w.steps.append(Transfer(f"regrid prol l{rl - 1}c{r.src}->l{rl}c{r.dst} g{g} t{tl}",
+                                            f"nl{rl}c{r.dst}g{g}t{tl}",
+                                            [f"nl{rl - 1}c{r.src}g{g}t{k}" for k in range(3)], ctimes,
+                                            Box(*r.recv), Box(*r.send), -0.5 * tl, op, centering,
+                                            L.prolongation_order, 2, phase=2 * rl + 1))
+    w.schedule = (s_old, s_new)
     return w

@@ -431,7 +431,7 @@ typedef struct cg_level_spec {
   int prolongation_order;   /* prolongation_orders_space[rl]; stencil = order / 2 */
 } cg_level_spec;
 enum { CG_SCHED_REF_PROL = 0, CG_SCHED_SYNC = 1, CG_SCHED_REF_BND_PROL = 2, CG_SCHED_REF_REST = 3,
-       CG_SCHED_NLISTS = 4 };
+       CG_SCHED_OLD2NEW_SYNC = 4, CG_SCHED_OLD2NEW_PROL = 5, CG_SCHED_NLISTS = 6 };
 /* boxes of a component (cg_schedule_component) */
 enum { CG_SCHED_EXTERIOR = 0, CG_SCHED_INTERIOR = 1, CG_SCHED_OWNED = 2, CG_SCHED_COMMUNICATED = 3 };
 /* region sets of a component (cg_schedule_set) */
@@ -454,6 +454,16 @@ int cg_schedule_component(const cg_schedule *s, int rl, int c, int which, int bb
 /* boxes of a region set: writes min(n, cap) boxes (9 ints each) when bboxes
    is not NULL and returns n */
 int cg_schedule_set(const cg_schedule *s, int rl, int c, int which, int cap, int *bboxes);
+/* regridding schedule of s after a regrid from old (dh.cc:1712-1850, the
+   lists ggf::recompose_fill walks, ggf.cc:175-215): for every component of
+   s, its active and overlap points from the owned regions of old's
+   components on the same level (CG_SCHED_OLD2NEW_SYNC, send = recv, the
+   source component indexes old), the rest by prolongation from s's next
+   coarser level (CG_SCHED_OLD2NEW_PROL).  old may be NULL (initial data:
+   prolongation only, nothing asserted); with an old hierarchy every point
+   must be filled (CG_ERR_SCHEDULE otherwise).  Replaces earlier regrid
+   lists of s. */
+int cg_schedule_regrid(cg_schedule *s, const cg_schedule *old);
 
 #ifdef __cplusplus
 }

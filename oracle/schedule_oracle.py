@@ -275,7 +275,8 @@ def build(centering, boundary, levels, ho_restrict=False, restrict_order=1):
                 k.buffers, k.overlaps, k.active = om & allbuffers, om & alloverlaps, om & L.allactive
             k.sync = lat.empty()
             k.bndref = lat.empty()
-        L.lists = {"ref_prol": {}, "sync": {}, "ref_bnd_prol": {}, "ref_rest": {}}
+        L.lists = {"ref_prol": {}, "sync": {}, "ref_bnd_prol": {}, "ref_rest": {}, "old2new_sync": {},
+                   "old2new_prol": {}}
 
     def prolong(rl, c, needrecv, name, filled):
         L, O = levels[rl], levels[rl - 1]
@@ -371,3 +372,47 @@ def _contract_box(b, t):
         rlo.append(a)
         rup.append(z)
     return (tuple(rlo), tuple(rup), tuple(ts))
+
+
+def regrid(centering, new_levels, old_levels):
+    """Regridding schedule (dh.cc:1712-1850) of new_levels (built by build())
+    after a regrid from old_levels (built, or None): per new component the
+    active and overlap points, first from the owned regions of the old
+    components of the same level ("old2new_sync", keys (old component, new
+    component)), the rest by prolongation from the new coarser level
+    ("old2new_prol")."""
+    have_old = old_levels is not None and len(old_levels) > 0
+    rel = [None] + [tuple(new_levels[rl].reffact[d] // new_levels[rl - 1].reffact[d] for d in range(3))
+                    for rl in range(1, len(new_levels))]
+    for rl, L in enumerate(new_levels):
+        lat = L.lat
+        L.lists["old2new_sync"] = {}
+        L.lists["old2new_prol"] = {}
+        for c, box in enumerate(L.comps):
+            needrecv = box.active | box.overlaps
+            if have_old and rl < len(old_levels):
+                for cc, obox in enumerate(old_levels[rl].comps):
+                    if not needrecv.any():
+                        break
+                    ovlp = needrecv & lat.box(obox.owned)
+                    if ovlp.any():
+                        _add(L.lists["old2new_sync"], (cc, c), ovlp)
+                    needrecv &= ~ovlp
+            if rl > 0 and needrecv.any():
+                O = new_levels[rl - 1]
+                e = [rel[rl][d] if centering == VERTEX else rel[rl][d] - 1 for d in range(3)]
+                sten = (L.order // 2,) * 3
+                for cc, obox in enumerate(O.comps):
+                    if not needrecv.any():
+                        break
+                    exp = lat.expand(O.lat.expanded_for(obox.active, lat), e, e)
+                    ovlp = needrecv & exp
+                    if ovlp.any():
+                        foot = O.lat.expand(lat.expanded_for(ovlp, O.lat), sten, sten)
+                        _check(not (foot & ~O.lat.box(obox.exterior)).any(),
+                               "Regridding prolongation: Send region must be contained in exterior")
+                        _add(L.lists["old2new_prol"], (cc, c), ovlp)
+                    needrecv &= ~ovlp
+            if have_old:
+                _check(not needrecv.any(), "Regridding prolongation: All points must have been received")
+    return new_levels

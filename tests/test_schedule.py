@@ -318,3 +318,79 @@ def itertools_product(b):
     import itertools
     rng = [range(lo, up + 1, st) for lo, up, st in zip(b.lower, b.upper, b.stride)]
     return itertools.product(*rng)
+
+
+def _olv(descs):
+    return [so.Level(L.domain, L.components, L.reffact, L.ghost_width, L.buffer_width, L.overlap_width,
+                     L.prolongation_order) for L in descs]
+
+
+@pytest.mark.parametrize("cent", [V, CC])
+@pytest.mark.parametrize("seed", range(8))
+def test_regrid_schedule_matches_oracle(cent, seed):
+    """Regridding lists (dh.cc:1712-1850) from an old to a new nested
+    hierarchy: copies from the old components of each level, prolongation
+    from the new coarser level for the rest; library against the oracle."""
+    from carpet_amd.workloads import nested_levels
+    rng = np.random.default_rng(500 + 10 * cent + seed)
+    nold = int(rng.integers(1, 4))
+    nnew = int(rng.integers(2, 4))
+    splits = [(2, 1, 1), (1, 2, 1), (1, 1, 2), (2, 2, 1), (1, 1, 1)]
+    order = 5 if cent == V else 3
+    kw = dict(ncoarse=int(rng.integers(28, 36)), centering=cent, buffer=int(rng.choice([3, 6])), order=order,
+              maxlevels=3)
+    old = nested_levels(nlevels=nold, split=splits[int(rng.integers(0, 5))],
+                        fraction=float(rng.choice([0.45, 0.5, 0.6])), **kw)
+    new = nested_levels(nlevels=nnew, split=splits[int(rng.integers(0, 5))],
+                        fraction=float(rng.choice([0.5, 0.6, 0.7])), **kw)
+    print(f"old {nold} levels, new {nnew} levels", flush=True)
+    bnd = (Z3, Z3)
+    so_old = so.build(cent, bnd, _olv(old))
+    so_new = so.build(cent, bnd, _olv(new))
+    s_old = S.Schedule(cent, bnd, old)
+    s_new = S.Schedule(cent, bnd, new)
+    try:
+        so.regrid(cent, so_new, so_old)
+        oerr = None
+    except so.ScheduleError as e:
+        oerr = e
+    try:
+        s_new.regrid(s_old)
+        perr = None
+    except cg.CarpetError as e:
+        perr = e
+    assert (oerr is None) == (perr is None), f"oracle: {oerr}; library: {perr}"
+    if oerr is not None:
+        return
+    for rl, L in enumerate(so_new):
+        lat = L.lat
+        for kind, name in ((S.OLD2NEW_SYNC, "old2new_sync"), (S.OLD2NEW_PROL, "old2new_prol")):
+            got = {}
+            for r in s_new.lists[rl][kind]:
+                m = lat.box(r.recv)
+                key = (r.src, r.dst)
+                assert not (got.get(key, lat.empty()) & m).any()
+                got[key] = got.get(key, lat.empty()) | m
+                if kind == S.OLD2NEW_SYNC:
+                    assert r.send == r.recv
+                else:
+                    O = so_new[rl - 1]
+                    send = O.lat.box(r.send)
+                    assert not (send & ~O.lat.box(O.comps[r.src].exterior)).any()
+                    foot = O.lat.expand(lat.expanded_for(m, O.lat), (L.order // 2,) * 3, (L.order // 2,) * 3)
+                    assert not (foot & ~send).any()
+            exp = L.lists[name]
+            assert set(got) == set(exp), (rl, name)
+            for key in exp:
+                assert np.array_equal(got[key], exp[key]), (rl, name, key)
+    # every active / overlap point of the new hierarchy is filled exactly once
+    for rl, L in enumerate(so_new):
+        filled = L.lat.empty()
+        for name in ("old2new_sync", "old2new_prol"):
+            for m in L.lists[name].values():
+                assert not (filled & m).any()
+                filled |= m
+        want = L.lat.empty()
+        for k in L.comps:
+            want |= k.active | k.overlaps
+        assert np.array_equal(filled, want)
