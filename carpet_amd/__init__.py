@@ -9,6 +9,7 @@ from .core import (  # noqa: F401
     CELL_CENTERED,
     OP_COPY,
     OP_ENO,
+    OP_WENO,
     OP_LAGRANGE,
     OP_NONE,
     OP_RESTRICT,
@@ -29,6 +30,7 @@ from .core import (  # noqa: F401
     interpolate_eno_3d_3tl,
     pad_shape,
     prolongate_3d_cc_eno_rf2,
+    prolongate_3d_eno,
     prolongate_3d_cc_rf2,
     prolongate_3d_cc_tvd,
     prolongate_3d_rf2,

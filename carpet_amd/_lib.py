@@ -76,6 +76,7 @@ _decl("cg_prolongate_3d_cc_eno_rf2_f64", i, i, *_OPARGS, vp)
 _decl("cg_prolongate_3d_cc_tvd_f64", i, *_OPARGS, vp)
 _decl("cg_restrict_3d_rf2_f64", i, *_OPARGS, vp)
 _decl("cg_restrict_3d_cc_rf2_f64", i, *_OPARGS, vp)
+_decl("cg_prolongate_3d_eno_f64", i, *_OPARGS, vp)
 _decl("cg_restrict_3d_cc_o3_rf2_f64", i, *_OPARGS, vp)
 _decl("cg_restrict_3d_cc_o5_rf2_f64", i, *_OPARGS, vp)
 _decl("cg_set_restriction", i, i, i)

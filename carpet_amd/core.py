@@ -286,6 +286,14 @@ def prolongate_3d_cc_eno_rf2(order: int, src: DeviceArray, dst: DeviceArray, dst
           "prolongate_3d_cc_eno_rf2")
 
 
+def prolongate_3d_eno(src: DeviceArray, dst: DeviceArray, dstregbbox: Box,
+                      srcregbbox: Optional[Box] = None, stream=None):
+    """Vertex-centred ENO (prolongate_3d_real8_eno.F90)."""
+    check(lib.cg_prolongate_3d_eno_f64(*_arr(src), *_arr(dst), src.box.as_c(), dst.box.as_c(),
+                                       (srcregbbox or dstregbbox).as_c(), dstregbbox.as_c(), _sh(stream)),
+          "prolongate_3d_eno")
+
+
 def prolongate_3d_cc_tvd(src: DeviceArray, dst: DeviceArray, dstregbbox: Box,
                          srcregbbox: Optional[Box] = None, stream=None):
     check(lib.cg_prolongate_3d_cc_tvd_f64(*_arr(src), *_arr(dst), src.box.as_c(), dst.box.as_c(),

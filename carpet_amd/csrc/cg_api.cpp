@@ -105,17 +105,41 @@ int resolve_transfer(const cg_gdata *dst, const cg_gdata *const *srcs,
       }
       break;
     case CG_OP_ENO:
-      if (cent == CG_VERTEX_CENTERED)
-        return set_error(CG_ERR_OPERATOR, "vertex-centred ENO is not provided");
+      if (cent == CG_VERTEX_CENTERED) {
+        // prolongate_3d_eno for order_space 3 and 5 (data.cc:658-690)
+        if (order_space == 1)
+          return set_error(CG_ERR_ORDER, "There is no stencil for op=\"ENO\" with order_space=1");
+        if (order_space != 3 && order_space != 5)
+          return set_error(CG_ERR_ORDER, "There is no stencil for op=\"ENO\" with order_space!=3");
+        kind = PK_VENO;
+        order = 3;
+        break;
+      }
       if (order_space < 2 || order_space > 5)
         return set_error(CG_ERR_ORDER, "There is no cell-centred stencil for "
                                        "op=\"ENO\" with order_space not in {2,3,4,5}");
       kind = PK_ENO;
       order = order_space == 5 ? 3 : 2; // data.cc:697-704
       break;
+    case CG_OP_WENO:
+      // vertex-centred WENO runs prolongate_3d_eno for order_space 5
+      // (data.cc:727-762)
+      if (cent != CG_VERTEX_CENTERED)
+        return set_error(CG_ERR_OPERATOR, "There are currently no cell-centred stencils for op=\"WENO\"");
+      if (order_space == 1 || order_space == 3)
+        return set_error(CG_ERR_ORDER, "There is no stencil for op=\"WENO\" with order_space=%d",
+                         order_space);
+      if (order_space != 5)
+        return set_error(CG_ERR_ORDER, "There is no stencil for op=\"WENO\" with order_space!=5");
+      kind = PK_VENO;
+      order = 3;
+      break;
     case CG_OP_TVD:
+      // the reference's vertex-centred TVD (prolongate_3d_real8_tvd.F90)
+      // aborts unconditionally
       if (cent == CG_VERTEX_CENTERED)
-        return set_error(CG_ERR_OPERATOR, "vertex-centred TVD is not provided");
+        return set_error(CG_ERR_OPERATOR, "vertex-centred TVD: the reference routine aborts (\"This "
+                                          "routine has been resurrected from an older version of Carpet\")");
       if (order_space != 1)
         return set_error(CG_ERR_ORDER, "There is no stencil for op=\"TVD\" with order_space!=1");
       kind = PK_TVD;
@@ -517,6 +541,15 @@ int cg_prolongate_3d_cc_eno_rf2_f64(int order, const double *src,
   (void)srcregbbox;
   return prolong_call(PK_ENO, order, src, srcpadext, srcext, dst, dstpadext,
                       dstext, srcbbox, dstbbox, dstregbbox, stream);
+}
+
+int cg_prolongate_3d_eno_f64(const double *src, const int srcpadext[3], const int srcext[3],
+                             double *dst, const int dstpadext[3], const int dstext[3],
+                             const int srcbbox[9], const int dstbbox[9], const int srcregbbox[9],
+                             const int dstregbbox[9], cg_stream_t stream) {
+  (void)srcregbbox;
+  return prolong_call(PK_VENO, 3, src, srcpadext, srcext, dst, dstpadext, dstext, srcbbox, dstbbox,
+                      dstregbbox, stream);
 }
 
 int cg_prolongate_3d_cc_tvd_f64(const double *src, const int srcpadext[3],

@@ -205,6 +205,7 @@ const void *launch_fn(const BatchLaunch &l, int *threads) {
   case LK_VLAG: return n ? batch_tensor_fn_kind<LK_VLAG, 1>(l, threads) : batch_tensor_fn_kind<LK_VLAG, 0>(l, threads);
   case LK_CLAG: return n ? batch_tensor_fn_kind<LK_CLAG, 1>(l, threads) : batch_tensor_fn_kind<LK_CLAG, 0>(l, threads);
   case LK_ENO: return n ? batch_tensor_fn_kind<LK_ENO, 1>(l, threads) : batch_tensor_fn_kind<LK_ENO, 0>(l, threads);
+  case LK_VENO: return n ? batch_tensor_fn_kind<LK_VENO, 1>(l, threads) : batch_tensor_fn_kind<LK_VENO, 0>(l, threads);
   }
   return nullptr;
 }
@@ -307,7 +308,7 @@ int batch_device_build(BatchDevice &dev, const std::vector<Transfer> &tr,
     return off;
   };
   for (const Group &g : groups) {
-    if (g.prec != CG_PREC_F64 && (!g.is_tensor || g.kind == LK_ENO))
+    if (g.prec != CG_PREC_F64 && (!g.is_tensor || fp64_only(g.kind)))
       return set_error(CG_ERR_OPERATOR, "mixed precision applies to Lagrange prolongation only");
     BatchLaunch l{};
     l.phase = g.phase;
@@ -583,6 +584,10 @@ static int run_launch(BatchDevice &dev, const BatchLaunch &l, hipStream_t st) {
     case LK_ENO:
       rc = n ? batch_tensor_kind<LK_ENO, 1>(l, td, items, seg, ctr, st)
              : batch_tensor_kind<LK_ENO, 0>(l, td, items, seg, ctr, st);
+      break;
+    case LK_VENO:
+      rc = n ? batch_tensor_kind<LK_VENO, 1>(l, td, items, seg, ctr, st)
+             : batch_tensor_kind<LK_VENO, 0>(l, td, items, seg, ctr, st);
       break;
     default: rc = -1;
     }

@@ -78,7 +78,7 @@ int batch_tensor_kind(const BatchLaunch &l, const TensorArgs *td, const Item *it
     };
     if (l.prec == 0)
       return by_ntl(std::integral_constant<int, 0>{});
-    if constexpr (KIND != LK_ENO) {
+    if constexpr (!fp64_only(KIND)) {
       if (l.prec == 1)
         return by_ntl(std::integral_constant<int, 1>{});
       if (l.prec == 2)
@@ -104,7 +104,7 @@ template <int KIND, int SH> const void *batch_tensor_fn_kind(const BatchLaunch &
     auto by_prec = [&](auto ntlc) {
       if (l.prec == 0)
         pick(ntlc, std::integral_constant<int, 0>{});
-      if constexpr (KIND != LK_ENO) {
+      if constexpr (!fp64_only(KIND)) {
         if (l.prec == 1)
           pick(ntlc, std::integral_constant<int, 1>{});
         if (l.prec == 2)

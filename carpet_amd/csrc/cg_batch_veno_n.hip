@@ -1,0 +1,9 @@
+// cg_batch_veno_n.hip -- batched tensor launches for LK_VENO line operators,
+// 16x16-base tiles.
+#include "cg_batch_tensor.hpp"
+
+namespace cg {
+template int batch_tensor_kind<LK_VENO, 1>(const BatchLaunch &, const TensorArgs *, const Item *,
+                                            const int *, int *, hipStream_t);
+template const void *batch_tensor_fn_kind<LK_VENO, 1>(const BatchLaunch &, int *);
+} // namespace cg

@@ -18,12 +18,13 @@ long long tensor_tiles(TensorArgs &a, const TileCfg &cfg) {
 
 int launch_tensor(int kind, int order, int ntl, int prec, const TensorArgs &a,
                   const TileCfg &cfg, hipStream_t s) {
-  if (prec < 0 || prec > 2 || (prec != 0 && kind == LK_ENO))
+  if (prec < 0 || prec > 2 || (prec != 0 && fp64_only(kind)))
     return -1;
   switch (kind) {
   case LK_VLAG: return launch_tensor_kind<LK_VLAG>(order, ntl, prec, a, cfg, s);
   case LK_CLAG: return launch_tensor_kind<LK_CLAG>(order, ntl, prec, a, cfg, s);
   case LK_ENO: return launch_tensor_kind<LK_ENO>(order, ntl, prec, a, cfg, s);
+  case LK_VENO: return launch_tensor_kind<LK_VENO>(order, ntl, prec, a, cfg, s);
   }
   return -1;
 }

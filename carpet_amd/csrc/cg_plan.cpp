@@ -98,6 +98,10 @@ static LineInfo line_info_rt(int kind, int order, bool &ok) {
     case 3: return line_info<LK_ENO, 3>();
     }
     break;
+  case PK_VENO:
+    if (order == 3)
+      return line_info<LK_VENO, 3>();
+    break;
   }
   ok = false;
   return LineInfo{};
@@ -197,9 +201,10 @@ int plan_prolong(int kind, int order, const Arr &src, const void *const *srcs,
     return set_error(CG_ERR_ORDER, "There is no %s stencil with order_space=%d",
                      kind == PK_VLAG ? "vertex-centred Lagrange"
                      : kind == PK_CLAG ? "cell-centred Lagrange"
+                     : kind == PK_VENO ? "vertex-centred ENO"
                                        : "cell-centred ENO",
                      order);
-  const bool cell = kind != PK_VLAG;
+  const bool cell = kind == PK_CLAG || kind == PK_ENO;
   for (int d = 0; d < 3; ++d)
     if (sb.str[d] <= reg.str[d] || db.str[d] != reg.str[d])
       return set_error(CG_ERR_STRIDE, "Internal error: strides disagree");

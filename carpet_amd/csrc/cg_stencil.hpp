@@ -13,7 +13,8 @@
 //
 // Sources: CarpetLib/src/prolongate_3d_rf2.cc:27-124, 158-295 (vertex
 // Lagrange), CarpetLib/src/prolongate_3d_cc_rf2.cc:49-195 (cell Lagrange),
-// CarpetLib/src/prolongate_3d_cc_eno_rf2.cc:51-881 (cell ENO).
+// CarpetLib/src/prolongate_3d_cc_eno_rf2.cc:51-881 (cell ENO),
+// CarpetLib/src/prolongate_3d_real8_eno.F90:8-67, 223-288 (vertex ENO).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -22,7 +23,11 @@ namespace cg {
 
 #define CG_HD __host__ __device__ __forceinline__
 
-enum LineKind { LK_VLAG = 0, LK_CLAG = 1, LK_ENO = 2 };
+// LK_VENO is 4: kinds share numbers with the prolongation kinds of the
+// planner, where 3 is the (point-wise) cell TVD
+enum LineKind { LK_VLAG = 0, LK_CLAG = 1, LK_ENO = 2, LK_VENO = 4 };
+// line kinds that exist in FP64 only (no FP32-storage modes)
+constexpr bool fp64_only(int kind) { return kind == LK_ENO || kind == LK_VENO; }
 
 // ---- weights -------------------------------------------------------------
 template <int O> struct VW;
@@ -270,6 +275,44 @@ template <int O> struct Line<LK_ENO, O> {
       res = linear<P>(v);
     return res;
   }
+};
+
+// vertex-centred ENO of the midpoint between q2 and q3
+// (prolongate_3d_real8_eno.F90:8-67): the quadratic through q1..q3 or
+// q2..q4, whichever has the smaller second difference (the right one on a
+// tie), evaluated at the midpoint; the average of q2 and q3 where that
+// value leaves [q2, q3] or the second differences differ in sign or vanish.
+// Evaluated as written, every product and sum rounded on its own.
+CG_HD double eno1d_v(double q1, double q2, double q3, double q4) {
+  const double dl = (q1 + q3) - 2.0 * q2;
+  const double dr = (q2 + q4) - 2.0 * q3;
+  double e;
+  if (__builtin_fabs(dl) < __builtin_fabs(dr))
+    e = 0.125 * ((-q1 + 6.0 * q2) + 3.0 * q3);
+  else
+    e = 0.125 * ((3.0 * q2 + 6.0 * q3) - q4);
+  if ((e - q2) * (q3 - e) < 0.0 || dl * dr <= 0.0)
+    e = 0.5 * (q2 + q3);
+  return e;
+}
+
+// vertex-centred ENO (prolongate_3d_real8_eno.F90:119-282): a coincident
+// child passes the coarse value through, a midpoint child is eno1d of the
+// coarse points b-1..b+2; one scheme (the dispatch maps ENO order_space 3
+// and 5 and WENO order_space 5 onto it, data.cc:654-762)
+template <int O> struct Line<LK_VENO, O> {
+  static constexpr int TMIN[2] = {0, -1};
+  static constexpr int NTAP[2] = {1, 4};
+  static constexpr int TMINALL = -1;
+  static constexpr int TMAXALL = 2;
+  static constexpr int R = 4;
+  template <int P, class V> static CG_HD double apply(V v) {
+    if constexpr (P == 0)
+      return v(0);
+    else
+      return eno1d_v(v(0), v(1), v(2), v(3));
+  }
+  template <int P, class V> static CG_HD float apply_f(V v) { return (float)apply<P>(v); }
 };
 
 // host-side view of a line operator's windows (for validation and

@@ -52,7 +52,7 @@ int launch_tensor_kind(int order, int ntl, int prec, const TensorArgs &a, const 
     };
     if (prec == 0)
       return by_ntl(std::integral_constant<int, 0>{});
-    if constexpr (KIND != LK_ENO) {
+    if constexpr (!fp64_only(KIND)) {
       if (prec == 1)
         return by_ntl(std::integral_constant<int, 1>{});
       if (prec == 2)

@@ -684,6 +684,9 @@ template <int KIND, class Fn> int with_order(int order, Fn &&fn) {
     case 2: return fn(Line<LK_ENO, 2>{});
     case 3: return fn(Line<LK_ENO, 3>{});
     }
+  } else if constexpr (KIND == LK_VENO) {
+    if (order == 3)
+      return fn(Line<LK_VENO, 3>{});
   }
   return -1;
 }
@@ -695,6 +698,7 @@ template <class Fn> int with_line(int kind, int order, Fn &&fn) {
   case LK_VLAG: return with_order<LK_VLAG>(order, fn);
   case LK_CLAG: return with_order<LK_CLAG>(order, fn);
   case LK_ENO: return with_order<LK_ENO>(order, fn);
+  case LK_VENO: return with_order<LK_VENO>(order, fn);
   }
   return -1;
 }

@@ -197,6 +197,17 @@ int cg_prolongate_3d_cc_rf2_f64(int order, const double *src,
                                 const int dstbbox[9], const int srcregbbox[9],
                                 const int dstregbbox[9], cg_stream_t stream);
 
+/* prolongate_3d_eno<CCTK_REAL8> (vertex-centred ENO, refinement factor 2):
+   CarpetLib/src/data.cc:201-215 -> prolongate_3d_real8_eno.F90:69-288.
+   The dispatch (cg_transfer_from) uses it for vertex-centred op ENO with
+   order_space 3 or 5 and op WENO with order_space 5, as data.cc:654-762
+   does.  The stencil footprint (coarse points b-1..b+2 around every
+   midpoint) must lie in the source array. */
+int cg_prolongate_3d_eno_f64(const double *src, const int srcpadext[3], const int srcext[3],
+                             double *dst, const int dstpadext[3], const int dstext[3],
+                             const int srcbbox[9], const int dstbbox[9], const int srcregbbox[9],
+                             const int dstregbbox[9], cg_stream_t stream);
+
 /* prolongate_3d_cc_eno_rf2<double,ORDER>, ORDER in {2,3}:
    CarpetLib/src/prolongate_3d_cc_eno_rf2.cc:943-1363 */
 int cg_prolongate_3d_cc_eno_rf2_f64(int order, const double *src,

@@ -377,6 +377,21 @@ void split_longest(const box3 &reg, int nthreads_hint, F &&body) {
 
 int g_use_openmp = 1;
 
+// eno1d of prolongate_3d_real8_eno.F90:8-67, each operation rounded on its
+// own in the order written
+double eno1d(const double *q) {
+  const double diffleft = (q[0] + q[2]) - 2.0 * q[1];
+  const double diffright = (q[1] + q[3]) - 2.0 * q[2];
+  double r;
+  if (std::fabs(diffleft) < std::fabs(diffright))
+    r = 0.125 * ((-q[0] + 6.0 * q[1]) + 3.0 * q[2]);
+  else
+    r = 0.125 * ((3.0 * q[1] + 6.0 * q[2]) - q[3]);
+  if ((r - q[1]) * (q[2] - r) < 0.0 || diffleft * diffright <= 0.0)
+    r = 0.5 * (q[1] + q[2]);
+  return r;
+}
+
 // ---- operators -----------------------------------------------------------
 
 int prolong_tensor(bool cell, int order, const double *src, const int *spad,
@@ -582,6 +597,136 @@ int orc_prolongate_3d_rf2(int order, const double *src, const int *spad,
     return CG_ERR_ORDER;
   return prolong_tensor(false, order, src, spad, sext, dst, dpad, dext, sb, db, drb);
 }
+
+// prolongate_3d_real8_eno (vertex-centred ENO, prolongate_3d_real8_eno.F90:
+// 69-288): per fine point the case of its parities (fi + 10 fj + 100 fk),
+// with the F90's temporaries -- x lines first, then y, then z; a
+// coincident direction takes the base value
+int orc_prolongate_3d_eno(const double *src, const int *spad, const int *sext, double *dst,
+                          const int *dpad, const int *dext, const int *sb, const int *db,
+                          const int *srb, const int *drb) {
+  (void)srb;
+  const box3 srcbbox = mkbox(sb), dstbbox = mkbox(db), reg = mkbox(drb);
+  int e;
+  if ((e = check_array(spad, sext, srcbbox)) || (e = check_array(dpad, dext, dstbbox)))
+    return e;
+  for (int d = 0; d < 3; ++d) {
+    if (reg.str[d] <= 0)
+      return CG_ERR_ARG;
+    if (srcbbox.str[d] <= reg.str[d] || dstbbox.str[d] != reg.str[d])
+      return CG_ERR_STRIDE;
+    if (srcbbox.str[d] != 2 * dstbbox.str[d])
+      return CG_ERR_REFFACT;
+    if (srcbbox.lo[d] % srcbbox.str[d] || dstbbox.lo[d] % dstbbox.str[d] || reg.lo[d] % reg.str[d])
+      return CG_ERR_ALIGN;
+  }
+  if (empty(reg))
+    return CG_ERR_EMPTY;
+  if (!contained(reg, dstbbox))
+    return CG_ERR_CONTAIN;
+  int off[3], doff[3], n[3];
+  for (int d = 0; d < 3; ++d) {
+    const int num = reg.lo[d] - srcbbox.lo[d];
+    if (num % reg.str[d] || (reg.lo[d] - dstbbox.lo[d]) % reg.str[d])
+      return CG_ERR_ALIGN;
+    off[d] = num / reg.str[d];
+    doff[d] = (reg.lo[d] - dstbbox.lo[d]) / reg.str[d];
+    n[d] = sizes(reg, d);
+    // exact footprint: b for a coincident child, b-1..b+2 for a midpoint
+    int lo = 1 << 30, hi = -(1 << 30);
+    for (int f : {0, 1, n[d] - 2, n[d] - 1}) {
+      if (f < 0 || f >= n[d])
+        continue;
+      const int g = off[d] + f, b = g >> 1;
+      lo = std::min(lo, (g & 1) ? b - 1 : b);
+      hi = std::max(hi, (g & 1) ? b + 2 : b);
+    }
+    if (off[d] < 0 || lo < 0 || hi >= sext[d])
+      return CG_ERR_CONTAIN;
+  }
+  auto at = [&](int i, int j, int k) { return src[idx3(i, j, k, spad)]; };
+  auto body = [&](const box3 &sub) {
+    int s0[3], m[3];
+    for (int d = 0; d < 3; ++d) {
+      s0[d] = (sub.lo[d] - reg.lo[d]) / reg.str[d];
+      m[d] = sizes(sub, d);
+    }
+    for (int k = s0[2]; k < s0[2] + m[2]; ++k)
+      for (int j = s0[1]; j < s0[1] + m[1]; ++j)
+        for (int i = s0[0]; i < s0[0] + m[0]; ++i) {
+          const int gi = off[0] + i, gj = off[1] + j, gk = off[2] + k;
+          const int i0 = gi >> 1, j0 = gj >> 1, k0 = gk >> 1;
+          const int fi = gi & 1, fj = gj & 1, fk = gk & 1;
+          double q[4], tmp2[4], tmp1[4][4], out = 0.0;
+          switch (fi + 10 * fj + 100 * fk) {
+          case 0:
+            out = at(i0, j0, k0);
+            break;
+          case 1:
+            for (int t = 0; t < 4; ++t)
+              q[t] = at(i0 - 1 + t, j0, k0);
+            out = eno1d(q);
+            break;
+          case 10:
+            for (int t = 0; t < 4; ++t)
+              q[t] = at(i0, j0 - 1 + t, k0);
+            out = eno1d(q);
+            break;
+          case 11:
+            for (int jj = 0; jj < 4; ++jj) {
+              for (int t = 0; t < 4; ++t)
+                q[t] = at(i0 - 1 + t, j0 - 1 + jj, k0);
+              tmp2[jj] = eno1d(q);
+            }
+            out = eno1d(tmp2);
+            break;
+          case 100:
+            for (int t = 0; t < 4; ++t)
+              q[t] = at(i0, j0, k0 - 1 + t);
+            out = eno1d(q);
+            break;
+          case 101:
+            for (int kk = 0; kk < 4; ++kk) {
+              for (int t = 0; t < 4; ++t)
+                q[t] = at(i0 - 1 + t, j0, k0 - 1 + kk);
+              tmp2[kk] = eno1d(q);
+            }
+            out = eno1d(tmp2);
+            break;
+          case 110:
+            for (int kk = 0; kk < 4; ++kk) {
+              for (int t = 0; t < 4; ++t)
+                q[t] = at(i0, j0 - 1 + t, k0 - 1 + kk);
+              tmp2[kk] = eno1d(q);
+            }
+            out = eno1d(tmp2);
+            break;
+          default: // 111
+            for (int jj = 0; jj < 4; ++jj)
+              for (int kk = 0; kk < 4; ++kk) {
+                for (int t = 0; t < 4; ++t)
+                  q[t] = at(i0 - 1 + t, j0 - 1 + jj, k0 - 1 + kk);
+                tmp1[jj][kk] = eno1d(q);
+              }
+            for (int ii = 0; ii < 4; ++ii) {
+              for (int t = 0; t < 4; ++t)
+                q[t] = tmp1[t][ii];
+              tmp2[ii] = eno1d(q);
+            }
+            out = eno1d(tmp2);
+            break;
+          }
+          dst[idx3(doff[0] + i, doff[1] + j, doff[2] + k, dpad)] = out;
+        }
+  };
+  if (g_use_openmp)
+    split_longest(reg, 0, body);
+  else
+    body(reg);
+  return CG_OK;
+}
+
+double orc_eno1d(const double *q) { return eno1d(q); }
 
 // prolongate_3d_cc_rf2<double,ORDER>
 int orc_prolongate_3d_cc_rf2(int order, const double *src, const int *spad,
@@ -1135,7 +1280,7 @@ int orc_prolongate_time(int kind, int order, const double *const *srcs,
                         const int *sext, double *dst, double time,
                         const int *dpad, const int *dext, const int *sb,
                         const int *db, const int *drb, int order_time) {
-  int op = kind == 2 ? CG_OP_ENO : kind == 3 ? CG_OP_TVD : CG_OP_LAGRANGE;
+  int op = (kind == 2 || kind == 4) ? CG_OP_ENO : kind == 3 ? CG_OP_TVD : CG_OP_LAGRANGE;
   int tl0, ntl, e;
   if ((e = orc_find_source_timelevel(times, ntimes, time, order_time, op, &tl0, &ntl)))
     return e;
@@ -1152,6 +1297,8 @@ int orc_prolongate_time(int kind, int order, const double *const *srcs,
       return orc_prolongate_3d_cc_eno_rf2(order, src, spad, sext, out, opad, oext, sb, ob, drb, drb);
     case 3:
       return orc_prolongate_3d_cc_tvd(src, spad, sext, out, opad, oext, sb, ob, drb, drb);
+    case 4:
+      return orc_prolongate_3d_eno(src, spad, sext, out, opad, oext, sb, ob, drb, drb);
     }
     return CG_ERR_OPERATOR;
   };
@@ -1173,7 +1320,7 @@ int orc_prolongate_time(int kind, int order, const double *const *srcs,
                          ntl > 2 ? tmp[2].data() : nullptr};
   // ENO and TVD groups interpolate 3 time levels with interpolate_eno_3d_3tl
   // (data.cc:1282-1292), Lagrange with interpolate_3d_2tl/3tl
-  if (ntl == 3 && (kind == 2 || kind == 3))
+  if (ntl == 3 && (kind == 2 || kind == 3 || kind == 4))
     return orc_interpolate_eno_3d_3tl(tp, times, text, text, dst, time, dpad, dext, drb, db,
                                       drb, drb);
   return orc_interpolate_3d(ntl, tp, times, text, text, dst, time, dpad, dext,

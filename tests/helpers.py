@@ -28,6 +28,8 @@ def windows(kind: str, order: int):
         return w[order]
     if kind == "tvd":
         return [(-1, 1), (-1, 1)]
+    if kind == "veno":  # vertex ENO: b for coincident, b-1..b+2 for midpoints
+        return [(0, 0), (-1, 2)]
     raise ValueError(kind)
 
 

@@ -41,7 +41,7 @@ def _src_host(rng, box, kind="mixed"):
 CASES = (
     [("vlag", o) for o in (1, 3, 5, 7, 9, 11)]
     + [("clag", o) for o in (0, 1, 2, 3, 4, 5)]
-    + [("eno", 2), ("eno", 3), ("tvd", 1)]
+    + [("eno", 2), ("eno", 3), ("tvd", 1), ("veno", 3)]
 )
 
 
@@ -53,6 +53,8 @@ def _run_prolong(kind, order, src, dst, reg, stream=None):
         cg.prolongate_3d_cc_rf2(order, src, dst, box, stream=stream)
     elif kind == "eno":
         cg.prolongate_3d_cc_eno_rf2(order, src, dst, box, stream=stream)
+    elif kind == "veno":
+        cg.prolongate_3d_eno(src, dst, box, stream=stream)
     else:
         cg.prolongate_3d_cc_tvd(src, dst, box, stream=stream)
 
@@ -64,6 +66,8 @@ def _orc_prolong(kind, order, src, dst, reg):
         return orc.prolongate_3d_cc_rf2(order, src, dst, reg)
     if kind == "eno":
         return orc.prolongate_3d_cc_eno_rf2(order, src, dst, reg)
+    if kind == "veno":
+        return orc.prolongate_3d_eno(src, dst, reg)
     return orc.prolongate_3d_cc_tvd(src, dst, reg)
 
 
@@ -190,7 +194,8 @@ def test_interpolate_parity(ntl):
 KIND_IDS = {"vlag": (orc.KIND_VLAG, cg.OP_LAGRANGE, cg.VERTEX_CENTERED),
             "clag": (orc.KIND_CLAG, cg.OP_LAGRANGE, cg.CELL_CENTERED),
             "eno": (orc.KIND_ENO, cg.OP_ENO, cg.CELL_CENTERED),
-            "tvd": (orc.KIND_TVD, cg.OP_TVD, cg.CELL_CENTERED)}
+            "tvd": (orc.KIND_TVD, cg.OP_TVD, cg.CELL_CENTERED),
+            "veno": (orc.KIND_VENO, cg.OP_ENO, cg.VERTEX_CENTERED)}
 
 
 def test_interpolate_eno_3d_3tl():
@@ -222,7 +227,8 @@ def test_interpolate_eno_3d_3tl():
         _check(exp, ddst, f"interp_eno case {c}")
 
 
-@pytest.mark.parametrize("kind,order,ntl", [("vlag", 5, 3), ("vlag", 3, 2), ("vlag", 1, 3),
+@pytest.mark.parametrize("kind,order,ntl", [("veno", 3, 3), ("veno", 3, 2), ("veno", 3, 1),
+                                            ("vlag", 5, 3), ("vlag", 3, 2), ("vlag", 1, 3),
                                             ("clag", 3, 3), ("eno", 3, 2), ("eno", 2, 2),
                                             ("eno", 3, 3), ("eno", 2, 3), ("tvd", 1, 2), ("tvd", 1, 3)])
 def test_transfer_time_interpolation(kind, order, ntl):
@@ -233,7 +239,7 @@ def test_transfer_time_interpolation(kind, order, ntl):
     rng = np.random.default_rng(31 * order + ntl)
     okind, op, cent = KIND_IDS[kind]
     # ENO order_space 5 -> ENO<3>, 2..4 -> ENO<2> (data.cc:697-704)
-    order_space = {("eno", 3): 5, ("eno", 2): 3}.get((kind, order), order)
+    order_space = {("eno", 3): 5, ("eno", 2): 3, ("veno", 3): 5}.get((kind, order), order)
     for c in range(4):
         print(f"transfer {kind} o{order} ntl{ntl} #{c}", flush=True)
         sb, db, rb = prolong_case(rng, kind, order, n=(70, 20, 40) if c == 0 else None)
@@ -358,3 +364,27 @@ def test_transfer_higher_order_restriction(order):
         cg.set_restriction(False, 3)
     with pytest.raises(cg.CarpetError):
         cg.set_restriction(True, 4)
+
+
+def test_vertex_eno_dispatch():
+    """Vertex-centred op ENO (order_space 3, 5) and op WENO (5) run
+    prolongate_3d_eno (data.cc:654-762); the other vertex-centred ENO/WENO
+    orders, cell-centred WENO and vertex-centred TVD (whose reference routine
+    aborts) are refused before any launch."""
+    rng = np.random.default_rng(4242)
+    sb, db, rb = prolong_case(rng, "veno", 3, n=(21, 14, 9))
+    src = _src_host(rng, sb, "mixed")
+    dst = host_array(db, pad_of(sizes(db), rng))
+    exp = orc.HostArr(db, dst.data.copy())
+    assert orc.prolongate_3d_eno(src, exp, rb) == 0
+    dsrc = _dev(src)
+    for op, order in ((cg.OP_ENO, 3), (cg.OP_ENO, 5), (cg.OP_WENO, 5)):
+        ddst = _dev(dst)
+        cg.transfer_from(ddst, [dsrc], [0.0], cg.Box(*rb), cg.Box(*rb), 0.0, op, cg.VERTEX_CENTERED, order, 0)
+        _check(exp, ddst, f"vertex op {op} order {order}")
+    ddst = _dev(dst)
+    for op, order, cent in ((cg.OP_ENO, 1, cg.VERTEX_CENTERED), (cg.OP_WENO, 3, cg.VERTEX_CENTERED),
+                            (cg.OP_WENO, 5, cg.CELL_CENTERED), (cg.OP_TVD, 1, cg.VERTEX_CENTERED)):
+        with pytest.raises(cg.CarpetError):
+            cg.transfer_from(ddst, [dsrc], [0.0], cg.Box(*rb), cg.Box(*rb), 0.0, op, cent, order, 0)
+    _check(orc.HostArr(db, dst.data.copy()), ddst, "refused transfers leave the destination untouched")

@@ -150,7 +150,10 @@ def test_transfer_dispatch_validation():
         return lib.cg_transfer_from(C.byref(f), srcs, tt, nt, reg, reg, None, time, op, cent, order, ot, None)
 
     assert call(cg.OP_LAGRANGE, cg.VERTEX_CENTERED, 4) == -7  # even vertex order
-    assert call(cg.OP_ENO, cg.VERTEX_CENTERED, 3) == -11  # vertex ENO out of scope
+    assert call(cg.OP_ENO, cg.VERTEX_CENTERED, 1) == -7  # vertex ENO: order_space 3 or 5
+    assert call(cg.OP_WENO, cg.VERTEX_CENTERED, 3) == -7  # vertex WENO: order_space 5
+    assert call(cg.OP_WENO, cg.CELL_CENTERED, 5) == -11  # no cell-centred WENO
+    assert call(cg.OP_TVD, cg.VERTEX_CENTERED, 1) == -11  # the reference's vertex TVD aborts
     assert call(cg.OP_TVD, cg.CELL_CENTERED, 3) == -7  # TVD order 1 only
     assert call(cg.OP_SYNC, cg.VERTEX_CENTERED, 3) == -11  # op_sync cannot prolongate
     assert call(cg.OP_NONE, cg.VERTEX_CENTERED, 3) == -11

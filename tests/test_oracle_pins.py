@@ -629,3 +629,80 @@ def test_cc_restrict_ho_brute_force_and_footprint(order):
         src2.view()[...] = 0.0
         dst2 = host_array(db, sizes(db))
         assert orc.restrict_3d_cc_ho_rf2(order, src2, dst2, rb) == rc_exp
+
+
+# ---- vertex-centred ENO (prolongate_3d_real8_eno.F90) ----------------------
+def test_vertex_eno1d_hand_cases():
+    """eno1d on hand-made lines: right quadratic on a tie, left quadratic
+    where its second difference is smaller, the average at a step, a local
+    extremum and where the result would leave [q2, q3]."""
+    assert orc.eno1d([0, 1, 3, 6]) == 1.875  # x(x+1)/2 at 1.5 (tie: right)
+    assert orc.eno1d([0, 1, 2.5, 5]) == 1.6875  # left quadratic: 0.25x^2+1.25x+1 at 0.5
+    assert orc.eno1d([0, 0, 1, 1]) == 0.5  # second differences of opposite sign
+    assert orc.eno1d([0, 2, 1, 5]) == 1.5  # extremum between q2 and q3
+    assert orc.eno1d([1, 1, 1, 1]) == 1.0
+    assert orc.eno1d([0, 1, 2, 3]) == 1.5  # linear: zero second differences -> average
+    # quadratic through q1..q3 overshoots q3: -> average
+    assert orc.eno1d([10, 0, 0.1, 100]) == 0.05
+
+
+def _veno_case(rng, n, data):
+    sb, db, rb = prolong_case(rng, "veno", 3, n=n)
+    src = host_array(sb, pad_of(sizes(sb), rng))
+    src.view()[...] = data(sb)
+    dst = host_array(db, pad_of(sizes(db), rng))
+    assert orc.prolongate_3d_eno(src, dst, rb) == 0
+    return src, dst, sb, db, rb
+
+
+def _coords(box):
+    lo, up, st = box
+    return [np.arange(lo[d], up[d] + 1, st[d], dtype=np.float64) for d in range(3)]
+
+
+def _region(h, rb):
+    """(z, y, x) values of h inside region rb."""
+    o = [(rb[0][d] - h.box[0][d]) // h.box[2][d] for d in range(3)]
+    n = sizes(rb)
+    return h.view()[o[2]:o[2] + n[2], o[1]:o[1] + n[1], o[0]:o[0] + n[0]]
+
+
+def test_vertex_eno_reproduces_monotone_quadratics():
+    """Products of increasing convex quadratics (second differences of one
+    sign, no extremum) are reproduced to rounding at every fine point."""
+    rng = np.random.default_rng(61)
+    f = [lambda x: (x + 40.0) ** 2, lambda y: (y + 45.0) ** 2 + 3.0, lambda z: 0.5 * (z + 50.0) ** 2]
+
+    def data(box):
+        x, y, z = _coords(box)
+        return f[2](z)[:, None, None] * f[1](y)[None, :, None] * f[0](x)[None, None, :]
+
+    for c in range(3):
+        src, dst, sb, db, rb = _veno_case(rng, None if c else (7, 6, 5), data)
+        x, y, z = _coords(rb)
+        exp = f[2](z)[:, None, None] * f[1](y)[None, :, None] * f[0](x)[None, None, :]
+        got = _region(dst, rb)
+        assert np.max(np.abs(got - exp) / np.abs(exp)) < 1e-14
+
+
+def test_vertex_eno_bounded_and_coincident():
+    """On random data every fine value lies between the smallest and largest
+    of its 2x2x2 nearest coarse values, and fine points on coarse points
+    take the coarse value exactly."""
+    rng = np.random.default_rng(62)
+    for c in range(4):
+        src, dst, sb, db, rb = _veno_case(rng, None, lambda box: rng.standard_normal(tuple(reversed(sizes(box)))))
+        got = _region(dst, rb)
+        xs = _coords(rb)
+        cs = _coords(sb)
+        sv = src.view()
+        for _ in range(200):
+            i, j, k = (int(rng.integers(0, len(v))) for v in xs)
+            p = (xs[0][i], xs[1][j], xs[2][k])
+            lo = [int(np.searchsorted(cs[d], p[d], side="right")) - 1 for d in range(3)]
+            hi = [lo[d] + (0 if cs[d][lo[d]] == p[d] else 1) for d in range(3)]
+            block = sv[lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1]
+            v = got[k, j, i]
+            assert block.min() <= v <= block.max()
+            if hi == lo:
+                assert v == block[0, 0, 0]
