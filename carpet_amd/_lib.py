@@ -77,6 +77,7 @@ _decl("cg_prolongate_3d_cc_tvd_f64", i, *_OPARGS, vp)
 _decl("cg_restrict_3d_rf2_f64", i, *_OPARGS, vp)
 _decl("cg_restrict_3d_cc_rf2_f64", i, *_OPARGS, vp)
 _decl("cg_prolongate_3d_eno_f64", i, *_OPARGS, vp)
+_decl("cg_prolongate_3d_o5_monotone_rf2_f64", i, *_OPARGS, vp)
 _decl("cg_restrict_3d_cc_o3_rf2_f64", i, *_OPARGS, vp)
 _decl("cg_restrict_3d_cc_o5_rf2_f64", i, *_OPARGS, vp)
 _decl("cg_set_restriction", i, i, i)

@@ -19,7 +19,7 @@ import numpy as np
 from ._lib import GData, I3, I9, Islab, check, lib
 
 # CarpetLib enums (operators.hh:8-25, defs.hh:134)
-OP_ERROR, OP_NONE, OP_SYNC, OP_RESTRICT, OP_COPY, OP_LAGRANGE, OP_ENO, OP_WENO, OP_TVD = range(9)
+OP_ERROR, OP_NONE, OP_SYNC, OP_RESTRICT, OP_COPY, OP_LAGRANGE, OP_ENO, OP_WENO, OP_TVD, OP_LAGRANGE_MONOTONE = range(10)
 VERTEX_CENTERED, CELL_CENTERED = 1, 2
 PREC_F64, PREC_MIXED, PREC_MIXED_DOT2 = 0, 1, 2
 
@@ -292,6 +292,15 @@ def prolongate_3d_eno(src: DeviceArray, dst: DeviceArray, dstregbbox: Box,
     check(lib.cg_prolongate_3d_eno_f64(*_arr(src), *_arr(dst), src.box.as_c(), dst.box.as_c(),
                                        (srcregbbox or dstregbbox).as_c(), dstregbbox.as_c(), _sh(stream)),
           "prolongate_3d_eno")
+
+
+def prolongate_3d_o5_monotone_rf2(src: DeviceArray, dst: DeviceArray, dstregbbox: Box,
+                                  srcregbbox: Optional[Box] = None, stream=None):
+    """Limited order-5 vertex Lagrange (prolongate_3d_o5_monotone_rf2.cc)."""
+    check(lib.cg_prolongate_3d_o5_monotone_rf2_f64(*_arr(src), *_arr(dst), src.box.as_c(), dst.box.as_c(),
+                                                   (srcregbbox or dstregbbox).as_c(), dstregbbox.as_c(),
+                                                   _sh(stream)),
+          "prolongate_3d_o5_monotone_rf2")
 
 
 def prolongate_3d_cc_tvd(src: DeviceArray, dst: DeviceArray, dstregbbox: Box,

@@ -145,6 +145,18 @@ int resolve_transfer(const cg_gdata *dst, const cg_gdata *const *srcs,
       kind = PK_TVD;
       order = 1;
       break;
+    case CG_OP_LAGRANGE_MONOTONE:
+      // prolongate_3d_o5_monotone_rf2 for order_space 5 (data.cc:807-832)
+      if (order_space == 1 || order_space == 3)
+        return set_error(CG_ERR_ORDER, "There is no stencil for op=\"Lagrange_monotone\" with order_space=%d",
+                         order_space);
+      if (order_space != 5)
+        return set_error(CG_ERR_ORDER, "There is no stencil for op=\"Lagrange_monotone\" with order_space!=5");
+      if (cent != CG_VERTEX_CENTERED)
+        return set_error(CG_ERR_OPERATOR, "op=\"Lagrange_monotone\" is provided for vertex centring only");
+      kind = PK_MONO5;
+      order = 5;
+      break;
     default:
       return set_error(CG_ERR_OPERATOR, "prolongation operator %d is not provided", op);
     }
@@ -549,6 +561,16 @@ int cg_prolongate_3d_eno_f64(const double *src, const int srcpadext[3], const in
                              const int dstregbbox[9], cg_stream_t stream) {
   (void)srcregbbox;
   return prolong_call(PK_VENO, 3, src, srcpadext, srcext, dst, dstpadext, dstext, srcbbox, dstbbox,
+                      dstregbbox, stream);
+}
+
+int cg_prolongate_3d_o5_monotone_rf2_f64(const double *src, const int srcpadext[3], const int srcext[3],
+                                         double *dst, const int dstpadext[3], const int dstext[3],
+                                         const int srcbbox[9], const int dstbbox[9],
+                                         const int srcregbbox[9], const int dstregbbox[9],
+                                         cg_stream_t stream) {
+  (void)srcregbbox;
+  return prolong_call(PK_MONO5, 5, src, srcpadext, srcext, dst, dstpadext, dstext, srcbbox, dstbbox,
                       dstregbbox, stream);
 }
 

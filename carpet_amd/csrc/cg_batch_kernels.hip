@@ -191,6 +191,7 @@ const void *persist_point_fn(int op, int pt) {
   case PO_TVD: return persist_point_pu<PO_TVD>(pu);
   case PO_RESTRICT_CC_O3: return persist_point_pu<PO_RESTRICT_CC_O3>(pu);
   case PO_RESTRICT_CC_O5: return persist_point_pu<PO_RESTRICT_CC_O5>(pu);
+  case PO_MONO5: return persist_point_pu<PO_MONO5>(pu);
   }
   return nullptr;
 }
@@ -268,7 +269,7 @@ int batch_device_build(BatchDevice &dev, const std::vector<Transfer> &tr,
     Group g{};
     g.phase = phase[i];
     g.prec = prec[i];
-    if (tr[i].is_prolong && tr[i].pr.kind != PK_TVD) {
+    if (tr[i].is_prolong && tr[i].pr.kind != PK_TVD && tr[i].pr.kind != PK_MONO5) {
       g.is_tensor = 1;
       g.kind = tr[i].pr.kind;
       g.order = tr[i].pr.order;
@@ -278,7 +279,7 @@ int batch_device_build(BatchDevice &dev, const std::vector<Transfer> &tr,
       g.is_tensor = 0;
       g.kind = g.order = -1;
       g.ntl = 1;
-      g.op = tr[i].is_prolong ? PO_TVD : tr[i].pt.op;
+      g.op = !tr[i].is_prolong ? tr[i].pt.op : tr[i].pr.kind == PK_MONO5 ? PO_MONO5 : PO_TVD;
     }
     auto it = std::find_if(groups.begin(), groups.end(), [&](const Group &x) { return x.same(g); });
     if (it == groups.end()) {
@@ -620,6 +621,7 @@ static int run_launch(BatchDevice &dev, const BatchLaunch &l, hipStream_t st) {
     case PO_TVD: rc = go(std::integral_constant<int, PO_TVD>{}); break;
     case PO_RESTRICT_CC_O3: rc = go(std::integral_constant<int, PO_RESTRICT_CC_O3>{}); break;
     case PO_RESTRICT_CC_O5: rc = go(std::integral_constant<int, PO_RESTRICT_CC_O5>{}); break;
+    case PO_MONO5: rc = go(std::integral_constant<int, PO_MONO5>{}); break;
     default: rc = -1;
     }
   }

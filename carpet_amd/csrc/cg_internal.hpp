@@ -65,7 +65,7 @@ int check_array(const Arr &a, const char *name);
 
 // kinds of tensor prolongation (cg::LineKind) plus TVD
 // the tensor kinds equal the line kinds (LK_*) of the kernels
-enum ProlongKind { PK_VLAG = 0, PK_CLAG = 1, PK_ENO = 2, PK_TVD = 3, PK_VENO = 4 };
+enum ProlongKind { PK_VLAG = 0, PK_CLAG = 1, PK_ENO = 2, PK_TVD = 3, PK_VENO = 4, PK_MONO5 = 5 };
 
 // A validated prolongation: kernel arguments for one (region, time levels)
 // plus the statistics the batch path reports.

@@ -51,7 +51,8 @@ __device__ __forceinline__ unsigned fastdiv(unsigned n, FastDiv f) {
 // Pointwise region operators: copy (with slab shift), injection
 // restriction, cell-centred restriction, time interpolation, cell TVD.
 enum PointOp { PO_COPY = 0, PO_RESTRICT_VC = 1, PO_RESTRICT_CC = 2,
-               PO_INTERP = 3, PO_TVD = 4, PO_RESTRICT_CC_O3 = 5, PO_RESTRICT_CC_O5 = 6 };
+               PO_INTERP = 3, PO_TVD = 4, PO_RESTRICT_CC_O3 = 5, PO_RESTRICT_CC_O5 = 6,
+               PO_MONO5 = 7 };
 
 struct PointArgs {
   const double *src[3]; // source element matching region point (0,0,0)

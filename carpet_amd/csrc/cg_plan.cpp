@@ -132,6 +132,60 @@ int plan_prolong(int kind, int order, const Arr &src, const void *const *srcs,
   pl.ntl = ntl;
   pl.es = es;
 
+  if (kind == PK_MONO5) {
+    // prolongate_3d_o5_monotone_rf2.cc:56-97, vertex-centred geometry; the
+    // exact footprint (b-2..b+3 where a point lies between coarse points)
+    // must lie in the source array
+    for (int d = 0; d < 3; ++d) {
+      if (sb.str[d] <= reg.str[d] || db.str[d] != reg.str[d])
+        return set_error(CG_ERR_STRIDE, "Internal error: strides disagree");
+      if (sb.str[d] != 2 * db.str[d])
+        return set_error(CG_ERR_REFFACT, "Internal error: source strides are not twice the destination strides");
+      if (sb.lo[d] % sb.str[d] || db.lo[d] % db.str[d] || reg.lo[d] % reg.str[d])
+        return set_error(CG_ERR_ALIGN, "Internal error: bboxes are not aligned with vertices");
+    }
+    if (reg.empty())
+      return set_error(CG_ERR_EMPTY, "Internal error: region extent is empty");
+    if (!reg.contained_in(db))
+      return set_error(CG_ERR_CONTAIN, "Internal error: region extent is not contained in array extent");
+    PointArgs &p = pl.p;
+    long long srcpts = 1;
+    int doff[3];
+    for (int d = 0; d < 3; ++d) {
+      if ((reg.lo[d] - sb.lo[d]) % reg.str[d] || (reg.lo[d] - db.lo[d]) % reg.str[d])
+        return set_error(CG_ERR_ALIGN, "region is not aligned with the source/destination grids (dir %d)", d);
+      p.off[d] = (reg.lo[d] - sb.lo[d]) / reg.str[d];
+      doff[d] = (reg.lo[d] - db.lo[d]) / reg.str[d];
+      p.n[d] = reg.sizes(d);
+      int lo = 1 << 30, hi = -(1 << 30);
+      for (int f : {0, 1, p.n[d] - 2, p.n[d] - 1}) {
+        if (f < 0 || f >= p.n[d])
+          continue;
+        const int g = p.off[d] + f, b = g >> 1;
+        lo = std::min(lo, (g & 1) ? b - 2 : b);
+        hi = std::max(hi, (g & 1) ? b + 3 : b);
+      }
+      if (p.off[d] < 0 || lo < 0 || hi >= src.ext[d])
+        return set_error(CG_ERR_CONTAIN,
+                         "Internal error: region extent is not contained in array extent (stencil "
+                         "footprint [%d,%d] vs source [0,%d) in dir %d)", lo, hi, src.ext[d], d);
+      srcpts *= (long long)(hi - lo + 1);
+    }
+    p.sj = src.sj();
+    p.sk = src.sk();
+    p.dst = static_cast<double *>(const_cast<void *>(dst.p)) + dst.at(doff[0], doff[1], doff[2]);
+    p.dj = dst.sj();
+    p.dk = dst.sk();
+    p.ntl = ntl;
+    for (int l = 0; l < ntl; ++l) {
+      p.src[l] = static_cast<const double *>(srcs[l]);
+      p.s[l] = s[l];
+    }
+    pl.dst_points = (long long)p.n[0] * p.n[1] * p.n[2];
+    pl.src_points = srcpts;
+    return CG_OK;
+  }
+
   if (kind == PK_TVD) {
     for (int d = 0; d < 3; ++d) {
       if (sb.str[d] <= reg.str[d] || db.str[d] != reg.str[d])
@@ -286,6 +340,8 @@ int launch_prolong(const ProlongPlan &pl, int prec, hipStream_t st) {
   int r;
   if (pl.kind == PK_TVD) {
     r = launch_point(PO_TVD, pl.p, st);
+  } else if (pl.kind == PK_MONO5) {
+    r = launch_point(PO_MONO5, pl.p, st);
   } else {
     TileCfg cfg{0, 0, 0};
     // z chunks: at most 128 planes (tiles that march much further apart in

@@ -410,6 +410,88 @@ __device__ __forceinline__ double minmod(double a, double b) {
   return __builtin_fabs(a) < __builtin_fabs(b) ? a : b;
 }
 
+// prolongate_3d_o5_monotone_rf2 at fine point (i,j,k) from one source
+// level (prolongate_3d_o5_monotone_rf2.cc:45-791): the order-5 stencil
+// (3,-25,150,150,-25,3)/256 on every axis where the point lies between
+// coarse points, summed flat in the reference's order -- one axis: its six
+// taps; two axes: the outer axis' offsets outside, the inner axis' inside;
+// three axes: one flat sum per z offset, the six added in z order -- each
+// product and sum rounded on its own; where the value leaves the range of
+// the 2, 4 or 8 nearest coarse values, their average (weights 1/2 per axis,
+// the same corner order) replaces it
+__device__ __forceinline__ double mono5_value(const double *src, long long J, long long K, int gi, int gj,
+                                              int gk) {
+  constexpr double f[6] = {3 / 256.0, -25 / 256.0, 150 / 256.0, 150 / 256.0, -25 / 256.0, 3 / 256.0};
+  const double *p = src + (gi >> 1) + J * (gj >> 1) + K * (gk >> 1);
+  const int pi = gi & 1, pj = gj & 1, pk = gk & 1;
+  const int nodd = pi + pj + pk;
+  if (nodd == 0)
+    return p[0];
+  // odd axes in x, y, z order: strides s[0] (inner) .. s[nodd-1] (outer)
+  long long s[3];
+  int m = 0;
+  if (pi)
+    s[m++] = 1;
+  if (pj)
+    s[m++] = J;
+  if (pk)
+    s[m++] = K;
+  if (nodd == 1) {
+    const long long a = s[0];
+    double r = f[0] * p[-2 * a];
+#pragma unroll
+    for (int t = 1; t < 6; ++t)
+      r = r + f[t] * p[(t - 2) * a];
+    const double c0 = p[0], c1 = p[a];
+    if (r > fmax(c0, c1) || r < fmin(c0, c1))
+      r = 0.5 * c0 + 0.5 * c1;
+    return r;
+  }
+  if (nodd == 2) {
+    const long long a = s[0], b = s[1];
+    double r = (f[0] * f[0]) * p[-2 * a - 2 * b];
+    for (int tb = 0; tb < 6; ++tb)
+#pragma unroll
+      for (int ta = 0; ta < 6; ++ta)
+        if (ta | tb)
+          r = r + (f[ta] * f[tb]) * p[(ta - 2) * a + (tb - 2) * b];
+    const double c00 = p[0], c10 = p[a], c01 = p[b], c11 = p[a + b];
+    if (r > fmax(fmax(c00, c10), fmax(c01, c11)) || r < fmin(fmin(c00, c10), fmin(c01, c11))) {
+      const double w = 0.5 * 0.5;
+      r = w * c00 + w * c10;
+      r = r + w * c01;
+      r = r + w * c11;
+    }
+    return r;
+  }
+  const long long a = s[0], b = s[1], c = s[2];
+  double r = 0.0;
+  for (int tc = 0; tc < 6; ++tc) {
+    double res = (f[0] * f[0] * f[tc]) * p[-2 * a - 2 * b + (tc - 2) * c];
+    for (int tb = 0; tb < 6; ++tb)
+#pragma unroll
+      for (int ta = 0; ta < 6; ++ta)
+        if (ta | tb)
+          res = res + (f[ta] * f[tb] * f[tc]) * p[(ta - 2) * a + (tb - 2) * b + (tc - 2) * c];
+    r = tc == 0 ? res : r + res;
+  }
+  const double c000 = p[0], c100 = p[a], c010 = p[b], c001 = p[c], c110 = p[a + b], c101 = p[a + c],
+               c011 = p[b + c], c111 = p[a + b + c];
+  const double mx = fmax(fmax(fmax(c000, c100), fmax(c010, c001)), fmax(fmax(c110, c101), fmax(c011, c111)));
+  const double mn = fmin(fmin(fmin(c000, c100), fmin(c010, c001)), fmin(fmin(c110, c101), fmin(c011, c111)));
+  if (r > mx || r < mn) {
+    const double w = 0.5 * 0.5 * 0.5;
+    double r1 = w * c000 + w * c100;
+    r1 = r1 + w * c010;
+    r1 = r1 + w * c110;
+    double r2 = w * c001 + w * c101;
+    r2 = r2 + w * c011;
+    r2 = r2 + w * c111;
+    r = r1 + r2;
+  }
+  return r;
+}
+
 // value of destination point (i,j,k) of a pointwise operator (loads only,
 // so that a caller can issue the loads of several points before storing)
 template <int OP>
@@ -434,6 +516,22 @@ __device__ __forceinline__ double point_value(const PointArgs &a, int i, int j, 
     r = r + w * p[J + K];
     r = r + w * p[1 + J + K];
     out = r;
+  } else if constexpr (OP == PO_MONO5) {
+    // each time level prolongated, then combined in time (data.cc:497-521)
+    const int gi = a.off[0] + i, gj = a.off[1] + j, gk = a.off[2] + k;
+    double v[3];
+    v[0] = mono5_value(a.src[0], a.sj, a.sk, gi, gj, gk);
+    if (a.ntl == 1) {
+      out = v[0];
+    } else {
+      v[1] = mono5_value(a.src[1], a.sj, a.sk, gi, gj, gk);
+      if (a.ntl == 2) {
+        out = time_combine<2>(v, a.s, a.s2, a.l0, 0);
+      } else {
+        v[2] = mono5_value(a.src[2], a.sj, a.sk, gi, gj, gk);
+        out = time_combine<3>(v, a.s, a.s2, a.l0, 0);
+      }
+    }
   } else if constexpr (OP == PO_RESTRICT_CC_O3) {
     // restrict_3d_cc_o3_rf2.cc:146-211: 64 terms, (w_a w_b w_c)/4096 with
     // w = {-1, 9, 9, -1}, a fastest, then b, then c; fine cells 2i-1..2i+2

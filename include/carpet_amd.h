@@ -208,6 +208,19 @@ int cg_prolongate_3d_eno_f64(const double *src, const int srcpadext[3], const in
                              const int srcbbox[9], const int dstbbox[9], const int srcregbbox[9],
                              const int dstregbbox[9], cg_stream_t stream);
 
+/* prolongate_3d_o5_monotone_rf2<double> (vertex-centred):
+   CarpetLib/src/prolongate_3d_o5_monotone_rf2.cc:45-791, the operator of
+   op Lagrange_monotone with order_space 5 (data.cc:807-832).  Order-5
+   Lagrange summed flat in the reference's term order, replaced by the
+   multilinear average where it leaves the range of the nearest coarse
+   values.  The footprint (coarse b-2..b+3 on every axis where a point lies
+   between coarse points) must lie in the source array. */
+int cg_prolongate_3d_o5_monotone_rf2_f64(const double *src, const int srcpadext[3], const int srcext[3],
+                                         double *dst, const int dstpadext[3], const int dstext[3],
+                                         const int srcbbox[9], const int dstbbox[9],
+                                         const int srcregbbox[9], const int dstregbbox[9],
+                                         cg_stream_t stream);
+
 /* prolongate_3d_cc_eno_rf2<double,ORDER>, ORDER in {2,3}:
    CarpetLib/src/prolongate_3d_cc_eno_rf2.cc:943-1363 */
 int cg_prolongate_3d_cc_eno_rf2_f64(int order, const double *src,

@@ -728,6 +728,150 @@ int orc_prolongate_3d_eno(const double *src, const int *spad, const int *sext, d
 
 double orc_eno1d(const double *q) { return eno1d(q); }
 
+// prolongate_3d_o5_monotone_rf2 (prolongate_3d_o5_monotone_rf2.cc:45-791):
+// the reference's eight parity cases, each with its own term order
+int orc_prolongate_3d_o5_monotone_rf2(const double *src, const int *spad, const int *sext, double *dst,
+                                      const int *dpad, const int *dext, const int *sb, const int *db,
+                                      const int *srb, const int *drb) {
+  (void)srb;
+  const box3 srcbbox = mkbox(sb), dstbbox = mkbox(db), reg = mkbox(drb);
+  int e;
+  if ((e = check_array(spad, sext, srcbbox)) || (e = check_array(dpad, dext, dstbbox)))
+    return e;
+  for (int d = 0; d < 3; ++d) {
+    if (reg.str[d] <= 0)
+      return CG_ERR_ARG;
+    if (srcbbox.str[d] <= reg.str[d] || dstbbox.str[d] != reg.str[d])
+      return CG_ERR_STRIDE;
+    if (srcbbox.str[d] != 2 * dstbbox.str[d])
+      return CG_ERR_REFFACT;
+    if (srcbbox.lo[d] % srcbbox.str[d] || dstbbox.lo[d] % dstbbox.str[d] || reg.lo[d] % reg.str[d])
+      return CG_ERR_ALIGN;
+  }
+  if (empty(reg))
+    return CG_ERR_EMPTY;
+  if (!contained(reg, dstbbox))
+    return CG_ERR_CONTAIN;
+  int off[3], doff[3], n[3];
+  for (int d = 0; d < 3; ++d) {
+    const int num = reg.lo[d] - srcbbox.lo[d];
+    if (num % reg.str[d] || (reg.lo[d] - dstbbox.lo[d]) % reg.str[d])
+      return CG_ERR_ALIGN;
+    off[d] = num / reg.str[d];
+    doff[d] = (reg.lo[d] - dstbbox.lo[d]) / reg.str[d];
+    n[d] = sizes(reg, d);
+    int lo = 1 << 30, hi = -(1 << 30);
+    for (int f : {0, 1, n[d] - 2, n[d] - 1}) {
+      if (f < 0 || f >= n[d])
+        continue;
+      const int g = off[d] + f, b = g >> 1;
+      lo = std::min(lo, (g & 1) ? b - 2 : b);
+      hi = std::max(hi, (g & 1) ? b + 3 : b);
+    }
+    if (off[d] < 0 || lo < 0 || hi >= sext[d])
+      return CG_ERR_CONTAIN;
+  }
+  const double f[6] = {3.0 / 256, -25.0 / 256, 150.0 / 256, 150.0 / 256, -25.0 / 256, 3.0 / 256};
+  const double h = 1.0 / 2;
+  auto S = [&](int i, int j, int k) { return src[idx3(i, j, k, spad)]; };
+  // one odd axis: six taps along it (cases l8001, l8010, l8100)
+  auto one = [&](int is, int js, int ks, int di, int dj, int dk) {
+    double r = f[0] * S(is - 2 * di, js - 2 * dj, ks - 2 * dk);
+    for (int t = 1; t < 6; ++t)
+      r = r + f[t] * S(is + (t - 2) * di, js + (t - 2) * dj, ks + (t - 2) * dk);
+    const double a = S(is, js, ks), b = S(is + di, js + dj, ks + dk);
+    if (r > std::max(a, b) || r < std::min(a, b))
+      r = h * a + h * b;
+    return r;
+  };
+  // two odd axes, u inner and v outer (cases l8011: x,y; l8101: x,z;
+  // l8110: y,z)
+  auto two = [&](int is, int js, int ks, const int *u, const int *v) {
+    double r = 0.0;
+    bool first = true;
+    for (int tv = 0; tv < 6; ++tv)
+      for (int tu = 0; tu < 6; ++tu) {
+        const double w = f[tu] * f[tv];
+        const double x = S(is + (tu - 2) * u[0] + (tv - 2) * v[0], js + (tu - 2) * u[1] + (tv - 2) * v[1],
+                           ks + (tu - 2) * u[2] + (tv - 2) * v[2]);
+        r = first ? w * x : r + w * x;
+        first = false;
+      }
+    const double c00 = S(is, js, ks), c10 = S(is + u[0], js + u[1], ks + u[2]),
+                 c01 = S(is + v[0], js + v[1], ks + v[2]),
+                 c11 = S(is + u[0] + v[0], js + u[1] + v[1], ks + u[2] + v[2]);
+    const double mx = std::max(std::max(c00, c10), std::max(c01, c11));
+    const double mn = std::min(std::min(c00, c10), std::min(c01, c11));
+    if (r > mx || r < mn)
+      r = h * h * c00 + h * h * c10 + h * h * c01 + h * h * c11;
+    return r;
+  };
+  // three odd axes (case l8111): res1..res6 per z offset, then their sum
+  auto three = [&](int is, int js, int ks) {
+    double res[6];
+    for (int c = 0; c < 6; ++c) {
+      double r = 0.0;
+      bool first = true;
+      for (int b = 0; b < 6; ++b)
+        for (int a = 0; a < 6; ++a) {
+          const double w = f[a] * f[b] * f[c];
+          const double x = S(is + a - 2, js + b - 2, ks + c - 2);
+          r = first ? w * x : r + w * x;
+          first = false;
+        }
+      res[c] = r;
+    }
+    double r = res[0] + res[1] + res[2] + res[3] + res[4] + res[5];
+    const double c[8] = {S(is, js, ks),         S(is + 1, js, ks),     S(is, js + 1, ks),
+                         S(is, js, ks + 1),     S(is + 1, js + 1, ks), S(is + 1, js, ks + 1),
+                         S(is, js + 1, ks + 1), S(is + 1, js + 1, ks + 1)};
+    const double mx = std::max(std::max(std::max(c[0], c[1]), std::max(c[2], c[3])),
+                               std::max(std::max(c[4], c[5]), std::max(c[6], c[7])));
+    const double mn = std::min(std::min(std::min(c[0], c[1]), std::min(c[2], c[3])),
+                               std::min(std::min(c[4], c[5]), std::min(c[6], c[7])));
+    if (r > mx || r < mn) {
+      const double w = h * h * h;
+      const double r1 = w * S(is, js, ks) + w * S(is + 1, js, ks) + w * S(is, js + 1, ks) +
+                        w * S(is + 1, js + 1, ks);
+      const double r2 = w * S(is, js, ks + 1) + w * S(is + 1, js, ks + 1) + w * S(is, js + 1, ks + 1) +
+                        w * S(is + 1, js + 1, ks + 1);
+      r = r1 + r2;
+    }
+    return r;
+  };
+  const int X[3] = {1, 0, 0}, Y[3] = {0, 1, 0}, Z[3] = {0, 0, 1};
+  auto body = [&](const box3 &sub) {
+    int s0[3], m[3];
+    for (int d = 0; d < 3; ++d) {
+      s0[d] = (sub.lo[d] - reg.lo[d]) / reg.str[d];
+      m[d] = sizes(sub, d);
+    }
+    for (int k = s0[2]; k < s0[2] + m[2]; ++k)
+      for (int j = s0[1]; j < s0[1] + m[1]; ++j)
+        for (int i = s0[0]; i < s0[0] + m[0]; ++i) {
+          const int gi = off[0] + i, gj = off[1] + j, gk = off[2] + k;
+          const int is = gi >> 1, js = gj >> 1, ks = gk >> 1;
+          double out;
+          switch ((gi & 1) + 10 * (gj & 1) + 100 * (gk & 1)) {
+          case 0: out = S(is, js, ks); break;
+          case 1: out = one(is, js, ks, 1, 0, 0); break;
+          case 10: out = one(is, js, ks, 0, 1, 0); break;
+          case 100: out = one(is, js, ks, 0, 0, 1); break;
+          case 11: out = two(is, js, ks, X, Y); break;
+          case 101: out = two(is, js, ks, X, Z); break;
+          case 110: out = two(is, js, ks, Y, Z); break;
+          default: out = three(is, js, ks); break;
+          }
+          dst[idx3(doff[0] + i, doff[1] + j, doff[2] + k, dpad)] = out;
+        }
+  };
+  if (g_use_openmp)
+    split_longest(reg, 0, body);
+  else
+    body(reg);
+  return CG_OK;
+}
+
 // prolongate_3d_cc_rf2<double,ORDER>
 int orc_prolongate_3d_cc_rf2(int order, const double *src, const int *spad,
                              const int *sext, double *dst, const int *dpad,
@@ -1299,6 +1443,8 @@ int orc_prolongate_time(int kind, int order, const double *const *srcs,
       return orc_prolongate_3d_cc_tvd(src, spad, sext, out, opad, oext, sb, ob, drb, drb);
     case 4:
       return orc_prolongate_3d_eno(src, spad, sext, out, opad, oext, sb, ob, drb, drb);
+    case 5:
+      return orc_prolongate_3d_o5_monotone_rf2(src, spad, sext, out, opad, oext, sb, ob, drb, drb);
     }
     return CG_ERR_OPERATOR;
   };

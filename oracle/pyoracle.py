@@ -54,6 +54,7 @@ _decl("orc_prolongate_3d_cc_eno_rf2", C.c_int, C.c_int, *_OP)
 _decl("orc_prolongate_3d_cc_tvd", C.c_int, *_OP)
 _decl("orc_prolongate_3d_eno", C.c_int, *_OP)
 _decl("orc_eno1d", C.c_double, _dp)
+_decl("orc_prolongate_3d_o5_monotone_rf2", C.c_int, *_OP)
 _decl("orc_restrict_3d_rf2", C.c_int, *_OP)
 _decl("orc_restrict_3d_cc_rf2", C.c_int, *_OP)
 _decl("orc_restrict_3d_cc_ho_rf2", C.c_int, C.c_int, *_OP)
@@ -67,7 +68,7 @@ _decl("orc_find_source_timelevel", C.c_int, _dp, C.c_int, C.c_double, C.c_int, C
 _decl("orc_prolongate_time", C.c_int, C.c_int, C.c_int, C.POINTER(_dp), _dp, C.c_int, _ip, _ip, _dp,
       C.c_double, _ip, _ip, _ip, _ip, _ip, C.c_int)
 
-KIND_VLAG, KIND_CLAG, KIND_ENO, KIND_TVD, KIND_VENO = 0, 1, 2, 3, 4
+KIND_VLAG, KIND_CLAG, KIND_ENO, KIND_TVD, KIND_VENO, KIND_MONO5 = 0, 1, 2, 3, 4, 5
 
 
 @dataclass
@@ -142,6 +143,11 @@ def prolongate_3d_cc_tvd(src, dst, dstreg) -> int:
 def prolongate_3d_eno(src, dst, dstreg) -> int:
     """prolongate_3d_real8_eno (vertex-centred ENO)"""
     return _lib.orc_prolongate_3d_eno(*_args(src, dst, dstreg, dstreg))
+
+
+def prolongate_3d_o5_monotone_rf2(src, dst, dstreg) -> int:
+    """prolongate_3d_o5_monotone_rf2 (vertex-centred, order 5, limited)"""
+    return _lib.orc_prolongate_3d_o5_monotone_rf2(*_args(src, dst, dstreg, dstreg))
 
 
 def eno1d(q) -> float:

@@ -41,7 +41,7 @@ def _src_host(rng, box, kind="mixed"):
 CASES = (
     [("vlag", o) for o in (1, 3, 5, 7, 9, 11)]
     + [("clag", o) for o in (0, 1, 2, 3, 4, 5)]
-    + [("eno", 2), ("eno", 3), ("tvd", 1), ("veno", 3)]
+    + [("eno", 2), ("eno", 3), ("tvd", 1), ("veno", 3), ("mono", 5)]
 )
 
 
@@ -55,6 +55,8 @@ def _run_prolong(kind, order, src, dst, reg, stream=None):
         cg.prolongate_3d_cc_eno_rf2(order, src, dst, box, stream=stream)
     elif kind == "veno":
         cg.prolongate_3d_eno(src, dst, box, stream=stream)
+    elif kind == "mono":
+        cg.prolongate_3d_o5_monotone_rf2(src, dst, box, stream=stream)
     else:
         cg.prolongate_3d_cc_tvd(src, dst, box, stream=stream)
 
@@ -68,6 +70,8 @@ def _orc_prolong(kind, order, src, dst, reg):
         return orc.prolongate_3d_cc_eno_rf2(order, src, dst, reg)
     if kind == "veno":
         return orc.prolongate_3d_eno(src, dst, reg)
+    if kind == "mono":
+        return orc.prolongate_3d_o5_monotone_rf2(src, dst, reg)
     return orc.prolongate_3d_cc_tvd(src, dst, reg)
 
 
@@ -195,7 +199,8 @@ KIND_IDS = {"vlag": (orc.KIND_VLAG, cg.OP_LAGRANGE, cg.VERTEX_CENTERED),
             "clag": (orc.KIND_CLAG, cg.OP_LAGRANGE, cg.CELL_CENTERED),
             "eno": (orc.KIND_ENO, cg.OP_ENO, cg.CELL_CENTERED),
             "tvd": (orc.KIND_TVD, cg.OP_TVD, cg.CELL_CENTERED),
-            "veno": (orc.KIND_VENO, cg.OP_ENO, cg.VERTEX_CENTERED)}
+            "veno": (orc.KIND_VENO, cg.OP_ENO, cg.VERTEX_CENTERED),
+            "mono": (orc.KIND_MONO5, cg.OP_LAGRANGE_MONOTONE, cg.VERTEX_CENTERED)}
 
 
 def test_interpolate_eno_3d_3tl():
@@ -227,7 +232,8 @@ def test_interpolate_eno_3d_3tl():
         _check(exp, ddst, f"interp_eno case {c}")
 
 
-@pytest.mark.parametrize("kind,order,ntl", [("veno", 3, 3), ("veno", 3, 2), ("veno", 3, 1),
+@pytest.mark.parametrize("kind,order,ntl", [("mono", 5, 3), ("mono", 5, 2), ("mono", 5, 1),
+                                            ("veno", 3, 3), ("veno", 3, 2), ("veno", 3, 1),
                                             ("vlag", 5, 3), ("vlag", 3, 2), ("vlag", 1, 3),
                                             ("clag", 3, 3), ("eno", 3, 2), ("eno", 2, 2),
                                             ("eno", 3, 3), ("eno", 2, 3), ("tvd", 1, 2), ("tvd", 1, 3)])
@@ -387,4 +393,27 @@ def test_vertex_eno_dispatch():
                             (cg.OP_WENO, 5, cg.CELL_CENTERED), (cg.OP_TVD, 1, cg.VERTEX_CENTERED)):
         with pytest.raises(cg.CarpetError):
             cg.transfer_from(ddst, [dsrc], [0.0], cg.Box(*rb), cg.Box(*rb), 0.0, op, cent, order, 0)
+    _check(orc.HostArr(db, dst.data.copy()), ddst, "refused transfers leave the destination untouched")
+
+
+def test_monotone_dispatch():
+    """op Lagrange_monotone with order_space 5 runs
+    prolongate_3d_o5_monotone_rf2 (data.cc:807-832); orders 1 and 3 (and
+    any other) are refused, as is cell centring, before any launch."""
+    rng = np.random.default_rng(4343)
+    sb, db, rb = prolong_case(rng, "mono", 5, n=(19, 12, 11))
+    src = _src_host(rng, sb, "random")
+    dst = host_array(db, pad_of(sizes(db), rng))
+    exp = orc.HostArr(db, dst.data.copy())
+    assert orc.prolongate_3d_o5_monotone_rf2(src, exp, rb) == 0
+    dsrc, ddst = _dev(src), _dev(dst)
+    cg.transfer_from(ddst, [dsrc], [0.0], cg.Box(*rb), cg.Box(*rb), 0.0, cg.OP_LAGRANGE_MONOTONE,
+                     cg.VERTEX_CENTERED, 5, 0)
+    _check(exp, ddst, "Lagrange_monotone order 5")
+    ddst = _dev(dst)
+    for order, cent in ((1, cg.VERTEX_CENTERED), (3, cg.VERTEX_CENTERED), (7, cg.VERTEX_CENTERED),
+                        (5, cg.CELL_CENTERED)):
+        with pytest.raises(cg.CarpetError):
+            cg.transfer_from(ddst, [dsrc], [0.0], cg.Box(*rb), cg.Box(*rb), 0.0, cg.OP_LAGRANGE_MONOTONE, cent,
+                             order, 0)
     _check(orc.HostArr(db, dst.data.copy()), ddst, "refused transfers leave the destination untouched")

@@ -706,3 +706,65 @@ def test_vertex_eno_bounded_and_coincident():
             assert block.min() <= v <= block.max()
             if hi == lo:
                 assert v == block[0, 0, 0]
+
+
+# ---- monotone order-5 Lagrange (prolongate_3d_o5_monotone_rf2.cc) ---------
+def _mono_case(rng, n, data):
+    sb, db, rb = prolong_case(rng, "mono", 5, n=n)
+    src = host_array(sb, pad_of(sizes(sb), rng))
+    src.view()[...] = data(sb)
+    dst = host_array(db, pad_of(sizes(db), rng))
+    assert orc.prolongate_3d_o5_monotone_rf2(src, dst, rb) == 0
+    return src, dst, sb, db, rb
+
+
+def test_monotone_reproduces_increasing_quintics():
+    """Order-5 Lagrange is exact for degree <= 5; on products of increasing
+    positive quintics the value never leaves the range of its nearest coarse
+    values, so no point is limited and all are exact to rounding."""
+    rng = np.random.default_rng(71)
+    f = [lambda x: (x + 60.0) ** 5, lambda y: (y + 70.0) ** 3 + (y + 70.0) ** 5, lambda z: (z + 80.0) ** 4]
+
+    def data(box):
+        x, y, z = _coords(box)
+        return f[2](z)[:, None, None] * f[1](y)[None, :, None] * f[0](x)[None, None, :]
+
+    for c in range(3):
+        src, dst, sb, db, rb = _mono_case(rng, None if c else (6, 5, 7), data)
+        x, y, z = _coords(rb)
+        exp = f[2](z)[:, None, None] * f[1](y)[None, :, None] * f[0](x)[None, None, :]
+        got = _region(dst, rb)
+        assert np.max(np.abs(got - exp) / np.abs(exp)) < 1e-13
+
+
+def test_monotone_limits_to_nearest_range():
+    """On random data every fine value lies within the range of its 1, 2,
+    4 or 8 nearest coarse values; where the order-5 value would leave it,
+    the value is their average (checked on a point-by-point restatement of
+    the limiter); coincident points are exact."""
+    rng = np.random.default_rng(72)
+    w = np.array([3, -25, 150, 150, -25, 3]) / 256.0
+    for c in range(3):
+        src, dst, sb, db, rb = _mono_case(rng, None, lambda box: rng.standard_normal(tuple(reversed(sizes(box)))))
+        got = _region(dst, rb)
+        xs, cs, sv = _coords(rb), _coords(sb), src.view()
+        limited = 0
+        for _ in range(300):
+            i, j, k = (int(rng.integers(0, len(v))) for v in xs)
+            p = (xs[0][i], xs[1][j], xs[2][k])
+            b = [int(np.searchsorted(cs[d], p[d], side="right")) - 1 for d in range(3)]
+            odd = [cs[d][b[d]] != p[d] for d in range(3)]
+            corners = sv[b[2]:b[2] + 1 + odd[2], b[1]:b[1] + 1 + odd[1], b[0]:b[0] + 1 + odd[0]]
+            v = got[k, j, i]
+            assert corners.min() <= v <= corners.max()
+            # the unlimited order-5 value, summed in float64 by numpy
+            ws = [w if odd[d] else np.array([1.0]) for d in range(3)]
+            win = sv[b[2] - 2 * odd[2]:b[2] + 1 + 3 * odd[2], b[1] - 2 * odd[1]:b[1] + 1 + 3 * odd[1],
+                     b[0] - 2 * odd[0]:b[0] + 1 + 3 * odd[0]]
+            hi = np.einsum("k,j,i,kji->", ws[2], ws[1], ws[0], win)
+            if hi > corners.max() + 1e-12 or hi < corners.min() - 1e-12:
+                limited += 1
+                assert abs(v - corners.mean()) < 1e-14 * max(1.0, abs(corners).max())
+            elif corners.min() + 1e-12 < hi < corners.max() - 1e-12:
+                assert abs(v - hi) < 1e-12
+        assert limited > 0

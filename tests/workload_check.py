@@ -10,7 +10,7 @@ import numpy as np
 
 import pyoracle as orc
 
-OP_COPY, OP_LAGRANGE, OP_ENO, OP_WENO, OP_TVD = 4, 5, 6, 7, 8
+OP_COPY, OP_LAGRANGE, OP_ENO, OP_WENO, OP_TVD, OP_LAGRANGE_MONOTONE = 4, 5, 6, 7, 8, 9
 VERTEX, CELL = 1, 2
 
 
@@ -41,6 +41,8 @@ def run_oracle(w) -> dict:
         elif all(a > b for a, b in zip(sst, dstst)):
             if t.op in (OP_COPY, OP_LAGRANGE):
                 kind, order = (orc.KIND_VLAG if t.cent == VERTEX else orc.KIND_CLAG), t.order_space
+            elif t.op == OP_LAGRANGE_MONOTONE:
+                kind, order = orc.KIND_MONO5, 5
             elif t.op in (OP_ENO, OP_WENO) and t.cent == VERTEX:
                 kind, order = orc.KIND_VENO, 3
             elif t.op == OP_ENO:
