@@ -128,6 +128,7 @@ static int save_cache(const char *path, const TuneMap &m) {
   f << "# carpet_amd batch tuner cache: <class key>|<zchunk> <points per item> "
        "<workgroups per CU: 0 resident capacity, -1 one per item> "
        "<tile shape: 0 wide (32x8 coarse bases, 64x8 in the FP32-storage FP64 mode), 1 16x16, "
+       "2 8x8 (FP64 regions thin in x and y or z, fixed), "
        "-1 by region size>\n";
   for (const auto &kv : m)
     f << kv.first << "|" << kv.second.zchunk << " " << kv.second.pt << " " << kv.second.wpc << " "
@@ -208,7 +209,7 @@ extern "C" int cg_batch_autotune(cg_batch *b, cg_stream_t stream, int reps,
       if (w != base.wpc && (w != -2 || l.is_tensor))
         if (int e = trial_run(Tune{base.zchunk, base.pt, w, base.shape}))
           return e;
-    if (l.is_tensor) // the other tile shape, in the best mode
+    if (l.is_tensor && l.shape < 2) // the other large tile shape, in the best mode
       if (int e = trial_run(Tune{bestc.zchunk, bestc.pt, bestc.wpc, 1 - l.shape}))
         return e;
     b->tuned[l.key] = bestc;

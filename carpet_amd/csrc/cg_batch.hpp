@@ -42,8 +42,9 @@ struct Item {
   unsigned tile;
 };
 
-// batched tensor launches per line kind (LK_*) and tile shape; defined in
-// cg_batch_tensor.hpp, instantiated in cg_batch_{vlag,clag,eno}_{w,n}.hip
+// batched tensor launches per line kind (LK_*) and tile shape (0 32x8 wide,
+// 1 16x16 narrow, 2 8x8 small, FP64 only); defined in cg_batch_tensor.hpp,
+// instantiated in cg_batch_{vlag,clag,eno,veno}_{w,n,s}.hip
 template <int KIND, int SH>
 int batch_tensor_kind(const BatchLaunch &l, const TensorArgs *td, const Item *items,
                       const int *seg, int *ctr, hipStream_t st);

@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <queue>
 #include <vector>
 
 #include "cg_batch_dev.hpp"
@@ -165,12 +166,71 @@ namespace {
 
 struct Group {
   int phase, is_tensor, kind, order, ntl, op, prec;
+  int small; // tensor regions routed to the 8x8-base tiles
   std::vector<int> members; // transfer indices
   bool same(const Group &g) const {
     return phase == g.phase && is_tensor == g.is_tensor && kind == g.kind &&
-           order == g.order && ntl == g.ntl && op == g.op && prec == g.prec;
+           order == g.order && ntl == g.ntl && op == g.op && prec == g.prec && small == g.small;
   }
 };
+
+// Lanes a region keeps busy in tiles of shape sh: tiles x (planes + stencil
+// span) x (tile width) x (tile rows + span), the work of the window loads
+// and of the x- and y-passes.
+double tile_busy(const TensorArgs &a, int ntl, int prec, int span, int sh) {
+  int cx, cy, th;
+  tile_shape(ntl, prec, &cx, &cy, &th, sh);
+  return (double)((a.nq[0] + cx - 1) / cx) * ((a.nq[1] + cy - 1) / cy) * (a.nq[2] + span) * cx *
+         (cy + span);
+}
+
+int line_span(int kind, int order) {
+  int span = 0;
+  with_line(kind, order, [&](auto tag) -> int {
+    using L = decltype(tag);
+    span = L::TMAXALL - L::TMINALL;
+    return 0;
+  });
+  return span;
+}
+
+// An FP64 region goes to the small tiles when they keep at most 1/1.3 of
+// the lanes busy that the better of the two large shapes would: a small
+// tile's lane costs more (more tiles per region, each with its own ramp-up
+// planes and halo).  (Measured, both workloads in one process: the level
+// step of a 3-level hierarchy, whose boundary prolongation regions are 3-
+// and 6-point shells, 1.14 ms against 1.61 ms without small tiles; 2000
+// regions of 8-96 points per axis 19.41 ms against 19.53 ms, and 19.83 ms
+// when routing at the break-even point 1.0.)
+bool small_tiles(const ProlongPlan &p, int prec) {
+  if (prec != CG_PREC_F64)
+    return false;
+  const int span = line_span(p.kind, p.order);
+  const double big = std::min(tile_busy(p.t, p.ntl, prec, span, 0), tile_busy(p.t, p.ntl, prec, span, 1));
+  return tile_busy(p.t, p.ntl, prec, span, 2) * 1.3 <= big;
+}
+
+template <int SH> const void *tensor_fn_sh(const BatchLaunch &l, int *threads) {
+  switch (l.kind) {
+  case LK_VLAG: return batch_tensor_fn_kind<LK_VLAG, SH>(l, threads);
+  case LK_CLAG: return batch_tensor_fn_kind<LK_CLAG, SH>(l, threads);
+  case LK_ENO: return batch_tensor_fn_kind<LK_ENO, SH>(l, threads);
+  case LK_VENO: return batch_tensor_fn_kind<LK_VENO, SH>(l, threads);
+  }
+  return nullptr;
+}
+
+template <int SH>
+int tensor_run_sh(const BatchLaunch &l, const TensorArgs *td, const Item *items, const int *seg, int *ctr,
+                  hipStream_t st) {
+  switch (l.kind) {
+  case LK_VLAG: return batch_tensor_kind<LK_VLAG, SH>(l, td, items, seg, ctr, st);
+  case LK_CLAG: return batch_tensor_kind<LK_CLAG, SH>(l, td, items, seg, ctr, st);
+  case LK_ENO: return batch_tensor_kind<LK_ENO, SH>(l, td, items, seg, ctr, st);
+  case LK_VENO: return batch_tensor_kind<LK_VENO, SH>(l, td, items, seg, ctr, st);
+  }
+  return -1;
+}
 
 template <int OP> const void *persist_point_pu(int pu) {
   switch (pu_inst<OP>(pu)) {
@@ -201,12 +261,10 @@ const void *launch_fn(const BatchLaunch &l, int *threads) {
     *threads = 256;
     return persist_point_fn(l.op, l.pt);
   }
-  const bool n = l.shape == 1;
-  switch (l.kind) {
-  case LK_VLAG: return n ? batch_tensor_fn_kind<LK_VLAG, 1>(l, threads) : batch_tensor_fn_kind<LK_VLAG, 0>(l, threads);
-  case LK_CLAG: return n ? batch_tensor_fn_kind<LK_CLAG, 1>(l, threads) : batch_tensor_fn_kind<LK_CLAG, 0>(l, threads);
-  case LK_ENO: return n ? batch_tensor_fn_kind<LK_ENO, 1>(l, threads) : batch_tensor_fn_kind<LK_ENO, 0>(l, threads);
-  case LK_VENO: return n ? batch_tensor_fn_kind<LK_VENO, 1>(l, threads) : batch_tensor_fn_kind<LK_VENO, 0>(l, threads);
+  switch (l.shape) {
+  case 0: return tensor_fn_sh<0>(l, threads);
+  case 1: return tensor_fn_sh<1>(l, threads);
+  case 2: return tensor_fn_sh<2>(l, threads);
   }
   return nullptr;
 }
@@ -271,6 +329,7 @@ int batch_device_build(BatchDevice &dev, const std::vector<Transfer> &tr,
     g.prec = prec[i];
     if (tr[i].is_prolong && tr[i].pr.kind != PK_TVD && tr[i].pr.kind != PK_MONO5) {
       g.is_tensor = 1;
+      g.small = small_tiles(tr[i].pr, prec[i]) ? 1 : 0;
       g.kind = tr[i].pr.kind;
       g.order = tr[i].pr.order;
       g.ntl = tr[i].pr.ntl;
@@ -321,46 +380,44 @@ int batch_device_build(BatchDevice &dev, const std::vector<Transfer> &tr,
     l.prec = g.prec;
     // key of the tuner cache: kernel class + box-size histogram bucket
     // (log2 of the median destination points per transfer) + count bucket
+    long long median_pts = 0;
     {
       std::vector<long long> pts;
       for (int m : g.members)
         pts.push_back(tr[m].dst_points());
       std::nth_element(pts.begin(), pts.begin() + pts.size() / 2, pts.end());
+      median_pts = pts[pts.size() / 2];
       char key[128];
-      snprintf(key, sizeof key, "%s k%d o%d t%d p%d op%d h%d n%d", g.is_tensor ? "tensor" : "point",
-               g.kind, g.order, g.ntl, g.prec, g.op, ilog2(pts[pts.size() / 2]),
+      snprintf(key, sizeof key, "%s k%d o%d t%d p%d op%d h%d n%d",
+               !g.is_tensor ? "point" : g.small ? "tensor8" : "tensor",
+               g.kind, g.order, g.ntl, g.prec, g.op, ilog2(median_pts),
                ilog2((long long)g.members.size()));
       l.key = key;
     }
     l.zchunk = tile_cfg < 0 ? ZCHUNK_DEFAULT : tile_cfg;
     l.pt = PT_DEFAULT;
-    // tensor tiles: persistent grid (its workgroups take the tiles in order,
-    // so neighbouring tiles march through z together and share halo rows in
-    // L2); pointwise runs: one workgroup per item
-    l.wpc = g.is_tensor ? 0 : -1;
+    // tensor tiles of large regions (median 2^20 points and up): persistent
+    // grid (its workgroups take the tiles in order, so neighbouring tiles
+    // march through z together and share halo rows in L2); other tensor
+    // classes and pointwise runs: one workgroup per item.  (Measured: on
+    // 2000 regions of 8-96 points per axis both modes are within 0.2%; on a
+    // hierarchy's shells one workgroup per item takes 0.45 ms less per
+    // step for the small tiles and 0.09 ms less for the large ones.)
+    l.wpc = g.is_tensor && !g.small && median_pts >= (1LL << 20) ? 0 : -1;
     // tile shape: the one that keeps fewer lanes busy in total -- tiles x
     // (planes + stencil span) x (tile width) x (tile rows + span), the work
     // of the window loads and of the x- and y-passes -- with a 25% margin
     // for the wide tile, which is faster per lane on large regions (single
     // 512^3 patch: narrow 0.81x the busy lanes of wide, yet 4% slower)
     l.shape = 0;
-    if (g.is_tensor) {
-      int span = 0;
-      with_line(g.kind, g.order, [&](auto tag) -> int {
-        using L = decltype(tag);
-        span = L::TMAXALL - L::TMINALL;
-        return 0;
-      });
+    if (g.is_tensor && g.small) {
+      l.shape = 2;
+    } else if (g.is_tensor) {
+      const int span = line_span(g.kind, g.order);
       double busy[2] = {0, 0};
-      for (int sh = 0; sh < 2; ++sh) {
-        int cx, cy, th;
-        tile_shape(g.ntl, g.prec, &cx, &cy, &th, sh);
-        for (int m : g.members) {
-          const TensorArgs &a = tr[m].pr.t;
-          busy[sh] += (double)((a.nq[0] + cx - 1) / cx) * ((a.nq[1] + cy - 1) / cy) *
-                      (a.nq[2] + span) * cx * (cy + span);
-        }
-      }
+      for (int sh = 0; sh < 2; ++sh)
+        for (int m : g.members)
+          busy[sh] += tile_busy(tr[m].pr.t, g.ntl, g.prec, span, sh);
       l.shape = busy[1] < 0.75 * busy[0] ? 1 : 0;
     }
     if (tuned) {
@@ -371,7 +428,7 @@ int batch_device_build(BatchDevice &dev, const std::vector<Transfer> &tr,
         l.wpc = it->second.wpc;
         if (!g.is_tensor && l.wpc == -2) // static rounds: tensor tiles only
           l.wpc = -1;
-        if (g.is_tensor && it->second.shape >= 0)
+        if (g.is_tensor && !g.small && it->second.shape >= 0 && it->second.shape < 2)
           l.shape = it->second.shape;
       }
     }
@@ -396,22 +453,43 @@ int batch_device_build(BatchDevice &dev, const std::vector<Transfer> &tr,
       TileCfg cfg;
       int tthreads = 0;
       tile_shape(g.ntl, g.prec, &cfg.cx, &cfg.cy, &tthreads, l.shape);
-      // z chunk: the longest of 128..8 planes that still gives the grid a few
-      // items per resident workgroup (tiles that march further apart in z
-      // stop sharing their halo rows through L2)
+      // z chunk: the one whose items finish soonest on the resident
+      // workgroups, under greedy list scheduling in dispatch order (what the
+      // work counters and the dispatcher do) with an item's cost its planes
+      // plus the ramp-up of the stencil span; a chunk adds those ramp-up
+      // planes, and whole-region items leave workgroups idle when regions
+      // are few.  Near-ties go to the longer chunk (neighbouring tiles
+      // then march through z together and share halo rows in L2).
       if (tile_cfg < 0 && !(tuned && tuned->count(l.key))) {
-        l.zchunk = 128;
-        for (int zc : {128, 64, 32, 16, 8}) {
-          l.zchunk = zc;
-          long long tiles = 0;
+        const int span = line_span(g.kind, g.order);
+        std::vector<double> mk;
+        const int zcs[5] = {128, 64, 32, 16, 8};
+        for (int zc : zcs) {
+          std::priority_queue<double, std::vector<double>, std::greater<double>> q;
+          for (int w = 0; w < full_grid; ++w)
+            q.push(0.0);
           for (int m : members) {
-            TensorArgs a = tr[m].pr.t;
-            cfg.zchunk = zc;
-            tiles += tensor_tiles(a, cfg);
+            const TensorArgs &a = tr[m].pr.t;
+            const long long nxy =
+                (long long)((a.nq[0] + cfg.cx - 1) / cfg.cx) * ((a.nq[1] + cfg.cy - 1) / cfg.cy);
+            for (int z0 = 0; z0 < a.nq[2]; z0 += zc)
+              for (long long t = 0; t < nxy; ++t) {
+                const double at = q.top();
+                q.pop();
+                q.push(at + std::min(zc, a.nq[2] - z0) + span + 1);
+              }
           }
-          if (tiles >= 4LL * full_grid)
-            break;
+          double end = 0;
+          for (; !q.empty(); q.pop())
+            end = std::max(end, q.top());
+          mk.push_back(end);
         }
+        const double best = *std::min_element(mk.begin(), mk.end());
+        for (int i = 0; i < 5; ++i)
+          if (mk[i] <= 1.03 * best) {
+            l.zchunk = zcs[i];
+            break;
+          }
       }
       std::vector<TensorArgs> descs;
       for (int m : members) {
@@ -572,24 +650,10 @@ static int run_launch(BatchDevice &dev, const BatchLaunch &l, hipStream_t st) {
   int rc = 0;
   if (l.is_tensor) {
     const TensorArgs *td = static_cast<const TensorArgs *>(descs);
-    const bool n = l.shape == 1;
-    switch (l.kind) {
-    case LK_VLAG:
-      rc = n ? batch_tensor_kind<LK_VLAG, 1>(l, td, items, seg, ctr, st)
-             : batch_tensor_kind<LK_VLAG, 0>(l, td, items, seg, ctr, st);
-      break;
-    case LK_CLAG:
-      rc = n ? batch_tensor_kind<LK_CLAG, 1>(l, td, items, seg, ctr, st)
-             : batch_tensor_kind<LK_CLAG, 0>(l, td, items, seg, ctr, st);
-      break;
-    case LK_ENO:
-      rc = n ? batch_tensor_kind<LK_ENO, 1>(l, td, items, seg, ctr, st)
-             : batch_tensor_kind<LK_ENO, 0>(l, td, items, seg, ctr, st);
-      break;
-    case LK_VENO:
-      rc = n ? batch_tensor_kind<LK_VENO, 1>(l, td, items, seg, ctr, st)
-             : batch_tensor_kind<LK_VENO, 0>(l, td, items, seg, ctr, st);
-      break;
+    switch (l.shape) {
+    case 0: rc = tensor_run_sh<0>(l, td, items, seg, ctr, st); break;
+    case 1: rc = tensor_run_sh<1>(l, td, items, seg, ctr, st); break;
+    case 2: rc = tensor_run_sh<2>(l, td, items, seg, ctr, st); break;
     default: rc = -1;
     }
   } else {

@@ -1,12 +1,15 @@
 // cg_batch_tensor.hpp -- batched tensor prolongation kernels (grid and
 // persistent modes), one launch template per line kind and tile shape;
-// instantiated in cg_batch_{vlag,clag,eno}_{w,n}.hip so they compile in
+// instantiated in cg_batch_{vlag,clag,eno,veno}_{w,n,s}.hip so they compile in
 // parallel.
 #pragma once
 
 #include "cg_batch_dev.hpp"
 
 namespace cg {
+
+// the small tile shape (2) is instantiated for FP64 storage only
+template <int KIND, int SH> constexpr bool has_mixed() { return !fp64_only(KIND) && SH < 2; }
 
 // 512-thread workgroups of orders up to 5 are held to 128 VGPRs: two
 // workgroups (4 waves per SIMD) per CU instead of one
@@ -78,7 +81,7 @@ int batch_tensor_kind(const BatchLaunch &l, const TensorArgs *td, const Item *it
     };
     if (l.prec == 0)
       return by_ntl(std::integral_constant<int, 0>{});
-    if constexpr (!fp64_only(KIND)) {
+    if constexpr (has_mixed<KIND, SH>()) {
       if (l.prec == 1)
         return by_ntl(std::integral_constant<int, 1>{});
       if (l.prec == 2)
@@ -104,7 +107,7 @@ template <int KIND, int SH> const void *batch_tensor_fn_kind(const BatchLaunch &
     auto by_prec = [&](auto ntlc) {
       if (l.prec == 0)
         pick(ntlc, std::integral_constant<int, 0>{});
-      if constexpr (!fp64_only(KIND)) {
+      if constexpr (has_mixed<KIND, SH>()) {
         if (l.prec == 1)
           pick(ntlc, std::integral_constant<int, 1>{});
         if (l.prec == 2)

@@ -704,6 +704,12 @@ template <> struct TileShape<3, 1, 0> {
 template <> struct TileShape<1, 0, 1> {
   static constexpr int CX = 64, CY = 8, IPT = 2, XQ = 2;
 };
+// small tiles (FP64 batch only) for regions a few fine points across in x
+// and in y or z: edges and corners of ghost and buffer shells, faces thin
+// in x, where a 32x8 or 16x16 tile keeps a tenth of its lanes busy or less
+template <int NTL> struct TileShape<NTL, 2, 0> {
+  static constexpr int CX = 8, CY = 8, IPT = 1, XQ = 2;
+};
 // the pipelined schedule (double-buffered window and x-pass results) where
 // its LDS fits the 64 KB of a static allocation, else two barriers per plane
 template <class L, int NTL, int CX, int CY, int XQ> constexpr int pipe_fits() {
@@ -733,8 +739,10 @@ inline void tile_shape(int ntl, int prec, int *cx, int *cy, int *threads, int sh
   auto by_sh = [&](auto ntlc, auto prc) {
     if (sh == 0)
       get(ntlc, std::integral_constant<int, 0>{}, prc);
-    else
+    else if (sh == 1)
       get(ntlc, std::integral_constant<int, 1>{}, prc);
+    else
+      get(ntlc, std::integral_constant<int, 2>{}, prc);
   };
   auto by_prec = [&](auto ntlc) {
     if (prec == 1)

@@ -398,12 +398,14 @@ int cg_batch_launch_graph(cg_batch *b, cg_stream_t stream);
    persistent grids of resident capacity and of 2 workgroups per CU) and,
    for tensor tiles, the other tile shape (wide: 32x8 coarse bases, 64x8
    for CG_PREC_MIXED; narrow: 16x16),
-   with HIP events, and keeps the fastest per launch class.
+   with HIP events, and keeps the fastest per launch class.  FP64 regions
+   thin in x and in y or z (shell edges and corners) form classes of their
+   own on 8x8-base tiles, whose shape is fixed.
    Choices are keyed on (kernel class, order, time levels, precision, log2
    of the median region size, log2 of the region count) and kept in the
    text file cache_path (may be NULL; lines "<key>|<zchunk> <points>
    <workgroups per CU: -1 one per item, 0 resident capacity> <tile shape:
-   0 wide, 1 narrow, -1 by region size>"): classes the file names take its
+   0 wide, 1 narrow, 2 small, -1 by region size>"): classes the file names take its
    parameters, the others are searched.  The
    search executes each launch (one phase's class of transfers) reps times
    on its own, outside phase order, so call it before the step whose
