@@ -81,6 +81,9 @@ _decl("cg_prolongate_3d_o5_monotone_rf2_f64", i, *_OPARGS, vp)
 _decl("cg_restrict_3d_cc_o3_rf2_f64", i, *_OPARGS, vp)
 _decl("cg_restrict_3d_cc_o5_rf2_f64", i, *_OPARGS, vp)
 _decl("cg_set_restriction", i, i, i)
+_decl("cg_prolongate_3d_stagger_f64", i, i, i, *_OPARGS, vp)
+_decl("cg_restrict_3d_stagger_f64", i, i, *_OPARGS, vp)
+_decl("cg_set_staggered_operators", i, i)
 _decl("cg_interpolate_3d_2tl_f64", i, vp, dp, vp, dp, ip, ip, vp, dp, ip, ip, ip, ip, ip, ip, vp)
 _decl("cg_interpolate_3d_3tl_f64", i, vp, dp, vp, dp, vp, dp, ip, ip, vp, dp, ip, ip, ip, ip, ip, ip, vp)
 _decl("cg_interpolate_eno_3d_3tl_f64", i, vp, dp, vp, dp, vp, dp, ip, ip, vp, dp, ip, ip, ip, ip, ip, ip, vp)

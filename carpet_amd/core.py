@@ -20,6 +20,10 @@ from ._lib import GData, I3, I9, Islab, check, lib
 
 # CarpetLib enums (operators.hh:8-25, defs.hh:134)
 OP_ERROR, OP_NONE, OP_SYNC, OP_RESTRICT, OP_COPY, OP_LAGRANGE, OP_ENO, OP_WENO, OP_TVD, OP_LAGRANGE_MONOTONE = range(10)
+OP_STAGGER011, OP_STAGGER101, OP_STAGGER110, OP_STAGGER111 = range(10, 14)
+# staggered directions of the staggered operators: bit d set = staggered in
+# direction d (CarpetLib's op_STAGGERxyz)
+STAGGER011, STAGGER101, STAGGER110, STAGGER111 = 6, 5, 3, 7
 VERTEX_CENTERED, CELL_CENTERED = 1, 2
 PREC_F64, PREC_MIXED, PREC_MIXED_DOT2 = 0, 1, 2
 
@@ -342,6 +346,32 @@ def restrict_3d_cc_o5_rf2(src: DeviceArray, dst: DeviceArray, dstregbbox: Box,
                                            (srcregbbox or dstregbbox).as_c(), dstregbbox.as_c(),
                                            _sh(stream)),
           "restrict_3d_cc_o5_rf2")
+
+
+def prolongate_3d_stagger(stagger: int, order: int, src: DeviceArray, dst: DeviceArray, dstregbbox: Box,
+                          srcregbbox: Optional[Box] = None, stream=None):
+    """Staggered prolongation (CarpetLib/src/prolongate_3d_stagger{011,101,110,111}.cc),
+    stagger one of STAGGER011..STAGGER111, order 2 or 3; needs
+    set_staggered_operators(True)."""
+    check(lib.cg_prolongate_3d_stagger_f64(int(stagger), int(order), *_arr(src), *_arr(dst), src.box.as_c(),
+                                           dst.box.as_c(), (srcregbbox or dstregbbox).as_c(), dstregbbox.as_c(),
+                                           _sh(stream)),
+          "prolongate_3d_stagger")
+
+
+def restrict_3d_stagger(stagger: int, src: DeviceArray, dst: DeviceArray, dstregbbox: Box,
+                        srcregbbox: Optional[Box] = None, stream=None):
+    """Staggered restriction (CarpetLib/src/restrict_3d_stagger{011,101,110,111}.cc);
+    needs set_staggered_operators(True)."""
+    check(lib.cg_restrict_3d_stagger_f64(int(stagger), *_arr(src), *_arr(dst), src.box.as_c(), dst.box.as_c(),
+                                         (srcregbbox or dstregbbox).as_c(), dstregbbox.as_c(), _sh(stream)),
+          "restrict_3d_stagger")
+
+
+def set_staggered_operators(support_staggered_operators: bool):
+    """CarpetLib's support_staggered_operators parameter (param.ccl:65): the
+    staggered operators refuse to run unless it is set."""
+    check(lib.cg_set_staggered_operators(int(bool(support_staggered_operators))), "set_staggered_operators")
 
 
 def set_restriction(use_higher_order_restriction: bool, restriction_order_space: int = 3):

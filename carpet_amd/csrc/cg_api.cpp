@@ -15,6 +15,17 @@ const char *last_error_message();
 
 static hipStream_t S(cg_stream_t s) { return static_cast<hipStream_t>(s); }
 
+// staggered directions of op STAGGER011..111 (bit d = direction d), else 0
+static int stagger_dirs(int op) {
+  switch (op) {
+  case CG_OP_STAGGER011: return CG_STAGGER011;
+  case CG_OP_STAGGER101: return CG_STAGGER101;
+  case CG_OP_STAGGER110: return CG_STAGGER110;
+  case CG_OP_STAGGER111: return CG_STAGGER111;
+  }
+  return 0;
+}
+
 // data<T>::transfer_time -> transfer_p_r -> transfer_prolongate /
 // transfer_restrict / copy_from_innerloop, with the time-level choice of
 // gdata::find_source_timelevel (CarpetLib/src/data.cc:485-556, 572-966,
@@ -157,6 +168,17 @@ int resolve_transfer(const cg_gdata *dst, const cg_gdata *const *srcs,
       kind = PK_MONO5;
       order = 5;
       break;
+    case CG_OP_STAGGER011:
+    case CG_OP_STAGGER101:
+    case CG_OP_STAGGER110:
+    case CG_OP_STAGGER111:
+      // prolongate_3d_stagger*<T,2> for order_space 2, <T,3> for 3..5
+      // (data.cc:834-920); other orders index past the reference's table
+      if (order_space < 2 || order_space > 5)
+        return set_error(CG_ERR_ORDER, "There is no staggered stencil with order_space=%d", order_space);
+      kind = PK_STAGGER;
+      order = order_space == 2 ? 2 : 3;
+      break;
     default:
       return set_error(CG_ERR_OPERATOR, "prolongation operator %d is not provided", op);
     }
@@ -180,13 +202,21 @@ int resolve_transfer(const cg_gdata *dst, const cg_gdata *const *srcs,
     }
     tr.is_prolong = true;
     return plan_prolong(kind, order, s0, sp, ntl, sf, d, dreg, tr.pr, es,
-                        eno_time ? &ew : nullptr);
+                        eno_time ? &ew : nullptr, stagger_dirs(op));
   }
   if (finer) {
     if (op == CG_OP_SYNC)
       return set_error(CG_ERR_OPERATOR, "op_sync cannot restrict");
     if (ntl != 1)
       return set_error(CG_ERR_OPERATOR, "restriction with time interpolation is not provided");
+    // the staggered groups restrict with their own operators whatever the
+    // centring (data.cc:989-1027)
+    if (stagger_dirs(op)) {
+      if (slab)
+        return set_error(CG_ERR_ARG, "staggered restriction takes no slab");
+      tr.is_prolong = false;
+      return plan_restrict_stagger(stagger_dirs(op), s0, d, dreg, tr.pt);
+    }
     if (cent == CG_VERTEX_CENTERED) {
       if (slab)
         return set_error(CG_ERR_ARG, "vertex-centred restriction takes no slab");
@@ -572,6 +602,40 @@ int cg_prolongate_3d_o5_monotone_rf2_f64(const double *src, const int srcpadext[
   (void)srcregbbox;
   return prolong_call(PK_MONO5, 5, src, srcpadext, srcext, dst, dstpadext, dstext, srcbbox, dstbbox,
                       dstregbbox, stream);
+}
+
+int cg_prolongate_3d_stagger_f64(int stagger, int order, const double *src, const int srcpadext[3],
+                                 const int srcext[3], double *dst, const int dstpadext[3],
+                                 const int dstext[3], const int srcbbox[9], const int dstbbox[9],
+                                 const int srcregbbox[9], const int dstregbbox[9], cg_stream_t stream) {
+  (void)srcregbbox;
+  if (!srcpadext || !srcext || !dstpadext || !dstext || !srcbbox || !dstbbox || !dstregbbox)
+    return set_error(CG_ERR_ARG, "null argument");
+  ProlongPlan pl;
+  const void *sp[1] = {src};
+  const double s1[1] = {1.0};
+  int e = plan_prolong(PK_STAGGER, order, arr_from(src, srcpadext, srcext, srcbbox), sp, 1, s1,
+                       arr_from(dst, dstpadext, dstext, dstbbox), Box::from(dstregbbox), pl, 8, nullptr,
+                       stagger);
+  if (e)
+    return e;
+  return launch_prolong(pl, 0, S(stream));
+}
+
+int cg_restrict_3d_stagger_f64(int stagger, const double *src, const int srcpadext[3],
+                               const int srcext[3], double *dst, const int dstpadext[3],
+                               const int dstext[3], const int srcbbox[9], const int dstbbox[9],
+                               const int srcregbbox[9], const int dstregbbox[9], cg_stream_t stream) {
+  (void)srcregbbox;
+  if (!srcpadext || !srcext || !dstpadext || !dstext || !srcbbox || !dstbbox || !dstregbbox)
+    return set_error(CG_ERR_ARG, "null argument");
+  Transfer tr;
+  tr.is_prolong = false;
+  int e = plan_restrict_stagger(stagger, arr_from(src, srcpadext, srcext, srcbbox),
+                                arr_from(dst, dstpadext, dstext, dstbbox), Box::from(dstregbbox), tr.pt);
+  if (e)
+    return e;
+  return run_transfer(tr, 0, S(stream));
 }
 
 int cg_prolongate_3d_cc_tvd_f64(const double *src, const int srcpadext[3],

@@ -153,7 +153,12 @@ __global__ __launch_bounds__(256) void k_grid_point(const PointArgs *__restrict_
 }
 
 // TVD items run per coarse cell with their own unrolling: one instance
-template <int OP> constexpr int pu_inst(int pu) { return OP == PO_TVD ? 8 : pu; }
+// points per thread and round instantiated for an operator: TVD always 8
+// (per coarse cell), the staggered operators 1 (up to 64 taps per point and
+// time level), the others as asked
+template <int OP> constexpr int pu_inst(int pu) {
+  return OP == PO_TVD ? 8 : (OP == PO_STAGGER || OP == PO_RESTRICT_STAGGER) ? 1 : pu;
+}
 
 __global__ void k_zero_counters(int *c, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -238,7 +243,7 @@ template <int OP> const void *persist_point_pu(int pu) {
   case 2: return reinterpret_cast<const void *>(&k_persist_point<OP, 2>);
   case 4: return reinterpret_cast<const void *>(&k_persist_point<OP, 4>);
   }
-  return reinterpret_cast<const void *>(&k_persist_point<OP, 8>);
+  return reinterpret_cast<const void *>(&k_persist_point<OP, pu_inst<OP>(8)>);
 }
 
 const void *persist_point_fn(int op, int pt) {
@@ -252,6 +257,8 @@ const void *persist_point_fn(int op, int pt) {
   case PO_RESTRICT_CC_O3: return persist_point_pu<PO_RESTRICT_CC_O3>(pu);
   case PO_RESTRICT_CC_O5: return persist_point_pu<PO_RESTRICT_CC_O5>(pu);
   case PO_MONO5: return persist_point_pu<PO_MONO5>(pu);
+  case PO_STAGGER: return persist_point_pu<PO_STAGGER>(pu);
+  case PO_RESTRICT_STAGGER: return persist_point_pu<PO_RESTRICT_STAGGER>(pu);
   }
   return nullptr;
 }
@@ -327,7 +334,8 @@ int batch_device_build(BatchDevice &dev, const std::vector<Transfer> &tr,
     Group g{};
     g.phase = phase[i];
     g.prec = prec[i];
-    if (tr[i].is_prolong && tr[i].pr.kind != PK_TVD && tr[i].pr.kind != PK_MONO5) {
+    if (tr[i].is_prolong && tr[i].pr.kind != PK_TVD && tr[i].pr.kind != PK_MONO5 &&
+        tr[i].pr.kind != PK_STAGGER) {
       g.is_tensor = 1;
       g.small = small_tiles(tr[i].pr, prec[i]) ? 1 : 0;
       g.kind = tr[i].pr.kind;
@@ -338,7 +346,10 @@ int batch_device_build(BatchDevice &dev, const std::vector<Transfer> &tr,
       g.is_tensor = 0;
       g.kind = g.order = -1;
       g.ntl = 1;
-      g.op = !tr[i].is_prolong ? tr[i].pt.op : tr[i].pr.kind == PK_MONO5 ? PO_MONO5 : PO_TVD;
+      g.op = !tr[i].is_prolong            ? tr[i].pt.op
+             : tr[i].pr.kind == PK_MONO5   ? PO_MONO5
+             : tr[i].pr.kind == PK_STAGGER ? PO_STAGGER
+                                           : PO_TVD;
     }
     auto it = std::find_if(groups.begin(), groups.end(), [&](const Group &x) { return x.same(g); });
     if (it == groups.end()) {
@@ -675,7 +686,7 @@ static int run_launch(BatchDevice &dev, const BatchLaunch &l, hipStream_t st) {
       case 2: return go_pu(opc, std::integral_constant<int, pu_inst<OP>(2)>{});
       case 4: return go_pu(opc, std::integral_constant<int, pu_inst<OP>(4)>{});
       }
-      return go_pu(opc, std::integral_constant<int, 8>{});
+      return go_pu(opc, std::integral_constant<int, pu_inst<OP>(8)>{});
     };
     switch (l.op) {
     case PO_COPY: rc = go(std::integral_constant<int, PO_COPY>{}); break;
@@ -686,6 +697,8 @@ static int run_launch(BatchDevice &dev, const BatchLaunch &l, hipStream_t st) {
     case PO_RESTRICT_CC_O3: rc = go(std::integral_constant<int, PO_RESTRICT_CC_O3>{}); break;
     case PO_RESTRICT_CC_O5: rc = go(std::integral_constant<int, PO_RESTRICT_CC_O5>{}); break;
     case PO_MONO5: rc = go(std::integral_constant<int, PO_MONO5>{}); break;
+    case PO_STAGGER: rc = go(std::integral_constant<int, PO_STAGGER>{}); break;
+    case PO_RESTRICT_STAGGER: rc = go(std::integral_constant<int, PO_RESTRICT_STAGGER>{}); break;
     default: rc = -1;
     }
   }

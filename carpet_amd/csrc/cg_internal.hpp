@@ -65,7 +65,8 @@ int check_array(const Arr &a, const char *name);
 
 // kinds of tensor prolongation (cg::LineKind) plus TVD
 // the tensor kinds equal the line kinds (LK_*) of the kernels
-enum ProlongKind { PK_VLAG = 0, PK_CLAG = 1, PK_ENO = 2, PK_TVD = 3, PK_VENO = 4, PK_MONO5 = 5 };
+enum ProlongKind { PK_VLAG = 0, PK_CLAG = 1, PK_ENO = 2, PK_TVD = 3, PK_VENO = 4, PK_MONO5 = 5,
+                   PK_STAGGER = 6 };
 
 // A validated prolongation: kernel arguments for one (region, time levels)
 // plus the statistics the batch path reports.
@@ -91,9 +92,10 @@ struct EnoTime {
 int eno_time_factors(const double *tt, double t, EnoTime &w);
 
 // eno: the time levels combine with interpolate_eno_3d_3tl's rule (3 levels)
+// stag: PK_STAGGER's directions (bit d set = staggered in direction d)
 int plan_prolong(int kind, int order, const Arr &src, const void *const *srcs,
                  int ntl, const double *s, const Arr &dst, const Box &reg,
-                 ProlongPlan &plan, int es = 8, const EnoTime *eno = nullptr);
+                 ProlongPlan &plan, int es = 8, const EnoTime *eno = nullptr, int stag = 0);
 int launch_prolong(const ProlongPlan &plan, int prec, hipStream_t st);
 
 struct PointPlan {
@@ -108,6 +110,12 @@ int plan_copy(const Arr &src, const Arr &dst, const Box &srcreg1,
 // restrict_3d_cc_o3_rf2 / restrict_3d_cc_o5_rf2)
 int plan_restrict(bool cell, const Arr &src, const Arr &dst, const Box &reg,
                   PointPlan &plan, int order = 1);
+// restrict_3d_stagger{011,101,110,111} (bit d of stag set = staggered in
+// direction d)
+int plan_restrict_stagger(int stag, const Arr &src, const Arr &dst, const Box &reg, PointPlan &plan);
+// CarpetLib's support_staggered_operators parameter (process-wide, default
+// off): the staggered operators refuse to run without it
+bool staggered_supported();
 // CarpetLib's use_higher_order_restriction / restriction_order_space
 // parameters, as seen by the dispatch (process-wide, default off / 1)
 void restriction_setting(int *use_higher_order, int *order_space);

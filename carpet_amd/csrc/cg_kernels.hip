@@ -91,6 +91,8 @@ int launch_point(int op, const PointArgs &a, hipStream_t s) {
   case PO_RESTRICT_CC_O3: hipLaunchKernelGGL(k_point<PO_RESTRICT_CC_O3>, grid, block, 0, s, a); break;
   case PO_RESTRICT_CC_O5: hipLaunchKernelGGL(k_point<PO_RESTRICT_CC_O5>, grid, block, 0, s, a); break;
   case PO_MONO5: hipLaunchKernelGGL(k_point<PO_MONO5>, grid, block, 0, s, a); break;
+  case PO_STAGGER: hipLaunchKernelGGL(k_point<PO_STAGGER>, grid, block, 0, s, a); break;
+  case PO_RESTRICT_STAGGER: hipLaunchKernelGGL(k_point<PO_RESTRICT_STAGGER>, grid, block, 0, s, a); break;
   default: return -1;
   }
   return hipGetLastError() == hipSuccess ? 0 : -2;

@@ -52,7 +52,7 @@ __device__ __forceinline__ unsigned fastdiv(unsigned n, FastDiv f) {
 // restriction, cell-centred restriction, time interpolation, cell TVD.
 enum PointOp { PO_COPY = 0, PO_RESTRICT_VC = 1, PO_RESTRICT_CC = 2,
                PO_INTERP = 3, PO_TVD = 4, PO_RESTRICT_CC_O3 = 5, PO_RESTRICT_CC_O5 = 6,
-               PO_MONO5 = 7 };
+               PO_MONO5 = 7, PO_STAGGER = 8, PO_RESTRICT_STAGGER = 9 };
 
 struct PointArgs {
   const double *src[3]; // source element matching region point (0,0,0)
@@ -67,6 +67,8 @@ struct PointArgs {
   double s2[2];         // ENO time rule: fallback pair weights
   int l0, teno;         //   first level of the pair; 1 = rule on (3 levels)
   int off[3];           // TVD: fine offsets (off+f)/2 = parent, (off+f)%2 = side
+  int stag;             // staggered operators: bit d set = staggered in direction d
+  int sorder;           //   prolongation order (2 or 3)
   FastDiv fx, fxy;      // division by n[0] and n[0]*n[1] (batch work items)
 };
 

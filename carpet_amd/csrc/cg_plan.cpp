@@ -112,7 +112,7 @@ static LineInfo line_info_rt(int kind, int order, bool &ok) {
 // prolongate_3d_cc_real8_tvd.F90:94-179 (cell TVD)
 int plan_prolong(int kind, int order, const Arr &src, const void *const *srcs,
                  int ntl, const double *s, const Arr &dst, const Box &reg,
-                 ProlongPlan &pl, int es, const EnoTime *eno) {
+                 ProlongPlan &pl, int es, const EnoTime *eno, int stag) {
   if (es != 8 && !(es == 4 && (kind == PK_VLAG || kind == PK_CLAG)))
     return set_error(CG_ERR_OPERATOR, "mixed precision applies to Lagrange prolongation only");
   int e;
@@ -132,10 +132,21 @@ int plan_prolong(int kind, int order, const Arr &src, const void *const *srcs,
   pl.ntl = ntl;
   pl.es = es;
 
-  if (kind == PK_MONO5) {
-    // prolongate_3d_o5_monotone_rf2.cc:56-97, vertex-centred geometry; the
-    // exact footprint (b-2..b+3 where a point lies between coarse points)
-    // must lie in the source array
+  if (kind == PK_MONO5 || kind == PK_STAGGER) {
+    // prolongate_3d_o5_monotone_rf2.cc:56-97 and
+    // prolongate_3d_stagger011.cc:30-90 (the other three alike): vertex
+    // geometry; the exact footprint must lie in the source array (monotone:
+    // b-2..b+3 where a point lies between coarse points; staggered: the tap
+    // sets of the direction's kind, b-2..b+2 at most)
+    if (kind == PK_STAGGER) {
+      if (order != 2 && order != 3)
+        return set_error(CG_ERR_ORDER, "There is no staggered stencil with order %d", order);
+      if (stag < 1 || stag > 7)
+        return set_error(CG_ERR_ARG, "staggered directions %d not in 1..7", stag);
+      if (!staggered_supported())
+        return set_error(CG_ERR_OPERATOR, "When using staggered restriction and prolongation operators "
+                                          "you need to set CarpetLib::support_staggered_operators");
+    }
     for (int d = 0; d < 3; ++d) {
       if (sb.str[d] <= reg.str[d] || db.str[d] != reg.str[d])
         return set_error(CG_ERR_STRIDE, "Internal error: strides disagree");
@@ -162,8 +173,19 @@ int plan_prolong(int kind, int order, const Arr &src, const void *const *srcs,
         if (f < 0 || f >= p.n[d])
           continue;
         const int g = p.off[d] + f, b = g >> 1;
-        lo = std::min(lo, (g & 1) ? b - 2 : b);
-        hi = std::max(hi, (g & 1) ? b + 3 : b);
+        if (kind == PK_MONO5) {
+          lo = std::min(lo, (g & 1) ? b - 2 : b);
+          hi = std::max(hi, (g & 1) ? b + 3 : b);
+        } else {
+          // tap sets: staggered f=0 b-1..b+1 (order 2) / b-2..b+1 (order 3),
+          // staggered f=1 and unstaggered f=1 b-1..b+order-1, unstaggered
+          // f=0 b (order 2) / b-1..b+1 (order 3)
+          const bool st = (stag >> d) & 1, odd = g & 1;
+          const int l = st && !odd && order == 3 ? b - 2 : (!st && !odd && order == 2) ? b : b - 1;
+          const int h = odd ? b + order - 1 : st ? b + 1 : (order == 2 ? b : b + 1);
+          lo = std::min(lo, l);
+          hi = std::max(hi, h);
+        }
       }
       if (p.off[d] < 0 || lo < 0 || hi >= src.ext[d])
         return set_error(CG_ERR_CONTAIN,
@@ -181,6 +203,8 @@ int plan_prolong(int kind, int order, const Arr &src, const void *const *srcs,
       p.src[l] = static_cast<const double *>(srcs[l]);
       p.s[l] = s[l];
     }
+    p.stag = stag;
+    p.sorder = order;
     pl.dst_points = (long long)p.n[0] * p.n[1] * p.n[2];
     pl.src_points = srcpts;
     return CG_OK;
@@ -342,6 +366,8 @@ int launch_prolong(const ProlongPlan &pl, int prec, hipStream_t st) {
     r = launch_point(PO_TVD, pl.p, st);
   } else if (pl.kind == PK_MONO5) {
     r = launch_point(PO_MONO5, pl.p, st);
+  } else if (pl.kind == PK_STAGGER) {
+    r = launch_point(PO_STAGGER, pl.p, st);
   } else {
     TileCfg cfg{0, 0, 0};
     // z chunks: at most 128 planes (tiles that march much further apart in
@@ -472,6 +498,71 @@ int plan_restrict(bool cell, const Arr &src, const Arr &dst, const Box &reg,
   pl.a.dk = dst.sk();
   pl.dst_points = (long long)pl.a.n[0] * pl.a.n[1] * pl.a.n[2];
   pl.src_points = pl.dst_points * (cell ? 8 : 1);
+  return CG_OK;
+}
+
+// restrict_3d_stagger011.cc:26-100 (the other three alike): vertex
+// geometry; every fine point read (2i-1 .. 2i+2 in staggered directions,
+// 2i-1 .. 2i+1 in the others) must lie in the source array
+int plan_restrict_stagger(int stag, const Arr &src, const Arr &dst, const Box &reg, PointPlan &pl) {
+  int e;
+  if (stag < 1 || stag > 7)
+    return set_error(CG_ERR_ARG, "staggered directions %d not in 1..7", stag);
+  if ((e = check_array(src, "source")) || (e = check_array(dst, "destination")))
+    return e;
+  if ((e = check_region_stride(reg)))
+    return e;
+  const Box &sb = src.box, &db = dst.box;
+  for (int d = 0; d < 3; ++d)
+    if (sb.str[d] >= reg.str[d] || db.str[d] != reg.str[d])
+      return set_error(CG_ERR_STRIDE, "Internal error: strides disagree");
+  for (int d = 0; d < 3; ++d)
+    if (2 * sb.str[d] != db.str[d])
+      return set_error(CG_ERR_REFFACT, "Internal error: destination strides are "
+                                       "not twice the source strides");
+  if (reg.empty())
+    return set_error(CG_ERR_EMPTY, "Internal error: region extent is empty");
+  if (!reg.contained_in(db))
+    return set_error(CG_ERR_CONTAIN, "Internal error: region extent is not contained in array extent");
+  if (!staggered_supported())
+    return set_error(CG_ERR_OPERATOR, "When using staggered restriction and prolongation operators "
+                                      "you need to set CarpetLib::support_staggered_operators");
+  std::memset(&pl, 0, sizeof pl);
+  int so[3], dof[3];
+  for (int d = 0; d < 3; ++d) {
+    pl.a.n[d] = reg.sizes(d);
+    if ((reg.lo[d] - sb.lo[d]) % sb.str[d] || (reg.lo[d] - db.lo[d]) % db.str[d])
+      return set_error(CG_ERR_ALIGN, "region is not aligned with the source/destination (dir %d)", d);
+    so[d] = (reg.lo[d] - sb.lo[d]) / sb.str[d];
+    dof[d] = (reg.lo[d] - db.lo[d]) / db.str[d];
+    const int hi = so[d] + 2 * (pl.a.n[d] - 1) + (((stag >> d) & 1) ? 2 : 1);
+    if (so[d] - 1 < 0 || hi >= src.ext[d])
+      return set_error(CG_ERR_CONTAIN, "Internal error: region extent is not contained in array extent "
+                                       "(stencil footprint [%d,%d] vs source [0,%d) in dir %d)",
+                       so[d] - 1, hi, src.ext[d], d);
+  }
+  pl.op = PO_RESTRICT_STAGGER;
+  pl.a.stag = stag;
+  pl.a.src[0] = static_cast<const double *>(src.p) + src.at(so[0], so[1], so[2]);
+  pl.a.sj = src.sj();
+  pl.a.sk = src.sk();
+  pl.a.dst = static_cast<double *>(const_cast<void *>(dst.p)) + dst.at(dof[0], dof[1], dof[2]);
+  pl.a.dj = dst.sj();
+  pl.a.dk = dst.sk();
+  pl.dst_points = (long long)pl.a.n[0] * pl.a.n[1] * pl.a.n[2];
+  long long sp = 1;
+  for (int d = 0; d < 3; ++d)
+    sp *= 2 * (long long)pl.a.n[d] + (((stag >> d) & 1) ? 2 : 1);
+  pl.src_points = sp;
+  return CG_OK;
+}
+
+static std::atomic<int> g_staggered{0};
+
+bool staggered_supported() { return g_staggered.load(std::memory_order_relaxed) != 0; }
+
+extern "C" int cg_set_staggered_operators(int support_staggered_operators) {
+  g_staggered.store(support_staggered_operators ? 1 : 0, std::memory_order_relaxed);
   return CG_OK;
 }
 

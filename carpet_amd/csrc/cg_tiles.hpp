@@ -492,6 +492,110 @@ __device__ __forceinline__ double mono5_value(const double *src, long long J, lo
   return r;
 }
 
+// Staggered operators (prolongate_3d_stagger{011,101,110,111}.cc,
+// restrict_3d_stagger{011,101,110,111}.cc).  Bit d of stag is set where the
+// field is staggered in direction d.  Per direction and fine parity f the
+// prolongation uses one tap set (coeffs_prolongate_3d_stagger.hh), with the
+// first tap at coarse base + stag_lo and stag_n taps:
+//   kind 0, staggered, f = 0: coeff[1] (to b - 1/4)
+//   kind 1, staggered, f = 1: coeff[2] (to b + 1/4)
+//   kind 2, unstaggered, f = 1: coeff[3] (to b + 1/2)
+//   kind 3, unstaggered, f = 0: coeff[4] for order 3; order 2 takes the
+//        coarse value (the reference's two-dimensional sums)
+__device__ __forceinline__ int stag_kind(int stag, int d, int f) {
+  return ((stag >> d) & 1) ? f : (f ? 2 : 3);
+}
+__device__ __forceinline__ int stag_lo(int order, int kind) {
+  return order == 3 && kind == 0 ? -2 : order == 2 && kind == 3 ? 0 : -1;
+}
+__device__ __forceinline__ int stag_n(int order, int kind) {
+  return order == 2 ? (kind == 3 ? 1 : 3) : (kind == 3 ? 3 : 4);
+}
+__device__ __forceinline__ double stag_coef(int order, int kind, int t) {
+  if (order == 2) {
+    switch (kind) {
+    case 0: return t == 0 ? 5.0 / 32.0 : t == 1 ? 15.0 / 16.0 : -3.0 / 32.0;
+    case 1: return t == 0 ? -3.0 / 32.0 : t == 1 ? 15.0 / 16.0 : 5.0 / 32.0;
+    case 2: return t == 0 ? -1.0 / 8.0 : t == 1 ? 3.0 / 4.0 : 3.0 / 8.0;
+    }
+    return 1.0;
+  }
+  switch (kind) {
+  case 0: return t == 0 ? -5.0 / 128.0 : t == 1 ? 35.0 / 128.0 : t == 2 ? 105.0 / 128.0 : -7.0 / 128.0;
+  case 1: return t == 0 ? -7.0 / 128.0 : t == 1 ? 105.0 / 128.0 : t == 2 ? 35.0 / 128.0 : -5.0 / 128.0;
+  case 2: return t == 0 ? -5.0 / 64.0 : t == 1 ? 37.0 / 64.0 : t == 2 ? 37.0 / 64.0 : -5.0 / 64.0;
+  }
+  return t == 1 ? 34.0 / 32.0 : -1.0 / 32.0;
+}
+
+// one staggered prolongation value at fine point (gi,gj,gk) relative to the
+// source array: sum from 0 of coeff_x * coeff_y * coeff_z * src, x taps
+// outermost and z innermost, each operation rounded as written (the
+// coefficient products are exact: dyadic numerators below 2^21)
+__device__ __forceinline__ double stagger_value(const double *src, long long J, long long K, int gi, int gj,
+                                                int gk, int stag, int order) {
+  const int kx = stag_kind(stag, 0, gi & 1), ky = stag_kind(stag, 1, gj & 1), kz = stag_kind(stag, 2, gk & 1);
+  const int nx = stag_n(order, kx), ny = stag_n(order, ky), nz = stag_n(order, kz);
+  const double *p = src + ((gi >> 1) + stag_lo(order, kx)) + J * ((gj >> 1) + stag_lo(order, ky)) +
+                    K * ((gk >> 1) + stag_lo(order, kz));
+  double r = 0.0;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    if (a >= nx)
+      break;
+    const double cx = stag_coef(order, kx, a);
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      if (b >= ny)
+        break;
+      const double cxy = cx * stag_coef(order, ky, b);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        if (c >= nz)
+          break;
+        const double t = (cxy * stag_coef(order, kz, c)) * p[a + J * b + K * c];
+        r = r + t;
+      }
+    }
+  }
+  return r;
+}
+
+// one staggered restriction value; p is the fine point 2i,2j,2k (the
+// coarse point's coincident child): staggered directions take fine points
+// 2i-1..2i+2 with {-1, 9, 9, -1}/16, unstaggered ones 2i-1..2i+1 with
+// {1, 6, 1}/8 (coeffs_restrict_3d_stagger.hh, ORDER_STAG 3)
+__device__ __forceinline__ double stagger_restrict_value(const double *p, long long J, long long K, int stag) {
+  auto coef = [](int s, int t) -> double {
+    if (s)
+      return t == 0 || t == 3 ? -1.0 / 16.0 : 9.0 / 16.0;
+    return t == 1 ? 6.0 / 8.0 : 1.0 / 8.0;
+  };
+  const int sx = stag & 1, sy = (stag >> 1) & 1, sz = (stag >> 2) & 1;
+  const double *q = p - 1 - J - K;
+  double r = 0.0;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    if (a >= 3 + sx)
+      break;
+    const double cx = coef(sx, a);
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      if (b >= 3 + sy)
+        break;
+      const double cxy = cx * coef(sy, b);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        if (c >= 3 + sz)
+          break;
+        const double t = (cxy * coef(sz, c)) * q[a + J * b + K * c];
+        r = r + t;
+      }
+    }
+  }
+  return r;
+}
+
 // value of destination point (i,j,k) of a pointwise operator (loads only,
 // so that a caller can issue the loads of several points before storing)
 template <int OP>
@@ -532,6 +636,24 @@ __device__ __forceinline__ double point_value(const PointArgs &a, int i, int j, 
         out = time_combine<3>(v, a.s, a.s2, a.l0, 0);
       }
     }
+  } else if constexpr (OP == PO_STAGGER) {
+    // each time level prolongated, then combined in time (data.cc:497-521)
+    const int gi = a.off[0] + i, gj = a.off[1] + j, gk = a.off[2] + k;
+    double v[3];
+    v[0] = stagger_value(a.src[0], a.sj, a.sk, gi, gj, gk, a.stag, a.sorder);
+    if (a.ntl == 1) {
+      out = v[0];
+    } else {
+      v[1] = stagger_value(a.src[1], a.sj, a.sk, gi, gj, gk, a.stag, a.sorder);
+      if (a.ntl == 2) {
+        out = time_combine<2>(v, a.s, a.s2, a.l0, 0);
+      } else {
+        v[2] = stagger_value(a.src[2], a.sj, a.sk, gi, gj, gk, a.stag, a.sorder);
+        out = time_combine<3>(v, a.s, a.s2, a.l0, 0);
+      }
+    }
+  } else if constexpr (OP == PO_RESTRICT_STAGGER) {
+    out = stagger_restrict_value(a.src[0] + 2 * i + a.sj * (2 * j) + a.sk * (2 * k), a.sj, a.sk, a.stag);
   } else if constexpr (OP == PO_RESTRICT_CC_O3) {
     // restrict_3d_cc_o3_rf2.cc:146-211: 64 terms, (w_a w_b w_c)/4096 with
     // w = {-1, 9, 9, -1}, a fastest, then b, then c; fine cells 2i-1..2i+2

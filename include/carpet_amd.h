@@ -76,8 +76,17 @@ enum {
   CG_OP_ENO = 6,
   CG_OP_WENO = 7,
   CG_OP_TVD = 8,
-  CG_OP_LAGRANGE_MONOTONE = 9
+  CG_OP_LAGRANGE_MONOTONE = 9,
+  CG_OP_STAGGER011 = 10,
+  CG_OP_STAGGER101 = 11,
+  CG_OP_STAGGER110 = 12,
+  CG_OP_STAGGER111 = 13
 };
+
+/* directions of the staggered operators: bit d set = the field is
+   staggered (sits half a coarse spacing off the vertices) in direction d;
+   CG_STAGGERxyz matches CarpetLib's op_STAGGERxyz */
+enum { CG_STAGGER011 = 6, CG_STAGGER101 = 5, CG_STAGGER110 = 3, CG_STAGGER111 = 7 };
 
 /* centering: same numbering as CarpetLib's enum centering
    (CarpetLib/src/defs.hh:134) */
@@ -220,6 +229,39 @@ int cg_prolongate_3d_o5_monotone_rf2_f64(const double *src, const int srcpadext[
                                          const int srcbbox[9], const int dstbbox[9],
                                          const int srcregbbox[9], const int dstregbbox[9],
                                          cg_stream_t stream);
+
+/* prolongate_3d_stagger011/101/110/111<double,ORDER>, ORDER in {2,3}:
+   CarpetLib/src/prolongate_3d_stagger011.cc:24-561 (the other three
+   alike), the operators of op STAGGER011..111 (data.cc:834-920, which maps
+   order_space 2 to ORDER 2 and 3..5 to ORDER 3).  stagger: CG_STAGGER*.
+   Per direction the tap set depends on staggering and fine parity
+   (coeffs_prolongate_3d_stagger.hh); the sum of coeff_x*coeff_y*coeff_z*src
+   runs x taps outermost, each operation rounded as written.  Needs
+   cg_set_staggered_operators(1) (CarpetLib::support_staggered_operators),
+   else CG_ERR_OPERATOR; the tap footprint must lie in the source array. */
+int cg_prolongate_3d_stagger_f64(int stagger, int order, const double *src, const int srcpadext[3],
+                                 const int srcext[3], double *dst, const int dstpadext[3],
+                                 const int dstext[3], const int srcbbox[9], const int dstbbox[9],
+                                 const int srcregbbox[9], const int dstregbbox[9], cg_stream_t stream);
+
+/* restrict_3d_stagger011/101/110/111<double>:
+   CarpetLib/src/restrict_3d_stagger011.cc:26-153 (the other three alike),
+   the restriction of op STAGGER011..111 (data.cc:989-1027): per staggered
+   direction the cubic through fine points 2i-1..2i+2 at 2i+1/2
+   ({-1,9,9,-1}/16), per unstaggered direction {1,6,1}/8 over 2i-1..2i+1
+   (coeffs_restrict_3d_stagger.hh, ORDER_STAG 3).  Needs
+   cg_set_staggered_operators(1); the footprint must lie in the source
+   array (the reference checks one fine point less on the upper side of
+   staggered directions and would read past it). */
+int cg_restrict_3d_stagger_f64(int stagger, const double *src, const int srcpadext[3],
+                               const int srcext[3], double *dst, const int dstpadext[3],
+                               const int dstext[3], const int srcbbox[9], const int dstbbox[9],
+                               const int srcregbbox[9], const int dstregbbox[9], cg_stream_t stream);
+
+/* CarpetLib's support_staggered_operators parameter (param.ccl:65,
+   default no): the staggered operators return CG_ERR_OPERATOR unless it is
+   set.  Process-wide. */
+int cg_set_staggered_operators(int support_staggered_operators);
 
 /* prolongate_3d_cc_eno_rf2<double,ORDER>, ORDER in {2,3}:
    CarpetLib/src/prolongate_3d_cc_eno_rf2.cc:943-1363 */

@@ -27,6 +27,9 @@
 //   CarpetLib/src/prolongate_3d_cc_real8_tvd.F90:55-244
 //   CarpetLib/src/restrict_3d_rf2.cc:25-121
 //   CarpetLib/src/restrict_3d_cc_rf2.cc:23-152
+//   CarpetLib/src/prolongate_3d_stagger{011,101,110,111}.cc:24-561,
+//     coeffs_prolongate_3d_stagger.hh; restrict_3d_stagger{...}.cc:26-153,
+//     coeffs_restrict_3d_stagger.hh
 //   CarpetLib/src/interpolate_3d_2tl.cc:24-117, interpolate_3d_3tl.cc:24-126
 //   CarpetLib/src/data.cc:34-105 (call_operator), 485-534 (transfer_time)
 //   CarpetLib/src/gdata.cc:288-349 (find_source_timelevel)
@@ -1416,9 +1419,199 @@ int orc_find_source_timelevel(const double *times, int ntimes, double time,
 
 // data<T>::transfer_time + transfer_prolongate for the operators in scope,
 // in the reference's shape: each needed time level is prolongated into a
+
+// ---- staggered operators --------------------------------------------------
+// prolongate_3d_stagger{011,101,110,111}.cc (the four differ only in which
+// directions are staggered) with coeffs_prolongate_3d_stagger.hh, and
+// restrict_3d_stagger{011,101,110,111}.cc with coeffs_restrict_3d_stagger.hh.
+// stag: bit d set = staggered in direction d (011 -> 6, 101 -> 5, 110 -> 3,
+// 111 -> 7).  The reference's loops are followed as written: per fine point
+// dst = 0, then dst += coeff * coeff * coeff * src with the x taps outermost,
+// and for ORDER 2 at a point coincident with a coarse vertex in the
+// unstaggered direction the two-dimensional sum over the other two.
+
+static int g_stagger = 7; // directions used by orc_prolongate_time's kind 6
+
+void orc_set_stagger(int stag) { g_stagger = stag; }
+
+// coeff[1..4][.] of coeffs_prolongate_3d_stagger.hh for ORDER 2 and 3
+// (index 0 of the second subscript is tap 1 of the reference)
+static const double stag_c1_o2[3] = {5.0 / 32.0, 15.0 / 16.0, -3.0 / 32.0};
+static const double stag_c2_o2[3] = {-3.0 / 32.0, 15.0 / 16.0, 5.0 / 32.0};
+static const double stag_c3_o2[3] = {-1.0 / 8.0, 3.0 / 4.0, 3.0 / 8.0};
+static const double stag_c1_o3[4] = {-5.0 / 128.0, 35.0 / 128.0, 105.0 / 128.0, -7.0 / 128.0};
+static const double stag_c2_o3[4] = {-7.0 / 128.0, 105.0 / 128.0, 35.0 / 128.0, -5.0 / 128.0};
+static const double stag_c3_o3[4] = {-5.0 / 64.0, 37.0 / 64.0, 37.0 / 64.0, -5.0 / 64.0};
+static const double stag_c4_o3[3] = {-1.0 / 32.0, 34.0 / 32.0, -1.0 / 32.0};
+
+double orc_stagger_coeff(int order, int which, int tap) {
+  if (order == 2) {
+    const double *c = which == 1 ? stag_c1_o2 : which == 2 ? stag_c2_o2 : which == 3 ? stag_c3_o2 : nullptr;
+    return c && tap >= 0 && tap < 3 ? c[tap] : 0.0;
+  }
+  if (order == 3) {
+    if (which == 4)
+      return tap >= 0 && tap < 3 ? stag_c4_o3[tap] : 0.0;
+    const double *c = which == 1 ? stag_c1_o3 : which == 2 ? stag_c2_o3 : which == 3 ? stag_c3_o3 : nullptr;
+    return c && tap >= 0 && tap < 4 ? c[tap] : 0.0;
+  }
+  return 0.0;
+}
+
+int orc_prolongate_3d_stagger(int stag, int order, const double *src, const int *spad, const int *sext,
+                              double *dst, const int *dpad, const int *dext, const int *sb, const int *db,
+                              const int *srb, const int *drb) {
+  (void)srb;
+  if (stag < 1 || stag > 7)
+    return CG_ERR_ARG;
+  if (order != 2 && order != 3)
+    return CG_ERR_ORDER;
+  const box3 srcbbox = mkbox(sb), dstbbox = mkbox(db), reg = mkbox(drb);
+  int e;
+  if ((e = check_array(spad, sext, srcbbox)) || (e = check_array(dpad, dext, dstbbox)))
+    return e;
+  for (int d = 0; d < 3; ++d) {
+    if (reg.str[d] <= 0)
+      return CG_ERR_ARG;
+    if (srcbbox.str[d] <= reg.str[d] || dstbbox.str[d] != reg.str[d])
+      return CG_ERR_STRIDE;
+    if (srcbbox.str[d] != 2 * dstbbox.str[d])
+      return CG_ERR_REFFACT;
+    if (srcbbox.lo[d] % srcbbox.str[d] || dstbbox.lo[d] % dstbbox.str[d] || reg.lo[d] % reg.str[d])
+      return CG_ERR_ALIGN;
+  }
+  if (empty(reg))
+    return CG_ERR_EMPTY;
+  if (!contained(reg, dstbbox))
+    return CG_ERR_CONTAIN;
+  int off[3], doff[3], n[3];
+  for (int d = 0; d < 3; ++d) {
+    const int num = reg.lo[d] - srcbbox.lo[d];
+    if (num % reg.str[d] || (reg.lo[d] - dstbbox.lo[d]) % reg.str[d])
+      return CG_ERR_ALIGN;
+    off[d] = num / reg.str[d];
+    doff[d] = (reg.lo[d] - dstbbox.lo[d]) / reg.str[d];
+    n[d] = sizes(reg, d);
+  }
+  // per direction and fine index: first coarse tap, tap count, coefficients
+  // (the index expressions of the reference's eight parity cases)
+  struct Taps {
+    int first, count;
+    const double *c;
+  };
+  auto taps = [&](int d, int g) -> Taps {
+    const int b = g >> 1;
+    const bool st = (stag >> d) & 1, odd = g & 1;
+    if (order == 2) {
+      if (st)
+        return {b - 1, 3, odd ? stag_c2_o2 : stag_c1_o2};
+      return odd ? Taps{b - 1, 3, stag_c3_o2} : Taps{b, 1, nullptr};
+    }
+    if (st)
+      return odd ? Taps{b - 1, 4, stag_c2_o3} : Taps{b - 2, 4, stag_c1_o3};
+    return odd ? Taps{b - 1, 4, stag_c3_o3} : Taps{b - 1, 3, stag_c4_o3};
+  };
+  // every tap of every point must lie in the source array
+  for (int d = 0; d < 3; ++d)
+    for (int f = 0; f < n[d]; ++f) {
+      const Taps t = taps(d, off[d] + f);
+      if (off[d] < 0 || t.first < 0 || t.first + t.count > sext[d])
+        return CG_ERR_CONTAIN;
+    }
+  auto body = [&](const box3 &r) {
+    for (int k = r.lo[2]; k <= r.up[2]; ++k)
+      for (int j = r.lo[1]; j <= r.up[1]; ++j)
+        for (int i = r.lo[0]; i <= r.up[0]; ++i) {
+          const Taps tx = taps(0, off[0] + i), ty = taps(1, off[1] + j), tz = taps(2, off[2] + k);
+          double acc = 0.0;
+          if (!tx.c) { // ORDER 2, coincident in x (unstaggered): y outer, z inner
+            for (int b = 0; b < ty.count; ++b)
+              for (int c = 0; c < tz.count; ++c)
+                acc += ty.c[b] * tz.c[c] * src[idx3(tx.first, ty.first + b, tz.first + c, spad)];
+          } else if (!ty.c) { // coincident in y: x outer, z inner
+            for (int a = 0; a < tx.count; ++a)
+              for (int c = 0; c < tz.count; ++c)
+                acc += tx.c[a] * tz.c[c] * src[idx3(tx.first + a, ty.first, tz.first + c, spad)];
+          } else if (!tz.c) { // coincident in z: x outer, y inner
+            for (int a = 0; a < tx.count; ++a)
+              for (int b = 0; b < ty.count; ++b)
+                acc += tx.c[a] * ty.c[b] * src[idx3(tx.first + a, ty.first + b, tz.first, spad)];
+          } else {
+            for (int a = 0; a < tx.count; ++a)
+              for (int b = 0; b < ty.count; ++b)
+                for (int c = 0; c < tz.count; ++c)
+                  acc += tx.c[a] * ty.c[b] * tz.c[c] *
+                         src[idx3(tx.first + a, ty.first + b, tz.first + c, spad)];
+          }
+          dst[idx3(doff[0] + i, doff[1] + j, doff[2] + k, dpad)] = acc;
+        }
+  };
+  box3 all{{0, 0, 0}, {n[0] - 1, n[1] - 1, n[2] - 1}, {1, 1, 1}};
+  if (g_use_openmp)
+    split_longest(all, 0, body);
+  else
+    body(all);
+  return CG_OK;
+}
+
+int orc_restrict_3d_stagger(int stag, const double *src, const int *spad, const int *sext, double *dst,
+                            const int *dpad, const int *dext, const int *sb, const int *db, const int *srb,
+                            const int *drb) {
+  (void)srb;
+  if (stag < 1 || stag > 7)
+    return CG_ERR_ARG;
+  const box3 srcbbox = mkbox(sb), dstbbox = mkbox(db), reg = mkbox(drb);
+  int e;
+  if ((e = check_array(spad, sext, srcbbox)) || (e = check_array(dpad, dext, dstbbox)))
+    return e;
+  for (int d = 0; d < 3; ++d) {
+    if (reg.str[d] <= 0)
+      return CG_ERR_ARG;
+    if (srcbbox.str[d] >= reg.str[d] || dstbbox.str[d] != reg.str[d])
+      return CG_ERR_STRIDE;
+    if (2 * srcbbox.str[d] != dstbbox.str[d])
+      return CG_ERR_REFFACT;
+  }
+  if (empty(reg))
+    return CG_ERR_EMPTY;
+  if (!contained(reg, dstbbox))
+    return CG_ERR_CONTAIN;
+  // coeff / coeff_i of coeffs_restrict_3d_stagger.hh (ORDER_STAG 3)
+  static const double cs[4] = {-1.0 / 16.0, 9.0 / 16.0, 9.0 / 16.0, -1.0 / 16.0};
+  static const double cv[3] = {1.0 / 8.0, 6.0 / 8.0, 1.0 / 8.0};
+  int so[3], doff[3], n[3], cnt[3];
+  const double *cf[3];
+  for (int d = 0; d < 3; ++d) {
+    if ((reg.lo[d] - srcbbox.lo[d]) % srcbbox.str[d] || (reg.lo[d] - dstbbox.lo[d]) % dstbbox.str[d])
+      return CG_ERR_ALIGN;
+    so[d] = (reg.lo[d] - srcbbox.lo[d]) / srcbbox.str[d];
+    doff[d] = (reg.lo[d] - dstbbox.lo[d]) / dstbbox.str[d];
+    n[d] = sizes(reg, d);
+    const bool st = (stag >> d) & 1;
+    cnt[d] = st ? 4 : 3;
+    cf[d] = st ? cs : cv;
+    if (so[d] - 1 < 0 || so[d] + 2 * (n[d] - 1) - 1 + cnt[d] > sext[d])
+      return CG_ERR_CONTAIN;
+  }
+  for (int k = 0; k < n[2]; ++k)
+    for (int j = 0; j < n[1]; ++j)
+      for (int i = 0; i < n[0]; ++i) {
+        double acc = 0.0;
+        for (int a = 0; a < cnt[0]; ++a)
+          for (int b = 0; b < cnt[1]; ++b)
+            for (int c = 0; c < cnt[2]; ++c)
+              acc += cf[0][a] * cf[1][b] * cf[2][c] *
+                     src[idx3(so[0] + 2 * i - 1 + a, so[1] + 2 * j - 1 + b, so[2] + 2 * k - 1 + c, spad)];
+        dst[idx3(doff[0] + i, doff[1] + j, doff[2] + k, dpad)] = acc;
+      }
+  return CG_OK;
+}
+
 // region-sized temporary (data.cc:504-515), then the temporaries are
 // interpolated in time (data.cc:517, 1182-1299).  kind: 0 vertex Lagrange,
-// 1 cell Lagrange, 2 cell ENO (order = ENO order), 3 cell TVD.
+// 1 cell Lagrange, 2 cell ENO (order = ENO order), 3 cell TVD, 4 vertex ENO,
+// 5 monotone order 5, 6 staggered (order 2 or 3, directions of
+// orc_set_stagger).
 int orc_prolongate_time(int kind, int order, const double *const *srcs,
                         const double *times, int ntimes, const int *spad,
                         const int *sext, double *dst, double time,
@@ -1445,6 +1638,8 @@ int orc_prolongate_time(int kind, int order, const double *const *srcs,
       return orc_prolongate_3d_eno(src, spad, sext, out, opad, oext, sb, ob, drb, drb);
     case 5:
       return orc_prolongate_3d_o5_monotone_rf2(src, spad, sext, out, opad, oext, sb, ob, drb, drb);
+    case 6:
+      return orc_prolongate_3d_stagger(g_stagger, order, src, spad, sext, out, opad, oext, sb, ob, drb, drb);
     }
     return CG_ERR_OPERATOR;
   };

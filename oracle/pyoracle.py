@@ -55,6 +55,10 @@ _decl("orc_prolongate_3d_cc_tvd", C.c_int, *_OP)
 _decl("orc_prolongate_3d_eno", C.c_int, *_OP)
 _decl("orc_eno1d", C.c_double, _dp)
 _decl("orc_prolongate_3d_o5_monotone_rf2", C.c_int, *_OP)
+_decl("orc_prolongate_3d_stagger", C.c_int, C.c_int, C.c_int, *_OP)
+_decl("orc_restrict_3d_stagger", C.c_int, C.c_int, *_OP)
+_decl("orc_stagger_coeff", C.c_double, C.c_int, C.c_int, C.c_int)
+_decl("orc_set_stagger", None, C.c_int)
 _decl("orc_restrict_3d_rf2", C.c_int, *_OP)
 _decl("orc_restrict_3d_cc_rf2", C.c_int, *_OP)
 _decl("orc_restrict_3d_cc_ho_rf2", C.c_int, C.c_int, *_OP)
@@ -68,7 +72,9 @@ _decl("orc_find_source_timelevel", C.c_int, _dp, C.c_int, C.c_double, C.c_int, C
 _decl("orc_prolongate_time", C.c_int, C.c_int, C.c_int, C.POINTER(_dp), _dp, C.c_int, _ip, _ip, _dp,
       C.c_double, _ip, _ip, _ip, _ip, _ip, C.c_int)
 
-KIND_VLAG, KIND_CLAG, KIND_ENO, KIND_TVD, KIND_VENO, KIND_MONO5 = 0, 1, 2, 3, 4, 5
+KIND_VLAG, KIND_CLAG, KIND_ENO, KIND_TVD, KIND_VENO, KIND_MONO5, KIND_STAGGER = 0, 1, 2, 3, 4, 5, 6
+# staggered directions, bit d = direction d (CarpetLib's op_STAGGERxyz)
+STAGGER011, STAGGER101, STAGGER110, STAGGER111 = 6, 5, 3, 7
 
 
 @dataclass
@@ -148,6 +154,26 @@ def prolongate_3d_eno(src, dst, dstreg) -> int:
 def prolongate_3d_o5_monotone_rf2(src, dst, dstreg) -> int:
     """prolongate_3d_o5_monotone_rf2 (vertex-centred, order 5, limited)"""
     return _lib.orc_prolongate_3d_o5_monotone_rf2(*_args(src, dst, dstreg, dstreg))
+
+
+def prolongate_3d_stagger(stag, order, src, dst, dstreg) -> int:
+    """prolongate_3d_stagger{011,101,110,111}<ORDER> (stag: bit d = staggered in d)"""
+    return _lib.orc_prolongate_3d_stagger(stag, order, *_args(src, dst, dstreg, dstreg))
+
+
+def restrict_3d_stagger(stag, src, dst, dstreg) -> int:
+    """restrict_3d_stagger{011,101,110,111}"""
+    return _lib.orc_restrict_3d_stagger(stag, *_args(src, dst, dstreg, dstreg))
+
+
+def stagger_coeff(order, which, tap) -> float:
+    """coeff[which][tap + 1] of coeffs_prolongate_3d_stagger.hh"""
+    return _lib.orc_stagger_coeff(order, which, tap)
+
+
+def set_stagger(stag):
+    """directions used by prolongate_time's KIND_STAGGER"""
+    _lib.orc_set_stagger(stag)
 
 
 def eno1d(q) -> float:

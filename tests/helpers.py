@@ -32,6 +32,8 @@ def windows(kind: str, order: int):
         return [(0, 0), (-1, 2)]
     if kind == "mono":  # monotone order 5: b, or b-2..b+3
         return [(0, 0), (-2, 3)]
+    if kind == "stag":  # staggered operators: the union over both direction kinds
+        return [(-1, 1), (-1, 1)] if order == 2 else [(-2, 1), (-1, 2)]
     raise ValueError(kind)
 
 

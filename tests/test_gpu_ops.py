@@ -417,3 +417,145 @@ def test_monotone_dispatch():
             cg.transfer_from(ddst, [dsrc], [0.0], cg.Box(*rb), cg.Box(*rb), 0.0, cg.OP_LAGRANGE_MONOTONE, cent,
                              order, 0)
     _check(orc.HostArr(db, dst.data.copy()), ddst, "refused transfers leave the destination untouched")
+
+
+# ---- staggered operators ---------------------------------------------------
+STAGGERS = [(cg.STAGGER011, cg.OP_STAGGER011), (cg.STAGGER101, cg.OP_STAGGER101),
+            (cg.STAGGER110, cg.OP_STAGGER110), (cg.STAGGER111, cg.OP_STAGGER111)]
+
+
+@pytest.fixture
+def staggered_on():
+    cg.set_staggered_operators(True)
+    yield
+    cg.set_staggered_operators(False)
+
+
+@pytest.mark.parametrize("order", [2, 3])
+@pytest.mark.parametrize("stag", [s for s, _ in STAGGERS], ids=["011", "101", "110", "111"])
+def test_stagger_prolong_parity(stag, order, staggered_on):
+    """prolongate_3d_stagger{011,101,110,111}<ORDER> against the oracle, bit
+    for bit, on regions of both parities (single points, thin slabs, blocks)."""
+    rng = np.random.default_rng(900 + 10 * stag + order)
+    for c, n in enumerate(((1, 1, 1), (2, 1, 3), (31, 3, 17), None, None)):
+        print(f"stagger {stag} o{order} #{c}", flush=True)
+        sb, db, rb = prolong_case(rng, "stag", order, n=n, scale=1 + (c % 2))
+        src = _src_host(rng, sb, ("mixed", "smooth", "random")[c % 3])
+        dst = host_array(db, pad_of(sizes(db), rng))
+        exp = orc.HostArr(db, dst.data.copy())
+        assert orc.prolongate_3d_stagger(stag, order, src, exp, rb) == 0
+        dsrc, ddst = _dev(src), _dev(dst)
+        cg.prolongate_3d_stagger(stag, order, dsrc, ddst, cg.Box(*rb))
+        _check(exp, ddst, f"stagger {stag} o{order} case {c} reg={rb}")
+
+
+def _stagger_restrict_case(rng, n, sf=1):
+    """Vertex geometry; the source holds one fine point below the region's
+    children and two above (the staggered directions' footprint), and 0..2
+    more."""
+    sc = 2 * sf
+    rlo, rup, slo, sup, dlo, dup = [], [], [], [], [], []
+    for d in range(3):
+        C0 = int(rng.integers(-4, 5))
+        rlo.append(C0 * sc)
+        rup.append((C0 + n[d] - 1) * sc)
+        slo.append(rlo[-1] - (1 + int(rng.integers(0, 3))) * sf)
+        sup.append(rup[-1] + (2 + int(rng.integers(0, 3))) * sf)
+        dlo.append(rlo[-1] - int(rng.integers(0, 2)) * sc)
+        dup.append(rup[-1] + int(rng.integers(0, 2)) * sc)
+    return ((tuple(slo), tuple(sup), (sf,) * 3), (tuple(dlo), tuple(dup), (sc,) * 3),
+            (tuple(rlo), tuple(rup), (sc,) * 3))
+
+
+@pytest.mark.parametrize("stag", [s for s, _ in STAGGERS], ids=["011", "101", "110", "111"])
+def test_stagger_restrict_parity(stag, staggered_on):
+    """restrict_3d_stagger{011,101,110,111} against the oracle, bit for bit."""
+    rng = np.random.default_rng(950 + stag)
+    for c in range(5):
+        n = (1, 1, 1) if c == 0 else tuple(int(v) for v in rng.integers(1, 30, size=3))
+        print(f"stagger restrict {stag} #{c} n={n}", flush=True)
+        sb, db, rb = _stagger_restrict_case(rng, n, sf=1 + c % 2)
+        src = _src_host(rng, sb, "random")
+        dst = host_array(db, pad_of(sizes(db), rng))
+        exp = orc.HostArr(db, dst.data.copy())
+        assert orc.restrict_3d_stagger(stag, src, exp, rb) == 0
+        dsrc, ddst = _dev(src), _dev(dst)
+        cg.restrict_3d_stagger(stag, dsrc, ddst, cg.Box(*rb))
+        _check(exp, ddst, f"stagger restrict {stag} case {c}")
+
+
+def test_stagger_requires_parameter():
+    """Without support_staggered_operators the staggered operators refuse
+    to run (prolongate_3d_stagger011.cc:60-63), per call and through the
+    dispatch, and leave the destination untouched."""
+    cg.set_staggered_operators(False)
+    rng = np.random.default_rng(4711)
+    sb, db, rb = prolong_case(rng, "stag", 3, n=(7, 5, 6))
+    src = _src_host(rng, sb, "random")
+    dst = host_array(db, pad_of(sizes(db), rng))
+    dsrc, ddst = _dev(src), _dev(dst)
+    with pytest.raises(cg.CarpetError) as ei:
+        cg.prolongate_3d_stagger(cg.STAGGER011, 3, dsrc, ddst, cg.Box(*rb))
+    assert ei.value.code == -11  # CG_ERR_OPERATOR
+    with pytest.raises(cg.CarpetError):
+        cg.transfer_from(ddst, [dsrc], [0.0], cg.Box(*rb), cg.Box(*rb), 0.0, cg.OP_STAGGER111,
+                         cg.VERTEX_CENTERED, 3, 0)
+    rsb, rdb, rrb = _stagger_restrict_case(rng, (4, 3, 5))
+    rsrc = _src_host(rng, rsb, "random")
+    rdst = host_array(rdb, pad_of(sizes(rdb), rng))
+    with pytest.raises(cg.CarpetError):
+        cg.restrict_3d_stagger(cg.STAGGER101, _dev(rsrc), _dev(rdst), cg.Box(*rrb))
+    _check(orc.HostArr(db, dst.data.copy()), ddst, "refused transfers leave the destination untouched")
+
+
+@pytest.mark.parametrize("stag,op", STAGGERS, ids=["011", "101", "110", "111"])
+def test_stagger_dispatch(stag, op, staggered_on):
+    """op STAGGERxyz through cg_transfer_from: prolongation with order_space
+    2 -> ORDER 2 and 3..5 -> ORDER 3 (data.cc:834-920), with 1-3 time levels
+    (each prolongated, then Lagrange in time) and through a batch; other
+    orders refused; restriction whatever the centring (data.cc:989-1027)."""
+    rng = np.random.default_rng(990 + stag)
+    orc.set_stagger(stag)
+    for c, (order_space, ntl) in enumerate(((2, 1), (3, 3), (5, 2), (4, 1))):
+        order = 2 if order_space == 2 else 3
+        print(f"stagger dispatch {stag} order_space {order_space} ntl {ntl}", flush=True)
+        sb, db, rb = prolong_case(rng, "stag", order)
+        pad = pad_of(sizes(sb), rng)
+        srcs = []
+        for _ in range(ntl):
+            h = host_array(sb, pad)
+            h.view()[...] = field(rng, sizes(sb), "mixed")
+            srcs.append(h)
+        times = [0.5, 0.25, 0.0][:ntl]
+        t = 0.3 if ntl > 1 else 0.5
+        dst = host_array(db, pad_of(sizes(db), rng))
+        exp = orc.HostArr(db, dst.data.copy())
+        assert orc.prolongate_time(orc.KIND_STAGGER, order, srcs, times, exp, t, rb, ntl - 1) == 0
+        ds = [_dev(h) for h in srcs]
+        ddst = _dev(dst)
+        cg.transfer_from(ddst, ds, times, cg.Box(*rb), cg.Box(*rb), t, op, cg.VERTEX_CENTERED, order_space, ntl - 1)
+        _check(exp, ddst, f"stagger dispatch {stag} case {c}")
+        if c == 1:
+            bdst = _dev(dst)
+            b = cg.Batch()
+            b.add(0, bdst, ds, times, cg.Box(*rb), cg.Box(*rb), t, op, cg.VERTEX_CENTERED, order_space, ntl - 1)
+            b.finalize()
+            b.run()
+            cg.synchronize()
+            _check(exp, bdst, f"batched stagger {stag}")
+            b.close()
+    sb, db, rb = prolong_case(rng, "stag", 3, n=(6, 5, 4))
+    dsrc, ddst = _dev(_src_host(rng, sb, "random")), _dev(host_array(db, pad_of(sizes(db), rng)))
+    for order_space in (1, 6):
+        with pytest.raises(cg.CarpetError):
+            cg.transfer_from(ddst, [dsrc], [0.0], cg.Box(*rb), cg.Box(*rb), 0.0, op, cg.VERTEX_CENTERED,
+                             order_space, 0)
+    for cent in (cg.VERTEX_CENTERED, cg.CELL_CENTERED):
+        rsb, rdb, rrb = _stagger_restrict_case(rng, (9, 4, 7))
+        rsrc = _src_host(rng, rsb, "random")
+        rdst = host_array(rdb, pad_of(sizes(rdb), rng))
+        rexp = orc.HostArr(rdb, rdst.data.copy())
+        assert orc.restrict_3d_stagger(stag, rsrc, rexp, rrb) == 0
+        drdst = _dev(rdst)
+        cg.transfer_from(drdst, [_dev(rsrc)], [0.0], cg.Box(*rrb), cg.Box(*rrb), 0.0, op, cent, 3, 0)
+        _check(rexp, drdst, f"stagger restriction dispatch {stag} cent {cent}")

@@ -768,3 +768,169 @@ def test_monotone_limits_to_nearest_range():
             elif corners.min() + 1e-12 < hi < corners.max() - 1e-12:
                 assert abs(v - hi) < 1e-12
         assert limited > 0
+
+
+# ---- staggered operators ---------------------------------------------------
+from fractions import Fraction as Fr  # noqa: E402
+
+
+def _lagrange_fr(nodes, x):
+    w = []
+    for i, xi in enumerate(nodes):
+        v = Fr(1)
+        for j, xj in enumerate(nodes):
+            if j != i:
+                v *= Fr(x - xj) / Fr(xi - xj)
+        w.append(v)
+    return w
+
+
+def _avg_monomial(k, a, b):
+    """average of x^k over [a, b], exact"""
+    return (Fr(b) ** (k + 1) - Fr(a) ** (k + 1)) / ((k + 1) * (Fr(b) - Fr(a)))
+
+
+def _cell_average_weights(nodes, a, b):
+    """weights w with sum_m w_m avg_[m-1/2, m+1/2] p = avg_[a, b] p for every
+    polynomial p of degree < len(nodes) (solved exactly)"""
+    n = len(nodes)
+    A = [[_avg_monomial(k, Fr(m) - Fr(1, 2), Fr(m) + Fr(1, 2)) for m in nodes] + [_avg_monomial(k, a, b)]
+         for k in range(n)]
+    for c in range(n):
+        p = next(r for r in range(c, n) if A[r][c] != 0)
+        A[c], A[p] = A[p], A[c]
+        for r in range(n):
+            if r != c and A[r][c] != 0:
+                f = A[r][c] / A[c][c]
+                A[r] = [x - f * y for x, y in zip(A[r], A[c])]
+    return [A[i][n] / A[i][i] for i in range(n)]
+
+
+Q = Fr(1, 4)
+H = Fr(1, 2)
+# coeffs_prolongate_3d_stagger.hh: what each table is, from the mathematics
+STAGGER_TABLES = {
+    (2, 1): _lagrange_fr((-1, 0, 1), -Q),  # staggered, to b - 1/4
+    (2, 2): _lagrange_fr((-1, 0, 1), Q),  # staggered, to b + 1/4
+    (2, 3): _lagrange_fr((-1, 0, 1), H),  # unstaggered, to b + 1/2
+    (3, 1): _lagrange_fr((-2, -1, 0, 1), -Q),
+    (3, 2): _lagrange_fr((-1, 0, 1, 2), Q),
+    # ORDER 3 unstaggered: cell averages in, fine-cell averages out
+    (3, 3): _cell_average_weights((-1, 0, 1, 2), Q, 3 * Q),
+    (3, 4): _cell_average_weights((-1, 0, 1), -Q, Q),
+}
+
+
+@pytest.mark.parametrize("key", sorted(STAGGER_TABLES))
+def test_stagger_coefficients(key):
+    """The oracle's staggered tables equal Lagrange weights at b -+ 1/4 (and
+    b + 1/2 for ORDER 2), and for ORDER 3 in the unstaggered direction the
+    cell-average reconstruction over the fine cell (exact rationals)."""
+    order, which = key
+    want = STAGGER_TABLES[key]
+    got = [orc.stagger_coeff(order, which, t) for t in range(len(want))]
+    assert got == [float(w) for w in want], (key, got, want)
+    assert orc.stagger_coeff(order, which, len(want)) == 0.0
+
+
+def _poly(rng, deg):
+    return np.polynomial.Polynomial(rng.uniform(-1, 1, deg + 1))
+
+
+def _avg(p, a, b):
+    P = p.integ()
+    return (P(b) - P(a)) / (b - a)
+
+
+@pytest.mark.parametrize("order", [2, 3])
+@pytest.mark.parametrize("stag", [6, 5, 3, 7], ids=["011", "101", "110", "111"])
+def test_stagger_prolong_reproduces_polynomials(stag, order):
+    """Separable data P(x)Q(y)R(z): staggered directions hold point values at
+    the coarse array indices, unstaggered ones point values (ORDER 2) or
+    cell averages (ORDER 3).  Every fine point must equal the product of the
+    per-direction targets: b -+ 1/4 (staggered), b or b + 1/2 (ORDER 2), the
+    fine-cell average over [b-1/4, b+1/4] or [b+1/4, b+3/4] (ORDER 3).
+    Pins tap positions, tables, parities and the direction bits."""
+    rng = np.random.default_rng(60 + stag + order)
+    for c in range(3):
+        sb, db, rb = prolong_case(rng, "stag", order, n=(1, 1, 1) if c == 0 else None, maxn=12)
+        ext = sizes(sb)
+        polys, coarse, targets = [], [], []
+        for d in range(3):
+            st = (stag >> d) & 1
+            p = _poly(rng, order if st else 2)
+            m = np.arange(ext[d], dtype=float)
+            coarse.append(p(m) if st or order == 2 else np.array([_avg(p, x - 0.5, x + 0.5) for x in m]))
+            polys.append((p, st))
+        src = host_array(sb, pad_of(ext, rng))
+        src.view()[...] = coarse[2][:, None, None] * coarse[1][None, :, None] * coarse[0][None, None, :]
+        dst = host_array(db, pad_of(sizes(db), rng))
+        assert orc.prolongate_3d_stagger(stag, order, src, dst, rb) == 0
+        n = sizes(rb)
+        for d in range(3):
+            p, st = polys[d]
+            off = (rb[0][d] - sb[0][d]) // rb[2][d]
+            vals = []
+            for f in range(n[d]):
+                g = off + f
+                b, odd = g >> 1, g & 1
+                if st:
+                    vals.append(p(b + (0.25 if odd else -0.25)))
+                elif order == 2:
+                    vals.append(p(b + (0.5 if odd else 0.0)))
+                else:
+                    vals.append(_avg(p, b + 0.25, b + 0.75) if odd else _avg(p, b - 0.25, b + 0.25))
+            targets.append(np.array(vals))
+        want = targets[2][:, None, None] * targets[1][None, :, None] * targets[0][None, None, :]
+        dlo = [(rb[0][d] - db[0][d]) // db[2][d] for d in range(3)]
+        got = dst.data[dlo[2]:dlo[2] + n[2], dlo[1]:dlo[1] + n[1], dlo[0]:dlo[0] + n[0]]
+        np.testing.assert_allclose(got, want, rtol=0, atol=1e-12 * max(1.0, np.abs(want).max()))
+
+
+@pytest.mark.parametrize("stag", [6, 5, 3, 7], ids=["011", "101", "110", "111"])
+def test_stagger_restrict_reproduces_polynomials(stag):
+    """Separable fine data: staggered directions hold point values at the fine
+    indices (the coarse value is the cubic at 2i + 1/2), unstaggered ones
+    fine-cell averages (the coarse value is the average over the coarse cell
+    [2i-1, 2i+1], exact for quadratics)."""
+    rng = np.random.default_rng(80 + stag)
+    for c in range(3):
+        n = (1, 1, 1) if c == 0 else tuple(int(v) for v in rng.integers(1, 8, size=3))
+        C0 = [int(v) for v in rng.integers(-3, 4, size=3)]
+        rb = (tuple(2 * c0 for c0 in C0), tuple(2 * (c0 + k - 1) for c0, k in zip(C0, n)), (2, 2, 2))
+        sb = (tuple(l - 1 - int(rng.integers(0, 2)) for l in rb[0]),
+              tuple(u + 2 + int(rng.integers(0, 2)) for u in rb[1]), (1, 1, 1))
+        db = (rb[0], rb[1], (2, 2, 2))
+        ext = sizes(sb)
+        fine, targets = [], []
+        for d in range(3):
+            st = (stag >> d) & 1
+            p = _poly(rng, 3 if st else 2)
+            x = np.arange(ext[d], dtype=float)
+            fine.append(p(x) if st else np.array([_avg(p, v - 0.5, v + 0.5) for v in x]))
+            so = rb[0][d] - sb[0][d]
+            X = so + 2 * np.arange(n[d])
+            targets.append(p(X + 0.5) if st else np.array([_avg(p, v - 1, v + 1) for v in X]))
+        src = host_array(sb, pad_of(ext, rng))
+        src.view()[...] = fine[2][:, None, None] * fine[1][None, :, None] * fine[0][None, None, :]
+        dst = host_array(db, pad_of(sizes(db), rng))
+        assert orc.restrict_3d_stagger(stag, src, dst, rb) == 0
+        want = targets[2][:, None, None] * targets[1][None, :, None] * targets[0][None, None, :]
+        got = dst.view()
+        np.testing.assert_allclose(got, want, rtol=0, atol=1e-12 * max(1.0, np.abs(want).max()))
+
+
+def test_stagger_footprint_checks():
+    """A region whose taps leave the source array is refused (the oracle
+    checks every tap; the reference's own check misses the last staggered
+    restriction tap)."""
+    sb = ((0, 0, 0), (8, 8, 8), (1, 1, 1))
+    src = host_array(sb, (9, 9, 9), fill=0.0)
+    # children 2..6; the staggered taps reach 7 (fits) and with children up
+    # to 8 they would reach 9 (outside)
+    ok = ((2, 2, 2), (6, 6, 6), (2, 2, 2))
+    bad = ((2, 2, 2), (8, 6, 6), (2, 2, 2))
+    dst = host_array(((0, 0, 0), (8, 8, 8), (2, 2, 2)), (5, 5, 5))
+    assert orc.restrict_3d_stagger(7, src, dst, ok) == 0
+    assert orc.restrict_3d_stagger(7, src, dst, bad) == -6  # CG_ERR_CONTAIN
+    assert orc.restrict_3d_stagger(6, src, dst, ((2, 2, 2), (6, 6, 6), (2, 2, 2))) == 0
