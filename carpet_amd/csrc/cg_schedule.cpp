@@ -499,11 +499,15 @@ int build(cg_schedule &S, const int bnd[2][3], int ho_restrict, int restrict_ord
         const Set tmp3 = tmp2.expand(one, one);
         allrestricted = Set::of(all_target) - tmp3;
       }
-      const int shrink = (ho_restrict && S.centering == CG_CELL_CENTERED) ? restrict_order / 2 : 0;
+      // with higher-order restriction or the staggered operators, restrict
+      // into interiors only, from sources shrunk by order/2 (cell centring)
+      // or by 2 (staggered) (dh.cc:1019-1050)
+      const bool stag = staggered_supported();
+      const int shrink = (ho_restrict && S.centering == CG_CELL_CENTERED) ? restrict_order / 2 : stag ? 2 : 0;
       const int sh[3] = {shrink, shrink, shrink}, msh[3] = {-shrink, -shrink, -shrink};
       for (size_t oc = 0; oc < O.comps.size(); ++oc) {
         const Comp &obox = O.comps[oc];
-        Set needrecv = ho_restrict ? allrestricted & obox.interior : allrestricted & obox.exterior;
+        Set needrecv = ho_restrict || stag ? allrestricted & obox.interior : allrestricted & obox.exterior;
         SCHED_CHECK((allrestricted & obox.buffers).empty(),
                     "Refinement restriction: cannot restrict into buffer zones (level %d, component %zu)",
                     rl - 1, oc);

@@ -509,7 +509,9 @@ enum { CG_SCHED_GHOSTS = 10, CG_SCHED_BUFFERS = 11, CG_SCHED_OVERLAPS = 12, CG_S
    boundary_width: h.boundary_width (lower x y z, upper x y z);
    use_higher_order_restriction / restriction_order_space: CarpetLib's
    parameters (restriction into interiors only, and for cell centring a
-   source shrunk by order/2, dh.cc:1019-1050) */
+   source shrunk by order/2, dh.cc:1019-1050).  The process-wide
+   support_staggered_operators (cg_set_staggered_operators) applies too, as
+   in the reference: restriction into interiors only, sources shrunk by 2. */
 int cg_schedule_build(int centering, const int boundary_width[6], int nlevels,
                       const cg_level_spec *levels, int use_higher_order_restriction,
                       int restriction_order_space, cg_schedule **out);

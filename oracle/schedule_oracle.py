@@ -212,7 +212,7 @@ def _add(lst, key, m):
         lst[key] = m.copy()
 
 
-def build(centering, boundary, levels, ho_restrict=False, restrict_order=1):
+def build(centering, boundary, levels, ho_restrict=False, restrict_order=1, staggered=False):
     """levels: list of Level (domain, interiors, reffact, widths, order).
     Returns the levels with regions and lists filled (masks)."""
     bnd = boundary
@@ -329,9 +329,11 @@ def build(centering, boundary, levels, ho_restrict=False, restrict_order=1):
                 tmp2 = all_target & ~tmp1
                 tmp3 = olat.expand(tmp2, (1, 1, 1), (1, 1, 1))
                 allrestricted = all_target & ~tmp3
-            shrink = restrict_order // 2 if (ho_restrict and centering == CELL) else 0
+            # dh.cc:1019-1050: support_staggered_operators also restricts into
+            # interiors only, from sources shrunk by 2
+            shrink = restrict_order // 2 if (ho_restrict and centering == CELL) else 2 if staggered else 0
             for oc, obox in enumerate(O.comps):
-                tgt = obox.interior if ho_restrict else obox.exterior
+                tgt = obox.interior if (ho_restrict or staggered) else obox.exterior
                 needrecv = allrestricted & olat.box(tgt)
                 _check(not (allrestricted & obox.buffers).any(),
                        "Refinement restriction: cannot restrict into buffer zones")

@@ -193,20 +193,23 @@ def _masks(lat, boxes):
     return m
 
 
-def _compare(levels, cent, bnd, ho, rorder):
+def _compare(levels, cent, bnd, ho, rorder, staggered=False):
     descs = [S.LevelDesc(L["domain"], L["comps"], L["reffact"], L["gw"], L["bw"], L["ow"], L["order"])
              for L in levels]
     olv = [so.Level(L["domain"], L["comps"], L["reffact"], L["gw"], L["bw"], L["ow"], L["order"]) for L in levels]
     try:
-        so.build(cent, bnd, olv, ho, rorder)
+        so.build(cent, bnd, olv, ho, rorder, staggered)
         oerr = None
     except so.ScheduleError as e:
         oerr = e
+    cg.set_staggered_operators(staggered)
     try:
         sch = S.Schedule(cent, bnd, descs, ho, rorder)
         perr = None
     except cg.CarpetError as e:
         perr = e
+    finally:
+        cg.set_staggered_operators(False)
     assert (oerr is None) == (perr is None), f"oracle: {oerr}; library: {perr}"
     if oerr is not None:
         return False
@@ -273,6 +276,23 @@ def test_schedule_matches_oracle(cent, seed):
     print(f"{'vertex' if cent == V else 'cell'} seed {seed}: {nlev} levels, "
           f"{[len(L['comps']) for L in levels]} components, boundary {bnd}, ho {ho}", flush=True)
     _compare(levels, cent, bnd, ho, rorder)
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_schedule_staggered_matches_oracle(seed):
+    """support_staggered_operators: restriction into interiors only, from
+    sources shrunk by 2 (dh.cc:1019-1050), library against the oracle."""
+    rng = np.random.default_rng(7000 + seed)
+    nlev = 3 if seed % 4 == 3 else 2
+    levels = None
+    for _ in range(500):
+        levels = _random_hierarchy(rng, V, nlev)
+        if levels is not None:
+            break
+    assert levels is not None, "no valid random hierarchy drawn"
+    bnd = (Z3, Z3) if seed % 2 == 0 else (tuple(int(v) for v in rng.integers(0, 3, 3)), Z3)
+    print(f"staggered seed {seed}: {nlev} levels, {[len(L['comps']) for L in levels]} components", flush=True)
+    _compare(levels, V, bnd, False, 1, staggered=True)
 
 
 def test_schedule_rejects_what_dh_asserts():
