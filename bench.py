@@ -408,6 +408,8 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-config4", action="store_true",
                     help="skip the 2000-region level step reported beside the headline line")
+    ap.add_argument("--no-config6", action="store_true",
+                    help="skip the hierarchy level step reported beside the headline line")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--config", type=int, default=2, choices=(2, 3, 4, 5, 6),
                     help="2: single large patch (default, the headline line); 3: cell-centred conservative "
@@ -542,6 +544,11 @@ def main(argv=None):
         # the 2000-region level step (batched vs plain) beside the headline
         print("# config4 beside the headline line", file=sys.stderr, flush=True)
         line["level_step_2000_regions"] = run_config4(2000, args.n_gf, 5, 2, 1)
+    if not args.no_config6:
+        # the level step of a nested 3-level hierarchy from its communication
+        # schedule (batched vs one launch per region)
+        print("# config6 beside the headline line", file=sys.stderr, flush=True)
+        line["hierarchy_step_3_levels"] = run_config6(args.n_gf, 10, 2, 1)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     print(json.dumps(line))
