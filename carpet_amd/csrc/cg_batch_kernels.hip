@@ -407,6 +407,12 @@ int batch_device_build(BatchDevice &dev, const std::vector<Transfer> &tr,
     }
     l.zchunk = tile_cfg < 0 ? ZCHUNK_DEFAULT : tile_cfg;
     l.pt = PT_DEFAULT;
+    // pointwise classes of small regions (median under 2048 points, such as
+    // the ghost and restriction slabs of a hierarchy's components): items of
+    // 256 points, one per thread, so a region spreads over several
+    // workgroups (the tuner's choice on such classes)
+    if (!g.is_tensor && median_pts < 2048)
+      l.pt = 256;
     // tensor tiles of large regions (median 2^20 points and up): persistent
     // grid (its workgroups take the tiles in order, so neighbouring tiles
     // march through z together and share halo rows in L2); other tensor
