@@ -158,3 +158,18 @@ def test_transfer_dispatch_validation():
     assert call(cg.OP_SYNC, cg.VERTEX_CENTERED, 3) == -11  # op_sync cannot prolongate
     assert call(cg.OP_NONE, cg.VERTEX_CENTERED, 3) == -11
     assert call(cg.OP_LAGRANGE, cg.VERTEX_CENTERED, 3, time=5.0) == -9  # extrapolation in time
+    # staggered operators: order_space 2..5 only, and only with
+    # support_staggered_operators (prolongate_3d_stagger011.cc:60-63)
+    cg.set_staggered_operators(False)
+    assert call(cg.OP_STAGGER011, cg.VERTEX_CENTERED, 3) == -11
+    cg.set_staggered_operators(True)
+    try:
+        assert call(cg.OP_STAGGER101, cg.VERTEX_CENTERED, 1) == -7
+        assert call(cg.OP_STAGGER110, cg.VERTEX_CENTERED, 6) == -7
+    finally:
+        cg.set_staggered_operators(False)
+    # restriction of a staggered group (fine -> coarse) without the parameter
+    csrc = (C.POINTER(GData) * 1)(C.pointer(f))
+    creg = cg.Box((8, 8, 8), (12, 12, 12), (2, 2, 2)).as_c()
+    assert lib.cg_transfer_from(C.byref(c), csrc, tt, 1, creg, creg, None, 0.0, cg.OP_STAGGER111,
+                                cg.CELL_CENTERED, 3, 0, None) == -11
