@@ -14,10 +14,16 @@ from carpet_amd.workloads import hierarchy_step, nested_levels  # noqa: E402
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("cent,order", [(cg.VERTEX_CENTERED, 5), (cg.CELL_CENTERED, 3)], ids=["vertex-o5", "cell-o3"])
-def test_hierarchy_step_matches_oracle(cent, order):
+@pytest.mark.parametrize("cent,order,op", [(cg.VERTEX_CENTERED, 5, cg.OP_LAGRANGE), (cg.CELL_CENTERED, 3, cg.OP_LAGRANGE),
+                                           (cg.VERTEX_CENTERED, 5, cg.OP_ENO), (cg.CELL_CENTERED, 3, cg.OP_ENO),
+                                           (cg.CELL_CENTERED, 5, cg.OP_ENO),
+                                           (cg.VERTEX_CENTERED, 5, cg.OP_LAGRANGE_MONOTONE)],
+                         ids=["vertex-o5", "cell-o3", "vertex-eno", "cell-eno2", "cell-eno3", "vertex-monotone"])
+def test_hierarchy_step_matches_oracle(cent, order, op):
+    """Every tensor kind's boundary prolongation shells go through the small
+    (8x8-base) tiles as well as the large ones."""
     lv = nested_levels(ncoarse=32, nlevels=3, split=(2, 1, 1), centering=cent, buffer=6, order=order)
-    w = hierarchy_step(lv, cent, n_gf=2)
+    w = hierarchy_step(lv, cent, n_gf=2, op=op)
     print(f"{w.name}: {len(w.steps)} transfers", flush=True)
     ref = run_oracle(w)
     st = cg.Stream()
@@ -25,6 +31,8 @@ def test_hierarchy_step_matches_oracle(cent, order):
     b = w.batch(dev)
     b.finalize()
     print(b.describe(), flush=True)
+    if op != cg.OP_LAGRANGE_MONOTONE:
+        assert "[tensor8 " in b.describe()
     b.run(st)
     st.synchronize()
     for k in ref:
