@@ -86,6 +86,9 @@ _decl("cg_restrict_3d_stagger_f64", i, i, *_OPARGS, vp)
 _decl("cg_set_staggered_operators", i, i)
 _decl("cg_interpolate_3d_2tl_f64", i, vp, dp, vp, dp, ip, ip, vp, dp, ip, ip, ip, ip, ip, ip, vp)
 _decl("cg_interpolate_3d_3tl_f64", i, vp, dp, vp, dp, vp, dp, ip, ip, vp, dp, ip, ip, ip, ip, ip, ip, vp)
+_decl("cg_interpolate_3d_4tl_f64", i, vp, dp, vp, dp, vp, dp, vp, dp, ip, ip, vp, dp, ip, ip, ip, ip, ip, ip, vp)
+_decl("cg_interpolate_3d_5tl_f64", i, vp, dp, vp, dp, vp, dp, vp, dp, vp, dp, ip, ip, vp, dp, ip, ip, ip, ip, ip,
+      ip, vp)
 _decl("cg_interpolate_eno_3d_3tl_f64", i, vp, dp, vp, dp, vp, dp, ip, ip, vp, dp, ip, ip, ip, ip, ip, ip, vp)
 _GP = C.POINTER(GData)
 _decl("cg_transfer_from", i, _GP, C.POINTER(_GP), C.POINTER(C.c_double), i, ip, ip,

@@ -400,6 +400,22 @@ def interpolate_3d_3tl(src1: DeviceArray, t1: float, src2: DeviceArray, t2: floa
           "interpolate_3d_3tl")
 
 
+def interpolate_3d_ntl(srcs: Sequence[DeviceArray], times: Sequence[float], dst: DeviceArray, t: float,
+                       dstregbbox: Box, stream=None):
+    """interpolate_3d_4tl / interpolate_3d_5tl (4 or 5 levels)."""
+    n = len(srcs)
+    if n not in (4, 5) or len(times) != n:
+        raise ValueError("interpolate_3d_ntl takes 4 or 5 levels and their times")
+    args = []
+    for a, tt in zip(srcs, times):
+        args += [a.ptr, tt]
+    s0 = srcs[0]
+    f = lib.cg_interpolate_3d_4tl_f64 if n == 4 else lib.cg_interpolate_3d_5tl_f64
+    check(f(*args, I3(*s0.padext), I3(*s0.ext), dst.ptr, t, I3(*dst.padext), I3(*dst.ext), s0.box.as_c(),
+            dst.box.as_c(), dstregbbox.as_c(), dstregbbox.as_c(), _sh(stream)),
+          f"interpolate_3d_{n}tl")
+
+
 def interpolate_eno_3d_3tl(src1: DeviceArray, t1: float, src2: DeviceArray, t2: float,
                            src3: DeviceArray, t3: float, dst: DeviceArray, t: float,
                            dstregbbox: Box, stream=None):

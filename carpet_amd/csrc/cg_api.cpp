@@ -60,9 +60,15 @@ int resolve_transfer(const cg_gdata *dst, const cg_gdata *const *srcs,
   }
   const Box dreg = Box::from(dstregbbox);
   const Box sreg = Box::from(srcregbbox);
-  const void *sp[3] = {nullptr, nullptr, nullptr};
-  for (int l = 0; l < ntl && l < 3; ++l)
+  const void *sp[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  for (int l = 0; l < ntl && l < 5; ++l)
     sp[l] = srcs[tl0 + l]->storage;
+  const bool eno_group = op == CG_OP_ENO || op == CG_OP_WENO || op == CG_OP_TVD || op == CG_OP_LAGRANGE_MONOTONE;
+  // 4 and 5 levels for Lagrange-type groups only: the ENO/WENO/TVD/
+  // monotone branch of data<T>::time_interpolate asserts beyond order_time
+  // 2 (data.cc:1260-1295)
+  if (ntl > 5 || (ntl > 3 && eno_group))
+    return set_error(CG_ERR_ORDER, "time interpolation order %d not supported", order_time);
 
   if (es != 8 && es != 4)
     return set_error(CG_ERR_ARG, "element size must be 8 or 4");
@@ -84,8 +90,6 @@ int resolve_transfer(const cg_gdata *dst, const cg_gdata *const *srcs,
     // interpolating the sources directly (a copy is exact)
     if (slab)
       return set_error(CG_ERR_OPERATOR, "slabbed copy with time interpolation");
-    if (ntl > 3)
-      return set_error(CG_ERR_ORDER, "time interpolation order %d not supported", order_time);
     // ENO and TVD groups interpolate 3 levels with interpolate_eno_3d_3tl
     // (data.cc:1262-1292)
     tr.is_prolong = false;
@@ -182,12 +186,49 @@ int resolve_transfer(const cg_gdata *dst, const cg_gdata *const *srcs,
     default:
       return set_error(CG_ERR_OPERATOR, "prolongation operator %d is not provided", op);
     }
+    if (ntl > 3) {
+      // every level prolongated into a temporary, then combined by
+      // interpolate_3d_4tl/5tl (data.cc:497-521, 1227-1256): validated here
+      // with a stand-in temporary, run by run_transfer
+      if (es != 8)
+        return set_error(CG_ERR_OPERATOR, "mixed precision with more than 3 time levels");
+      double tw[5];
+      for (int l = 0; l < ntl; ++l)
+        tw[l] = times[tl0 + l];
+      double sfx[5];
+      if ((e = time_factors(ntl, tw, time, sfx)))
+        return e;
+      Arr tmp;
+      tmp.p = s0.p; // stand-in address, never dereferenced
+      for (int k = 0; k < 3; ++k)
+        tmp.pad[k] = tmp.ext[k] = dreg.sizes(k);
+      tmp.box = dreg;
+      const double one[1] = {1.0};
+      if ((e = plan_prolong(kind, order, s0, sp, 1, one, tmp, dreg, tr.pr, 8, nullptr, stagger_dirs(op))))
+        return e;
+      PointPlan ip;
+      if ((e = plan_interp(ntl, sp, tw, time, tmp, d, dreg, ip)))
+        return e;
+      tr.is_prolong = true;
+      tr.tmp_ntl = ntl;
+      tr.tmp_kind = kind;
+      tr.tmp_order = order;
+      tr.tmp_stag = stagger_dirs(op);
+      tr.tmp_src = s0;
+      tr.tmp_dst = d;
+      tr.tmp_reg = dreg;
+      for (int l = 0; l < ntl; ++l) {
+        tr.tmp_srcs[l] = sp[l];
+        tr.tmp_times[l] = tw[l];
+      }
+      tr.tmp_time = time;
+      tr.pr.dst_points = dreg.size();
+      return CG_OK;
+    }
     double sf[3] = {1.0, 0.0, 0.0};
     EnoTime ew;
     const bool eno_time = ntl == 3 && (op == CG_OP_ENO || op == CG_OP_TVD);
     if (ntl > 1) {
-      if (ntl > 3)
-        return set_error(CG_ERR_ORDER, "time interpolation order %d not supported", order_time);
       // each level prolongated, then combined in time (data.cc:497-521):
       // ENO/TVD groups with 3 levels by interpolate_eno_3d_3tl, the others
       // by interpolate_3d_2tl/3tl (data.cc:1182-1299)
@@ -237,7 +278,52 @@ int resolve_transfer(const cg_gdata *dst, const cg_gdata *const *srcs,
   return set_error(CG_ERR_STRIDE, "source and destination strides are not ordered");
 }
 
+// 4 or 5 time levels: region-sized temporaries from a stream-ordered
+// allocation, one single-level prolongation per level, the interpolation,
+// then the temporaries are released in stream order
+static int run_transfer_tmp(const Transfer &tr, hipStream_t st) {
+  const Box &reg = tr.tmp_reg;
+  const long long npts = reg.size();
+  double *tmp = nullptr;
+  hipError_t he = hipMallocAsync(reinterpret_cast<void **>(&tmp), (size_t)tr.tmp_ntl * npts * sizeof(double), st);
+  if (he != hipSuccess)
+    return hip_error(he, "hipMallocAsync (time-level temporaries)");
+  int e = CG_OK;
+  const void *tp[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  Arr ta;
+  for (int k = 0; k < 3; ++k)
+    ta.pad[k] = ta.ext[k] = reg.sizes(k);
+  ta.box = reg;
+  const double one[1] = {1.0};
+  for (int l = 0; l < tr.tmp_ntl && !e; ++l) {
+    ta.p = tmp + l * npts;
+    tp[l] = ta.p;
+    ProlongPlan pl;
+    const void *s1[1] = {tr.tmp_srcs[l]};
+    e = plan_prolong(tr.tmp_kind, tr.tmp_order, tr.tmp_src, s1, 1, one, ta, reg, pl, 8, nullptr, tr.tmp_stag);
+    if (!e)
+      e = launch_prolong(pl, 0, st);
+  }
+  if (!e) {
+    ta.p = tmp;
+    PointPlan ip;
+    e = plan_interp(tr.tmp_ntl, tp, tr.tmp_times, tr.tmp_time, ta, tr.tmp_dst, reg, ip);
+    if (!e) {
+      const int r = launch_point(ip.op, ip.a, st);
+      e = r == -1 ? set_error(CG_ERR_ARG, "region too large for the pointwise kernel")
+          : r     ? hip_error(hipGetLastError(), "kernel launch")
+                  : CG_OK;
+    }
+  }
+  he = hipFreeAsync(tmp, st);
+  if (!e && he != hipSuccess)
+    e = hip_error(he, "hipFreeAsync (time-level temporaries)");
+  return e;
+}
+
 int run_transfer(const Transfer &tr, int prec, hipStream_t st) {
+  if (tr.tmp_ntl)
+    return run_transfer_tmp(tr, st);
   if (tr.is_prolong)
     return launch_prolong(tr.pr, tr.pr.es == 4 && prec == CG_PREC_F64 ? CG_PREC_MIXED : prec, st);
   const int r = launch_point(tr.pt.op, tr.pt.a, st);
@@ -528,7 +614,7 @@ int cg_copy_3d_f64(const double *src, const int srcpadext[3], const int srcext[3
                     Box::from(srcregbbox), Box::from(dstregbbox), slab, pl);
   if (e)
     return e;
-  Transfer tr;
+  Transfer tr{};
   tr.is_prolong = false;
   tr.pt = pl;
   return run_transfer(tr, 0, S(stream));
@@ -629,7 +715,7 @@ int cg_restrict_3d_stagger_f64(int stagger, const double *src, const int srcpade
   (void)srcregbbox;
   if (!srcpadext || !srcext || !dstpadext || !dstext || !srcbbox || !dstbbox || !dstregbbox)
     return set_error(CG_ERR_ARG, "null argument");
-  Transfer tr;
+  Transfer tr{};
   tr.is_prolong = false;
   int e = plan_restrict_stagger(stagger, arr_from(src, srcpadext, srcext, srcbbox),
                                 arr_from(dst, dstpadext, dstext, dstbbox), Box::from(dstregbbox), tr.pt);
@@ -656,7 +742,7 @@ static int restrict_call(bool cell, const double *src, const int *srcpadext,
                          cg_stream_t stream, int order = 1) {
   if (!srcpadext || !srcext || !dstpadext || !dstext || !srcbbox || !dstbbox || !dstregbbox)
     return set_error(CG_ERR_ARG, "null argument");
-  Transfer tr;
+  Transfer tr{};
   tr.is_prolong = false;
   int e = plan_restrict(cell, arr_from(src, srcpadext, srcext, srcbbox),
                         arr_from(dst, dstpadext, dstext, dstbbox),
@@ -716,9 +802,11 @@ static int interp_call(int ntl, const double *const *srcs, const double *times,
                        const int *dstregbbox, cg_stream_t stream, bool eno = false) {
   if (!srcpadext || !srcext || !dstpadext || !dstext || !srcbbox || !dstbbox || !dstregbbox)
     return set_error(CG_ERR_ARG, "null argument");
-  Transfer tr;
+  Transfer tr{};
   tr.is_prolong = false;
-  const void *sp[3] = {srcs[0], srcs[1], ntl > 2 ? srcs[2] : nullptr};
+  const void *sp[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  for (int l = 0; l < ntl && l < 5; ++l)
+    sp[l] = srcs[l];
   int e = plan_interp(ntl, sp, times, t, arr_from(srcs[0], srcpadext, srcext, srcbbox),
                       arr_from(dst, dstpadext, dstext, dstbbox),
                       Box::from(dstregbbox), tr.pt, eno);
@@ -752,6 +840,32 @@ int cg_interpolate_3d_3tl_f64(const double *src1, double t1, const double *src2,
   const double tt[3] = {t1, t2, t3};
   return interp_call(3, srcs, tt, srcpadext, srcext, dst, t, dstpadext, dstext,
                      srcbbox, dstbbox, dstregbbox, stream);
+}
+
+int cg_interpolate_3d_4tl_f64(const double *src1, double t1, const double *src2, double t2,
+                              const double *src3, double t3, const double *src4, double t4,
+                              const int srcpadext[3], const int srcext[3], double *dst, double t,
+                              const int dstpadext[3], const int dstext[3], const int srcbbox[9],
+                              const int dstbbox[9], const int srcregbbox[9], const int dstregbbox[9],
+                              cg_stream_t stream) {
+  (void)srcregbbox;
+  const double *srcs[4] = {src1, src2, src3, src4};
+  const double tt[4] = {t1, t2, t3, t4};
+  return interp_call(4, srcs, tt, srcpadext, srcext, dst, t, dstpadext, dstext, srcbbox, dstbbox, dstregbbox,
+                     stream);
+}
+
+int cg_interpolate_3d_5tl_f64(const double *src1, double t1, const double *src2, double t2,
+                              const double *src3, double t3, const double *src4, double t4,
+                              const double *src5, double t5, const int srcpadext[3], const int srcext[3],
+                              double *dst, double t, const int dstpadext[3], const int dstext[3],
+                              const int srcbbox[9], const int dstbbox[9], const int srcregbbox[9],
+                              const int dstregbbox[9], cg_stream_t stream) {
+  (void)srcregbbox;
+  const double *srcs[5] = {src1, src2, src3, src4, src5};
+  const double tt[5] = {t1, t2, t3, t4, t5};
+  return interp_call(5, srcs, tt, srcpadext, srcext, dst, t, dstpadext, dstext, srcbbox, dstbbox, dstregbbox,
+                     stream);
 }
 
 int cg_interpolate_eno_3d_3tl_f64(const double *src1, double t1, const double *src2,
@@ -808,7 +922,7 @@ int cg_transfer_from_mixed(const cg_gdata *dst, const cg_gdata *const *srcs,
                            const double *times, int ntimes, const int dstregbbox[9],
                            const int srcregbbox[9], double time, int transport_operator,
                            int centering, int order_space, int order_time, cg_stream_t stream) {
-  Transfer tr;
+  Transfer tr{};
   int e = resolve_transfer(dst, srcs, times, ntimes, dstregbbox, srcregbbox, nullptr, time,
                            transport_operator, centering, order_space, order_time, tr, 4);
   if (e)
@@ -821,7 +935,7 @@ int cg_transfer_from(const cg_gdata *dst, const cg_gdata *const *srcs,
                      const int srcregbbox[9], const cg_islab *slab, double time,
                      int transport_operator, int centering, int order_space,
                      int order_time, cg_stream_t stream) {
-  Transfer tr;
+  Transfer tr{};
   int e = resolve_transfer(dst, srcs, times, ntimes, dstregbbox, srcregbbox, slab,
                            time, transport_operator, centering, order_space,
                            order_time, tr);

@@ -69,12 +69,15 @@ int cg_batch_add(cg_batch *b, int phase, const cg_gdata *dst,
     return set_error(CG_ERR_ARG, "phase must be in [0,15]");
   if (prec != CG_PREC_F64 && prec != CG_PREC_MIXED && prec != CG_PREC_MIXED_DOT2)
     return set_error(CG_ERR_ARG, "unknown precision %d", prec);
-  Transfer tr;
+  Transfer tr{};
   int e = resolve_transfer(dst, srcs, times, ntimes, dstregbbox, srcregbbox, slab,
                            time, transport_operator, centering, order_space,
                            order_time, tr, prec == CG_PREC_F64 ? 8 : 4);
   if (e)
     return e;
+  if (tr.tmp_ntl)
+    return set_error(CG_ERR_ORDER, "prolongation with %d time levels runs per call (cg_transfer_from), "
+                                   "not in a batch", tr.tmp_ntl);
   b->transfers.push_back(tr);
   b->phase.push_back(phase);
   b->prec.push_back(prec);

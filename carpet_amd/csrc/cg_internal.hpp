@@ -136,6 +136,16 @@ struct Transfer {
   bool is_prolong;
   PointPlan pt;
   ProlongPlan pr;
+  // prolongation with 4 or 5 time levels (order_time 3, 4): as in
+  // data.cc:497-521 each level is prolongated into a region-sized temporary
+  // (stream-ordered allocation), then interpolate_3d_4tl/5tl combines them;
+  // per call only (the batch refuses it)
+  int tmp_ntl;
+  int tmp_kind, tmp_order, tmp_stag;
+  Arr tmp_src, tmp_dst;
+  Box tmp_reg;
+  const void *tmp_srcs[5];
+  double tmp_times[5], tmp_time;
   long long dst_points() const { return is_prolong ? pr.dst_points : pt.dst_points; }
 };
 int resolve_transfer(const cg_gdata *dst, const cg_gdata *const *srcs,

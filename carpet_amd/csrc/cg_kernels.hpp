@@ -55,15 +55,16 @@ enum PointOp { PO_COPY = 0, PO_RESTRICT_VC = 1, PO_RESTRICT_CC = 2,
                PO_MONO5 = 7, PO_STAGGER = 8, PO_RESTRICT_STAGGER = 9 };
 
 struct PointArgs {
-  const double *src[3]; // source element matching region point (0,0,0)
+  const double *src[5]; // source element matching region point (0,0,0)
                         // (copy/interp); for restrict: first child; for
                         // TVD: source element (0,0,0) of the array
   long long sj, sk;
   double *dst;          // destination element at the region origin
   long long dj, dk;
   int n[3];             // region extents (destination points)
-  int ntl;              // time levels (interp, TVD)
-  double s[3];          // time weights (interp, TVD)
+  int ntl;              // time levels (interp: 2..5; TVD and the other
+                        // prolongations: 1..3)
+  double s[5];          // time weights
   double s2[2];         // ENO time rule: fallback pair weights
   int l0, teno;         //   first level of the pair; 1 = rule on (3 levels)
   int off[3];           // TVD: fine offsets (off+f)/2 = parent, (off+f)%2 = side

@@ -611,6 +611,34 @@ int time_factors(int ntl, const double *tt, double t, double *s) {
     s[2] = (t - t1) * (t - t2) / ((t3 - t1) * (t3 - t2));
     return CG_OK;
   }
+  if (ntl == 4 || ntl == 5) {
+    // interpolate_3d_4tl.cc:98-119, interpolate_3d_5tl.cc:101-128: per level
+    // the product of (t - t_m) over the other levels in level order over the
+    // product of (t_k - t_m) in the same order
+    double tmin = tt[0], tmax = tt[0];
+    for (int a = 0; a < ntl; ++a) {
+      for (int b = a + 1; b < ntl; ++b)
+        if (std::fabs(tt[a] - tt[b]) < eps)
+          return set_error(CG_ERR_TIME_SAME, "Internal error: arrays have same time");
+      tmin = std::fmin(tmin, tt[a]);
+      tmax = std::fmax(tmax, tt[a]);
+    }
+    if (t < tmin - eps || t > tmax + eps)
+      return set_error(CG_ERR_TIME_EXTRAP, "Internal error: extrapolation in time");
+    for (int k = 0; k < ntl; ++k) {
+      double num = 0.0, den = 0.0;
+      bool first = true;
+      for (int m = 0; m < ntl; ++m) {
+        if (m == k)
+          continue;
+        num = first ? (t - tt[m]) : num * (t - tt[m]);
+        den = first ? (tt[k] - tt[m]) : den * (tt[k] - tt[m]);
+        first = false;
+      }
+      s[k] = num / den;
+    }
+    return CG_OK;
+  }
   return set_error(CG_ERR_ORDER, "no time interpolation with %d time levels", ntl);
 }
 
@@ -675,8 +703,8 @@ int plan_interp(int ntl, const void *const *srcs, const double *times, double t,
   } else if ((e = time_factors(ntl, times, t, pl.a.s))) {
     return e;
   }
-  if (ntl < 2)
-    return set_error(CG_ERR_ORDER, "time interpolation needs 2 or 3 time levels");
+  if (ntl < 2 || ntl > 5)
+    return set_error(CG_ERR_ORDER, "time interpolation needs 2 to 5 time levels");
   int so[3], dof[3];
   for (int d = 0; d < 3; ++d) {
     if ((reg.lo[d] - sb.lo[d]) % sb.str[d] || (reg.lo[d] - db.lo[d]) % db.str[d])

@@ -694,7 +694,16 @@ __device__ __forceinline__ double point_value(const PointArgs &a, int i, int j, 
     out = r2;
   } else if constexpr (OP == PO_INTERP) {
     const long long o = i + a.sj * j + a.sk * k;
-    if (a.ntl == 3) {
+    if (a.ntl > 3) {
+      // interpolate_3d_4tl / _5tl: s1*a, then fma(s_l, level l, .)
+      double r = a.s[0] * a.src[0][o];
+      r = fmad(a.s[1], a.src[1][o], r);
+      r = fmad(a.s[2], a.src[2][o], r);
+      r = fmad(a.s[3], a.src[3][o], r);
+      if (a.ntl == 5)
+        r = fmad(a.s[4], a.src[4][o], r);
+      out = r;
+    } else if (a.ntl == 3) {
       const double v[3] = {a.src[0][o], a.src[1][o], a.src[2][o]};
       out = time_combine<3>(v, a.s, a.s2, a.l0, a.teno);
     } else {

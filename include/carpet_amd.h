@@ -309,6 +309,23 @@ int cg_interpolate_3d_2tl_f64(const double *src1, double t1,
                               const int dstbbox[9], const int srcregbbox[9],
                               const int dstregbbox[9], cg_stream_t stream);
 
+/* interpolate_3d_4tl<double> / interpolate_3d_5tl<double>:
+   CarpetLib/src/interpolate_3d_4tl.cc:24-131, interpolate_3d_5tl.cc:24-140
+   (cubic / quartic Lagrange in time, data.cc:1227-1256); the sum in the
+   canonical order s1*src1, then fma(s_l, src_l, .) */
+int cg_interpolate_3d_4tl_f64(const double *src1, double t1, const double *src2, double t2,
+                              const double *src3, double t3, const double *src4, double t4,
+                              const int srcpadext[3], const int srcext[3], double *dst, double t,
+                              const int dstpadext[3], const int dstext[3], const int srcbbox[9],
+                              const int dstbbox[9], const int srcregbbox[9], const int dstregbbox[9],
+                              cg_stream_t stream);
+int cg_interpolate_3d_5tl_f64(const double *src1, double t1, const double *src2, double t2,
+                              const double *src3, double t3, const double *src4, double t4,
+                              const double *src5, double t5, const int srcpadext[3], const int srcext[3],
+                              double *dst, double t, const int dstpadext[3], const int dstext[3],
+                              const int srcbbox[9], const int dstbbox[9], const int srcregbbox[9],
+                              const int dstregbbox[9], cg_stream_t stream);
+
 /* interpolate_3d_3tl<double>: CarpetLib/src/interpolate_3d_3tl.cc:24-126 */
 int cg_interpolate_3d_3tl_f64(const double *src1, double t1,
                               const double *src2, double t2,
@@ -391,7 +408,11 @@ int cg_transfer_from_mixed(const cg_gdata *dst, const cg_gdata *const *srcs,
    times[] their times.  Where the reference prolongates every time level
    into a temporary and then interpolates in time, this call runs one fused
    kernel that keeps the prolongated values in registers; the arithmetic is
-   the same.  Cell-centred restrictions follow cg_set_restriction. */
+   the same.  With 4 or 5 time levels (order_time 3 or 4; Lagrange, copy
+   and staggered groups only, as in data.cc:1192-1258) the levels go
+   through region-sized temporaries from a stream-ordered allocation, as in
+   the reference; cg_batch_add refuses such prolongations.  Cell-centred
+   restrictions follow cg_set_restriction. */
 int cg_transfer_from(const cg_gdata *dst, const cg_gdata *const *srcs,
                      const double *times, int ntimes,
                      const int dstregbbox[9], const int srcregbbox[9],

@@ -1246,10 +1246,41 @@ int orc_time_factors(int ntl, const double *tt, double t, double *s) {
     s[2] = (t - t1) * (t - t2) / ((t3 - t1) * (t3 - t2));
     return CG_OK;
   }
+  if (ntl == 4 || ntl == 5) {
+    // interpolate_3d_4tl.cc:98-119, interpolate_3d_5tl.cc:101-128: every
+    // pair apart by eps, no extrapolation, and per level the product of
+    // (t - t_m) over the other levels in level order divided by the
+    // product of (t_k - t_m) in the same order
+    double tmin = tt[0], tmax = tt[0];
+    for (int a = 0; a < ntl; ++a) {
+      for (int b = a + 1; b < ntl; ++b)
+        if (std::fabs(tt[a] - tt[b]) < eps)
+          return CG_ERR_TIME_SAME;
+      tmin = std::min(tmin, tt[a]);
+      tmax = std::max(tmax, tt[a]);
+    }
+    if (t < tmin - eps || t > tmax + eps)
+      return CG_ERR_TIME_EXTRAP;
+    for (int k = 0; k < ntl; ++k) {
+      double num = 0.0, den = 0.0;
+      bool first = true;
+      for (int m = 0; m < ntl; ++m) {
+        if (m == k)
+          continue;
+        num = first ? (t - tt[m]) : num * (t - tt[m]);
+        den = first ? (tt[k] - tt[m]) : den * (tt[k] - tt[m]);
+        first = false;
+      }
+      s[k] = num / den;
+    }
+    return CG_OK;
+  }
   return CG_ERR_ORDER;
 }
 
-// interpolate_3d_2tl / _3tl on same-stride arrays (ntl in {2,3})
+// interpolate_3d_2tl / _3tl / _4tl / _5tl on same-stride arrays (ntl in
+// 2..5); the sum in canonical order: s1*a, then fma(s2, b, .), ...
+// (the reference writes s1fac*src1 + s2fac*src2 + ..., interpolate_3d_4tl.cc:125-128)
 int orc_interpolate_3d(int ntl, const double *const *srcs, const double *tt,
                        const int *spad, const int *sext, double *dst, double t,
                        const int *dpad, const int *dext, const int *sb,
@@ -1269,8 +1300,8 @@ int orc_interpolate_3d(int ntl, const double *const *srcs, const double *tt,
     return CG_ERR_EMPTY;
   if (!contained(r, s) || !contained(r, dbx))
     return CG_ERR_CONTAIN;
-  double f[3];
-  if ((e = orc_time_factors(ntl, tt, t, f)) || ntl < 2)
+  double f[5];
+  if (ntl > 5 || (e = orc_time_factors(ntl, tt, t, f)) || ntl < 2)
     return e ? e : CG_ERR_ORDER;
   int n[3], so[3], dof[3];
   for (int d = 0; d < 3; ++d) {
@@ -1645,7 +1676,9 @@ int orc_prolongate_time(int kind, int order, const double *const *srcs,
   };
   if (ntl == 1)
     return prolong(srcs[tl0], dst, dpad, dext, db);
-  if (ntl > 3)
+  // 4 and 5 levels: Lagrange-type groups only (data.cc:1192-1258; the
+  // ENO/WENO/TVD/monotone branch asserts beyond order_time 2, :1260-1295)
+  if (ntl > 5 || (ntl > 3 && kind != 0 && kind != 1 && kind != 6))
     return CG_ERR_ORDER;
   // temporaries with extent = region, unpadded (data.cc:507-509)
   const box3 r = mkbox(drb);
@@ -1657,8 +1690,9 @@ int orc_prolongate_time(int kind, int order, const double *const *srcs,
   for (int l = 0; l < ntl; ++l)
     if ((e = prolong(srcs[l], tmp[l].data(), text, text, drb)))
       return e;
-  const double *tp[3] = {tmp[0].data(), ntl > 1 ? tmp[1].data() : nullptr,
-                         ntl > 2 ? tmp[2].data() : nullptr};
+  const double *tp[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  for (int l = 0; l < ntl; ++l)
+    tp[l] = tmp[l].data();
   // ENO and TVD groups interpolate 3 time levels with interpolate_eno_3d_3tl
   // (data.cc:1282-1292), Lagrange with interpolate_3d_2tl/3tl
   if (ntl == 3 && (kind == 2 || kind == 3 || kind == 4))

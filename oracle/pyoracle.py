@@ -198,7 +198,7 @@ def restrict_3d_cc_ho_rf2(order, src, dst, dstreg) -> int:
 
 def time_factors(times, t):
     n = len(times)
-    s = (C.c_double * 3)()
+    s = (C.c_double * 5)()
     tt = (C.c_double * n)(*times)
     rc = _lib.orc_time_factors(n, tt, t, s)
     return rc, list(s)[:n]

@@ -559,3 +559,85 @@ def test_stagger_dispatch(stag, op, staggered_on):
         drdst = _dev(rdst)
         cg.transfer_from(drdst, [_dev(rsrc)], [0.0], cg.Box(*rrb), cg.Box(*rrb), 0.0, op, cent, 3, 0)
         _check(rexp, drdst, f"stagger restriction dispatch {stag} cent {cent}")
+
+
+# ---- 4 and 5 time levels ---------------------------------------------------
+@pytest.mark.parametrize("ntl", [4, 5])
+def test_interpolate_4tl_5tl_parity(ntl):
+    """interpolate_3d_4tl / interpolate_3d_5tl against the oracle, bit for
+    bit, per call and through cg_transfer_from (same level, order_time
+    ntl-1)."""
+    rng = np.random.default_rng(40 + ntl)
+    times = [0.5, 0.25, 0.0, -0.25, -0.5][:ntl]
+    for c in range(3):
+        print(f"interp ntl={ntl} #{c}", flush=True)
+        n = tuple(int(v) for v in rng.integers(1, 40, size=3))
+        rlo = tuple(int(v) for v in rng.integers(-5, 5, size=3))
+        rup = tuple(l + k - 1 for l, k in zip(rlo, n))
+        sb = (tuple(l - 1 for l in rlo), tuple(u + 2 for u in rup), (1,) * 3)
+        db = (tuple(l - 2 for l in rlo), tuple(u + 1 for u in rup), (1,) * 3)
+        rb = (rlo, rup, (1,) * 3)
+        pad = pad_of(sizes(sb), rng)
+        srcs = []
+        for _ in range(ntl):
+            h = host_array(sb, pad)
+            h.view()[...] = field(rng, sizes(sb), "random")
+            srcs.append(h)
+        t = float(rng.uniform(min(times), max(times)))
+        dst = host_array(db, pad_of(sizes(db), rng))
+        exp = orc.HostArr(db, dst.data.copy())
+        assert orc.interpolate_3d(srcs, times, exp, t, rb) == 0
+        ds = [_dev(h) for h in srcs]
+        ddst = _dev(dst)
+        cg.interpolate_3d_ntl(ds, times, ddst, t, cg.Box(*rb))
+        _check(exp, ddst, f"interp ntl={ntl} case {c}")
+        ddst = _dev(dst)
+        cg.transfer_from(ddst, ds, times, cg.Box(*rb), cg.Box(*rb), t, cg.OP_LAGRANGE, cg.VERTEX_CENTERED, 3,
+                         ntl - 1)
+        _check(exp, ddst, f"transfer interp ntl={ntl} case {c}")
+
+
+@pytest.mark.parametrize("kind,order,ntl", [("vlag", 5, 4), ("vlag", 3, 5), ("clag", 3, 4), ("clag", 2, 5),
+                                            ("stag", 3, 4), ("stag", 2, 5)])
+def test_transfer_4tl_5tl_prolongation(kind, order, ntl):
+    """Prolongation with 4 or 5 time levels (order_time 3, 4): each level
+    into a temporary, then interpolate_3d_4tl/5tl, as data.cc:497-521 does;
+    equal to the oracle's replay of that shape bit for bit.  The batch
+    refuses it, and ENO-type groups refuse more than 3 levels."""
+    rng = np.random.default_rng(60 * ntl + order + len(kind))
+    okind, op, cent = {"vlag": (orc.KIND_VLAG, cg.OP_LAGRANGE, cg.VERTEX_CENTERED),
+                       "clag": (orc.KIND_CLAG, cg.OP_LAGRANGE, cg.CELL_CENTERED),
+                       "stag": (orc.KIND_STAGGER, cg.OP_STAGGER111, cg.VERTEX_CENTERED)}[kind]
+    order_space = {("stag", 3): 4}.get((kind, order), order)
+    if kind == "stag":
+        cg.set_staggered_operators(True)
+        orc.set_stagger(cg.STAGGER111)
+    try:
+        times = [0.5, 0.25, 0.0, -0.25, -0.5][:ntl]
+        for c in range(3):
+            print(f"transfer {kind} o{order} ntl{ntl} #{c}", flush=True)
+            sb, db, rb = prolong_case(rng, kind, order, n=(40, 9, 13) if c == 0 else None)
+            pad = pad_of(sizes(sb), rng)
+            srcs = []
+            for _ in range(ntl):
+                h = host_array(sb, pad)
+                h.view()[...] = field(rng, sizes(sb), "mixed")
+                srcs.append(h)
+            t = [0.3, 0.1, -0.2][c]
+            dst = host_array(db, pad_of(sizes(db), rng))
+            exp = orc.HostArr(db, dst.data.copy())
+            assert orc.prolongate_time(okind, order, srcs, times, exp, t, rb, ntl - 1) == 0
+            ds = [_dev(h) for h in srcs]
+            ddst = _dev(dst)
+            cg.transfer_from(ddst, ds, times, cg.Box(*rb), cg.Box(*rb), t, op, cent, order_space, ntl - 1)
+            _check(exp, ddst, f"transfer {kind} ntl {ntl} case {c}")
+        b = cg.Batch()
+        with pytest.raises(cg.CarpetError):
+            b.add(0, ddst, ds, times, cg.Box(*rb), cg.Box(*rb), t, op, cent, order_space, ntl - 1)
+        b.close()
+    finally:
+        cg.set_staggered_operators(False)
+    if kind != "stag":
+        with pytest.raises(cg.CarpetError):
+            cg.transfer_from(ddst, ds, times, cg.Box(*rb), cg.Box(*rb), t, cg.OP_TVD if kind == "clag" else cg.OP_ENO,
+                             cent, 1 if kind == "clag" else 5, ntl - 1)

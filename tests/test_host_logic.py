@@ -114,6 +114,11 @@ def test_time_interpolation_validation():
     # extrapolation
     assert lib.cg_interpolate_3d_3tl_f64(FAKE, 1.0, FAKE, 0.0, FAKE, -1.0, e, e, FAKE, 1.5, e, e, box, box, box,
                                          box, None) == -9
+    # 4 and 5 levels (interpolate_3d_4tl.cc:98-108, interpolate_3d_5tl.cc:101-113)
+    assert lib.cg_interpolate_3d_4tl_f64(FAKE, 0.5, FAKE, 0.25, FAKE, 0.25 + 1e-13, FAKE, -0.25, e, e, FAKE, 0.3,
+                                         e, e, box, box, box, box, None) == -8
+    assert lib.cg_interpolate_3d_5tl_f64(FAKE, 0.5, FAKE, 0.25, FAKE, 0.0, FAKE, -0.25, FAKE, -0.5, e, e, FAKE,
+                                         0.6, e, e, box, box, box, box, None) == -9
 
 
 def test_copy_validation():

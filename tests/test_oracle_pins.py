@@ -390,14 +390,27 @@ def test_eno_line_fallbacks(order, di):
 
 # ---- time interpolation -----------------------------------------------------
 def test_time_factors_reproduce_polynomials_in_time():
-    for times in ([1.0, 0.0], [1.0, 0.0, -1.0], [0.3, 0.2, 0.1]):
+    for times in ([1.0, 0.0], [1.0, 0.0, -1.0], [0.3, 0.2, 0.1], [1.0, 0.5, 0.0, -0.5],
+                  [0.0, -0.25, -0.5, -0.75, -1.0], [2.0, 0.0, 1.0, -1.0]):
         n = len(times)
         for t in np.linspace(min(times), max(times), 7):
             rc, s = orc.time_factors(times, float(t))
             assert rc == 0
+            # Lagrange weights: exact on polynomials of degree < n
             for deg in range(n):
                 val = sum(si * ti**deg for si, ti in zip(s, times))
-                assert val == pytest.approx(t**deg, abs=1e-14)
+                assert val == pytest.approx(t**deg, abs=1e-13)
+            # and equal to the textbook product formula in exact rationals
+            from fractions import Fraction as F
+            ft = [F(x) for x in times]
+            for k in range(n):
+                w = F(1)
+                for m in range(n):
+                    if m != k:
+                        w *= (F(float(t)) - ft[m]) / (ft[k] - ft[m])
+                assert s[k] == pytest.approx(float(w), rel=1e-13, abs=1e-15)
+    assert orc.time_factors([1.0, 0.5, 0.5 + 1e-13, 0.0], 0.7)[0] == -8  # CG_ERR_TIME_SAME, 4 levels
+    assert orc.time_factors([1.0, 0.5, 0.0, -0.5, -1.0], 1.1)[0] == -9  # CG_ERR_TIME_EXTRAP, 5 levels
     assert orc.time_factors([1.0, 1.0 + 1e-13], 1.0)[0] == -8  # CG_ERR_TIME_SAME
     assert orc.time_factors([1.0, 0.0], 1.1)[0] == -9  # CG_ERR_TIME_EXTRAP
     assert orc.time_factors([1.0, 0.0, -1.0], -1.0 - 1e-13)[0] == 0  # within eps
