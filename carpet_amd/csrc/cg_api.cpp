@@ -26,6 +26,65 @@ static int stagger_dirs(int op) {
   return 0;
 }
 
+// region-sized, unpadded temporary over the transfer's destination region
+static Arr tmp_arr(const Transfer &tr, const void *p) {
+  Arr a;
+  a.p = p;
+  for (int k = 0; k < 3; ++k)
+    a.pad[k] = a.ext[k] = tr.tmp_reg.sizes(k);
+  a.box = tr.tmp_reg;
+  return a;
+}
+
+// one time level of a transfer through temporaries, written to out
+static int plan_level(const Transfer &tr, const void *src, const Arr &out, Transfer &one) {
+  std::memset(&one, 0, sizeof one);
+  Arr sa = tr.tmp_src;
+  sa.p = src;
+  switch (tr.tmp_step) {
+  case 0: {
+    const void *s1[1] = {src};
+    const double w[1] = {1.0};
+    one.is_prolong = true;
+    return plan_prolong(tr.tmp_kind, tr.tmp_order, sa, s1, 1, w, out, tr.tmp_reg, one.pr, 8, nullptr,
+                        tr.tmp_stag);
+  }
+  case 1:
+    return plan_copy(sa, out, tr.tmp_sreg, tr.tmp_reg, tr.tmp_has_slab ? &tr.tmp_slab : nullptr, one.pt);
+  case 2:
+    return tr.tmp_stag ? plan_restrict_stagger(tr.tmp_stag, sa, out, tr.tmp_reg, one.pt)
+                       : plan_restrict(tr.tmp_cell != 0, sa, out, tr.tmp_reg, one.pt, tr.tmp_rorder);
+  }
+  return set_error(CG_ERR_ARG, "unknown transfer step");
+}
+
+// completes a transfer through temporaries and validates it with a
+// stand-in temporary (its address is never dereferenced)
+static int setup_tmp(Transfer &tr, int ntl, const void *const *sp, const double *tw, double time,
+                     const Arr &s0, const Arr &d, const Box &sreg, const Box &dreg) {
+  tr.tmp_src = s0;
+  tr.tmp_dst = d;
+  tr.tmp_sreg = sreg;
+  tr.tmp_reg = dreg;
+  for (int l = 0; l < ntl; ++l) {
+    tr.tmp_srcs[l] = sp[l];
+    tr.tmp_times[l] = tw[l];
+  }
+  tr.tmp_time = time;
+  const Arr stand_in = tmp_arr(tr, s0.p);
+  Transfer one;
+  int e;
+  if ((e = plan_level(tr, sp[0], stand_in, one)))
+    return e;
+  PointPlan ip;
+  if ((e = plan_interp(ntl, sp, tw, time, stand_in, d, dreg, ip, tr.tmp_eno)))
+    return e;
+  tr.tmp_ntl = ntl;
+  tr.is_prolong = tr.tmp_step == 0;
+  tr.pr.dst_points = tr.pt.dst_points = dreg.size();
+  return CG_OK;
+}
+
 // data<T>::transfer_time -> transfer_p_r -> transfer_prolongate /
 // transfer_restrict / copy_from_innerloop, with the time-level choice of
 // gdata::find_source_timelevel (CarpetLib/src/data.cc:485-556, 572-966,
@@ -63,6 +122,8 @@ int resolve_transfer(const cg_gdata *dst, const cg_gdata *const *srcs,
   const void *sp[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   for (int l = 0; l < ntl && l < 5; ++l)
     sp[l] = srcs[tl0 + l]->storage;
+  // ENO, WENO, TVD and monotone groups interpolate 3 time levels with
+  // interpolate_eno_3d_3tl (data.cc:1260-1292)
   const bool eno_group = op == CG_OP_ENO || op == CG_OP_WENO || op == CG_OP_TVD || op == CG_OP_LAGRANGE_MONOTONE;
   // 4 and 5 levels for Lagrange-type groups only: the ENO/WENO/TVD/
   // monotone branch of data<T>::time_interpolate asserts beyond order_time
@@ -87,14 +148,23 @@ int resolve_transfer(const cg_gdata *dst, const cg_gdata *const *srcs,
       return plan_copy(s0, d, sreg, dreg, slab, tr.pt);
     }
     // copying each level into a temporary and interpolating in time equals
-    // interpolating the sources directly (a copy is exact)
-    if (slab)
-      return set_error(CG_ERR_OPERATOR, "slabbed copy with time interpolation");
+    // interpolating the sources directly (a copy is exact); with a slab
+    // offset the copies go through temporaries as in the reference
+    if (slab) {
+      double tw[5];
+      for (int l = 0; l < ntl; ++l)
+        tw[l] = times[tl0 + l];
+      tr.tmp_step = 1;
+      tr.tmp_has_slab = true;
+      tr.tmp_slab = *slab;
+      tr.tmp_eno = ntl == 3 && eno_group;
+      return setup_tmp(tr, ntl, sp, tw, time, s0, d, sreg, dreg);
+    }
     // ENO and TVD groups interpolate 3 levels with interpolate_eno_3d_3tl
     // (data.cc:1262-1292)
     tr.is_prolong = false;
     return plan_interp(ntl, sp, times, time, s0, d, dreg, tr.pt,
-                       op == CG_OP_ENO || op == CG_OP_TVD);
+                       eno_group);
   }
   if (coarser) {
     if (op == CG_OP_SYNC || op == CG_OP_RESTRICT)
@@ -188,49 +258,25 @@ int resolve_transfer(const cg_gdata *dst, const cg_gdata *const *srcs,
     }
     if (ntl > 3) {
       // every level prolongated into a temporary, then combined by
-      // interpolate_3d_4tl/5tl (data.cc:497-521, 1227-1256): validated here
-      // with a stand-in temporary, run by run_transfer
+      // interpolate_3d_4tl/5tl (data.cc:497-521, 1227-1256)
       if (es != 8)
         return set_error(CG_ERR_OPERATOR, "mixed precision with more than 3 time levels");
       double tw[5];
       for (int l = 0; l < ntl; ++l)
         tw[l] = times[tl0 + l];
-      double sfx[5];
-      if ((e = time_factors(ntl, tw, time, sfx)))
-        return e;
-      Arr tmp;
-      tmp.p = s0.p; // stand-in address, never dereferenced
-      for (int k = 0; k < 3; ++k)
-        tmp.pad[k] = tmp.ext[k] = dreg.sizes(k);
-      tmp.box = dreg;
-      const double one[1] = {1.0};
-      if ((e = plan_prolong(kind, order, s0, sp, 1, one, tmp, dreg, tr.pr, 8, nullptr, stagger_dirs(op))))
-        return e;
-      PointPlan ip;
-      if ((e = plan_interp(ntl, sp, tw, time, tmp, d, dreg, ip)))
-        return e;
-      tr.is_prolong = true;
-      tr.tmp_ntl = ntl;
+      tr.tmp_step = 0;
       tr.tmp_kind = kind;
       tr.tmp_order = order;
       tr.tmp_stag = stagger_dirs(op);
-      tr.tmp_src = s0;
-      tr.tmp_dst = d;
-      tr.tmp_reg = dreg;
-      for (int l = 0; l < ntl; ++l) {
-        tr.tmp_srcs[l] = sp[l];
-        tr.tmp_times[l] = tw[l];
-      }
-      tr.tmp_time = time;
-      tr.pr.dst_points = dreg.size();
-      return CG_OK;
+      return setup_tmp(tr, ntl, sp, tw, time, s0, d, sreg, dreg);
     }
     double sf[3] = {1.0, 0.0, 0.0};
     EnoTime ew;
-    const bool eno_time = ntl == 3 && (op == CG_OP_ENO || op == CG_OP_TVD);
+    const bool eno_time = ntl == 3 && eno_group;
     if (ntl > 1) {
       // each level prolongated, then combined in time (data.cc:497-521):
-      // ENO/TVD groups with 3 levels by interpolate_eno_3d_3tl, the others
+      // ENO/WENO/TVD/monotone groups with 3 levels by
+      // interpolate_eno_3d_3tl, the others
       // by interpolate_3d_2tl/3tl (data.cc:1182-1299)
       if (eno_time) {
         if ((e = eno_time_factors(times, time, ew)))
@@ -248,72 +294,74 @@ int resolve_transfer(const cg_gdata *dst, const cg_gdata *const *srcs,
   if (finer) {
     if (op == CG_OP_SYNC)
       return set_error(CG_ERR_OPERATOR, "op_sync cannot restrict");
-    if (ntl != 1)
-      return set_error(CG_ERR_OPERATOR, "restriction with time interpolation is not provided");
     // the staggered groups restrict with their own operators whatever the
     // centring (data.cc:989-1027)
-    if (stagger_dirs(op)) {
+    const int stag = stagger_dirs(op);
+    int cell = 0, rorder = 1;
+    if (stag) {
       if (slab)
         return set_error(CG_ERR_ARG, "staggered restriction takes no slab");
-      tr.is_prolong = false;
-      return plan_restrict_stagger(stagger_dirs(op), s0, d, dreg, tr.pt);
-    }
-    if (cent == CG_VERTEX_CENTERED) {
+    } else if (cent == CG_VERTEX_CENTERED) {
       if (slab)
         return set_error(CG_ERR_ARG, "vertex-centred restriction takes no slab");
-      tr.is_prolong = false;
-      return plan_restrict(false, s0, d, dreg, tr.pt);
+    } else {
+      if (slab && !(slab->is_centered[0] == 1 && slab->is_centered[1] == 1 &&
+                    slab->is_centered[2] == 1))
+        return set_error(CG_ERR_OPERATOR, "restrict_3d_vc_rf2 is not provided");
+      // data.cc:1064-1094: higher-order restriction when the parameter asks
+      // for it, except for ENO and WENO groups
+      int use_ho = 0, ro = 1;
+      restriction_setting(&use_ho, &ro);
+      cell = 1;
+      rorder = use_ho && op != CG_OP_ENO && op != CG_OP_WENO ? ro : 1;
     }
-    if (slab && !(slab->is_centered[0] == 1 && slab->is_centered[1] == 1 &&
-                  slab->is_centered[2] == 1))
-      return set_error(CG_ERR_OPERATOR, "restrict_3d_vc_rf2 is not provided");
-    // data.cc:1064-1094: higher-order restriction when the parameter asks
-    // for it, except for ENO and WENO groups
-    int use_ho = 0, rorder = 1;
-    restriction_setting(&use_ho, &rorder);
+    if (ntl > 1) {
+      // every level restricted into a temporary, then interpolated in time
+      // (data.cc:497-521)
+      double tw[5];
+      for (int l = 0; l < ntl; ++l)
+        tw[l] = times[tl0 + l];
+      tr.tmp_step = 2;
+      tr.tmp_stag = stag;
+      tr.tmp_cell = cell;
+      tr.tmp_rorder = rorder;
+      tr.tmp_eno = ntl == 3 && eno_group;
+      return setup_tmp(tr, ntl, sp, tw, time, s0, d, sreg, dreg);
+    }
     tr.is_prolong = false;
-    return plan_restrict(true, s0, d, dreg, tr.pt,
-                         use_ho && op != CG_OP_ENO && op != CG_OP_WENO ? rorder : 1);
+    if (stag)
+      return plan_restrict_stagger(stag, s0, d, dreg, tr.pt);
+    return plan_restrict(cell != 0, s0, d, dreg, tr.pt, rorder);
   }
   return set_error(CG_ERR_STRIDE, "source and destination strides are not ordered");
 }
 
-// 4 or 5 time levels: region-sized temporaries from a stream-ordered
-// allocation, one single-level prolongation per level, the interpolation,
-// then the temporaries are released in stream order
+// transfers through temporaries: region-sized temporaries from a
+// stream-ordered allocation, one single-level transfer per time level into
+// them, the interpolation in time, then the temporaries are released in
+// stream order
 static int run_transfer_tmp(const Transfer &tr, hipStream_t st) {
-  const Box &reg = tr.tmp_reg;
-  const long long npts = reg.size();
+  const long long npts = tr.tmp_reg.size();
   double *tmp = nullptr;
   hipError_t he = hipMallocAsync(reinterpret_cast<void **>(&tmp), (size_t)tr.tmp_ntl * npts * sizeof(double), st);
   if (he != hipSuccess)
     return hip_error(he, "hipMallocAsync (time-level temporaries)");
   int e = CG_OK;
   const void *tp[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
-  Arr ta;
-  for (int k = 0; k < 3; ++k)
-    ta.pad[k] = ta.ext[k] = reg.sizes(k);
-  ta.box = reg;
-  const double one[1] = {1.0};
   for (int l = 0; l < tr.tmp_ntl && !e; ++l) {
-    ta.p = tmp + l * npts;
-    tp[l] = ta.p;
-    ProlongPlan pl;
-    const void *s1[1] = {tr.tmp_srcs[l]};
-    e = plan_prolong(tr.tmp_kind, tr.tmp_order, tr.tmp_src, s1, 1, one, ta, reg, pl, 8, nullptr, tr.tmp_stag);
+    tp[l] = tmp + l * npts;
+    Transfer one;
+    e = plan_level(tr, tr.tmp_srcs[l], tmp_arr(tr, tp[l]), one);
     if (!e)
-      e = launch_prolong(pl, 0, st);
+      e = run_transfer(one, 0, st);
   }
   if (!e) {
-    ta.p = tmp;
-    PointPlan ip;
-    e = plan_interp(tr.tmp_ntl, tp, tr.tmp_times, tr.tmp_time, ta, tr.tmp_dst, reg, ip);
-    if (!e) {
-      const int r = launch_point(ip.op, ip.a, st);
-      e = r == -1 ? set_error(CG_ERR_ARG, "region too large for the pointwise kernel")
-          : r     ? hip_error(hipGetLastError(), "kernel launch")
-                  : CG_OK;
-    }
+    Transfer ip;
+    std::memset(&ip, 0, sizeof ip);
+    e = plan_interp(tr.tmp_ntl, tp, tr.tmp_times, tr.tmp_time, tmp_arr(tr, tmp), tr.tmp_dst, tr.tmp_reg, ip.pt,
+                    tr.tmp_eno);
+    if (!e)
+      e = run_transfer(ip, 0, st);
   }
   he = hipFreeAsync(tmp, st);
   if (!e && he != hipSuccess)

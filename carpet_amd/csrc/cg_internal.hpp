@@ -136,14 +136,20 @@ struct Transfer {
   bool is_prolong;
   PointPlan pt;
   ProlongPlan pr;
-  // prolongation with 4 or 5 time levels (order_time 3, 4): as in
-  // data.cc:497-521 each level is prolongated into a region-sized temporary
-  // (stream-ordered allocation), then interpolate_3d_4tl/5tl combines them;
-  // per call only (the batch refuses it)
-  int tmp_ntl;
-  int tmp_kind, tmp_order, tmp_stag;
+  // transfers through temporaries, as data<T>::transfer_time does them
+  // (data.cc:497-521): each time level is prolongated (4 or 5 levels),
+  // copied through a slab or restricted into a region-sized temporary
+  // (stream-ordered allocation), then the temporaries are interpolated in
+  // time; per call only (the batch refuses them)
+  int tmp_ntl;  // 0: not such a transfer
+  int tmp_step; // 0 prolongation, 1 slabbed copy, 2 restriction
+  int tmp_kind, tmp_order, tmp_stag; // prolongation kind; staggered directions
+  int tmp_cell, tmp_rorder;          // restriction: cell-centred, order
+  bool tmp_has_slab;
+  cg_islab tmp_slab;
+  bool tmp_eno; // interpolate_eno_3d_3tl for the 3 levels
   Arr tmp_src, tmp_dst;
-  Box tmp_reg;
+  Box tmp_sreg, tmp_reg;
   const void *tmp_srcs[5];
   double tmp_times[5], tmp_time;
   long long dst_points() const { return is_prolong ? pr.dst_points : pt.dst_points; }

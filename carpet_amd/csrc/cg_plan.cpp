@@ -205,6 +205,12 @@ int plan_prolong(int kind, int order, const Arr &src, const void *const *srcs,
     }
     p.stag = stag;
     p.sorder = order;
+    if (eno && ntl == 3) { // monotone groups: interpolate_eno_3d_3tl (data.cc:1260-1292)
+      p.teno = 1;
+      p.l0 = eno->l0;
+      p.s2[0] = eno->s2[0];
+      p.s2[1] = eno->s2[1];
+    }
     pl.dst_points = (long long)p.n[0] * p.n[1] * p.n[2];
     pl.src_points = srcpts;
     return CG_OK;

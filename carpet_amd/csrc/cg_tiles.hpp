@@ -633,7 +633,7 @@ __device__ __forceinline__ double point_value(const PointArgs &a, int i, int j, 
         out = time_combine<2>(v, a.s, a.s2, a.l0, 0);
       } else {
         v[2] = mono5_value(a.src[2], a.sj, a.sk, gi, gj, gk);
-        out = time_combine<3>(v, a.s, a.s2, a.l0, 0);
+        out = time_combine<3>(v, a.s, a.s2, a.l0, a.teno);
       }
     }
   } else if constexpr (OP == PO_STAGGER) {

@@ -1693,9 +1693,11 @@ int orc_prolongate_time(int kind, int order, const double *const *srcs,
   const double *tp[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   for (int l = 0; l < ntl; ++l)
     tp[l] = tmp[l].data();
-  // ENO and TVD groups interpolate 3 time levels with interpolate_eno_3d_3tl
+  // ENO, WENO, TVD and monotone groups interpolate 3 time levels with
+  // interpolate_eno_3d_3tl
   // (data.cc:1282-1292), Lagrange with interpolate_3d_2tl/3tl
-  if (ntl == 3 && (kind == 2 || kind == 3 || kind == 4))
+  // (kinds 2 cell ENO, 3 TVD, 4 vertex ENO/WENO, 5 monotone: data.cc:1260-1292)
+  if (ntl == 3 && (kind == 2 || kind == 3 || kind == 4 || kind == 5))
     return orc_interpolate_eno_3d_3tl(tp, times, text, text, dst, time, dpad, dext, drb, db,
                                       drb, drb);
   return orc_interpolate_3d(ntl, tp, times, text, text, dst, time, dpad, dext,

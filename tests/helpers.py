@@ -28,7 +28,7 @@ def windows(kind: str, order: int):
         return w[order]
     if kind == "tvd":
         return [(-1, 1), (-1, 1)]
-    if kind == "veno":  # vertex ENO: b for coincident, b-1..b+2 for midpoints
+    if kind in ("veno", "weno"):  # vertex ENO: b for coincident, b-1..b+2 for midpoints
         return [(0, 0), (-1, 2)]
     if kind == "mono":  # monotone order 5: b, or b-2..b+3
         return [(0, 0), (-2, 3)]

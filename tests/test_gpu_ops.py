@@ -200,7 +200,8 @@ KIND_IDS = {"vlag": (orc.KIND_VLAG, cg.OP_LAGRANGE, cg.VERTEX_CENTERED),
             "eno": (orc.KIND_ENO, cg.OP_ENO, cg.CELL_CENTERED),
             "tvd": (orc.KIND_TVD, cg.OP_TVD, cg.CELL_CENTERED),
             "veno": (orc.KIND_VENO, cg.OP_ENO, cg.VERTEX_CENTERED),
-            "mono": (orc.KIND_MONO5, cg.OP_LAGRANGE_MONOTONE, cg.VERTEX_CENTERED)}
+            "mono": (orc.KIND_MONO5, cg.OP_LAGRANGE_MONOTONE, cg.VERTEX_CENTERED),
+            "weno": (orc.KIND_VENO, cg.OP_WENO, cg.VERTEX_CENTERED)}
 
 
 def test_interpolate_eno_3d_3tl():
@@ -232,7 +233,7 @@ def test_interpolate_eno_3d_3tl():
         _check(exp, ddst, f"interp_eno case {c}")
 
 
-@pytest.mark.parametrize("kind,order,ntl", [("mono", 5, 3), ("mono", 5, 2), ("mono", 5, 1),
+@pytest.mark.parametrize("kind,order,ntl", [("mono", 5, 3), ("mono", 5, 2), ("mono", 5, 1), ("weno", 3, 3),
                                             ("veno", 3, 3), ("veno", 3, 2), ("veno", 3, 1),
                                             ("vlag", 5, 3), ("vlag", 3, 2), ("vlag", 1, 3),
                                             ("clag", 3, 3), ("eno", 3, 2), ("eno", 2, 2),
@@ -240,12 +241,12 @@ def test_interpolate_eno_3d_3tl():
 def test_transfer_time_interpolation(kind, order, ntl):
     """cg_transfer_from with time interpolation (one fused kernel) against the
     reference's shape (prolongate every level into a temporary, then
-    interpolate_3d_2tl/3tl, or interpolate_eno_3d_3tl for ENO/TVD groups with
+    interpolate_3d_2tl/3tl, or interpolate_eno_3d_3tl for ENO/WENO/TVD/monotone groups with
     3 levels), data.cc:497-521, 1182-1299; also through a batch."""
     rng = np.random.default_rng(31 * order + ntl)
     okind, op, cent = KIND_IDS[kind]
     # ENO order_space 5 -> ENO<3>, 2..4 -> ENO<2> (data.cc:697-704)
-    order_space = {("eno", 3): 5, ("eno", 2): 3, ("veno", 3): 5}.get((kind, order), order)
+    order_space = {("eno", 3): 5, ("eno", 2): 3, ("veno", 3): 5, ("weno", 3): 5}.get((kind, order), order)
     for c in range(4):
         print(f"transfer {kind} o{order} ntl{ntl} #{c}", flush=True)
         sb, db, rb = prolong_case(rng, kind, order, n=(70, 20, 40) if c == 0 else None)
@@ -641,3 +642,104 @@ def test_transfer_4tl_5tl_prolongation(kind, order, ntl):
         with pytest.raises(cg.CarpetError):
             cg.transfer_from(ddst, ds, times, cg.Box(*rb), cg.Box(*rb), t, cg.OP_TVD if kind == "clag" else cg.OP_ENO,
                              cent, 1 if kind == "clag" else 5, ntl - 1)
+
+
+# ---- restriction and slabbed copies with time interpolation ----------------
+def _tmp_interp(srcs, times, t, rb, per_level, eno):
+    """The reference's transfer_time (data.cc:497-521): each level into a
+    region-sized temporary, then interpolated in time."""
+    tmps = []
+    for h in srcs:
+        tmp = host_array(rb, sizes(rb))
+        assert per_level(h, tmp) == 0
+        tmps.append(tmp)
+
+    def finish(exp):
+        if eno:
+            return orc.interpolate_eno_3d_3tl(tmps, times, exp, t, rb)
+        return orc.interpolate_3d(tmps, times, exp, t, rb)
+    return finish
+
+
+@pytest.mark.parametrize("what,op,ntl", [("vertex", cg.OP_LAGRANGE, 2), ("vertex", cg.OP_LAGRANGE, 3),
+                                         ("cell", cg.OP_LAGRANGE, 3), ("cell", cg.OP_ENO, 3),
+                                         ("cell", cg.OP_LAGRANGE, 5), ("stagger", cg.OP_STAGGER011, 3)])
+def test_restriction_with_time_interpolation(what, op, ntl):
+    """Restriction between levels at a time between the source levels:
+    every level restricted into a temporary, then interpolated in time
+    (interpolate_eno_3d_3tl for ENO groups), bit for bit against the oracle
+    replay of that shape; the batch refuses it."""
+    rng = np.random.default_rng(500 + ntl + 7 * len(what) + op)
+    times = [0.5, 0.25, 0.0, -0.25, -0.5][:ntl]
+    cent = cg.CELL_CENTERED if what == "cell" else cg.VERTEX_CENTERED
+    if what == "stagger":
+        cg.set_staggered_operators(True)
+    try:
+        for c in range(3):
+            n = (1, 1, 1) if c == 0 else tuple(int(v) for v in rng.integers(2, 20, size=3))
+            print(f"restrict+time {what} op {op} ntl {ntl} #{c} n={n}", flush=True)
+            if what == "cell":
+                sb, db, rb = _cc_restrict_case(rng, 0, n)
+                per = orc.restrict_3d_cc_rf2
+            else:
+                sb, db, rb = _stagger_restrict_case(rng, n)
+                per = (lambda s, d, r: orc.restrict_3d_stagger(cg.STAGGER011, s, d, r)) if what == "stagger" \
+                    else orc.restrict_3d_rf2
+            pad = pad_of(sizes(sb), rng)
+            srcs = []
+            for _ in range(ntl):
+                h = host_array(sb, pad)
+                h.view()[...] = field(rng, sizes(sb), "random")
+                srcs.append(h)
+            t = min(times) + [0.3, 0.7, 0.1][c] * (max(times) - min(times))
+            dst = host_array(db, pad_of(sizes(db), rng))
+            exp = orc.HostArr(db, dst.data.copy())
+            finish = _tmp_interp(srcs, times, t, rb, lambda h, tmp: per(h, tmp, rb), ntl == 3 and op == cg.OP_ENO)
+            assert finish(exp) == 0
+            ds = [_dev(h) for h in srcs]
+            ddst = _dev(dst)
+            cg.transfer_from(ddst, ds, times, cg.Box(*rb), cg.Box(*rb), t, op, cent, 3, ntl - 1)
+            _check(exp, ddst, f"restrict+time {what} case {c}")
+        b = cg.Batch()
+        with pytest.raises(cg.CarpetError):
+            b.add(0, ddst, ds, times, cg.Box(*rb), cg.Box(*rb), t, op, cent, 3, ntl - 1)
+        b.close()
+    finally:
+        cg.set_staggered_operators(False)
+
+
+@pytest.mark.parametrize("ntl", [2, 3])
+def test_slabbed_copy_with_time_interpolation(ntl):
+    """A copy with a slab offset at a time between the source levels: each
+    level copied through the slab into a temporary, then interpolated."""
+    rng = np.random.default_rng(620 + ntl)
+    times = [1.0, 0.0, -1.0][:ntl]
+    for c in range(3):
+        print(f"slab copy+time ntl {ntl} #{c}", flush=True)
+        n = tuple(int(v) for v in rng.integers(1, 30, size=3))
+        rlo = tuple(int(v) for v in rng.integers(-10, 10, size=3))
+        rup = tuple(l + k - 1 for l, k in zip(rlo, n))
+        off = tuple(int(v) for v in rng.integers(-3, 4, size=3))
+        srlo = tuple(l + o for l, o in zip(rlo, off))
+        srup = tuple(u + o for u, o in zip(rup, off))
+        sb = (tuple(l - int(rng.integers(0, 3)) for l in srlo), tuple(u + int(rng.integers(0, 3)) for u in srup),
+              (1,) * 3)
+        db = (tuple(l - int(rng.integers(0, 3)) for l in rlo), tuple(u + int(rng.integers(0, 3)) for u in rup),
+              (1,) * 3)
+        rb, srb = (rlo, rup, (1,) * 3), (srlo, srup, (1,) * 3)
+        pad = pad_of(sizes(sb), rng)
+        srcs = []
+        for _ in range(ntl):
+            h = host_array(sb, pad)
+            h.view()[...] = field(rng, sizes(sb), "random")
+            srcs.append(h)
+        t = min(times) + [0.4, 0.15, 0.9][c] * (max(times) - min(times))
+        dst = host_array(db, pad_of(sizes(db), rng))
+        exp = orc.HostArr(db, dst.data.copy())
+        finish = _tmp_interp(srcs, times, t, rb, lambda h, tmp: orc.copy_3d(h, tmp, srb, rb, off), False)
+        assert finish(exp) == 0
+        ds = [_dev(h) for h in srcs]
+        ddst = _dev(dst)
+        cg.transfer_from(ddst, ds, times, cg.Box(*rb), cg.Box(*srb), t, cg.OP_LAGRANGE, cg.VERTEX_CENTERED, 1,
+                         ntl - 1, slab_offset=off)
+        _check(exp, ddst, f"slab copy+time case {c}")
