@@ -415,8 +415,10 @@ int batch_device_build(BatchDevice &dev, const std::vector<Transfer> &tr,
       l.pt = 256;
     // TVD items count coarse parent cells (8 fine points each): 1024 cells
     // (measured on 8 variables of 192^3 -> 384^3: 0.98 ms against 1.05 ms
-    // with 4096 and 1.11 ms with 256)
-    else if (g.op == PO_TVD)
+    // with 4096 and 1.11 ms with 256); cell-centred restriction (8 or more
+    // children per point) likewise 1024 points (the 384^3 -> 192^3
+    // restriction of the same set: 0.77 ms against 0.81 ms with 4096)
+    else if (g.op == PO_TVD || g.op == PO_RESTRICT_CC || g.op == PO_RESTRICT_CC_O3 || g.op == PO_RESTRICT_CC_O5)
       l.pt = 1024;
     // tensor tiles of large regions (median 2^20 points and up): persistent
     // grid (its workgroups take the tiles in order, so neighbouring tiles
