@@ -98,6 +98,44 @@ def test_prolongate_parity(kind, order):
         _check(exp, ddst, f"{kind} o{order} case {c} reg={rb}")
 
 
+@pytest.mark.parametrize("order", [2, 3])
+@pytest.mark.parametrize("shape", ["sum", "product"])
+def test_eno_kinked_field_parity(order, shape):
+    """The kinked fields that pin the oracle's ENO stencil choice
+    (test_oracle_eno_pins): g(x)+h(y)+k(z) and g(x)*h(y)*k(z), each axis a
+    polynomial with kinks so that neighbouring fine points select left,
+    centred and right stencils.  Random padding and region offsets; the HIP
+    operator must equal the oracle bit for bit."""
+    from fractions import Fraction as F
+
+    from test_oracle_eno_pins import kinked_axis
+
+    rng = np.random.default_rng(70 + order + 3 * (shape == "product"))
+    n = 40
+    axes = [kinked_axis(n, order, [(F(2 * int(rng.integers(6, n - 6)) + 1, 2) + F(1, 2), +1),
+                                   (F(2 * int(rng.integers(6, n - 6)) + 1, 2) - F(1, 2), -1)])
+            for _ in range(3)]
+    gx, gy, gz = (np.array([float(v) for v in a]) for a in axes)
+    gx, gy, gz = gx / gx.max(), gy / gy.max(), gz / gz.max()
+    if shape == "sum":
+        vol = gz[:, None, None] + gy[None, :, None] + gx[None, None, :]
+    else:
+        vol = gz[:, None, None] * gy[None, :, None] * gx[None, None, :]
+    sb = ((2, 2, 2), (2 + 4 * (n - 1),) * 3, (4, 4, 4))
+    src = host_array(sb, pad_of(sizes(sb), rng))
+    src.view()[...] = vol
+    lo = [int(rng.integers(6, 9)) for _ in range(3)]
+    hi = [2 * n - int(rng.integers(7, 10)) for _ in range(3)]
+    rb = (tuple(1 + 2 * v for v in lo), tuple(1 + 2 * v for v in hi), (2, 2, 2))
+    db = (tuple(1 + 2 * (v - 1) for v in lo), tuple(1 + 2 * (v + 1) for v in hi), (2, 2, 2))
+    dst = host_array(db, pad_of(sizes(db), rng))
+    exp = orc.HostArr(db, dst.data.copy())
+    assert orc.prolongate_3d_cc_eno_rf2(order, src, exp, rb) == 0
+    dsrc, ddst = _dev(src), _dev(dst)
+    cg.prolongate_3d_cc_eno_rf2(order, dsrc, ddst, cg.Box(*rb))
+    _check(exp, ddst, f"eno{order} kinked {shape} reg={rb}")
+
+
 @pytest.mark.parametrize("cell", [False, True])
 def test_restrict_parity(cell):
     rng = np.random.default_rng(7 + cell)

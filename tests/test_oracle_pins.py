@@ -194,6 +194,105 @@ def test_cell_lagrange_polynomial_exactness(order):
     assert en <= 8 * gold + 8 * EPS * 100, (en, gold)
 
 
+def _point_value(kind, order, x, y, z, powers, dx=0.02):
+    """The oracle's prolongated value at one fine point of the
+    CarpetProlongateTest grid (faces at -1 + k*dx, refined level dx/2) from
+    the polynomial on the coarse level; returns (value, max |coarse value| in
+    the block read, fine indices)."""
+    if kind == "vlag":  # vertex: coarse point c at c*dx, fine point f at f*dx/2
+        f = [round(v / (dx / 2)) for v in (x, y, z)]
+        h = order // 2 + 2
+        lo = [(fi >> 1) - h for fi in f]
+        hi = [(fi >> 1) + h + 1 for fi in f]
+        sb = (tuple(2 * v for v in lo), tuple(2 * v for v in hi), (2, 2, 2))
+        xs = [dx * np.arange(a, b + 1) for a, b in zip(lo, hi)]
+        rb = (tuple(f), tuple(f), (1, 1, 1))
+        op = orc.prolongate_3d_rf2
+    else:  # cell: coarse cell c centred at -1+dx/2+c*dx, fine cell f at -1+dx/4+f*dx/2
+        f = [round((v + 1 - dx / 4) / (dx / 2)) for v in (x, y, z)]
+        h = order // 2 + 3
+        lo = [(fi >> 1) - h for fi in f]
+        hi = [(fi >> 1) + h for fi in f]
+        sb = (tuple(2 + 4 * v for v in lo), tuple(2 + 4 * v for v in hi), (4, 4, 4))
+        xs = [-1 + dx / 2 + dx * np.arange(a, b + 1) for a, b in zip(lo, hi)]
+        rb = (tuple(1 + 2 * v for v in f),) * 2 + ((2, 2, 2),)
+        op = orc.prolongate_3d_cc_rf2
+    src = host_array(sb, sizes(sb))
+    Z, Y, X = np.meshgrid(xs[2], xs[1], xs[0], indexing="ij")
+    src.view()[...] = density_sum(X + 1, Y + 1, Z + 1, powers)
+    dst = host_array(rb, (1, 1, 1))
+    assert op(order, src, dst, rb) == 0
+    return float(dst.view()[0, 0, 0]), float(np.abs(src.view()).max()), f
+
+
+@pytest.mark.parametrize("test", ["test_o1", "test_o3", "test_o5", "test_o7", "test_o9", "test_o11",
+                                  "test_cc_o1", "test_cc_o2", "test_cc_o3", "test_cc_o4", "test_cc_o5",
+                                  "test_cc_horest_o3", "test_cc_horest_o5"])
+def test_lagrange_matches_golden_du_samples(test):
+    """Every stored du = u - exact sample of the refined level: the oracle's
+    value at that grid point, minus the exact polynomial, agrees with the
+    reference's du to within the rounding of the two evaluations (16 ulps of
+    the largest coarse value the stencil reads).  The samples are the points
+    where the reference's rounding error was not zero, so they must be
+    interpolated points: no vertex sample may sit on a coincident point,
+    where the operator copies.  Rounding bound of each side: one rounding per
+    operation of the nested sums (3(O+1)) and of the reference's 3-level time
+    interpolation (6), each growing with the cube of the 1-D weights'
+    absolute sum L, on the largest coarse value read."""
+    cell = test.startswith("test_cc")
+    order = int(test[-1]) if cell else int(test[len("test_o"):])
+    if cell:  # cell-centred: nodes at the coarse centres nearest x = 1/4
+        nodes = np.arange(order + 1) - (order + 1) // 2 + 0.5
+        x0 = 0.25
+    else:  # vertex-centred midpoint
+        nodes = np.arange(order + 1) - (order - 1) // 2
+        x0 = 0.5
+    L = float(np.sum(np.abs(_lagrange_weights(nodes, x0))))
+    ops = 3 * (order + 1) + 6
+    samples = GOLD[test]["samples"]
+    assert samples
+    coincident = 0
+    for s in samples:
+        v, scale, f = _point_value("clag" if cell else "vlag", order, s["x"], s["y"], s["z"], (order,) * 3)
+        du = v - density_sum(s["x"] + 1, s["y"] + 1, s["z"] + 1, (order,) * 3)
+        assert abs(du - s["du"]) <= 2 * ops * L**3 * np.spacing(scale), (s, du)
+        coincident += (not cell) and all(fi % 2 == 0 for fi in f)
+    # test_o9 stores one coincident sample (|du| = 1 ulp: the reference's
+    # time interpolation of three equal levels); none of the others do
+    assert coincident <= (1 if test == "test_o9" else 0)
+
+
+@pytest.mark.parametrize("p", range(6))
+def test_cc_restrict_matches_golden_errornorm(p):
+    """restrict_3d_cc_rf2 on the degree-p CarpetProlongateTest polynomial:
+    the coarse cells under the refined level (centres within +-0.19, dx
+    0.02) get the mean of their 8 children, whose error against the exact
+    value is the test_cc_rest_o<p> errornorm (0 for p=0, rounding for p=1,
+    a second-derivative term up to 0.083 for p=5).  A wrong weight, child or
+    region changes it."""
+    dx, c0, c1 = 0.02, 40, 59  # coarse cells 40..59: centres -0.19 .. 0.19
+    f0, f1 = 2 * c0, 2 * c1 + 1
+    sb = ((1 + 2 * f0,) * 3, (1 + 2 * f1,) * 3, (2, 2, 2))
+    rb = ((2 + 4 * c0,) * 3, (2 + 4 * c1,) * 3, (4, 4, 4))
+    xf = -1 + dx / 4 + dx / 2 * np.arange(f0, f1 + 1)
+    xc = -1 + dx / 2 + dx * np.arange(c0, c1 + 1)
+    assert abs(xc[0] + 0.19) < 1e-12 and abs(xc[-1] - 0.19) < 1e-12
+    Z, Y, X = np.meshgrid(xf, xf, xf, indexing="ij")
+    src = host_array(sb, sizes(sb))
+    src.view()[...] = density_sum(X + 1, Y + 1, Z + 1, (p,) * 3)
+    dst = host_array(rb, sizes(rb))
+    assert orc.restrict_3d_cc_rf2(src, dst, rb) == 0
+    Zc, Yc, Xc = np.meshgrid(xc, xc, xc, indexing="ij")
+    en = float(np.max(np.abs(dst.view() - density_sum(Xc + 1, Yc + 1, Zc + 1, (p,) * 3))))
+    gold = GOLD[f"test_cc_rest_o{p}"]["errornorm"]
+    # both errors are differences of values up to max|u|: a few of its ulps
+    tol = 16 * np.spacing(float(np.abs(src.view()).max()))
+    if gold > 1e-10:
+        assert abs(en - gold) <= tol, (en, gold, tol)
+    else:
+        assert en <= gold + tol, (en, gold, tol)
+
+
 # ---- brute force: tensor-product Lagrange through the nearest nodes -------
 def _lagrange_weights(nodes, x):
     w = []
