@@ -1,19 +1,23 @@
 """Benchmark of the transfer operators on one MI355X.
 
-Default workload (the single-large-patch configuration): 24 FP64
-vertex-centred grid functions, coarse 257^3 (+3 ghosts) with 3 time levels
-at t = 1, 0, -1, prolongated by prolongate_3d_rf2<5> fused with
-interpolate_3d_3tl onto the fine region [2,513]^3 at t = 1/3 and 2/3.
-One step = all 48 (grid function, target time) transfers of the level step.
+Default workload (the batched uneven level step): 2000 regions per grid
+function with edges uniform in [8,96] fine points per axis, spread over 5
+refinement levels, 24 FP64 vertex-centred grid functions x 3 time levels;
+a third of the regions are copies (a component's box from a neighbour's
+array, copy_3d), a third order-5 Lagrange prolongation fused with 3-level
+time interpolation, a third rf2 restriction.  One step = all 48000
+(region, grid function) transfers of the level step.
 
-The batched path (one launch over the device descriptor list, a workgroup
-per tile column, replayed as a captured hipGraph) gives `value`; the plain
-path (one launch per transfer, the reference's per-region call shape) is
-timed beside it.
-Kernel time is measured with HIP events on the stream the work runs on.
-`cpu_baseline` times the CPU oracle (the reference's algorithm restated,
-prolongating every time level into a temporary and interpolating in time,
-OpenMP over the host cores) on a bounded sample of the same workload.
+The batched path (one launch per phase and kernel class over the
+device-resident descriptor list, replayed as a captured hipGraph) gives
+`value`; the plain path (one launch per transfer, the reference's
+per-region call shape) is timed beside it.  Kernel times are measured with
+HIP events on the stream the work runs on; the dominant launch of the step
+gives `roofline`.  `cpu_baseline` times the CPU oracle (the reference's
+algorithm restated, OpenMP over the host cores) on a bounded sample of the
+same transfers, in the reference's shape (one operator call per region,
+time interpolation through temporaries) and fused (one parallel loop over
+all regions, time interpolation in place).
 """
 from __future__ import annotations
 
@@ -142,12 +146,39 @@ def run_config2(steps: int, warmup: int, n_gf: int, plain_steps: int, tune_cache
     return cfg, stats, ms_step, wall, ms_plain
 
 
+def _launch_times(b, st, steps: int):
+    """Per-launch ms (HIP events on st around each launch run on its own,
+    averaged over steps) with each launch's phase, points and algorithmic
+    bytes."""
+    import carpet_amd as cg
+
+    n = b.launch_count()
+    ev = [cg.Event() for _ in range(n + 1)]
+    tot = [0.0] * n
+    for _ in range(steps):
+        ev[0].record(st)
+        for i in range(n):
+            b.run_launch(i, st)
+            ev[i + 1].record(st)
+        st.synchronize()
+        for i in range(n):
+            tot[i] += ev[i].elapsed_ms(ev[i + 1])
+    out = []
+    for i in range(n):
+        ph, pts, alg = b.launch_stats(i)
+        out.append({"launch": i, "phase": ph, "ms": round(tot[i] / steps, 4), "dst_points": pts,
+                    "algorithmic_bytes": alg, "gbytes_per_s": round(alg / (tot[i] / steps) / 1e6, 1)})
+    return out
+
+
 def run_config4(n_regions: int, n_gf: int, steps: int, warmup: int, plain_steps: int,
-                tune_cache=None):
-    """Batched uneven hierarchy: n_regions regions (copy sync / O=5 prolongation
+                tune_cache=None, cpu_seconds: float = 0.0):
+    """Batched uneven level step: n_regions regions (copies / O=5 prolongation
     with 3 time levels / rf2 restriction, edges uniform in [8,96]) per grid
-    function; batched graph replay vs one launch per transfer, and the two
-    paths' outputs compared bit for bit on a sample of destinations."""
+    function; batched graph replay vs one launch per transfer, per-launch
+    kernel times, the step advancing in time through the per-step
+    parameter path, and the two paths' outputs compared bit for bit on a
+    sample of destinations."""
     import numpy as np
 
     import carpet_amd as cg
@@ -156,14 +187,17 @@ def run_config4(n_regions: int, n_gf: int, steps: int, warmup: int, plain_steps:
     w = config4(n_regions=n_regions, n_gf=n_gf)
     t0 = time.perf_counter()
     dev = w.allocate(pool=True)
+    pool = getattr(w, "pool_bytes", None)
+    print(f"# config4: pool [{w._pool:#x}, {w._pool + pool:#x}) {pool} bytes, {len(w.arrays)} arrays, "
+          f"{w.setup_launches} setup launches", file=sys.stderr, flush=True)
     st = cg.Stream()
     b = w.batch(dev)
     b.finalize()
     if tune_cache:
         b.autotune(st, reps=3, cache_path=tune_cache)
     desc = b.describe()
-    print(f"# config4: {len(w.steps)} transfers, {len(w.arrays)} arrays, set up in "
-          f"{time.perf_counter() - t0:.1f} s\n" + desc, file=sys.stderr, flush=True)
+    print(f"# config4: {len(w.steps)} transfers, set up in {time.perf_counter() - t0:.1f} s\n" + desc,
+          file=sys.stderr, flush=True)
     stats = b.stats()
     b.capture_graph(st)
     for _ in range(warmup):
@@ -176,7 +210,23 @@ def run_config4(n_regions: int, n_gf: int, steps: int, warmup: int, plain_steps:
     e1.record(st)
     st.synchronize()
     ms_b = e0.elapsed_ms(e1) / steps
+    launches = _launch_times(b, st, max(2, steps // 4))
+    print("# config4 launches: " + json.dumps(launches), file=sys.stderr, flush=True)
+    # the step advancing in time: the prolongation phase alternates between
+    # the intermediate fine time (-1/2, 3 levels) and the coarse-aligned time
+    # (0, exact match), new weights uploaded before each replay
+    prol_phase = 1
+    e0.record(st)
+    for k in range(steps):
+        b.set_time(prol_phase, [0.0, -1.0, -2.0], 0.0 if k % 2 else -0.5)
+        b.launch_graph(st)
+    e1.record(st)
+    st.synchronize()
+    ms_retimed = e0.elapsed_ms(e1) / steps
+    b.set_time(prol_phase, [0.0, -1.0, -2.0], -0.5)
     sample = w.outputs[:: max(1, len(w.outputs) // 64)]
+    b.launch_graph(st)
+    st.synchronize()
     got_b = {k: dev[k].download() for k in sample}
     for k in sample:  # back to the initial state (NaN, then the seeded values)
         dev[k].fill(np.nan)
@@ -194,18 +244,83 @@ def run_config4(n_regions: int, n_gf: int, steps: int, warmup: int, plain_steps:
     ms_p = e0.elapsed_ms(e1) / plain_steps
     b.close()
     w.free(dev)
-    pool = getattr(w, "pool_bytes", None)
     desc_bytes = int(desc.split("device descriptor bytes ")[1].split()[0]) if "descriptor bytes" in desc else None
     pts, alg = stats["dst_points"], stats["alg_bytes"]
-    return {"regions": n_regions, "grid_functions": n_gf, "transfers": stats["transfers"],
-            "dst_points_per_step": pts, "algorithmic_bytes_per_step": alg,
-            "resident_array_bytes": pool, "device_descriptor_bytes": desc_bytes,
-            "batched_ms": round(ms_b, 4), "batched_gpoints_per_s": round(pts / ms_b / 1e6, 3),
-            "batched_gbytes_per_s": round(alg / ms_b / 1e6, 1),
-            "batched_roofline_frac": round(alg / ms_b / 1e6 / HBM_PEAK_GBS, 4),
-            "plain_ms": round(ms_p, 4), "plain_gpoints_per_s": round(pts / ms_p / 1e6, 3),
-            "plain_launches_per_step": stats["transfers"], "batched_vs_plain": round(ms_p / ms_b, 3),
-            "batched_equals_plain_on_sample": bool(same), "sample_destinations": len(sample)}
+    out = {"regions": n_regions, "grid_functions": n_gf, "transfers": stats["transfers"],
+           "dst_points_per_step": pts, "algorithmic_bytes_per_step": alg,
+           "resident_array_bytes": pool, "device_descriptor_bytes": desc_bytes,
+           "batched_ms": round(ms_b, 4), "batched_gpoints_per_s": round(pts / ms_b / 1e6, 3),
+           "batched_gbytes_per_s": round(alg / ms_b / 1e6, 1),
+           "batched_roofline_frac": round(alg / ms_b / 1e6 / HBM_PEAK_GBS, 4),
+           "launches": launches,
+           "retimed_step_ms": round(ms_retimed, 4),
+           "plain_ms": round(ms_p, 4), "plain_gpoints_per_s": round(pts / ms_p / 1e6, 3),
+           "plain_launches_per_step": stats["transfers"], "batched_vs_plain": round(ms_p / ms_b, 3),
+           "batched_equals_plain_on_sample": bool(same), "sample_destinations": len(sample)}
+    if cpu_seconds > 0:
+        out["cpu"] = cpu_baseline_config4(w, cpu_seconds)
+    return out
+
+
+def cpu_baseline_config4(w, target_seconds: float):
+    """The oracle on the first regions of the level step (one grid function,
+    the three kinds in turn) until about target_seconds of CPU work per
+    shape: the reference's shape (one operator call per region, each
+    parallelised inside, time interpolation through temporaries) and fused
+    (all regions in one parallel loop, time interpolation in place); both
+    must give the same bits."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    sys.path.insert(0, str(ROOT / "tests"))
+    import numpy as np
+
+    import pyoracle as orc
+    from workload_check import _host
+
+    steps = [t for t in w.steps if t.name.endswith("g0")]
+    # the first transfers, up to about 4e7 destination points
+    pick, pts = [], 0
+    for t in steps:
+        if pts > 4e7:
+            break
+        pick.append(t)
+        pts += t.dstreg.size()
+    names = sorted({t.dst for t in pick} | {s for t in pick for s in t.srcs})
+    from carpet_amd.workloads import hash_field
+    host = {}
+    for k in names:
+        spec = w.arrays[k]
+        e = spec.box.sizes()
+        host[k] = _host(spec, hash_field(spec.seed, e) if spec.seed is not None else np.full(e[::-1], np.nan))
+    tasks = orc.level_step_tasks(pick, host)
+    res = {}
+    outs = {}
+    orc.set_openmp(True)
+    for mode, key in ((0, "reference_shaped"), (1, "fused")):
+        assert orc.replay(mode, tasks) == 0
+        reps, t0 = 0, time.perf_counter()
+        while True:
+            assert orc.replay(mode, tasks) == 0
+            reps += 1
+            el = time.perf_counter() - t0
+            if el >= target_seconds / 2 or reps >= 500:
+                break
+        outs[key] = {t.dst: host[t.dst].data.copy() for t in pick}
+        res[key] = {"value": pts * reps / el / 1e9, "seconds": round(el, 2), "repetitions": reps}
+    same = all(np.array_equal(outs["reference_shaped"][k], outs["fused"][k], equal_nan=True)
+               for k in outs["fused"])
+    kinds = {"copy": 0, "prolongation": 0, "restriction": 0}
+    for t in pick:
+        kinds["copy" if t.phase == 2 else "prolongation" if t.phase == 1 else "restriction"] += 1
+    return {"value": res["reference_shaped"]["value"], "unit": "Gpoints/s", "cores": orc.num_threads(),
+            "kind": "port",
+            "sample": f"oracle replay of the first {len(pick)} transfers of grid function 0 ({kinds}), "
+                      f"{pts} destination points, one operator call per region as the reference does "
+                      f"(OpenMP inside each call, time interpolation through temporaries); "
+                      f"{res['reference_shaped']['repetitions']} repetitions in "
+                      f"{res['reference_shaped']['seconds']} s",
+            "best_cpu_fused": {"value": res["fused"]["value"], "unit": "Gpoints/s",
+                               "repetitions": res["fused"]["repetitions"], "seconds": res["fused"]["seconds"],
+                               "same_bits_as_reference_shaped": bool(same)}}
 
 
 def run_config6(n_gf: int, steps: int, warmup: int, plain_steps: int, ncoarse: int = 128, tune_cache=None):
@@ -406,14 +521,15 @@ def main(argv=None):
     ap.add_argument("--n-gf", type=int, default=24)
     ap.add_argument("--plain-steps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-config4", action="store_true",
-                    help="skip the 2000-region level step reported beside the headline line")
+    ap.add_argument("--no-config2", action="store_true",
+                    help="skip the single-patch prolongation reported beside the headline line")
     ap.add_argument("--no-config6", action="store_true",
                     help="skip the hierarchy level step reported beside the headline line")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--config", type=int, default=2, choices=(2, 3, 4, 5, 6),
-                    help="2: single large patch (default, the headline line); 3: cell-centred conservative "
-                         "set; 4: batched uneven hierarchy; 5: mixed precision vs FP64; 6: level step of a nested hierarchy from its communication schedule")
+    ap.add_argument("--cpu-seconds", type=float, default=16.0)
+    ap.add_argument("--config", type=int, default=4, choices=(2, 3, 4, 5, 6),
+                    help="4: batched uneven level step (default, the headline line); 2: single large patch; "
+                         "3: cell-centred conservative set; 5: mixed precision vs FP64; 6: level step of a "
+                         "nested hierarchy from its communication schedule")
     ap.add_argument("--n-var", type=int, default=8, help="config 3 variables")
     ap.add_argument("--n-regions", type=int, default=2000)
     ap.add_argument("--tune-cache", default=None,
@@ -485,74 +601,78 @@ def main(argv=None):
                              "unit": "GB/s", "frac": r["batched_roofline_frac"], "traffic": None}}
         print(json.dumps(line))
         return 0
-    if args.config == 4:
-        r = run_config4(args.n_regions, args.n_gf, args.steps, args.warmup, args.plain_steps,
-                        args.tune_cache)
-        line = {"metric": METRIC, "value": r["batched_gpoints_per_s"], "unit": "Gpoints/s",
-                "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": r["batched_ms"],
-                "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-                "data": "synthetic",
-                "config": {"workload": f"batched uneven level step: {args.n_regions} regions x {args.n_gf} grid "
-                                       "functions (edges U[8,96]; copy sync, O=5 prolongation + 3 time levels, "
-                                       "rf2 restriction), one batched launch per phase + hipGraph", **r},
-                "gbytes_per_s": r["batched_gbytes_per_s"],
-                "plain": {"ms_per_step": r["plain_ms"], "gpoints_per_s": r["plain_gpoints_per_s"],
-                          "launches_per_step": r["plain_launches_per_step"]},
-                "batched_vs_plain": r["batched_vs_plain"],
-                "roofline": {"bound": "hbm", "achieved": r["batched_gbytes_per_s"], "peak": HBM_PEAK_GBS,
-                             "unit": "GB/s", "frac": r["batched_roofline_frac"], "traffic": None}}
+    if args.config == 2:
+        print(f"# config2: {args.n_gf} grid functions, {args.steps} steps", file=sys.stderr, flush=True)
+        line = config2_line(args)
         print(json.dumps(line))
         return 0
-    print(f"# config2: {args.n_gf} grid functions, {args.steps} steps", file=sys.stderr, flush=True)
+    # default: the batched uneven level step (2000 regions x 24 grid functions)
+    r = run_config4(args.n_regions, args.n_gf, args.steps, args.warmup, args.plain_steps, args.tune_cache,
+                    cpu_seconds=0.0 if args.no_cpu else args.cpu_seconds)
+    dom = max(r["launches"], key=lambda x: x["ms"])
+    cpu = r.pop("cpu", None)
+    line = {"metric": METRIC, "value": r["batched_gpoints_per_s"], "unit": "Gpoints/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": r["batched_ms"],
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic",
+            "config": {"workload": f"batched uneven level step: {args.n_regions} regions x {args.n_gf} grid "
+                                   "functions (edges U[8,96] over 5 refinement levels; whole-box copies, O=5 "
+                                   "prolongation + 3-level time interpolation, rf2 restriction), one launch "
+                                   "per phase and kernel class + hipGraph", **r},
+            "gbytes_per_s": r["batched_gbytes_per_s"],
+            "plain": {"ms_per_step": r["plain_ms"], "gpoints_per_s": r["plain_gpoints_per_s"],
+                      "launches_per_step": r["plain_launches_per_step"]},
+            "batched_vs_plain": r["batched_vs_plain"],
+            "step_roofline_frac": r["batched_roofline_frac"],
+            # the dominant launch of the step (HIP events around each launch)
+            "roofline": {"bound": "hbm", "achieved": dom["gbytes_per_s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(dom["gbytes_per_s"] / HBM_PEAK_GBS, 4),
+                         "traffic": _profile_traffic(f"config4_launch{dom['launch']}"),
+                         "kernel": f"launch {dom['launch']} (phase {dom['phase']})", "ms": dom["ms"],
+                         "algorithmic_bytes": dom["algorithmic_bytes"]}}
+    if not args.no_config2:
+        print("# config2 beside the headline line", file=sys.stderr, flush=True)
+        line["single_patch_512"] = config2_line(args, side=True)
+    if not args.no_config6:
+        print("# config6 beside the headline line", file=sys.stderr, flush=True)
+        line["hierarchy_step_3_levels"] = run_config6(args.n_gf, 10, 2, 1)
+    if cpu is not None:
+        line["cpu_baseline"] = cpu
+    print(json.dumps(line))
+    return 0
+
+
+def config2_line(args, side: bool = False):
     cfg, stats, ms_step, wall, ms_plain = run_config2(args.steps, args.warmup, args.n_gf, args.plain_steps,
                                                       args.tune_cache)
-    print(f"# timed: {ms_step:.3f} ms/step batched, {ms_plain:.3f} ms/step plain", file=sys.stderr, flush=True)
+    print(f"# config2 timed: {ms_step:.3f} ms/step batched, {ms_plain:.3f} ms/step plain", file=sys.stderr,
+          flush=True)
     pts = cfg.dst_points()
     alg = cfg.alg_bytes()
     gpts = pts / (ms_step * 1e-3) / 1e9
     gbs = alg / (ms_step * 1e-3) / 1e9
-    line = {
-        "metric": METRIC,
-        "value": round(gpts, 3),
-        "unit": "Gpoints/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms_step, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
-        "config": {"workload": "single large patch: 24 FP64 vertex-centred grid functions, coarse 257^3 "
-                               "(+3 ghosts, 3 time levels t=1,0,-1) -> fine 512^3, prolongate_3d_rf2<5> fused "
-                               "with interpolate_3d_3tl at t=1/3 and 2/3",
-                   "grid_functions": cfg.n_gf, "fine_points_per_step": pts,
-                   "algorithmic_bytes_per_step": alg,
-                   "path": "one batched launch (resident workgroups taking 32x8-base x 128-plane tile "
-                           "columns from per-XCD counters), hipGraph replay"},
-        "gbytes_per_s": round(gbs, 1),
-        "plain": {"ms_per_step": round(ms_plain, 4), "gpoints_per_s": round(pts / (ms_plain * 1e-3) / 1e9, 3),
-                  "launches_per_step": stats["transfers"]},
-        "batched_vs_plain": round(ms_plain / ms_step, 3),
-        "wall_s_timed": round(wall, 3),
-        "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(gbs / HBM_PEAK_GBS, 4),
-                     "traffic": _profile_traffic("k_persist_tensor_vlag5_3tl_config2")},
-    }
-    if not args.no_config4:
-        # the 2000-region level step (batched vs plain) beside the headline
-        print("# config4 beside the headline line", file=sys.stderr, flush=True)
-        line["level_step_2000_regions"] = run_config4(2000, args.n_gf, 5, 2, 1)
-    if not args.no_config6:
-        # the level step of a nested 3-level hierarchy from its communication
-        # schedule (batched vs one launch per region)
-        print("# config6 beside the headline line", file=sys.stderr, flush=True)
-        line["hierarchy_step_3_levels"] = run_config6(args.n_gf, 10, 2, 1)
+    out = {"value": round(gpts, 3), "unit": "Gpoints/s", "ms_per_step": round(ms_step, 4),
+           "config": {"workload": "single large patch: 24 FP64 vertex-centred grid functions, coarse 257^3 "
+                                  "(+3 ghosts, 3 time levels t=1,0,-1) -> fine 512^3, prolongate_3d_rf2<5> fused "
+                                  "with interpolate_3d_3tl at t=1/3 and 2/3",
+                      "grid_functions": cfg.n_gf, "fine_points_per_step": pts, "algorithmic_bytes_per_step": alg},
+           "gbytes_per_s": round(gbs, 1),
+           "plain": {"ms_per_step": round(ms_plain, 4), "gpoints_per_s": round(pts / (ms_plain * 1e-3) / 1e9, 3),
+                     "launches_per_step": stats["transfers"]},
+           "batched_vs_plain": round(ms_plain / ms_step, 3),
+           "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(gbs / HBM_PEAK_GBS, 4),
+                        "traffic": _profile_traffic("k_persist_tensor_vlag5_3tl_config2")}}
+    if side:
+        return out
+    out.update({"metric": METRIC, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+                "data": "synthetic"})
+    out["config"]["path"] = ("one batched launch (resident workgroups taking 32x8-base x 128-plane tile "
+                             "columns from per-XCD counters), hipGraph replay")
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
-    print(json.dumps(line))
-    return 0
+        out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+    return out
 
 
 if __name__ == "__main__":

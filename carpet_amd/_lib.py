@@ -59,6 +59,14 @@ _decl("cg_memset_d8", i, vp, i, C.c_size_t, vp)
 _decl("cg_fill_f64", i, vp, dp, C.c_size_t, vp)
 _decl("cg_fill_hash_f64", i, vp, ip, ip, C.c_ulonglong, dp, dp, vp)
 _decl("cg_fill_fourier_f64", i, C.POINTER(GData), dp, dp, i, ip, C.POINTER(C.c_double), vp)
+
+
+class HashFill(C.Structure):
+    _fields_ = [("dst", C.c_void_p), ("padext", C.c_int * 3), ("ext", C.c_int * 3), ("seed", C.c_ulonglong),
+                ("lo", C.c_double), ("hi", C.c_double)]
+
+
+_decl("cg_fill_hash_many_f64", i, i, C.POINTER(HashFill), vp)
 _decl("cg_stream_create", i, C.POINTER(C.c_void_p))
 _decl("cg_stream_destroy", i, vp)
 _decl("cg_stream_synchronize", i, vp)
@@ -81,6 +89,8 @@ _decl("cg_prolongate_3d_o5_monotone_rf2_f64", i, *_OPARGS, vp)
 _decl("cg_restrict_3d_cc_o3_rf2_f64", i, *_OPARGS, vp)
 _decl("cg_restrict_3d_cc_o5_rf2_f64", i, *_OPARGS, vp)
 _decl("cg_set_restriction", i, i, i)
+_decl("cg_restrict_3d_vc_rf2_f64", i, i, i, i, *_OPARGS, vp)
+_decl("cg_set_dgfe", i, i)
 _decl("cg_prolongate_3d_stagger_f64", i, i, i, *_OPARGS, vp)
 _decl("cg_restrict_3d_stagger_f64", i, i, *_OPARGS, vp)
 _decl("cg_set_staggered_operators", i, i)
@@ -102,6 +112,11 @@ _decl("cg_batch_run", i, vp, vp)
 _decl("cg_batch_run_plain", i, vp, vp)
 _decl("cg_batch_capture_graph", i, vp, vp)
 _decl("cg_batch_launch_graph", i, vp, vp)
+_decl("cg_batch_set_time", i, vp, i, C.POINTER(C.c_double), i, dp)
+_decl("cg_batch_set_storage", i, vp, i, _GP, _GP)
+_decl("cg_batch_launch_count", i, vp)
+_decl("cg_batch_run_launch", i, vp, i, vp)
+_decl("cg_batch_launch_stats", i, vp, i, ip, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong))
 _decl("cg_batch_stats", i, vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), C.POINTER(C.c_longlong))
 
 

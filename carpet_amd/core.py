@@ -253,6 +253,23 @@ class DeviceArray:
             pass
 
 
+def fill_hash_many(fills, lo: float = -1.0, hi: float = 1.0, stream=None) -> int:
+    """Seeded uniform data in many arrays at once: fills = [(DeviceArray,
+    seed), ...] (cg_fill_hash_many_f64, one launch); returns once done."""
+    from ._lib import HashFill
+    n = len(fills)
+    arr = (HashFill * max(1, n))()
+    for k, (a, seed) in enumerate(fills):
+        arr[k].dst = a.ptr
+        arr[k].padext[:] = list(a.padext)
+        arr[k].ext[:] = list(a.ext)
+        arr[k].seed = seed & (2**64 - 1)
+        arr[k].lo = lo
+        arr[k].hi = hi
+    check(lib.cg_fill_hash_many_f64(n, arr, _sh(stream)), "cg_fill_hash_many_f64")
+    return 1
+
+
 def _arr(a: DeviceArray):
     return (a.ptr, I3(*a.padext), I3(*a.ext))
 
@@ -374,6 +391,19 @@ def set_staggered_operators(support_staggered_operators: bool):
     check(lib.cg_set_staggered_operators(int(bool(support_staggered_operators))), "set_staggered_operators")
 
 
+def restrict_3d_vc_rf2(cent, src: DeviceArray, dst: DeviceArray, dstregbbox: Box, stream=None):
+    """restrict_3d_vc_rf2<T, centi, centj, centk>; cent = (centi, centj, centk)."""
+    check(lib.cg_restrict_3d_vc_rf2_f64(int(cent[0]), int(cent[1]), int(cent[2]), *_arr(src), *_arr(dst),
+                                        src.box.as_c(), dst.box.as_c(), dstregbbox.as_c(), dstregbbox.as_c(),
+                                        _sh(stream)),
+          "restrict_3d_vc_rf2")
+
+
+def set_dgfe(use_dgfe: bool):
+    """CarpetLib::use_dgfe: the DGFE operators are refused while it is set."""
+    check(lib.cg_set_dgfe(1 if use_dgfe else 0), "cg_set_dgfe")
+
+
 def set_restriction(use_higher_order_restriction: bool, restriction_order_space: int = 3):
     """CarpetLib's use_higher_order_restriction / restriction_order_space
     parameters (CarpetLib/param.ccl) as seen by transfer_from: cell-centred
@@ -431,12 +461,19 @@ def _gptrs(srcs: Sequence[DeviceArray]):
     return arr
 
 
+def _slab(slab_offset, is_centered):
+    if slab_offset is None and is_centered is None:
+        return None
+    return Islab(I3(*(slab_offset or (0, 0, 0))), I3(*(is_centered or (1, 1, 1))))
+
+
 def transfer_from(dst: DeviceArray, srcs: Sequence[DeviceArray], times: Sequence[float],
                   dstregbbox: Box, srcregbbox: Box, time: float, transport_operator: int,
                   centering: int, order_space: int, order_time: int, slab_offset=None,
-                  stream=None):
-    """data<T>::transfer_from_innerloop with gdata::find_source_timelevel."""
-    slab = Islab(I3(*slab_offset), I3(1, 1, 1)) if slab_offset is not None else None
+                  stream=None, is_centered=None):
+    """data<T>::transfer_from_innerloop with gdata::find_source_timelevel;
+    a slab (offset, is_centered) is passed when either is given."""
+    slab = _slab(slab_offset, is_centered)
     tt = (C.c_double * len(times))(*times)
     check(lib.cg_transfer_from(C.byref(dst.g), _gptrs(srcs), tt, len(times), dstregbbox.as_c(),
                                srcregbbox.as_c(), C.byref(slab) if slab else None, time,
@@ -455,8 +492,9 @@ class Batch:
 
     def add(self, phase: int, dst: DeviceArray, srcs: Sequence[DeviceArray], times: Sequence[float],
             dstregbbox: Box, srcregbbox: Box, time: float, transport_operator: int, centering: int,
-            order_space: int, order_time: int, slab_offset=None, prec: int = PREC_F64) -> int:
-        slab = Islab(I3(*slab_offset), I3(1, 1, 1)) if slab_offset is not None else None
+            order_space: int, order_time: int, slab_offset=None, prec: int = PREC_F64,
+            is_centered=None) -> int:
+        slab = _slab(slab_offset, is_centered)
         tt = (C.c_double * len(times))(*times)
         return check(lib.cg_batch_add(self.handle, phase, C.byref(dst.g), _gptrs(srcs), tt, len(times),
                                       dstregbbox.as_c(), srcregbbox.as_c(),
@@ -478,6 +516,20 @@ class Batch:
     def launch_graph(self, stream: Stream):
         check(lib.cg_batch_launch_graph(self.handle, _sh(stream)), "cg_batch_launch_graph")
 
+    def set_time(self, phase: int, times: Sequence[float], time: float) -> int:
+        """New level times and target time for the transfers of `phase`
+        added with len(times) source levels (cg_batch_set_time)."""
+        tt = (C.c_double * len(times))(*times)
+        return check(lib.cg_batch_set_time(self.handle, phase, tt, len(times), time), "cg_batch_set_time")
+
+    def set_storage(self, pairs) -> int:
+        """Rebinds storage: [(from DeviceArray, to DeviceArray), ...], all at
+        once (cg_batch_set_storage; ggf::cycle_all's rotation)."""
+        n = len(pairs)
+        fr = (GData * max(1, n))(*[a.g for a, _ in pairs])
+        to = (GData * max(1, n))(*[b.g for _, b in pairs])
+        return check(lib.cg_batch_set_storage(self.handle, n, fr, to), "cg_batch_set_storage")
+
     def autotune(self, stream=None, reps: int = 3, cache_path: Optional[str] = None):
         """Searches the work-item sizes per launch class (cg_batch_autotune)."""
         check(lib.cg_batch_autotune(self.handle, _sh(stream), reps,
@@ -487,6 +539,20 @@ class Batch:
         buf = C.create_string_buffer(1 << 16)
         check(lib.cg_batch_describe(self.handle, buf, len(buf)), "cg_batch_describe")
         return buf.value.decode()
+
+    def launch_count(self) -> int:
+        return check(lib.cg_batch_launch_count(self.handle), "cg_batch_launch_count")
+
+    def run_launch(self, idx: int, stream=None):
+        """Launch idx of the finalized batch on its own (per-kernel timing)."""
+        check(lib.cg_batch_run_launch(self.handle, idx, _sh(stream)), "cg_batch_run_launch")
+
+    def launch_stats(self, idx: int):
+        """(phase, destination points, algorithmic bytes) of launch idx."""
+        ph, p, b = C.c_int(), C.c_longlong(), C.c_longlong()
+        check(lib.cg_batch_launch_stats(self.handle, idx, C.byref(ph), C.byref(p), C.byref(b)),
+              "cg_batch_launch_stats")
+        return ph.value, p.value, b.value
 
     def stats(self):
         n, p, b = C.c_longlong(), C.c_longlong(), C.c_longlong()

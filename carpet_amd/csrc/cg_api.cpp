@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <vector>
 
 #include "cg_internal.hpp"
 
@@ -52,6 +53,8 @@ static int plan_level(const Transfer &tr, const void *src, const Arr &out, Trans
   case 1:
     return plan_copy(sa, out, tr.tmp_sreg, tr.tmp_reg, tr.tmp_has_slab ? &tr.tmp_slab : nullptr, one.pt);
   case 2:
+    if (tr.tmp_vc)
+      return plan_restrict_vc(tr.tmp_vc & 7, sa, out, tr.tmp_reg, one.pt);
     return tr.tmp_stag ? plan_restrict_stagger(tr.tmp_stag, sa, out, tr.tmp_reg, one.pt)
                        : plan_restrict(tr.tmp_cell != 0, sa, out, tr.tmp_reg, one.pt, tr.tmp_rorder);
   }
@@ -182,6 +185,10 @@ int resolve_transfer(const cg_gdata *dst, const cg_gdata *const *srcs,
                            "op=\"COPY\" with order_space not in {1, 3, 5, 7, 9, 11}");
         kind = PK_VLAG;
       } else {
+        // use_dgfe selects prolongate_3d_dgfe_rf2<T,5> here (data.cc:617-627)
+        if (dgfe_requested())
+          return set_error(CG_ERR_OPERATOR, "use_dgfe: prolongate_3d_dgfe_rf2 evaluates HRSCCore's "
+                                            "GLLElement<5>, which this library does not provide");
         if (order_space < 0 || order_space > 5)
           return set_error(CG_ERR_ORDER,
                            "There is no cell-centred stencil for op=\"LAGRANGE\" or "
@@ -297,17 +304,31 @@ int resolve_transfer(const cg_gdata *dst, const cg_gdata *const *srcs,
     // the staggered groups restrict with their own operators whatever the
     // centring (data.cc:989-1027)
     const int stag = stagger_dirs(op);
-    int cell = 0, rorder = 1;
+    int cell = 0, rorder = 1, vc = 0;
     if (stag) {
       if (slab)
         return set_error(CG_ERR_ARG, "staggered restriction takes no slab");
     } else if (cent == CG_VERTEX_CENTERED) {
       if (slab)
         return set_error(CG_ERR_ARG, "vertex-centred restriction takes no slab");
+    } else if (slab && !(slab->is_centered[0] == 1 && slab->is_centered[1] == 1 &&
+                         slab->is_centered[2] == 1)) {
+      // restrict_3d_vc_rf2 for a slab with one vertex-centred direction
+      // (data.cc:1106-1127); other centrings reach the reference's assert(0)
+      const int *c = slab->is_centered;
+      for (int k = 0; k < 3; ++k)
+        if (c[k] != 0 && c[k] != 1)
+          return set_error(CG_ERR_ARG, "is_centered must hold 0 or 1");
+      if (c[0] + c[1] + c[2] != 2)
+        return set_error(CG_ERR_OPERATOR, "cell-centred restriction with is_centered (%d,%d,%d): the reference "
+                                          "provides (1,1,1), (0,1,1), (1,0,1) and (1,1,0) only",
+                         c[0], c[1], c[2]);
+      vc = c[0] | (c[1] << 1) | (c[2] << 2);
     } else {
-      if (slab && !(slab->is_centered[0] == 1 && slab->is_centered[1] == 1 &&
-                    slab->is_centered[2] == 1))
-        return set_error(CG_ERR_OPERATOR, "restrict_3d_vc_rf2 is not provided");
+      // use_dgfe selects restrict_3d_dgfe_rf2<T,5> here (data.cc:1055-1062)
+      if (dgfe_requested())
+        return set_error(CG_ERR_OPERATOR, "use_dgfe: restrict_3d_dgfe_rf2 evaluates HRSCCore's "
+                                          "GLLElement<5>, which this library does not provide");
       // data.cc:1064-1094: higher-order restriction when the parameter asks
       // for it, except for ENO and WENO groups
       int use_ho = 0, ro = 1;
@@ -323,12 +344,15 @@ int resolve_transfer(const cg_gdata *dst, const cg_gdata *const *srcs,
         tw[l] = times[tl0 + l];
       tr.tmp_step = 2;
       tr.tmp_stag = stag;
+      tr.tmp_vc = vc;
       tr.tmp_cell = cell;
       tr.tmp_rorder = rorder;
       tr.tmp_eno = ntl == 3 && eno_group;
       return setup_tmp(tr, ntl, sp, tw, time, s0, d, sreg, dreg);
     }
     tr.is_prolong = false;
+    if (vc)
+      return plan_restrict_vc(vc, s0, d, dreg, tr.pt);
     if (stag)
       return plan_restrict_stagger(stag, s0, d, dreg, tr.pt);
     return plan_restrict(cell != 0, s0, d, dreg, tr.pt, rorder);
@@ -336,16 +360,11 @@ int resolve_transfer(const cg_gdata *dst, const cg_gdata *const *srcs,
   return set_error(CG_ERR_STRIDE, "source and destination strides are not ordered");
 }
 
-// transfers through temporaries: region-sized temporaries from a
-// stream-ordered allocation, one single-level transfer per time level into
-// them, the interpolation in time, then the temporaries are released in
-// stream order
-static int run_transfer_tmp(const Transfer &tr, hipStream_t st) {
+// transfers through temporaries: one single-level transfer per time level
+// into region-sized temporaries at tmp (tmp_ntl x region points), then the
+// interpolation in time; every temporary point is written before it is read
+static int run_transfer_tmp_into(const Transfer &tr, double *tmp, hipStream_t st) {
   const long long npts = tr.tmp_reg.size();
-  double *tmp = nullptr;
-  hipError_t he = hipMallocAsync(reinterpret_cast<void **>(&tmp), (size_t)tr.tmp_ntl * npts * sizeof(double), st);
-  if (he != hipSuccess)
-    return hip_error(he, "hipMallocAsync (time-level temporaries)");
   int e = CG_OK;
   const void *tp[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   for (int l = 0; l < tr.tmp_ntl && !e; ++l) {
@@ -363,6 +382,17 @@ static int run_transfer_tmp(const Transfer &tr, hipStream_t st) {
     if (!e)
       e = run_transfer(ip, 0, st);
   }
+  return e;
+}
+
+// the temporaries from a stream-ordered allocation, released in stream order
+static int run_transfer_tmp(const Transfer &tr, hipStream_t st) {
+  double *tmp = nullptr;
+  hipError_t he = hipMallocAsync(reinterpret_cast<void **>(&tmp),
+                                 (size_t)tr.tmp_ntl * tr.tmp_reg.size() * sizeof(double), st);
+  if (he != hipSuccess)
+    return hip_error(he, "hipMallocAsync (time-level temporaries)");
+  int e = run_transfer_tmp_into(tr, tmp, st);
   he = hipFreeAsync(tmp, st);
   if (!e && he != hipSuccess)
     e = hip_error(he, "hipFreeAsync (time-level temporaries)");
@@ -592,6 +622,43 @@ int cg_fill_hash_f64(double *dst, const int padext[3], const int ext[3],
              ? hip_error(hipGetLastError(), "fill_hash")
              : CG_OK;
 }
+int cg_fill_hash_many_f64(int n, const cg_hash_fill *fills, cg_stream_t s) {
+  if (n < 0 || (n > 0 && !fills))
+    return set_error(CG_ERR_ARG, "null argument");
+  if (n == 0)
+    return CG_OK;
+  std::vector<HashFill> f((size_t)n);
+  std::vector<long long> first((size_t)n + 1, 0);
+  for (int a = 0; a < n; ++a) {
+    const cg_hash_fill &h = fills[a];
+    if (!h.dst)
+      return set_error(CG_ERR_ARG, "fill %d: null array", a);
+    for (int d = 0; d < 3; ++d)
+      if (h.ext[d] < 0 || h.padext[d] < h.ext[d])
+        return set_error(CG_ERR_SHAPE, "fill %d: extent does not fit the padding", a);
+    f[a] = HashFill{h.dst, h.padext[0], (long long)h.padext[0] * h.padext[1], h.ext[0], h.ext[1], h.seed, h.lo,
+                    h.hi};
+    first[a + 1] = first[a] + (long long)h.ext[0] * h.ext[1] * h.ext[2];
+  }
+  const size_t fb = (size_t)n * sizeof(HashFill), pb = ((size_t)n + 1) * sizeof(long long);
+  char *buf = nullptr;
+  hipError_t he = hipMalloc(reinterpret_cast<void **>(&buf), fb + pb);
+  if (he != hipSuccess)
+    return set_error(CG_ERR_NOMEM, "fill descriptors: %s", hipGetErrorString(he));
+  int e = CG_OK;
+  if ((he = hipMemcpy(buf, f.data(), fb, hipMemcpyHostToDevice)) != hipSuccess ||
+      (he = hipMemcpy(buf + fb, first.data(), pb, hipMemcpyHostToDevice)) != hipSuccess)
+    e = hip_error(he, "fill descriptor upload");
+  if (!e && launch_fill_hash_many(reinterpret_cast<const HashFill *>(buf),
+                                  reinterpret_cast<const long long *>(buf + fb), n, first[n], S(s)))
+    e = hip_error(hipGetLastError(), "fill_hash_many");
+  // the descriptors are released once the fill has run
+  if ((he = hipStreamSynchronize(S(s))) != hipSuccess && !e)
+    e = hip_error(he, "hipStreamSynchronize (fill_hash_many)");
+  (void)hipFree(buf);
+  return e;
+}
+
 int cg_fill_fourier_f64(const cg_gdata *g, double scale, double amp, int nmodes,
                         const int *kvec, const double *phase, cg_stream_t s) {
   if (!g || !g->storage || (nmodes > 0 && (!kvec || !phase)))
@@ -819,6 +886,24 @@ int cg_restrict_3d_cc_rf2_f64(const double *src, const int srcpadext[3],
   (void)srcregbbox;
   return restrict_call(true, src, srcpadext, srcext, dst, dstpadext, dstext,
                        srcbbox, dstbbox, dstregbbox, stream);
+}
+
+int cg_restrict_3d_vc_rf2_f64(int centi, int centj, int centk, const double *src,
+                              const int srcpadext[3], const int srcext[3], double *dst,
+                              const int dstpadext[3], const int dstext[3], const int srcbbox[9],
+                              const int dstbbox[9], const int srcregbbox[9], const int dstregbbox[9],
+                              cg_stream_t stream) {
+  (void)srcregbbox;
+  if (!src || !dst || !srcpadext || !srcext || !dstpadext || !dstext || !srcbbox || !dstbbox || !dstregbbox)
+    return set_error(CG_ERR_ARG, "null argument");
+  if ((centi != 0 && centi != 1) || (centj != 0 && centj != 1) || (centk != 0 && centk != 1))
+    return set_error(CG_ERR_ARG, "centi, centj, centk must be 0 or 1");
+  Transfer tr{};
+  int e = plan_restrict_vc(centi | (centj << 1) | (centk << 2), arr_from(src, srcpadext, srcext, srcbbox),
+                           arr_from(dst, dstpadext, dstext, dstbbox), Box::from(dstregbbox), tr.pt);
+  if (e)
+    return e;
+  return run_transfer(tr, 0, S(stream));
 }
 
 int cg_restrict_3d_cc_o3_rf2_f64(const double *src, const int srcpadext[3],

@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <map>
 #include <new>
 #include <sstream>
 #include <vector>
@@ -23,16 +24,39 @@
 
 using namespace cg;
 
+// the arguments of one cg_batch_add, kept so that a transfer can be
+// resolved again with a new target time or new storage
+struct BatchAddArgs {
+  cg_gdata dst;
+  std::vector<cg_gdata> srcs;
+  std::vector<double> times;
+  int dreg[9], sreg[9];
+  bool has_slab;
+  cg_islab slab;
+  double time;
+  int op, cent, order_space, order_time;
+};
+
 struct cg_batch {
   std::vector<Transfer> transfers;
   std::vector<int> phase;
   std::vector<int> prec;
+  std::vector<int> cls_ntl; // time levels of each transfer's launch class
+  std::vector<BatchAddArgs> args;
   cg::BatchDevice dev;
   cg::TuneMap tuned;
   int tile_cfg = -1;
   bool finalized = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // per-step parameter path: launches whose descriptors changed since the
+  // last upload, and a pinned staging image of the descriptor blob
+  std::vector<int> launch_of; // transfer -> launch index
+  std::vector<char> dirty;    // per launch
+  char *stage = nullptr;
+  size_t stage_bytes = 0;
+  hipEvent_t stage_done = nullptr; // the last upload from stage has completed
+  bool stage_used = false;
 };
 
 extern "C" {
@@ -51,6 +75,12 @@ int cg_batch_destroy(cg_batch *b) {
     (void)hipGraphExecDestroy(b->exec);
   if (b->graph)
     (void)hipGraphDestroy(b->graph);
+  if (b->stage_done) {
+    (void)hipEventSynchronize(b->stage_done);
+    (void)hipEventDestroy(b->stage_done);
+  }
+  if (b->stage)
+    (void)hipHostFree(b->stage);
   batch_device_free(b->dev);
   delete b;
   return CG_OK;
@@ -78,10 +108,37 @@ int cg_batch_add(cg_batch *b, int phase, const cg_gdata *dst,
   if (tr.tmp_ntl)
     return set_error(CG_ERR_ORDER, "prolongation with %d time levels runs per call (cg_transfer_from), "
                                    "not in a batch", tr.tmp_ntl);
+  BatchAddArgs a{};
+  a.dst = *dst;
+  a.srcs.reserve(ntimes);
+  for (int l = 0; l < ntimes; ++l)
+    a.srcs.push_back(*srcs[l]);
+  a.times.assign(times, times + ntimes);
+  std::memcpy(a.dreg, dstregbbox, sizeof a.dreg);
+  std::memcpy(a.sreg, srcregbbox, sizeof a.sreg);
+  a.has_slab = slab != nullptr;
+  if (slab)
+    a.slab = *slab;
+  a.time = time;
+  a.op = transport_operator;
+  a.cent = centering;
+  a.order_space = order_space;
+  a.order_time = order_time;
   b->transfers.push_back(tr);
   b->phase.push_back(phase);
   b->prec.push_back(prec);
+  b->cls_ntl.push_back(tr.is_prolong ? tr.pr.ntl : 1);
+  b->args.push_back(std::move(a));
   return (int)b->transfers.size() - 1;
+}
+
+// transfer -> launch map after a (re)build; nothing is pending after it
+static void index_launches(cg_batch *b) {
+  b->launch_of.assign(b->transfers.size(), -1);
+  for (size_t l = 0; l < b->dev.launches.size(); ++l)
+    for (int m : b->dev.launches[l].members)
+      b->launch_of[m] = (int)l;
+  b->dirty.assign(b->dev.launches.size(), 0);
 }
 
 int cg_batch_finalize(cg_batch *b, int tile_cfg) {
@@ -90,9 +147,10 @@ int cg_batch_finalize(cg_batch *b, int tile_cfg) {
   if (b->finalized)
     return set_error(CG_ERR_STATE, "batch already finalized");
   b->tile_cfg = tile_cfg;
-  int e = batch_device_build(b->dev, b->transfers, b->phase, b->prec, tile_cfg, &b->tuned);
+  int e = batch_device_build(b->dev, b->transfers, b->phase, b->prec, b->cls_ntl, tile_cfg, &b->tuned);
   if (e)
     return e;
+  index_launches(b);
   b->finalized = true;
   return CG_OK;
 }
@@ -192,7 +250,7 @@ extern "C" int cg_batch_autotune(cg_batch *b, cg_stream_t stream, int reps,
     auto trial_run = [&](const Tune &c) -> int {
       TuneMap trial = b->tuned;
       trial[l.key] = c;
-      int e = batch_device_build(b->dev, b->transfers, b->phase, b->prec, b->tile_cfg, &trial);
+      int e = batch_device_build(b->dev, b->transfers, b->phase, b->prec, b->cls_ntl, b->tile_cfg, &trial);
       if (e)
         return e;
       float ms = 0;
@@ -217,9 +275,10 @@ extern "C" int cg_batch_autotune(cg_batch *b, cg_stream_t stream, int reps,
         return e;
     b->tuned[l.key] = bestc;
   }
-  int e = batch_device_build(b->dev, b->transfers, b->phase, b->prec, b->tile_cfg, &b->tuned);
+  int e = batch_device_build(b->dev, b->transfers, b->phase, b->prec, b->cls_ntl, b->tile_cfg, &b->tuned);
   if (e)
     return e;
+  index_launches(b);
   return save_cache(cache_path, b->tuned);
 }
 
@@ -243,11 +302,178 @@ extern "C" int cg_batch_describe(const cg_batch *b, char *buf, size_t len) {
   return (int)s.size();
 }
 
+// ---- per-step parameters ----------------------------------------------------
+// A transfer resolved again (new target time or storage) must run in the
+// launch class it was finalized in: the same operator and stencil, and for
+// tensor classes the class's time levels or one (an exact time match, run
+// with nlev = 1); pointwise prolongations combine their levels at run time.
+static int same_class(const Transfer &old, const Transfer &nu, int cls_ntl, int idx) {
+  if (nu.tmp_ntl)
+    return set_error(CG_ERR_STATE, "transfer %d would need time-level temporaries, which a batch does not run",
+                     idx);
+  if (old.is_prolong != nu.is_prolong)
+    return set_error(CG_ERR_STATE, "transfer %d changes its operator; build a new batch", idx);
+  if (old.is_prolong) {
+    if (old.pr.kind != nu.pr.kind || old.pr.order != nu.pr.order)
+      return set_error(CG_ERR_STATE, "transfer %d changes its stencil; build a new batch", idx);
+    const bool tensor = nu.pr.kind != PK_TVD && nu.pr.kind != PK_MONO5 && nu.pr.kind != PK_STAGGER;
+    if (tensor && nu.pr.ntl != cls_ntl && nu.pr.ntl != 1)
+      return set_error(CG_ERR_STATE,
+                       "transfer %d now combines %d time levels; its launch was built for %d (or an exact "
+                       "time match); build a new batch",
+                       idx, nu.pr.ntl, cls_ntl);
+    return CG_OK;
+  }
+  if (old.pt.op == nu.pt.op || (old.pt.op == PO_INTERP && nu.pt.op == PO_COPY))
+    return CG_OK;
+  return set_error(CG_ERR_STATE,
+                   "transfer %d changes from %s to %s; build a new batch", idx,
+                   old.pt.op == PO_COPY ? "a copy" : "its operator",
+                   nu.pt.op == PO_INTERP ? "a time interpolation" : "another operator");
+}
+
+// resolves the transfers idx[] with their (updated) arguments; all or nothing
+static int reresolve(cg_batch *b, const std::vector<int> &idx, const std::vector<BatchAddArgs> &na) {
+  std::vector<Transfer> nt(idx.size());
+  for (size_t i = 0; i < idx.size(); ++i) {
+    const BatchAddArgs &a = na[i];
+    std::vector<const cg_gdata *> sp(a.srcs.size());
+    for (size_t l = 0; l < a.srcs.size(); ++l)
+      sp[l] = &a.srcs[l];
+    const int t = idx[i];
+    int e = resolve_transfer(&a.dst, sp.data(), a.times.data(), (int)a.times.size(), a.dreg, a.sreg,
+                             a.has_slab ? &a.slab : nullptr, a.time, a.op, a.cent, a.order_space,
+                             a.order_time, nt[i], b->prec[t] == CG_PREC_F64 ? 8 : 4);
+    if (e)
+      return e;
+    if ((e = same_class(b->transfers[t], nt[i], b->cls_ntl[t], t)))
+      return e;
+  }
+  for (size_t i = 0; i < idx.size(); ++i) {
+    const int t = idx[i];
+    b->transfers[t] = nt[i];
+    b->args[t] = na[i];
+    if (b->finalized && b->launch_of[t] >= 0)
+      b->dirty[b->launch_of[t]] = 1;
+  }
+  return CG_OK;
+}
+
+// uploads the descriptors of the launches changed since the last upload, in
+// stream order (a pinned staging image, reused once its last copy is done)
+static int upload_pending(cg_batch *b, hipStream_t st) {
+  bool any = false;
+  for (char d : b->dirty)
+    any = any || d;
+  if (!any)
+    return CG_OK;
+  hipError_t he;
+  if (b->stage_bytes < b->dev.blob_bytes) {
+    if (b->stage) {
+      (void)hipEventSynchronize(b->stage_done);
+      (void)hipHostFree(b->stage);
+      b->stage = nullptr;
+      b->stage_bytes = 0;
+    }
+    he = hipHostMalloc(reinterpret_cast<void **>(&b->stage), b->dev.blob_bytes, hipHostMallocDefault);
+    if (he != hipSuccess)
+      return set_error(CG_ERR_NOMEM, "pinned staging of the batch descriptors: %s", hipGetErrorString(he));
+    b->stage_bytes = b->dev.blob_bytes;
+  }
+  if (!b->stage_done) {
+    he = hipEventCreateWithFlags(&b->stage_done, hipEventDisableTiming);
+    if (he != hipSuccess)
+      return hip_error(he, "hipEventCreate");
+  }
+  if (b->stage_used) {
+    he = hipEventSynchronize(b->stage_done);
+    if (he != hipSuccess)
+      return hip_error(he, "hipEventSynchronize (descriptor staging)");
+  }
+  for (size_t l = 0; l < b->dev.launches.size(); ++l) {
+    if (!b->dirty[l])
+      continue;
+    const BatchLaunch &L = b->dev.launches[l];
+    batch_launch_descs(L, b->transfers, b->stage + L.desc_off);
+    he = hipMemcpyAsync(b->dev.blob + L.desc_off, b->stage + L.desc_off, L.desc_bytes, hipMemcpyHostToDevice,
+                        st);
+    if (he != hipSuccess)
+      return hip_error(he, "descriptor upload");
+    b->dirty[l] = 0;
+  }
+  he = hipEventRecord(b->stage_done, st);
+  if (he != hipSuccess)
+    return hip_error(he, "hipEventRecord (descriptor staging)");
+  b->stage_used = true;
+  return CG_OK;
+}
+
+int cg_batch_set_time(cg_batch *b, int phase, const double *times, int ntimes, double time) {
+  if (!b || !times || ntimes <= 0)
+    return set_error(CG_ERR_ARG, "null argument");
+  std::vector<int> idx;
+  std::vector<BatchAddArgs> na;
+  for (size_t t = 0; t < b->transfers.size(); ++t)
+    if (b->phase[t] == phase && (int)b->args[t].times.size() == ntimes) {
+      idx.push_back((int)t);
+      na.push_back(b->args[t]);
+      na.back().times.assign(times, times + ntimes);
+      na.back().time = time;
+    }
+  int e = reresolve(b, idx, na);
+  return e ? e : (int)idx.size();
+}
+
+int cg_batch_set_storage(cg_batch *b, int n, const cg_gdata *from, const cg_gdata *to) {
+  if (!b || n < 0 || (n > 0 && (!from || !to)))
+    return set_error(CG_ERR_ARG, "null argument");
+  std::map<const void *, int> map;
+  for (int i = 0; i < n; ++i) {
+    if (std::memcmp(from[i].padext, to[i].padext, sizeof from[i].padext) ||
+        std::memcmp(from[i].ext, to[i].ext, sizeof from[i].ext) ||
+        std::memcmp(from[i].bbox, to[i].bbox, sizeof from[i].bbox))
+      return set_error(CG_ERR_ARG, "storage pair %d: the arrays have different layouts", i);
+    if (!map.emplace(from[i].storage, i).second)
+      return set_error(CG_ERR_ARG, "storage pair %d: the same storage appears twice", i);
+  }
+  std::vector<int> idx;
+  std::vector<BatchAddArgs> na;
+  for (size_t t = 0; t < b->transfers.size(); ++t) {
+    BatchAddArgs a = b->args[t];
+    bool hit = false;
+    auto swap = [&](cg_gdata &g) -> int {
+      auto it = map.find(g.storage);
+      if (it == map.end())
+        return CG_OK;
+      const cg_gdata &f = from[it->second];
+      if (std::memcmp(g.padext, f.padext, sizeof g.padext) || std::memcmp(g.bbox, f.bbox, sizeof g.bbox))
+        return set_error(CG_ERR_ARG, "storage pair %d: transfer %zu holds that storage with another layout",
+                         it->second, t);
+      g = to[it->second];
+      hit = true;
+      return CG_OK;
+    };
+    int e = swap(a.dst);
+    for (size_t l = 0; l < a.srcs.size() && !e; ++l)
+      e = swap(a.srcs[l]);
+    if (e)
+      return e;
+    if (hit) {
+      idx.push_back((int)t);
+      na.push_back(std::move(a));
+    }
+  }
+  int e = reresolve(b, idx, na);
+  return e ? e : (int)idx.size();
+}
+
 int cg_batch_run(cg_batch *b, cg_stream_t stream) {
   if (!b)
     return set_error(CG_ERR_ARG, "null batch");
   if (!b->finalized)
     return set_error(CG_ERR_STATE, "batch not finalized");
+  if (int e = upload_pending(b, static_cast<hipStream_t>(stream)))
+    return e;
   return batch_device_run(b->dev, static_cast<hipStream_t>(stream));
 }
 
@@ -287,6 +513,9 @@ int cg_batch_capture_graph(cg_batch *b, cg_stream_t stream) {
     (void)hipGraphDestroy(b->graph);
     b->graph = nullptr;
   }
+  // pending descriptor changes go up before (not into) the captured step
+  if (int e = upload_pending(b, st))
+    return e;
   hipError_t he = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
   if (he != hipSuccess)
     return hip_error(he, "hipStreamBeginCapture");
@@ -309,8 +538,69 @@ int cg_batch_launch_graph(cg_batch *b, cg_stream_t stream) {
     return set_error(CG_ERR_ARG, "null batch");
   if (!b->exec)
     return set_error(CG_ERR_STATE, "no captured graph");
+  if (int e = upload_pending(b, static_cast<hipStream_t>(stream)))
+    return e;
   hipError_t he = hipGraphLaunch(b->exec, static_cast<hipStream_t>(stream));
   return he == hipSuccess ? CG_OK : hip_error(he, "hipGraphLaunch");
+}
+
+// destination points and algorithmic bytes of transfer i: element size x
+// (destination points + unique source footprint per time level)
+static void transfer_bytes(const cg_batch *b, size_t i, long long *pts, long long *bytes) {
+  const Transfer &t = b->transfers[i];
+  const long long es = b->prec[i] == CG_PREC_F64 ? 8 : 4;
+  if (t.is_prolong) {
+    *pts = t.pr.dst_points;
+    *bytes = es * (t.pr.dst_points + t.pr.src_points * t.pr.ntl);
+  } else {
+    const int ntl = t.pt.op == PO_INTERP ? t.pt.a.ntl : 1;
+    *pts = t.pt.dst_points;
+    *bytes = 8 * (t.pt.dst_points + t.pt.src_points * ntl);
+  }
+}
+
+int cg_batch_launch_count(const cg_batch *b) {
+  if (!b)
+    return set_error(CG_ERR_ARG, "null batch");
+  if (!b->finalized)
+    return set_error(CG_ERR_STATE, "batch not finalized");
+  return (int)b->dev.launches.size();
+}
+
+int cg_batch_run_launch(cg_batch *b, int idx, cg_stream_t stream) {
+  if (!b)
+    return set_error(CG_ERR_ARG, "null batch");
+  if (!b->finalized)
+    return set_error(CG_ERR_STATE, "batch not finalized");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (int e = upload_pending(b, st))
+    return e;
+  return batch_device_run_one(b->dev, idx, st);
+}
+
+int cg_batch_launch_stats(const cg_batch *b, int idx, int *phase, long long *dst_points,
+                          long long *alg_bytes) {
+  if (!b)
+    return set_error(CG_ERR_ARG, "null batch");
+  if (!b->finalized)
+    return set_error(CG_ERR_STATE, "batch not finalized");
+  if (idx < 0 || idx >= (int)b->dev.launches.size())
+    return set_error(CG_ERR_ARG, "no launch %d", idx);
+  const BatchLaunch &l = b->dev.launches[idx];
+  long long pts = 0, bytes = 0;
+  for (int m : l.members) {
+    long long p, q;
+    transfer_bytes(b, (size_t)m, &p, &q);
+    pts += p;
+    bytes += q;
+  }
+  if (phase)
+    *phase = l.phase;
+  if (dst_points)
+    *dst_points = pts;
+  if (alg_bytes)
+    *alg_bytes = bytes;
+  return CG_OK;
 }
 
 int cg_batch_stats(const cg_batch *b, long long *ntransfers, long long *dst_points,
@@ -319,17 +609,11 @@ int cg_batch_stats(const cg_batch *b, long long *ntransfers, long long *dst_poin
     return set_error(CG_ERR_ARG, "null batch");
   long long n = 0, pts = 0, bytes = 0;
   for (size_t i = 0; i < b->transfers.size(); ++i) {
-    const Transfer &t = b->transfers[i];
-    const long long es = b->prec[i] == CG_PREC_F64 ? 8 : 4;
+    long long p, q;
+    transfer_bytes(b, i, &p, &q);
     ++n;
-    if (t.is_prolong) {
-      pts += t.pr.dst_points;
-      bytes += es * (t.pr.dst_points + t.pr.src_points * t.pr.ntl);
-    } else {
-      const int ntl = t.pt.op == PO_INTERP ? t.pt.a.ntl : 1;
-      pts += t.pt.dst_points;
-      bytes += 8 * (t.pt.dst_points + t.pt.src_points * ntl);
-    }
+    pts += p;
+    bytes += q;
   }
   if (ntransfers)
     *ntransfers = n;

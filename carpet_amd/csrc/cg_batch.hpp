@@ -33,6 +33,8 @@ struct BatchLaunch {
                    // in fixed rounds (tensor tiles)
   int shape;       // tensor tile shape (0: 32 x 8 bases, 1: 16 x 16)
   std::string key; // tuner cache key (class + box-size histogram bucket)
+  std::vector<int> members; // transfer indices in descriptor order
+  size_t desc_bytes;        // bytes of the descriptor array
 };
 
 // one work item: a tile (tensor) or a run of points (pointwise) of one
@@ -63,13 +65,24 @@ struct BatchDevice {
   size_t blob_bytes = 0;
   int *counters = nullptr; // one work counter per launch (device)
   int ncounters = 0;
+  // launches of one phase after the first run on helper streams forked from
+  // the caller's stream and joined back (graph branches under capture)
+  std::vector<hipStream_t> aux;
+  std::vector<hipEvent_t> fork, join;
 };
 
 // tile_cfg selects the z chunk of tensor tiles in coarse bases (0 = whole
-// region, < 0 = automatic)
+// region, < 0 = automatic); cls_ntl: per transfer the time levels of its
+// launch class (a tensor transfer resolved to fewer levels -- an exact time
+// match -- runs in its class with nlev = 1)
 int batch_device_build(BatchDevice &dev, const std::vector<Transfer> &tr,
                        const std::vector<int> &phase, const std::vector<int> &prec,
-                       int tile_cfg, const TuneMap *tuned = nullptr);
+                       const std::vector<int> &cls_ntl, int tile_cfg,
+                       const TuneMap *tuned = nullptr);
+// the descriptor array of launch l from the current transfers, in the
+// layout batch_device_build uploads (the per-step parameter path rewrites
+// it in place: same items, new time weights and array pointers)
+void batch_launch_descs(const BatchLaunch &l, const std::vector<Transfer> &tr, char *out);
 int batch_device_run(BatchDevice &dev, hipStream_t st);
 // runs launch idx alone (its counter cleared first)
 int batch_device_run_one(BatchDevice &dev, int idx, hipStream_t st);

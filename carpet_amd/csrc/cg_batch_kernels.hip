@@ -207,12 +207,44 @@ int line_span(int kind, int order) {
 // and 6-point shells, 1.14 ms against 1.61 ms without small tiles; 2000
 // regions of 8-96 points per axis 19.41 ms against 19.53 ms, and 19.83 ms
 // when routing at the break-even point 1.0.)
-bool small_tiles(const ProlongPlan &p, int prec) {
+bool small_tiles(const ProlongPlan &p, int ntl, int prec) {
   if (prec != CG_PREC_F64)
     return false;
   const int span = line_span(p.kind, p.order);
-  const double big = std::min(tile_busy(p.t, p.ntl, prec, span, 0), tile_busy(p.t, p.ntl, prec, span, 1));
-  return tile_busy(p.t, p.ntl, prec, span, 2) * 1.3 <= big;
+  const double big = std::min(tile_busy(p.t, ntl, prec, span, 0), tile_busy(p.t, ntl, prec, span, 1));
+  return tile_busy(p.t, ntl, prec, span, 2) * 1.3 <= big;
+}
+
+// one transfer's descriptor in launch l: a tensor transfer resolved to one
+// time level (its target time matches level 0 exactly) inside a class of
+// more levels keeps nlev = 1 and reads level 0 for every level slot; a
+// same-level interpolation resolved to a copy stays an interpolation of one
+// level
+TensorArgs tensor_desc(const BatchLaunch &l, const Transfer &t) {
+  TensorArgs a = t.pr.t;
+  for (int k = a.nlev; k < l.ntl; ++k)
+    a.src[k] = a.src[0];
+  TileCfg cfg;
+  int th = 0;
+  tile_shape(l.ntl, l.prec, &cfg.cx, &cfg.cy, &th, l.shape);
+  cfg.zchunk = l.zchunk;
+  (void)tensor_tiles(a, cfg);
+  return a;
+}
+
+PointArgs point_desc(const BatchLaunch &l, const Transfer &t) {
+  PointArgs a = t.is_prolong ? t.pr.p : t.pt.a;
+  if (l.op == PO_INTERP && !t.is_prolong && t.pt.op == PO_COPY) {
+    a.ntl = 1;
+    a.s[0] = 1.0;
+  }
+  // work units: destination points, or TVD's coarse parent cells
+  long long u[3];
+  for (int d = 0; d < 3; ++d)
+    u[d] = l.op == PO_TVD ? ((a.off[d] + a.n[d] - 1) >> 1) - (a.off[d] >> 1) + 1 : a.n[d];
+  a.fx = make_fastdiv((unsigned)u[0]);
+  a.fxy = make_fastdiv((unsigned)(u[0] * u[1]));
+  return a;
 }
 
 template <int SH> const void *tensor_fn_sh(const BatchLaunch &l, int *threads) {
@@ -259,6 +291,7 @@ const void *persist_point_fn(int op, int pt) {
   case PO_MONO5: return persist_point_pu<PO_MONO5>(pu);
   case PO_STAGGER: return persist_point_pu<PO_STAGGER>(pu);
   case PO_RESTRICT_STAGGER: return persist_point_pu<PO_RESTRICT_STAGGER>(pu);
+  case PO_RESTRICT_VC2: return persist_point_pu<PO_RESTRICT_VC2>(pu);
   }
   return nullptr;
 }
@@ -279,6 +312,17 @@ const void *launch_fn(const BatchLaunch &l, int *threads) {
 } // namespace
 
 void batch_device_free(BatchDevice &dev) {
+  for (hipStream_t s : dev.aux)
+    (void)hipStreamSynchronize(s);
+  for (hipStream_t s : dev.aux)
+    (void)hipStreamDestroy(s);
+  for (hipEvent_t e : dev.join)
+    (void)hipEventDestroy(e);
+  for (hipEvent_t e : dev.fork)
+    (void)hipEventDestroy(e);
+  dev.aux.clear();
+  dev.join.clear();
+  dev.fork.clear();
   if (dev.blob)
     (void)hipFree(dev.blob);
   if (dev.counters)
@@ -324,9 +368,22 @@ static int ilog2(long long v) {
   return r;
 }
 
+void batch_launch_descs(const BatchLaunch &l, const std::vector<Transfer> &tr, char *out) {
+  for (size_t i = 0; i < l.members.size(); ++i) {
+    const Transfer &t = tr[l.members[i]];
+    if (l.is_tensor) {
+      const TensorArgs a = tensor_desc(l, t);
+      std::memcpy(out + i * sizeof(TensorArgs), &a, sizeof a);
+    } else {
+      const PointArgs a = point_desc(l, t);
+      std::memcpy(out + i * sizeof(PointArgs), &a, sizeof a);
+    }
+  }
+}
+
 int batch_device_build(BatchDevice &dev, const std::vector<Transfer> &tr,
                        const std::vector<int> &phase, const std::vector<int> &prec,
-                       int tile_cfg, const TuneMap *tuned) {
+                       const std::vector<int> &cls_ntl, int tile_cfg, const TuneMap *tuned) {
   batch_device_free(dev);
   // group transfers by (phase, kernel)
   std::vector<Group> groups;
@@ -337,10 +394,10 @@ int batch_device_build(BatchDevice &dev, const std::vector<Transfer> &tr,
     if (tr[i].is_prolong && tr[i].pr.kind != PK_TVD && tr[i].pr.kind != PK_MONO5 &&
         tr[i].pr.kind != PK_STAGGER) {
       g.is_tensor = 1;
-      g.small = small_tiles(tr[i].pr, prec[i]) ? 1 : 0;
+      g.small = small_tiles(tr[i].pr, cls_ntl[i], prec[i]) ? 1 : 0;
       g.kind = tr[i].pr.kind;
       g.order = tr[i].pr.order;
-      g.ntl = tr[i].pr.ntl;
+      g.ntl = cls_ntl[i];
       g.op = -1;
     } else {
       g.is_tensor = 0;
@@ -526,9 +583,8 @@ int batch_device_build(BatchDevice &dev, const std::vector<Transfer> &tr,
       }
       std::vector<TensorArgs> descs;
       for (int m : members) {
-        TensorArgs a = tr[m].pr.t;
-        cfg.zchunk = l.zchunk;
-        const long long nt = tensor_tiles(a, cfg);
+        const TensorArgs a = tensor_desc(l, tr[m]);
+        const long long nt = (long long)a.ntiles[0] * a.ntiles[1] * a.ntiles[2];
         const unsigned d = (unsigned)descs.size();
         descs.push_back(a);
         first.push_back((int)items.size());
@@ -542,10 +598,11 @@ int batch_device_build(BatchDevice &dev, const std::vector<Transfer> &tr,
         }
       }
       l.desc_off = append(descs.data(), descs.size() * sizeof(TensorArgs));
+      l.desc_bytes = descs.size() * sizeof(TensorArgs);
     } else {
       std::vector<PointArgs> descs;
       for (int m : members) {
-        PointArgs a = tr[m].is_prolong ? tr[m].pr.p : tr[m].pt.a;
+        const PointArgs a = point_desc(l, tr[m]);
         // work units: destination points, or TVD's coarse parent cells
         long long u[3];
         for (int d = 0; d < 3; ++d)
@@ -553,8 +610,6 @@ int batch_device_build(BatchDevice &dev, const std::vector<Transfer> &tr,
         const long long tot = u[0] * u[1] * u[2];
         if (tot >= (1LL << 31) - l.pt)
           return set_error(CG_ERR_ARG, "region too large for the batch");
-        a.fx = make_fastdiv((unsigned)u[0]);
-        a.fxy = make_fastdiv((unsigned)(u[0] * u[1]));
         const unsigned d = (unsigned)descs.size();
         descs.push_back(a);
         first.push_back((int)items.size());
@@ -565,7 +620,9 @@ int batch_device_build(BatchDevice &dev, const std::vector<Transfer> &tr,
         }
       }
       l.desc_off = append(descs.data(), descs.size() * sizeof(PointArgs));
+      l.desc_bytes = descs.size() * sizeof(PointArgs);
     }
+    l.members = members;
     first.push_back((int)items.size());
     l.nitems = (int)items.size();
     // One segment per XCD.  Every transfer is striped over the XCDs: its
@@ -648,6 +705,31 @@ int batch_device_build(BatchDevice &dev, const std::vector<Transfer> &tr,
 
 static int run_launch(BatchDevice &dev, const BatchLaunch &l, hipStream_t st);
 
+// helper streams (default priority, non-blocking) and the fork/join events
+// of the launches that share a phase
+static int ensure_aux(BatchDevice &dev, size_t n) {
+  if (dev.fork.empty()) {
+    hipEvent_t e;
+    hipError_t he = hipEventCreateWithFlags(&e, hipEventDisableTiming);
+    if (he != hipSuccess)
+      return hip_error(he, "hipEventCreate");
+    dev.fork.push_back(e);
+  }
+  while (dev.aux.size() < n) {
+    hipStream_t s;
+    hipEvent_t e;
+    hipError_t he = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (he != hipSuccess)
+      return hip_error(he, "hipStreamCreate");
+    dev.aux.push_back(s);
+    he = hipEventCreateWithFlags(&e, hipEventDisableTiming);
+    if (he != hipSuccess)
+      return hip_error(he, "hipEventCreate");
+    dev.join.push_back(e);
+  }
+  return CG_OK;
+}
+
 int batch_device_run(BatchDevice &dev, hipStream_t st) {
   if (dev.launches.empty())
     return CG_OK;
@@ -658,10 +740,39 @@ int batch_device_run(BatchDevice &dev, hipStream_t st) {
   if (persistent)
     hipLaunchKernelGGL(k_zero_counters, dim3((8 * dev.ncounters + 255) / 256), dim3(256), 0, st,
                        dev.counters, 8 * dev.ncounters);
-  for (const BatchLaunch &l : dev.launches) {
-    int e = run_launch(dev, l, st);
-    if (e)
-      return e;
+  // The launches of one phase write disjoint regions (dh's lists never send
+  // one point twice), so the classes of a phase run side by side: the first
+  // on st, the others on helper streams forked from st by an event and joined
+  // back before the next phase.  Under stream capture the fork and join
+  // become branches of the graph.
+  const size_t nl = dev.launches.size();
+  for (size_t i = 0; i < nl;) {
+    size_t j = i + 1;
+    while (j < nl && dev.launches[j].phase == dev.launches[i].phase)
+      ++j;
+    const size_t extra = j - i - 1;
+    if (extra > 0) {
+      if (int e = ensure_aux(dev, extra))
+        return e;
+      hipError_t he = hipEventRecord(dev.fork[0], st);
+      for (size_t k = 0; k < extra && he == hipSuccess; ++k)
+        he = hipStreamWaitEvent(dev.aux[k], dev.fork[0], 0);
+      if (he != hipSuccess)
+        return hip_error(he, "phase fork");
+    }
+    for (size_t k = i; k < j; ++k) {
+      int e = run_launch(dev, dev.launches[k], k == i ? st : dev.aux[k - i - 1]);
+      if (e)
+        return e;
+    }
+    for (size_t k = 0; k < extra; ++k) {
+      hipError_t he = hipEventRecord(dev.join[k], dev.aux[k]);
+      if (he == hipSuccess)
+        he = hipStreamWaitEvent(st, dev.join[k], 0);
+      if (he != hipSuccess)
+        return hip_error(he, "phase join");
+    }
+    i = j;
   }
   return CG_OK;
 }
@@ -721,6 +832,7 @@ static int run_launch(BatchDevice &dev, const BatchLaunch &l, hipStream_t st) {
     case PO_MONO5: rc = go(std::integral_constant<int, PO_MONO5>{}); break;
     case PO_STAGGER: rc = go(std::integral_constant<int, PO_STAGGER>{}); break;
     case PO_RESTRICT_STAGGER: rc = go(std::integral_constant<int, PO_RESTRICT_STAGGER>{}); break;
+    case PO_RESTRICT_VC2: rc = go(std::integral_constant<int, PO_RESTRICT_VC2>{}); break;
     default: rc = -1;
     }
   }

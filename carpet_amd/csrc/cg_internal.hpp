@@ -113,6 +113,12 @@ int plan_restrict(bool cell, const Arr &src, const Arr &dst, const Box &reg,
 // restrict_3d_stagger{011,101,110,111} (bit d of stag set = staggered in
 // direction d)
 int plan_restrict_stagger(int stag, const Arr &src, const Arr &dst, const Box &reg, PointPlan &plan);
+// restrict_3d_vc_rf2 (bit d of cmask set = direction d cell-centred)
+int plan_restrict_vc(int cmask, const Arr &src, const Arr &dst, const Box &reg, PointPlan &plan);
+// CarpetLib's use_dgfe parameter (process-wide, default off): the DGFE
+// operators it selects are not provided, so the transfers that would use
+// them are refused
+bool dgfe_requested();
 // CarpetLib's support_staggered_operators parameter (process-wide, default
 // off): the staggered operators refuse to run without it
 bool staggered_supported();
@@ -145,6 +151,7 @@ struct Transfer {
   int tmp_step; // 0 prolongation, 1 slabbed copy, 2 restriction
   int tmp_kind, tmp_order, tmp_stag; // prolongation kind; staggered directions
   int tmp_cell, tmp_rorder;          // restriction: cell-centred, order
+  int tmp_vc;                        // restriction: restrict_3d_vc_rf2 centring mask (0: none)
   bool tmp_has_slab;
   cg_islab tmp_slab;
   bool tmp_eno; // interpolate_eno_3d_3tl for the 3 levels

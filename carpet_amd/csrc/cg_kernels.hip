@@ -93,6 +93,7 @@ int launch_point(int op, const PointArgs &a, hipStream_t s) {
   case PO_MONO5: hipLaunchKernelGGL(k_point<PO_MONO5>, grid, block, 0, s, a); break;
   case PO_STAGGER: hipLaunchKernelGGL(k_point<PO_STAGGER>, grid, block, 0, s, a); break;
   case PO_RESTRICT_STAGGER: hipLaunchKernelGGL(k_point<PO_RESTRICT_STAGGER>, grid, block, 0, s, a); break;
+  case PO_RESTRICT_VC2: hipLaunchKernelGGL(k_point<PO_RESTRICT_VC2>, grid, block, 0, s, a); break;
   default: return -1;
   }
   return hipGetLastError() == hipSuccess ? 0 : -2;
@@ -126,6 +127,54 @@ __global__ void k_fill_hash(double *d, long long sj, long long sk, int nx, int n
     const double u = (double)(z >> 11) * (1.0 / 9007199254740992.0);
     d[i + sj * j + sk * k] = lo + (hi - lo) * u;
   }
+}
+
+// Many seeded fills in one launch: array a owns the global point range
+// [first[a], first[a+1]) (x fastest within its extent); workgroup b fills
+// the FILL_CHUNK points from b * FILL_CHUNK, starting at the last array
+// whose range begins at or before them.  Same values as k_fill_hash.
+constexpr long long FILL_CHUNK = 8192;
+
+__global__ void k_fill_hash_many(const HashFill *__restrict__ f, const long long *__restrict__ first, int n,
+                                 long long total) {
+  const long long c0 = (long long)blockIdx.x * FILL_CHUNK;
+  const long long c1 = min(total, c0 + FILL_CHUNK);
+  __shared__ int s_a;
+  if (threadIdx.x == 0) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) / 2;
+      if (first[mid] <= c0)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    s_a = lo;
+  }
+  __syncthreads();
+  int a = s_a;
+  for (long long e = c0 + threadIdx.x; e < c1; e += blockDim.x) {
+    while (e >= first[a + 1]) // e < total = first[n], so a stays below n
+      ++a;
+    const HashFill &h = f[a];
+    const long long l = e - first[a];
+    const long long nxy = (long long)h.nx * h.ny;
+    const long long k = l / nxy, r = l - k * nxy;
+    const long long j = r / h.nx, i = r - j * h.nx;
+    const unsigned long long z = splitmix64(h.seed + (unsigned long long)l);
+    const double u = (double)(z >> 11) * (1.0 / 9007199254740992.0);
+    h.d[i + h.sj * j + h.sk * k] = h.lo + (h.hi - h.lo) * u;
+  }
+}
+
+int launch_fill_hash_many(const HashFill *f, const long long *first, int n, long long total, hipStream_t s) {
+  if (total <= 0 || n <= 0)
+    return 0;
+  const long long blocks = (total + FILL_CHUNK - 1) / FILL_CHUNK;
+  if (blocks > 0x7fffffffLL)
+    return -1;
+  hipLaunchKernelGGL(k_fill_hash_many, dim3((unsigned)blocks), dim3(256), 0, s, f, first, n, total);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
 int launch_fill_hash(double *d, long long sj, long long sk, const int *ext,

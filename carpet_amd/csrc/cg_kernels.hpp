@@ -27,6 +27,9 @@ struct TensorArgs {
   double s2[2];       // ENO time rule: weights of the fallback pair
   int l0;             //   first level of the fallback pair (0: 1-2, 1: 2-3)
   int teno;           //   1: interpolate_eno_3d_3tl's rule for 3 levels
+  int nlev;           // time levels this transfer combines: the kernel's NTL,
+                      // or 1 (an exact time match inside a class of NTL
+                      // levels: the value of level 0, every src[] = src[0])
 };
 
 // Division by a fixed d >= 1 for dividends n < 2^31 as a multiply and a
@@ -52,7 +55,7 @@ __device__ __forceinline__ unsigned fastdiv(unsigned n, FastDiv f) {
 // restriction, cell-centred restriction, time interpolation, cell TVD.
 enum PointOp { PO_COPY = 0, PO_RESTRICT_VC = 1, PO_RESTRICT_CC = 2,
                PO_INTERP = 3, PO_TVD = 4, PO_RESTRICT_CC_O3 = 5, PO_RESTRICT_CC_O5 = 6,
-               PO_MONO5 = 7, PO_STAGGER = 8, PO_RESTRICT_STAGGER = 9 };
+               PO_MONO5 = 7, PO_STAGGER = 8, PO_RESTRICT_STAGGER = 9, PO_RESTRICT_VC2 = 10 };
 
 struct PointArgs {
   const double *src[5]; // source element matching region point (0,0,0)
@@ -68,7 +71,8 @@ struct PointArgs {
   double s2[2];         // ENO time rule: fallback pair weights
   int l0, teno;         //   first level of the pair; 1 = rule on (3 levels)
   int off[3];           // TVD: fine offsets (off+f)/2 = parent, (off+f)%2 = side
-  int stag;             // staggered operators: bit d set = staggered in direction d
+  int stag;             // staggered operators: bit d set = staggered in direction d;
+                        // restrict_3d_vc_rf2: bit d set = direction d cell-centred
   int sorder;           //   prolongation order (2 or 3)
   FastDiv fx, fxy;      // division by n[0] and n[0]*n[1] (batch work items)
 };
@@ -98,5 +102,15 @@ int launch_fill_fourier(double *d, long long sj, long long sk, const int *ext,
                         int nmodes, const int *kvec, const double *phase, hipStream_t s);
 int launch_fill_hash(double *d, long long sj, long long sk, const int *ext,
                      unsigned long long seed, double lo, double hi, hipStream_t s);
+// one array of a batched seeded fill (cg_fill_hash_many_f64)
+struct HashFill {
+  double *d;
+  long long sj, sk;
+  int nx, ny;
+  unsigned long long seed;
+  double lo, hi;
+};
+// f and first (n + 1 prefix counts of points, first[n] = total) in device memory
+int launch_fill_hash_many(const HashFill *f, const long long *first, int n, long long total, hipStream_t s);
 
 } // namespace cg

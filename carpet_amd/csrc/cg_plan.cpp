@@ -354,6 +354,7 @@ int plan_prolong(int kind, int order, const Arr &src, const void *const *srcs,
   t.dk = dst.sk();
   for (int l = 0; l < 3; ++l)
     t.s[l] = l < ntl ? s[l] : 0.0;
+  t.nlev = ntl;
   if (eno && ntl == 3) {
     t.teno = 1;
     t.l0 = eno->l0;
@@ -507,6 +508,62 @@ int plan_restrict(bool cell, const Arr &src, const Arr &dst, const Box &reg,
   return CG_OK;
 }
 
+// restrict_3d_vc_rf2<T, centi, centj, centk> (restrict_3d_vc_rf2.cc:
+// 107-200): a cell-centred group's restriction whose slab has vertex-centred
+// directions (cmask bit d set = direction d cell-centred); the children of
+// coarse point i start at fine index srcoff + 2i, srcoff = (reg.lo - src.lo
+// - src.str/2) / src.str, and every fine point read (srcoff + 2i, and
+// srcoff + 2i + 1 in cell-centred directions) must lie in the source array
+int plan_restrict_vc(int cmask, const Arr &src, const Arr &dst, const Box &reg, PointPlan &pl) {
+  int e;
+  if (cmask < 0 || cmask > 7)
+    return set_error(CG_ERR_ARG, "centring mask %d not in 0..7", cmask);
+  if ((e = check_array(src, "source")) || (e = check_array(dst, "destination")))
+    return e;
+  if ((e = check_region_stride(reg)))
+    return e;
+  const Box &sb = src.box, &db = dst.box;
+  for (int d = 0; d < 3; ++d)
+    if (sb.str[d] >= reg.str[d] || db.str[d] != reg.str[d])
+      return set_error(CG_ERR_STRIDE, "Internal error: strides disagree");
+  for (int d = 0; d < 3; ++d)
+    if (2 * sb.str[d] != db.str[d])
+      return set_error(CG_ERR_REFFACT, "Internal error: destination strides are "
+                                       "not twice the source strides");
+  if (reg.empty())
+    return set_error(CG_ERR_EMPTY, "Internal error: region extent is empty");
+  std::memset(&pl, 0, sizeof pl);
+  int so[3], dof[3];
+  for (int d = 0; d < 3; ++d) {
+    pl.a.n[d] = reg.sizes(d);
+    if (sb.str[d] % 2)
+      return set_error(CG_ERR_ALIGN, "source stride is not even");
+    const int num = reg.lo[d] - sb.lo[d] - sb.str[d] / 2;
+    if (num % sb.str[d] || (reg.lo[d] - db.lo[d]) % db.str[d])
+      return set_error(CG_ERR_ALIGN, "region is not aligned with the source/destination (dir %d)", d);
+    so[d] = num / sb.str[d];
+    dof[d] = (reg.lo[d] - db.lo[d]) / db.str[d];
+    const int hi = so[d] + 2 * (pl.a.n[d] - 1) + ((cmask >> d) & 1);
+    if (so[d] < 0 || hi >= src.ext[d])
+      return set_error(CG_ERR_CONTAIN, "Internal error: region extent is not contained in array extent "
+                                       "(footprint [%d,%d] vs source [0,%d) in dir %d)",
+                       so[d], hi, src.ext[d], d);
+  }
+  if (!reg.contained_in(db))
+    return set_error(CG_ERR_CONTAIN, "Internal error: region extent is not contained in array extent");
+  pl.op = PO_RESTRICT_VC2;
+  pl.a.stag = cmask;
+  pl.a.src[0] = static_cast<const double *>(src.p) + src.at(so[0], so[1], so[2]);
+  pl.a.sj = src.sj();
+  pl.a.sk = src.sk();
+  pl.a.dst = static_cast<double *>(const_cast<void *>(dst.p)) + dst.at(dof[0], dof[1], dof[2]);
+  pl.a.dj = dst.sj();
+  pl.a.dk = dst.sk();
+  pl.dst_points = (long long)pl.a.n[0] * pl.a.n[1] * pl.a.n[2];
+  pl.src_points = pl.dst_points * (1 + (cmask & 1)) * (1 + ((cmask >> 1) & 1)) * (1 + ((cmask >> 2) & 1));
+  return CG_OK;
+}
+
 // restrict_3d_stagger011.cc:26-100 (the other three alike): vertex
 // geometry; every fine point read (2i-1 .. 2i+2 in staggered directions,
 // 2i-1 .. 2i+1 in the others) must lie in the source array
@@ -569,6 +626,15 @@ bool staggered_supported() { return g_staggered.load(std::memory_order_relaxed) 
 
 extern "C" int cg_set_staggered_operators(int support_staggered_operators) {
   g_staggered.store(support_staggered_operators ? 1 : 0, std::memory_order_relaxed);
+  return CG_OK;
+}
+
+static std::atomic<int> g_dgfe{0};
+
+bool dgfe_requested() { return g_dgfe.load(std::memory_order_relaxed) != 0; }
+
+extern "C" int cg_set_dgfe(int use_dgfe) {
+  g_dgfe.store(use_dgfe ? 1 : 0, std::memory_order_relaxed);
   return CG_OK;
 }
 

@@ -73,11 +73,15 @@ template <class L, int NTL, int CX, int CY, int XQ = 1, int NB = 1> struct Tenso
 // Combination of one point's values on ntl time levels (canonical order:
 // s0*v0, then fma(s1, v1, .), fma(s2, v2, .)); with the ENO rule
 // (interpolate_eno_3d_3tl.cc:146-159) a 3-level result outside the range of
-// the three values falls back to the pair l0, l0+1.
+// the three values falls back to the pair l0, l0+1.  nlev = 1 (a transfer
+// whose target time matches level 0 exactly, run by a kernel built for NTL
+// levels) gives level 0's value untouched, as a one-level transfer does.
 template <int NTL>
 __device__ __forceinline__ double time_combine(const double *v, const double *s, const double *s2,
-                                               int l0, int teno) {
+                                               int l0, int teno, int nlev = NTL) {
   if constexpr (NTL == 1) {
+    return v[0];
+  } else if (nlev == 1) {
     return v[0];
   } else if constexpr (NTL == 2) {
     return fmad(s[1], v[1], s[0] * v[0]);
@@ -253,7 +257,7 @@ __device__ __forceinline__ void tensor_tile(const TensorArgs &a, int b, double *
 #pragma unroll
           for (int tl = 0; tl < NSRC; ++tl)
             v[tl] = (double)pre[SL][tl][k];
-          (&A[0][0][0])[e] = time_combine<NSRC>(v, a.s, a.s2, a.l0, 0);
+          (&A[0][0][0])[e] = time_combine<NSRC>(v, a.s, a.s2, a.l0, 0, a.nlev);
         } else {
 #pragma unroll
           for (int tl = 0; tl < NTL; ++tl)
@@ -341,7 +345,7 @@ __device__ __forceinline__ void tensor_tile(const TensorArgs &a, int b, double *
             v[tl] = (double)line_apply<L, PZ, CT>(
                 [&](int t) { return ring[tl][(J - NTP + 1 + t + 2 * R) % R][m][py]; });
           // time interpolation in FP64 in every precision mode
-          const double out = time_combine<NTL>(v, a.s, a.s2, a.l0, a.teno);
+          const double out = time_combine<NTL>(v, a.s, a.s2, a.l0, a.teno, a.nlev);
           if (xin && fy >= 0 && fy < a.n[1])
             __builtin_nontemporal_store((TS)out, &static_cast<TS *>(a.dst)[gfx + a.dj * fy + a.dk * fz]);
         }
@@ -652,6 +656,16 @@ __device__ __forceinline__ double point_value(const PointArgs &a, int i, int j, 
         out = time_combine<3>(v, a.s, a.s2, a.l0, 0);
       }
     }
+  } else if constexpr (OP == PO_RESTRICT_VC2) {
+    // restrict_3d_vc_rf2.cc:26-97: nested half-sums, x inside y inside z; a
+    // cell-centred direction (bit of a.stag) averages its two children, a
+    // vertex-centred one takes the first; the halves are exact, so each
+    // half*a + half*b is one rounding however it is contracted
+    const double *p = a.src[0] + 2 * i + a.sj * (2 * j) + a.sk * (2 * k);
+    const int cm = a.stag;
+    auto r1 = [&](const double *q) { return (cm & 1) ? 0.5 * q[0] + 0.5 * q[1] : q[0]; };
+    auto r2 = [&](const double *q) { return (cm & 2) ? 0.5 * r1(q) + 0.5 * r1(q + a.sj) : r1(q); };
+    out = (cm & 4) ? 0.5 * r2(p) + 0.5 * r2(p + a.sk) : r2(p);
   } else if constexpr (OP == PO_RESTRICT_STAGGER) {
     out = stagger_restrict_value(a.src[0] + 2 * i + a.sj * (2 * j) + a.sk * (2 * k), a.sj, a.sk, a.stag);
   } else if constexpr (OP == PO_RESTRICT_CC_O3) {
@@ -706,6 +720,10 @@ __device__ __forceinline__ double point_value(const PointArgs &a, int i, int j, 
     } else if (a.ntl == 3) {
       const double v[3] = {a.src[0][o], a.src[1][o], a.src[2][o]};
       out = time_combine<3>(v, a.s, a.s2, a.l0, a.teno);
+    } else if (a.ntl == 1) {
+      // a batched same-level interpolation whose target time has become an
+      // exact match: the copy of that level
+      out = a.src[0][o];
     } else {
       const double v[2] = {a.src[0][o], a.src[1][o]};
       out = time_combine<2>(v, a.s, a.s2, a.l0, a.teno);

@@ -16,7 +16,8 @@ from typing import Dict, List
 import numpy as np
 
 from .core import (CELL_CENTERED, OP_ENO, OP_LAGRANGE, OP_RESTRICT, OP_SYNC, OP_TVD, PREC_F64,
-                   PREC_MIXED, VERTEX_CENTERED, Batch, Box, DeviceArray, pad_shape, transfer_from)
+                   PREC_MIXED, VERTEX_CENTERED, Batch, Box, DeviceArray, fill_hash_many, pad_shape,
+                   transfer_from)
 from ._lib import check, lib
 
 GOLDEN = 0x9E3779B97F4A7C15
@@ -106,6 +107,18 @@ class Workload:
         else:
             for k, spec in self.arrays.items():
                 dev[k] = DeviceArray(spec.box, owned=spec.owned)
+        if pool:
+            # the whole pool poisoned with NaN (padding and unseeded arrays
+            # included), then every seeded array in one launch
+            check(lib.cg_fill_f64(self._pool, float("nan"), self.pool_bytes // 8, None), "cg_fill_f64")
+            seeded = [(dev[k], spec.seed) for k, spec in self.arrays.items()
+                      if spec.init is None and spec.seed is not None]
+            self.setup_launches = 1 + fill_hash_many(seeded)
+            for k, spec in self.arrays.items():
+                if spec.init is not None:
+                    dev[k].upload(spec.init, fill=np.nan)
+                    self.setup_launches += 1
+            return dev
         for k, spec in self.arrays.items():
             a = dev[k]
             if spec.init is not None:
@@ -212,10 +225,13 @@ def config4(n_regions: int = 2000, n_gf: int = 24, edge=(8, 96), seed: int = 4,
             levels: int = 5, order: int = 5) -> Workload:
     """Batched uneven level step: n_regions destination regions per grid
     function with edges uniform in [edge[0], edge[1]] fine points per axis,
-    split evenly between copy sync (ghost width 3 taken from a neighbour
-    component), order-5 Lagrange prolongation with 3-level time interpolation
-    (the refinement-boundary + buffer shell, sources at t = 0, -dt, -2dt
-    interpolated to t = -dt/2) and rf2 restriction.  Regions belong to one of
+    split evenly between copies (copy_3d of a whole region box from another
+    array whose box is offset from it, the kernel of a sync), order-5
+    Lagrange prolongation with 3-level time interpolation (sources at t = 0,
+    -dt, -2dt interpolated to t = -dt/2) and rf2 restriction.  Regions are
+    whole boxes, not the thin ghost and buffer shells of a real hierarchy
+    (those come from its communication schedule: hierarchy_step); each
+    destination array holds its region plus 3 ghost points per side.  Regions belong to one of
     `levels` refinement levels (stride 2^(levels-1-rl) in finest units).
     Every region has its own destination and source arrays, so the
     destinations of a phase are disjoint, as dh's lists guarantee.
@@ -235,7 +251,7 @@ def config4(n_regions: int = 2000, n_gf: int = 24, edge=(8, 96), seed: int = 4,
         for g in range(n_gf):
             tag = f"r{r}g{g}"
             sd = seed * 1000003 + r * 1009 + g
-            if kind == 0:  # sync: dst ghost slab region from a neighbour array
+            if kind == 0:  # copy: the region from another array offset from it
                 lo = tuple(b * sf for b in base)
                 reg = Box(lo, tuple(l + (k - 1) * sf for l, k in zip(lo, n)), (sf,) * 3)
                 dbox = reg.expand((3, 3, 3), (3, 3, 3))

@@ -164,6 +164,17 @@ int cg_fill_f64(double *dst, double value, size_t n, cg_stream_t s);
    ext[0]*(j + ext[1]*k)) scaled to [0,1) */
 int cg_fill_hash_f64(double *dst, const int padext[3], const int ext[3],
                      unsigned long long seed, double lo, double hi, cg_stream_t s);
+/* no reference counterpart: many cg_fill_hash_f64 fills in one launch
+   (setting up thousands of arrays without a launch per array); returns
+   once the fills have run on stream s */
+typedef struct cg_hash_fill {
+  double *dst;       /* element (0,0,0) */
+  int padext[3];
+  int ext[3];
+  unsigned long long seed;
+  double lo, hi;
+} cg_hash_fill;
+int cg_fill_hash_many_f64(int n, const cg_hash_fill *fills, cg_stream_t s);
 /* no reference counterpart: smooth synthetic data; every point of g's
    extent at global index X becomes amp * sum_m sin(2 pi k_m.(X*scale) +
    phase_m), nmodes <= 8, kvec[3*m+q] */
@@ -354,6 +365,31 @@ int cg_restrict_3d_cc_o5_rf2_f64(const double *src, const int srcpadext[3],
                                  const int srcbbox[9], const int dstbbox[9],
                                  const int srcregbbox[9], const int dstregbbox[9],
                                  cg_stream_t stream);
+/* restrict_3d_vc_rf2<double, centi, centj, centk>:
+   CarpetLib/src/restrict_3d_vc_rf2.cc:26-260 -- the restriction of a
+   cell-centred group's slab with one vertex-centred direction (is_centered
+   (0,1,1), (1,0,1) or (1,1,0), data.cc:1106-1127; face-centred data such
+   as refluxing fluxes).  Per coarse point nested half-sums, x innermost: a
+   cell-centred direction (cent = 1) averages fine points srcoff+2i and
+   srcoff+2i+1, a vertex-centred one takes srcoff+2i, with srcoff = (reg.lo
+   - src.lo - src.stride/2) / src.stride; every point read must lie in the
+   source array.  cg_transfer_from dispatches it for a cell-centred
+   restriction whose slab has such an is_centered. */
+int cg_restrict_3d_vc_rf2_f64(int centi, int centj, int centk, const double *src,
+                              const int srcpadext[3], const int srcext[3], double *dst,
+                              const int dstpadext[3], const int dstext[3], const int srcbbox[9],
+                              const int dstbbox[9], const int srcregbbox[9], const int dstregbbox[9],
+                              cg_stream_t stream);
+/* CarpetLib's use_dgfe parameter (param.ccl:30, default no).  When set, the
+   reference replaces cell-centred Lagrange prolongation and full-cell
+   restriction by prolongate_3d_dgfe_rf2 / restrict_3d_dgfe_rf2<T,5>
+   (data.cc:617-627, 1055-1062), which evaluate the discontinuous-Galerkin
+   element operators of the optional HRSCCore thorn (hrscc.hh,
+   GLLElement<5>); that code is not part of the reference tree and this
+   library does not provide it, so with the parameter set those transfers
+   return CG_ERR_OPERATOR instead of silently running Lagrange.
+   Process-wide. */
+int cg_set_dgfe(int use_dgfe);
 /* CarpetLib's parameters use_higher_order_restriction and
    restriction_order_space (CarpetLib/param.ccl) as the transfer dispatch
    sees them: cell-centred restrictions of groups whose operator is not
@@ -429,7 +465,8 @@ int cg_transfer_from(const cg_gdata *dst, const cg_gdata *const *srcs,
    class) runs as one launch over it -- one workgroup per work item, or a
    persistent grid taking items from per-XCD counters; the whole step can
    be captured as one hipGraph and replayed while the hierarchy is
-   unchanged. */
+   unchanged, with new target times and rotated time levels between
+   replays (cg_batch_set_time, cg_batch_set_storage). */
 typedef struct cg_batch cg_batch;
 int cg_batch_create(cg_batch **out);
 int cg_batch_destroy(cg_batch *b);
@@ -453,6 +490,33 @@ int cg_batch_run_plain(cg_batch *b, cg_stream_t stream);
 /* captures cg_batch_run into a hipGraph, then replays it */
 int cg_batch_capture_graph(cg_batch *b, cg_stream_t stream);
 int cg_batch_launch_graph(cg_batch *b, cg_stream_t stream);
+/* Per-step parameters of a batch: what changes between the reference's
+   level steps without a regrid -- the target time of each substep, which
+   ggf::transfer_from_all passes down to data<T>::transfer_time and
+   gdata::find_source_timelevel (CarpetLib/src/ggf.cc:500-591,
+   data.cc:485-534, gdata.cc:288-349), and the rotation of the time levels'
+   storage by ggf::cycle_all (CarpetLib/src/ggf.cc:248-265).  Each call
+   resolves the affected transfers again exactly as cg_batch_add did (same
+   validation, time-level choice and weights) and keeps the launches, work
+   items and a captured graph: the changed descriptors are uploaded in
+   stream order by the next cg_batch_run or cg_batch_launch_graph on that
+   stream (a pinned staging copy; nothing inside the graph changes).  A
+   transfer must stay in its launch class: a tensor prolongation built with
+   2 or 3 time levels may also resolve to 1 (a target time that matches a
+   level exactly runs in the same class on that level alone, bit-identical
+   to a one-level transfer); one built at an exact match (1 level) cannot
+   later interpolate; operators and stencils cannot change.  Otherwise
+   CG_ERR_STATE, and the batch is unchanged.
+   cg_batch_set_time: every transfer of `phase` added with exactly ntimes
+   source levels gets these level times and target time; returns the number
+   of transfers updated.
+   cg_batch_set_storage: every destination or source level of any transfer
+   whose storage is from[i].storage now uses to[i] (the same layout:
+   padext, ext, bbox), all pairs applied at once -- ggf::cycle_all is the
+   rotation from = {tl0, tl1, tl2}, to = {tl2, tl0, tl1} per grid function;
+   returns the number of transfers updated. */
+int cg_batch_set_time(cg_batch *b, int phase, const double *times, int ntimes, double time);
+int cg_batch_set_storage(cg_batch *b, int n, const cg_gdata *from, const cg_gdata *to);
 /* Autotuner: times each launch of the finalized batch under candidate
    work-item sizes (tensor tiles: z chunk of 8/16/32/64/128/all coarse bases;
    pointwise items: 2048..65536 points) in the class's default mode
@@ -480,6 +544,14 @@ int cg_batch_autotune(cg_batch *b, cg_stream_t stream, int reps,
    class key, work items, grid, tile choice, grid mode); returns the full
    length */
 int cg_batch_describe(const cg_batch *b, char *buf, size_t len);
+/* no reference counterpart: the launches of a finalized batch one at a
+   time, for per-kernel timing -- their number, launch idx run on its own
+   (its work counters cleared first), and its phase, destination points and
+   algorithmic bytes */
+int cg_batch_launch_count(const cg_batch *b);
+int cg_batch_run_launch(cg_batch *b, int idx, cg_stream_t stream);
+int cg_batch_launch_stats(const cg_batch *b, int idx, int *phase, long long *dst_points,
+                          long long *alg_bytes);
 /* statistics: number of transfers, destination points and algorithmic
    bytes (destination + unique source footprint per time level) */
 int cg_batch_stats(const cg_batch *b, long long *ntransfers,

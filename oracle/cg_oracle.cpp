@@ -27,6 +27,7 @@
 //   CarpetLib/src/prolongate_3d_cc_real8_tvd.F90:55-244
 //   CarpetLib/src/restrict_3d_rf2.cc:25-121
 //   CarpetLib/src/restrict_3d_cc_rf2.cc:23-152
+//   CarpetLib/src/restrict_3d_vc_rf2.cc:26-260
 //   CarpetLib/src/prolongate_3d_stagger{011,101,110,111}.cc:24-561,
 //     coeffs_prolongate_3d_stagger.hh; restrict_3d_stagger{...}.cc:26-153,
 //     coeffs_restrict_3d_stagger.hh
@@ -35,6 +36,7 @@
 //   CarpetLib/src/gdata.cc:288-349 (find_source_timelevel)
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstddef>
 #include <cstdint>
@@ -1122,6 +1124,73 @@ int orc_restrict_3d_cc_rf2(const double *src, const int *spad, const int *sext,
   return CG_OK;
 }
 
+// restrict_3d_vc_rf2<T, centi, centj, centk>, restrict_3d_vc_rf2.cc:26-260:
+// restriction of a cell-centred group's slab whose is_centered vector has
+// one vertex-centred direction (data.cc:1106-1127; cmask bit d set =
+// direction d cell-centred).  Per coarse point the nested half-sums of
+// restrict1/2/3 (:26-97): a cell-centred direction averages its two
+// children p and p + d (half*a + half*b), a vertex-centred one takes p;
+// x innermost, then y, then z.  The children start at fine index
+// srcoff + 2i, srcoff = (reg.lo - src.lo - src.str/2) / src.str (:161-171).
+static double vc_restrict1(const double *p, long long d1, int cx) {
+  const double half = 1.0 / 2;
+  return cx ? half * p[0] + half * p[d1] : p[0];
+}
+static double vc_restrict2(const double *p, long long d1, long long d2, int cx, int cy) {
+  const double half = 1.0 / 2;
+  return cy ? half * vc_restrict1(p, d1, cx) + half * vc_restrict1(p + d2, d1, cx) : vc_restrict1(p, d1, cx);
+}
+static double vc_restrict3(const double *p, long long d1, long long d2, long long d3, int cx, int cy, int cz) {
+  const double half = 1.0 / 2;
+  return cz ? half * vc_restrict2(p, d1, d2, cx, cy) + half * vc_restrict2(p + d3, d1, d2, cx, cy)
+            : vc_restrict2(p, d1, d2, cx, cy);
+}
+
+int orc_restrict_3d_vc_rf2(int cmask, const double *src, const int *spad, const int *sext, double *dst,
+                           const int *dpad, const int *dext, const int *sb, const int *db, const int *srb,
+                           const int *drb) {
+  (void)srb;
+  const box3 s = mkbox(sb), dbx = mkbox(db), r = mkbox(drb);
+  int e;
+  if ((e = check_array(spad, sext, s)) || (e = check_array(dpad, dext, dbx)))
+    return e;
+  for (int d = 0; d < 3; ++d) {
+    if (r.str[d] <= 0)
+      return CG_ERR_ARG;
+    if (s.str[d] >= r.str[d] || dbx.str[d] != r.str[d])
+      return CG_ERR_STRIDE;
+    if (2 * s.str[d] != dbx.str[d])
+      return CG_ERR_REFFACT;
+    if (s.str[d] % 2)
+      return CG_ERR_ALIGN;
+  }
+  if (empty(r))
+    return CG_ERR_EMPTY;
+  int n[3], so[3], dof[3], cen[3];
+  for (int d = 0; d < 3; ++d) {
+    cen[d] = (cmask >> d) & 1;
+    if ((r.lo[d] - s.lo[d] - s.str[d] / 2) % s.str[d] || (r.lo[d] - dbx.lo[d]) % dbx.str[d])
+      return CG_ERR_ALIGN;
+    n[d] = shape(r, d) / r.str[d];
+    so[d] = (r.lo[d] - s.lo[d] - s.str[d] / 2) / s.str[d];
+    dof[d] = (r.lo[d] - dbx.lo[d]) / dbx.str[d];
+    // fine points srcoff + 2i (+1 in a cell-centred direction) in the source
+    if (so[d] < 0 || so[d] + 2 * (n[d] - 1) + cen[d] >= sext[d])
+      return CG_ERR_CONTAIN;
+  }
+  if (!contained(r, dbx))
+    return CG_ERR_CONTAIN;
+  const long long d1 = 1, d2 = spad[0], d3 = (long long)spad[0] * spad[1];
+#pragma omp parallel for collapse(2) if (g_use_openmp)
+  for (int k = 0; k < n[2]; ++k)
+    for (int j = 0; j < n[1]; ++j)
+      for (int i = 0; i < n[0]; ++i)
+        dst[idx3(dof[0] + i, dof[1] + j, dof[2] + k, dpad)] =
+            vc_restrict3(&src[idx3(so[0] + 2 * i, so[1] + 2 * j, so[2] + 2 * k, spad)], d1, d2, d3, cen[0],
+                         cen[1], cen[2]);
+  return CG_OK;
+}
+
 // Higher-order cell-centred restriction (use_higher_order_restriction,
 // data.cc:1064-1094): restrict_3d_cc_o3_rf2.cc:33-212 evaluates at the coarse
 // cell centre the cubic through 4 fine cells per axis, 64 terms with weights
@@ -1702,6 +1771,131 @@ int orc_prolongate_time(int kind, int order, const double *const *srcs,
                                       drb, drb);
   return orc_interpolate_3d(ntl, tp, times, text, text, dst, time, dpad, dext,
                             drb, db, drb, drb);
+}
+
+// ---- a level step's transfers on the CPU (the timed CPU baseline) ---------
+// One transfer of a vertex-centred level step: 0 copy_3d (no slab),
+// 1 restrict_3d_rf2, 2 prolongate_3d_rf2<order> with ntl source levels
+// interpolated in time (ntl = 1: none).
+struct orc_task {
+  int op, order, ntl;
+  const double *src[3];
+  int spad[3], sext[3], sb[9];
+  double *dst;
+  int dpad[3], dext[3], db[9];
+  int rb[9];
+  double times[3], time;
+};
+
+// mode 0, the reference's shape: one operator call per transfer in order,
+// each parallelised inside as the reference does it (call_operator's split
+// along the longest axis for prolongation, collapse loops for copies and
+// restriction) and time interpolation through region-sized temporaries
+// (data.cc:56-100, 447-452, 497-521, 1100-1105).
+// mode 1, the best this restatement does on a CPU: the transfers' planes
+// as one work list under one parallel loop (no fork/join per region), each
+// destination point of a prolongation evaluated on its ntl levels and
+// combined at once (no temporaries); the same arithmetic, so the same bits.
+// Returns 0 or the first failing operator's code.
+int orc_replay(int mode, int n, const orc_task *t) {
+  if (mode == 0) {
+    for (int i = 0; i < n; ++i) {
+      const orc_task &x = t[i];
+      int e;
+      switch (x.op) {
+      case 0:
+        e = orc_copy_3d(x.src[0], x.spad, x.sext, x.dst, x.dpad, x.dext, x.sb, x.db, x.rb, x.rb, nullptr);
+        break;
+      case 1:
+        e = orc_restrict_3d_rf2(x.src[0], x.spad, x.sext, x.dst, x.dpad, x.dext, x.sb, x.db, x.rb, x.rb);
+        break;
+      case 2:
+        e = orc_prolongate_time(0, x.order, x.src, x.times, x.ntl, x.spad, x.sext, x.dst, x.time, x.dpad,
+                                x.dext, x.sb, x.db, x.rb, x.ntl - 1);
+        break;
+      default:
+        e = CG_ERR_OPERATOR;
+      }
+      if (e)
+        return e;
+    }
+    return CG_OK;
+  }
+  // fused: validate every transfer by its per-call operator first (on a
+  // copy of nothing: the checks run before any store), then one work list
+  struct Job {
+    int task, k;
+  };
+  std::vector<Job> jobs;
+  std::vector<std::array<int, 9>> geo((size_t)n); // n[3], source offset[3], destination offset[3]
+  std::vector<std::array<double, 3>> w((size_t)n);
+  for (int i = 0; i < n; ++i) {
+    const orc_task &x = t[i];
+    const box3 s = mkbox(x.sb), d = mkbox(x.db), r = mkbox(x.rb);
+    int e;
+    if ((e = check_array(x.spad, x.sext, s)) || (e = check_array(x.dpad, x.dext, d)))
+      return e;
+    if (empty(r))
+      return CG_ERR_EMPTY;
+    if (!contained(r, d))
+      return CG_ERR_CONTAIN;
+    for (int q = 0; q < 3; ++q) {
+      const int m = sizes(r, q);
+      geo[i][q] = m;
+      geo[i][6 + q] = (r.lo[q] - d.lo[q]) / d.str[q];
+      if (x.op == 2) {
+        // fine index of the region start relative to the source array
+        geo[i][3 + q] = (r.lo[q] - s.lo[q]) / r.str[q];
+        const Stencil1D st = vc_stencil(x.order);
+        const int g0 = geo[i][3 + q], g1 = g0 + m - 1;
+        if ((g0 >> 1) + st.tmin[1] < 0 || (g1 >> 1) + st.tmin[1] + st.ntap[1] - 1 >= x.sext[q])
+          return CG_ERR_CONTAIN;
+      } else {
+        geo[i][3 + q] = (r.lo[q] - s.lo[q]) / s.str[q];
+        const int hi = geo[i][3 + q] + (x.op == 1 ? 2 * (m - 1) : m - 1);
+        if (geo[i][3 + q] < 0 || hi >= x.sext[q])
+          return CG_ERR_CONTAIN;
+      }
+    }
+    w[i] = {1.0, 0.0, 0.0};
+    if (x.op == 2 && x.ntl > 1 && (e = orc_time_factors(x.ntl, x.times, x.time, w[i].data())))
+      return e;
+    for (int k = 0; k < geo[i][2]; ++k)
+      jobs.push_back({i, k});
+  }
+#pragma omp parallel for schedule(dynamic, 1)
+  for (size_t jb = 0; jb < jobs.size(); ++jb) {
+    const orc_task &x = t[jobs[jb].task];
+    const std::array<int, 9> &g = geo[jobs[jb].task];
+    const int k = jobs[jb].k;
+    if (x.op == 2) {
+      const Stencil1D st = vc_stencil(x.order);
+      const std::array<double, 3> &s3 = w[jobs[jb].task];
+      const int gk = g[5] + k;
+      for (int j = 0; j < g[1]; ++j) {
+        const int gj = g[4] + j;
+        for (int i = 0; i < g[0]; ++i) {
+          const int gi = g[3] + i;
+          double v[3] = {0.0, 0.0, 0.0};
+          for (int l = 0; l < x.ntl; ++l)
+            v[l] = lag3(st, x.src[l], x.spad, gi >> 1, gj >> 1, gk >> 1, gi & 1, gj & 1, gk & 1);
+          double out = v[0];
+          if (x.ntl == 2)
+            out = std::fma(s3[1], v[1], s3[0] * v[0]);
+          else if (x.ntl == 3)
+            out = std::fma(s3[2], v[2], std::fma(s3[1], v[1], s3[0] * v[0]));
+          x.dst[idx3(g[6] + i, g[7] + j, g[8] + k, x.dpad)] = out;
+        }
+      }
+    } else {
+      const int f = x.op == 1 ? 2 : 1;
+      for (int j = 0; j < g[1]; ++j)
+        for (int i = 0; i < g[0]; ++i)
+          x.dst[idx3(g[6] + i, g[7] + j, g[8] + k, x.dpad)] =
+              x.src[0][idx3(g[3] + f * i, g[4] + f * j, g[5] + f * k, x.spad)];
+    }
+  }
+  return CG_OK;
 }
 
 } // extern "C"

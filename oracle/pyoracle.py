@@ -62,6 +62,7 @@ _decl("orc_set_stagger", None, C.c_int)
 _decl("orc_restrict_3d_rf2", C.c_int, *_OP)
 _decl("orc_restrict_3d_cc_rf2", C.c_int, *_OP)
 _decl("orc_restrict_3d_cc_ho_rf2", C.c_int, C.c_int, *_OP)
+_decl("orc_restrict_3d_vc_rf2", C.c_int, C.c_int, *_OP)
 _decl("orc_time_factors", C.c_int, C.c_int, _dp, C.c_double, _dp)
 _decl("orc_interpolate_3d", C.c_int, C.c_int, C.POINTER(_dp), _dp, _ip, _ip, _dp, C.c_double,
       _ip, _ip, _ip, _ip, _ip, _ip)
@@ -191,6 +192,12 @@ def restrict_3d_cc_rf2(src, dst, dstreg) -> int:
     return _lib.orc_restrict_3d_cc_rf2(*_args(src, dst, dstreg, dstreg))
 
 
+def restrict_3d_vc_rf2(cent, src, dst, dstreg) -> int:
+    """restrict_3d_vc_rf2<T, centi, centj, centk>; cent = (centi, centj, centk)"""
+    cmask = int(cent[0]) | (int(cent[1]) << 1) | (int(cent[2]) << 2)
+    return _lib.orc_restrict_3d_vc_rf2(cmask, *_args(src, dst, dstreg, dstreg))
+
+
 def restrict_3d_cc_ho_rf2(order, src, dst, dstreg) -> int:
     """restrict_3d_cc_o3_rf2 (order 3) / restrict_3d_cc_o5_rf2 (order 5)"""
     return _lib.orc_restrict_3d_cc_ho_rf2(order, *_args(src, dst, dstreg, dstreg))
@@ -271,3 +278,58 @@ def cc_coeff_selftest(order, di) -> float:
 
 def eno_coeff_selftest(order, di, offset) -> float:
     return _lib.orc_eno_coeff_selftest(order, di, offset)
+
+
+class Task(C.Structure):
+    """orc_task: one transfer of a vertex-centred level step (op 0 copy_3d,
+    1 restrict_3d_rf2, 2 prolongate_3d_rf2 + time interpolation)."""
+    _fields_ = [("op", C.c_int), ("order", C.c_int), ("ntl", C.c_int), ("src", C.c_void_p * 3),
+                ("spad", C.c_int * 3), ("sext", C.c_int * 3), ("sb", C.c_int * 9), ("dst", C.c_void_p),
+                ("dpad", C.c_int * 3), ("dext", C.c_int * 3), ("db", C.c_int * 9), ("rb", C.c_int * 9),
+                ("times", C.c_double * 3), ("time", C.c_double)]
+
+
+_decl("orc_replay", C.c_int, C.c_int, C.c_int, C.POINTER(Task))
+
+
+def level_step_tasks(steps, host):
+    """orc_task list for workload transfers (copies, rf2 restrictions and
+    vertex Lagrange prolongations with 1-3 time levels) over HostArr arrays;
+    keeps the arrays alive through the returned object."""
+    arr = (Task * len(steps))()
+    for k, t in enumerate(steps):
+        d = host[t.dst]
+        ss = [host[x] for x in t.srcs]
+        sst, dst_ = ss[0].box[2], d.box[2]
+        x = arr[k]
+        if tuple(sst) == tuple(dst_):
+            x.op = 0
+        elif all(a > b for a, b in zip(sst, dst_)):
+            x.op = 2
+        else:
+            x.op = 1
+        x.order = t.order_space
+        x.ntl = len(ss) if x.op == 2 else 1
+        for l, h in enumerate(ss[:3]):
+            x.src[l] = h.data.ctypes.data
+        s0 = ss[0]
+        x.spad[:] = list(s0.padext)
+        x.sext[:] = list(s0.ext)
+        x.sb[:] = list(s0.box[0]) + list(s0.box[1]) + list(s0.box[2])
+        x.dst = d.data.ctypes.data
+        x.dpad[:] = list(d.padext)
+        x.dext[:] = list(d.ext)
+        x.db[:] = list(d.box[0]) + list(d.box[1]) + list(d.box[2])
+        r = t.dstreg
+        x.rb[:] = list(r.lower) + list(r.upper) + list(r.stride)
+        for l in range(min(3, len(t.times))):
+            x.times[l] = t.times[l]
+        x.time = t.time
+    return arr
+
+
+def replay(mode, tasks) -> int:
+    """orc_replay: mode 0 one operator call per transfer (the reference's
+    shape), mode 1 all transfers in one parallel loop, time interpolation
+    fused."""
+    return _lib.orc_replay(mode, len(tasks), tasks)

@@ -80,3 +80,60 @@ def test_recompose_fill_matches_oracle(cent, order):
             if not same(got, ref[k]):
                 raise AssertionError(f"{mode}: {k} differs {first_diff(got, ref[k])}")
         w.free(dev)
+
+
+@pytest.mark.parametrize("cent,order,op", [(cg.VERTEX_CENTERED, 5, cg.OP_LAGRANGE), (cg.CELL_CENTERED, 5, cg.OP_ENO)],
+                         ids=["vertex-o5", "cell-eno3"])
+def test_captured_step_advances_in_time(cent, order, op):
+    """One captured level step replayed as a run advances: the boundary
+    prolongation at the intermediate fine time (-1/2, interpolating), at the
+    coarse-aligned time (0, an exact match: one level), then after every
+    level's time levels rotate (ggf::cycle_all) at 1/2 and 1 against new
+    level times -- new target times and storage go in through
+    cg_batch_set_time / cg_batch_set_storage, the graph is never rebuilt.
+    After each replay every array equals the oracle's sequential replay of
+    the same steps, bit for bit."""
+    from dataclasses import replace
+
+    from workload_check import host_state
+
+    lv = nested_levels(ncoarse=32, nlevels=3, split=(2, 1, 1), centering=cent, buffer=6, order=order)
+    w = hierarchy_step(lv, cent, n_gf=2, op=op)
+    host = host_state(w)
+    st = cg.Stream()
+    dev = w.allocate()
+    b = w.batch(dev)
+    b.finalize()
+    b.capture_graph(st)
+    prol = sorted({t.phase for t in w.steps if len(t.srcs) == 3})
+    rot = {"t0": "t2", "t1": "t0", "t2": "t1"}  # storage of level tl after one cycle: old tl-1
+
+    def ren(name):
+        return name[:-2] + rot[name[-2:]]
+
+    steps = list(w.steps)
+    plan = [("interpolating -1/2", None, [0.0, -1.0, -2.0], -0.5), ("exact 0", None, [0.0, -1.0, -2.0], 0.0),
+            ("rotated, interpolating 1/2", "cycle", [1.0, 0.0, -1.0], 0.5),
+            ("rotated, exact 1", None, [1.0, 0.0, -1.0], 1.0)]
+    for k, (what, cycle, times, t) in enumerate(plan):
+        print(f"step {k}: {what}", flush=True)
+        if cycle:
+            n = b.set_storage([(dev[a], dev[ren(a)]) for a in dev])
+            assert n == len(w.steps)
+            steps = [replace(s, dst=ren(s.dst), srcs=[ren(x) for x in s.srcs]) for s in steps]
+        if k:
+            for ph in prol:
+                assert b.set_time(ph, times, t) > 0
+        steps = [replace(s, times=list(times), time=t) if len(s.srcs) == 3 else s for s in steps]
+        b.launch_graph(st)
+        st.synchronize()
+        run_oracle(w, host, steps)
+        for name, h in host.items():
+            got = dev[name].download()
+            if not same(got, h.data):
+                raise AssertionError(f"step {k} ({what}): {name} differs {first_diff(got, h.data)}")
+    # a target time outside the levels is refused and changes nothing
+    with pytest.raises(cg.CarpetError):
+        b.set_time(prol[0], [1.0, 0.0, -1.0], 5.0)
+    b.close()
+    w.free(dev)

@@ -781,3 +781,52 @@ def test_slabbed_copy_with_time_interpolation(ntl):
         cg.transfer_from(ddst, ds, times, cg.Box(*rb), cg.Box(*srb), t, cg.OP_LAGRANGE, cg.VERTEX_CENTERED, 1,
                          ntl - 1, slab_offset=off)
         _check(exp, ddst, f"slab copy+time case {c}")
+
+
+# ---- restrict_3d_vc_rf2 -------------------------------------------------------
+VC_CENTS = [(0, 1, 1), (1, 0, 1), (1, 1, 0)]
+
+
+@pytest.mark.parametrize("cent", VC_CENTS, ids=["011", "101", "110"])
+def test_restrict_vc_parity(cent):
+    """restrict_3d_vc_rf2 against the oracle, bit for bit: per call, through
+    cg_transfer_from with a slab of that is_centered (data.cc:1106-1127),
+    with 2-level time interpolation (levels restricted into temporaries),
+    and in a batch."""
+    from test_oracle_pins import _vc_case
+    rng = np.random.default_rng(1200 + cent[0] + 2 * cent[1])
+    for c in range(4):
+        n = (1, 1, 1) if c == 0 else tuple(int(v) for v in rng.integers(2, 40, size=3))
+        print(f"restrict_vc {cent} #{c} n={n}", flush=True)
+        sb, db, rb = _vc_case(rng, cent, n)
+        src = _src_host(rng, sb, "random")
+        dst = host_array(db, pad_of(sizes(db), rng))
+        exp = orc.HostArr(db, dst.data.copy())
+        assert orc.restrict_3d_vc_rf2(cent, src, exp, rb) == 0
+        dsrc, ddst = _dev(src), _dev(dst)
+        cg.restrict_3d_vc_rf2(cent, dsrc, ddst, cg.Box(*rb))
+        _check(exp, ddst, f"restrict_vc {cent} case {c}")
+        ddst = _dev(dst)
+        cg.transfer_from(ddst, [dsrc], [0.0], cg.Box(*rb), cg.Box(*rb), 0.0, cg.OP_LAGRANGE, cg.CELL_CENTERED, 3, 0,
+                         is_centered=cent)
+        _check(exp, ddst, f"transfer restrict_vc {cent} case {c}")
+        ddst = _dev(dst)
+        b = cg.Batch()
+        b.add(0, ddst, [dsrc], [0.0], cg.Box(*rb), cg.Box(*rb), 0.0, cg.OP_LAGRANGE, cg.CELL_CENTERED, 3, 0,
+              is_centered=cent)
+        b.finalize()
+        b.run()
+        cg.synchronize()
+        _check(exp, ddst, f"batched restrict_vc {cent} case {c}")
+        b.close()
+        # two time levels: each restricted into a temporary, then interpolated
+        src2 = orc.HostArr(sb, np.full_like(src.data, np.nan))  # same padding as level 1
+        src2.view()[...] = field(rng, sizes(sb), "random")
+        texp = orc.HostArr(db, dst.data.copy())
+        finish = _tmp_interp([src, src2], [1.0, 0.0], 0.3, rb, lambda h, tmp: orc.restrict_3d_vc_rf2(cent, h, tmp, rb),
+                             False)
+        assert finish(texp) == 0
+        ddst = _dev(dst)
+        cg.transfer_from(ddst, [dsrc, _dev(src2)], [1.0, 0.0], cg.Box(*rb), cg.Box(*rb), 0.3, cg.OP_LAGRANGE,
+                         cg.CELL_CENTERED, 3, 1, is_centered=cent)
+        _check(texp, ddst, f"restrict_vc {cent} with time case {c}")

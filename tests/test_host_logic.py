@@ -26,7 +26,7 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_error_strings():
-    for code in range(0, -16, -1):
+    for code in range(0, -17, -1):
         assert lib.cg_error_string(code).decode() != "unknown error"
 
 
@@ -178,3 +178,94 @@ def test_transfer_dispatch_validation():
     creg = cg.Box((8, 8, 8), (12, 12, 12), (2, 2, 2)).as_c()
     assert lib.cg_transfer_from(C.byref(c), csrc, tt, 1, creg, creg, None, 0.0, cg.OP_STAGGER111,
                                 cg.CELL_CENTERED, 3, 0, None) == -11
+
+
+def test_batch_set_time_keeps_launch_classes():
+    """cg_batch_set_time resolves the phase's transfers again; a transfer
+    added at an exact time match (one level) cannot start to interpolate
+    (its launch class has one level): CG_ERR_STATE, and the batch keeps its
+    state; a target outside the levels is the reference's extrapolation
+    error.  Host logic only (the batch is never finalized)."""
+    from carpet_amd._lib import GData
+    c = GData()
+    c.storage = FAKE
+    c.padext[:] = [21, 21, 21]
+    c.ext[:] = [21, 21, 21]
+    c.bbox[:] = [0, 0, 0, 40, 40, 40, 2, 2, 2]
+    f = GData()
+    f.storage = FAKE
+    f.padext[:] = [41, 41, 41]
+    f.ext[:] = [41, 41, 41]
+    f.bbox[:] = [0, 0, 0, 40, 40, 40, 1, 1, 1]
+    srcs = (C.POINTER(GData) * 3)(C.pointer(c), C.pointer(c), C.pointer(c))
+    reg = cg.Box((8, 8, 8), (12, 12, 12), (1, 1, 1)).as_c()
+    tt = (C.c_double * 3)(0.0, -1.0, -2.0)
+    h = C.c_void_p()
+    assert lib.cg_batch_create(C.byref(h)) == 0
+    try:
+        assert lib.cg_batch_add(h, 1, C.byref(f), srcs, tt, 3, reg, reg, None, 0.0, cg.OP_LAGRANGE,
+                                cg.VERTEX_CENTERED, 5, 2, 0) == 0
+        assert lib.cg_batch_add(h, 1, C.byref(f), srcs, tt, 3, reg, reg, None, -0.5, cg.OP_LAGRANGE,
+                                cg.VERTEX_CENTERED, 5, 2, 0) == 1
+        nt = (C.c_double * 3)(1.0, 0.0, -1.0)
+        assert lib.cg_batch_set_time(h, 1, nt, 3, 0.5) == -14  # transfer 0 was built on one level
+        assert "time levels" in lib.cg_last_error_message().decode()
+        assert lib.cg_batch_set_time(h, 1, nt, 3, 3.0) == -9  # extrapolation
+        assert lib.cg_batch_set_time(h, 1, nt, 3, 1.0) == 2  # exact matches: both stay in class
+        assert lib.cg_batch_set_time(h, 2, nt, 3, 0.5) == 0  # no transfer in phase 2
+        assert lib.cg_batch_set_time(h, 1, nt, 2, 0.5) == 0  # none added with 2 levels
+    finally:
+        lib.cg_batch_destroy(h)
+
+
+def test_vc_restriction_and_dgfe_dispatch():
+    """A cell-centred restriction whose slab is_centered is not one of the
+    reference's four cases is refused (data.cc:1106-1129 asserts), and with
+    use_dgfe set the DGFE transfers are refused instead of running
+    Lagrange (data.cc:617-627, 1055-1062)."""
+    from carpet_amd._lib import GData, Islab
+    cbox = [2, 2, 2, 2 + 4 * 9, 2 + 4 * 9, 2 + 4 * 9, 4, 4, 4]
+    c = GData()
+    c.storage = FAKE
+    c.padext[:] = [10, 10, 10]
+    c.ext[:] = [10, 10, 10]
+    c.bbox[:] = cbox
+    f = GData()
+    f.storage = FAKE
+    f.padext[:] = [24, 24, 24]
+    f.ext[:] = [24, 24, 24]
+    f.bbox[:] = [-1, -1, -1, -1 + 2 * 23, -1 + 2 * 23, -1 + 2 * 23, 2, 2, 2]
+    tt = (C.c_double * 1)(0.0)
+    creg = cg.Box((6, 6, 6), (18, 18, 18), (4, 4, 4)).as_c()
+    fsrc = (C.POINTER(GData) * 1)(C.pointer(f))
+
+    def rest(cent, op=cg.OP_LAGRANGE):
+        slab = Islab(I3(0, 0, 0), I3(*cent))
+        return lib.cg_transfer_from(C.byref(c), fsrc, tt, 1, creg, creg, C.byref(slab), 0.0, op,
+                                    cg.CELL_CENTERED, 3, 0, None)
+
+    for bad in [(0, 0, 1), (1, 0, 0), (0, 0, 0)]:
+        assert rest(bad) == -11, bad
+    assert rest((2, 1, 1)) == -1
+    assert lib.cg_restrict_3d_vc_rf2_f64(0, 1, 2, FAKE, I3(24, 24, 24), I3(24, 24, 24), FAKE, I3(10, 10, 10),
+                                         I3(10, 10, 10), f.bbox, c.bbox, creg, creg, None) == -1
+    # refinement factor and footprint checks of the per-call entry
+    assert lib.cg_restrict_3d_vc_rf2_f64(0, 1, 1, FAKE, I3(10, 10, 10), I3(10, 10, 10), FAKE, I3(10, 10, 10),
+                                         I3(10, 10, 10), c.bbox, c.bbox, creg, creg, None) == -2
+    far = cg.Box((6, 6, 6), (46, 18, 18), (4, 4, 4)).as_c()
+    assert lib.cg_restrict_3d_vc_rf2_f64(0, 1, 1, FAKE, I3(24, 24, 24), I3(24, 24, 24), FAKE, I3(10, 10, 10),
+                                         I3(10, 10, 10), f.bbox, c.bbox, far, far, None) == -6
+    cg.set_dgfe(True)
+    try:
+        # full-cell restriction and cell-centred Lagrange prolongation
+        assert lib.cg_transfer_from(C.byref(c), fsrc, tt, 1, creg, creg, None, 0.0, cg.OP_LAGRANGE,
+                                    cg.CELL_CENTERED, 3, 0, None) == -11
+        assert "dgfe" in lib.cg_last_error_message().decode().lower()
+        csrc = (C.POINTER(GData) * 1)(C.pointer(c))
+        freg = cg.Box((11, 11, 11), (21, 21, 21), (2, 2, 2)).as_c()
+        assert lib.cg_transfer_from(C.byref(f), csrc, tt, 1, freg, freg, None, 0.0, cg.OP_LAGRANGE,
+                                    cg.CELL_CENTERED, 3, 0, None) == -11
+        # the vc restriction does not go through DGFE (data.cc:1106-1127)
+        assert rest((0, 1, 1)) != -11
+    finally:
+        cg.set_dgfe(False)

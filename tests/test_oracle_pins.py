@@ -1046,3 +1046,117 @@ def test_stagger_footprint_checks():
     assert orc.restrict_3d_stagger(7, src, dst, ok) == 0
     assert orc.restrict_3d_stagger(7, src, dst, bad) == -6  # CG_ERR_CONTAIN
     assert orc.restrict_3d_stagger(6, src, dst, ((2, 2, 2), (6, 6, 6), (2, 2, 2))) == 0
+
+
+# ---- restrict_3d_vc_rf2 -------------------------------------------------------
+def _vc_case(rng, cent, n):
+    """A cell-centred coarse region (stride 4, cells at 2 + 4c) restricted
+    from fine stride 2; the source holds the children 0..2 points beyond."""
+    sf, sc = 2, 4
+    rlo, rup, slo, sup, dlo, dup = [], [], [], [], [], []
+    for d in range(3):
+        C0 = int(rng.integers(-4, 5))
+        rlo.append(sc // 2 + C0 * sc)
+        rup.append(sc // 2 + (C0 + n[d] - 1) * sc)
+        F0 = 2 * C0 - int(rng.integers(0, 3))
+        slo.append(sf // 2 + F0 * sf)
+        sup.append(sf // 2 + (2 * (C0 + n[d]) - 1 + int(rng.integers(0, 3))) * sf)
+        dlo.append(rlo[-1] - int(rng.integers(0, 2)) * sc)
+        dup.append(rup[-1] + int(rng.integers(0, 2)) * sc)
+    return ((tuple(slo), tuple(sup), (sf,) * 3), (tuple(dlo), tuple(dup), (sc,) * 3),
+            (tuple(rlo), tuple(rup), (sc,) * 3))
+
+
+@pytest.mark.parametrize("cent", [(0, 1, 1), (1, 0, 1), (1, 1, 0), (1, 1, 1), (0, 0, 0)])
+def test_restrict_vc_positions(cent):
+    """restrict_3d_vc_rf2 pinned in position space: the coarse value at X
+    is the mean of the fine values at X - s/2 and X + s/2 along each
+    cell-centred direction and the fine value at X - s/2 along a
+    vertex-centred one (s = fine stride; the srcoff + 2i children of
+    restrict_3d_vc_rf2.cc:161-171 for boxes whose lower bounds sit on that
+    lattice).  Integer-valued data keep every half-sum exact, so the
+    expectation is computed by plain averaging; a swapped axis, a wrong
+    child offset or a wrong cell/vertex choice fails it.  (1,1,1) equals
+    restrict_3d_cc_rf2 and (0,0,0) injection at X - s/2."""
+    rng = np.random.default_rng(sum(c << i for i, c in enumerate(cent)) + 70)
+    for c in range(4):
+        n = tuple(int(v) for v in rng.integers(1, 9, size=3))
+        sb, db, rb = _vc_case(rng, cent, n)
+        src = host_array(sb, pad_of(sizes(sb), rng))
+        vals = rng.integers(-2**20, 2**20, size=sizes(sb)[::-1]).astype(np.float64)
+        src.view()[...] = vals
+        dst = host_array(db, pad_of(sizes(db), rng))
+        assert orc.restrict_3d_vc_rf2(cent, src, dst, rb) == 0
+        got = dst.view()
+        sl, sfs = sb[0], sb[2]
+        for k in range(n[2]):
+            for j in range(n[1]):
+                for i in range(n[0]):
+                    X = [rb[0][d] + (i, j, k)[d] * rb[2][d] for d in range(3)]
+                    idx = []
+                    for d in range(3):
+                        lo = (X[d] - sfs[d] // 2 - sl[d]) // sfs[d]
+                        idx.append([lo, lo + 1] if cent[d] else [lo])
+                    exp = np.mean([vals[z, y, x] for z in idx[2] for y in idx[1] for x in idx[0]])
+                    o = [(X[d] - db[0][d]) // db[2][d] for d in range(3)]
+                    assert got[o[2], o[1], o[0]] == exp, (cent, c, i, j, k)
+        # outside the region the destination keeps its poison
+        mask = np.ones(got.shape, bool)
+        o = [(rb[0][d] - db[0][d]) // db[2][d] for d in range(3)]
+        mask[o[2]:o[2] + n[2], o[1]:o[1] + n[1], o[0]:o[0] + n[0]] = False
+        assert np.isnan(got[mask]).all()
+        if cent == (1, 1, 1):
+            d2 = host_array(db, sizes(db))
+            assert orc.restrict_3d_cc_rf2(src, d2, rb) == 0
+            assert np.array_equal(d2.view()[~mask], got[~mask])
+
+
+def test_restrict_vc_validation():
+    """The footprint must lie in the source (one fine point short in a
+    cell-centred direction is refused), strides must be related by 2 and the
+    region must sit on the source's cell lattice."""
+    rng = np.random.default_rng(5)
+    sb, db, rb = _vc_case(rng, (0, 1, 1), (3, 3, 3))
+    src = host_array(sb, sizes(sb), 0.0)
+    dst = host_array(db, sizes(db), 0.0)
+    # cut the source one fine point short above the last y child
+    short = (sb[0], (sb[1][0], rb[1][1] + 1 - 2, sb[1][2]), sb[2])
+    s2 = host_array(short, sizes(short), 0.0)
+    assert orc.restrict_3d_vc_rf2((0, 1, 1), s2, dst, rb) == -6
+    # the same cut is fine where y is vertex-centred (only the first child is read)
+    assert orc.restrict_3d_vc_rf2((1, 0, 1), s2, dst, rb) == 0
+    bad = (tuple(l + 2 for l in rb[0]), tuple(u + 2 for u in rb[1]), rb[2])
+    assert orc.restrict_3d_vc_rf2((0, 1, 1), src, dst, bad) == -4
+    wrong = (sb[0], sb[1], (4, 4, 4))
+    s3 = host_array(wrong, sizes(wrong), 0.0)
+    assert orc.restrict_3d_vc_rf2((0, 1, 1), s3, dst, rb) == -2
+
+
+# ---- the level step on the CPU (the timed baseline) ---------------------------
+def test_level_step_replay_shapes_agree():
+    """orc_replay runs a level step's transfers in the reference's shape
+    (one operator call per region, time interpolation through temporaries)
+    or fused (one parallel loop over all regions, time interpolation in
+    place); both give the bits of the per-transfer oracle replay."""
+    from carpet_amd.workloads import config4
+    from workload_check import host_state, run_oracle
+
+    w = config4(n_regions=24, n_gf=1, edge=(6, 30))
+    ref = run_oracle(w)
+    for mode in (0, 1):
+        host = host_state(w)
+        tasks = orc.level_step_tasks(w.steps, host)
+        assert orc.replay(mode, tasks) == 0
+        for k in w.outputs:
+            assert np.array_equal(host[k].data, ref[k], equal_nan=True), (mode, k)
+    # a transfer whose footprint leaves its source array is refused in both shapes
+    bad = host_state(w)
+    t = [x for x in w.steps if x.phase == 1][0]
+    s0 = bad[t.srcs[0]]
+    lo, up, st = s0.box
+    cut = (lo[0] + 3 * st[0], lo[1], lo[2])
+    small = orc.HostArr((cut, up, st), np.ascontiguousarray(s0.data[:, :, 3:]))
+    for name in t.srcs:
+        bad[name] = small
+    for mode in (0, 1):
+        assert orc.replay(mode, orc.level_step_tasks([t], bad)) == -6, mode

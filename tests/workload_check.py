@@ -23,10 +23,19 @@ def _b(box):
     return (box.lower, box.upper, box.stride)
 
 
-def run_oracle(w) -> dict:
+def host_state(w) -> dict:
+    """Oracle arrays holding every array's initial extent."""
     inputs = w.host_inputs()
-    host = {k: _host(s, inputs[k]) for k, s in w.arrays.items()}
-    for t in sorted(w.steps, key=lambda t: t.phase):
+    return {k: _host(s, inputs[k]) for k, s in w.arrays.items()}
+
+
+def run_oracle(w, host=None, steps=None) -> dict:
+    """Replays w's transfers (or `steps`) in phase order on the oracle,
+    starting from `host` (updated in place) or from the initial arrays;
+    returns the outputs."""
+    if host is None:
+        host = host_state(w)
+    for t in sorted(w.steps if steps is None else steps, key=lambda t: t.phase):
         dst = host[t.dst]
         srcs = [host[s] for s in t.srcs]
         sst, dstst = srcs[0].box[2], dst.box[2]
@@ -54,4 +63,4 @@ def run_oracle(w) -> dict:
             f = orc.restrict_3d_rf2 if t.cent == VERTEX else orc.restrict_3d_cc_rf2
             rc = f(srcs[tl0], dst, reg)
         assert rc == 0, (t.name, rc)
-    return {k: host[k].data for k in w.outputs}
+    return {k: host[k].data for k in w.outputs if k in host}
