@@ -103,6 +103,8 @@ _decl("cg_interpolate_eno_3d_3tl_f64", i, vp, dp, vp, dp, vp, dp, ip, ip, vp, dp
 _GP = C.POINTER(GData)
 _decl("cg_transfer_from", i, _GP, C.POINTER(_GP), C.POINTER(C.c_double), i, ip, ip,
       C.POINTER(Islab), dp, i, i, i, i, vp)
+_decl("cg_transfer_from_noalloc", i, _GP, C.POINTER(_GP), C.POINTER(C.c_double), i, ip, ip,
+      C.POINTER(Islab), dp, i, i, i, i, vp)
 _decl("cg_batch_create", i, C.POINTER(C.c_void_p))
 _decl("cg_batch_destroy", i, vp)
 _decl("cg_batch_add", i, vp, i, _GP, C.POINTER(_GP), C.POINTER(C.c_double), i, ip, ip,

@@ -481,6 +481,20 @@ def transfer_from(dst: DeviceArray, srcs: Sequence[DeviceArray], times: Sequence
           "transfer_from")
 
 
+def transfer_from_noalloc(dst: DeviceArray, srcs: Sequence[DeviceArray], times: Sequence[float],
+                          dstregbbox: Box, srcregbbox: Box, time: float, transport_operator: int,
+                          centering: int, order_space: int, order_time: int, slab_offset=None,
+                          stream=None):
+    """transfer_from that allocates no device memory: time-level temporaries
+    live in the library's per-GPU scratch array (cg_transfer_from_noalloc)."""
+    slab = _slab(slab_offset, None)
+    tt = (C.c_double * len(times))(*times)
+    check(lib.cg_transfer_from_noalloc(C.byref(dst.g), _gptrs(srcs), tt, len(times), dstregbbox.as_c(),
+                                       srcregbbox.as_c(), C.byref(slab) if slab else None, time,
+                                       transport_operator, centering, order_space, order_time, _sh(stream)),
+          "transfer_from_noalloc")
+
+
 class Batch:
     """A batched level step (replaces ggf::transfer_from_all's region walk)."""
 

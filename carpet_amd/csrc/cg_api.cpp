@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "cg_internal.hpp"
@@ -399,6 +400,37 @@ static int run_transfer_tmp(const Transfer &tr, hipStream_t st) {
   return e;
 }
 
+// Temporaries of the allocation-free transfer: one library-owned device
+// array per GPU (a static device symbol, so loading the library reserves it
+// and no call allocates).  In host code the symbol's name is not a device
+// address: its address on the current device comes from
+// hipGetSymbolAddress, taken once per device on that device's first call and
+// used from then on for every kernel argument.
+static __device__ __align__(256) double scratch[CG_NOALLOC_SCRATCH_BYTES / sizeof(double)];
+static std::mutex scratch_mu;
+static std::vector<void *> scratch_at; // per device: address of scratch
+
+static int scratch_base(double **out) {
+  int dev = 0;
+  hipError_t he = hipGetDevice(&dev);
+  if (he != hipSuccess)
+    return hip_error(he, "hipGetDevice");
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if ((size_t)dev >= scratch_at.size())
+    scratch_at.resize((size_t)dev + 1, nullptr);
+  if (!scratch_at[dev]) {
+    void *p = nullptr;
+    he = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (he != hipSuccess)
+      return hip_error(he, "hipGetSymbolAddress (scratch)");
+    if (!p)
+      return set_error(CG_ERR_HIP, "hipGetSymbolAddress (scratch) gave no address on device %d", dev);
+    scratch_at[dev] = p;
+  }
+  *out = static_cast<double *>(scratch_at[dev]);
+  return CG_OK;
+}
+
 int run_transfer(const Transfer &tr, int prec, hipStream_t st) {
   if (tr.tmp_ntl)
     return run_transfer_tmp(tr, st);
@@ -439,6 +471,7 @@ const char *cg_error_string(int code) {
   case CG_ERR_STATE: return "batch used in the wrong state";
   case CG_ERR_DEVICE: return "device-side check failed";
   case CG_ERR_SCHEDULE: return "schedule consistency check failed";
+  case CG_ERR_SCRATCH: return "allocation-free scratch too small";
   }
   return "unknown error";
 }
@@ -1075,6 +1108,32 @@ int cg_transfer_from(const cg_gdata *dst, const cg_gdata *const *srcs,
   if (e)
     return e;
   return run_transfer(tr, 0, S(stream));
+}
+
+int cg_transfer_from_noalloc(const cg_gdata *dst, const cg_gdata *const *srcs,
+                             const double *times, int ntimes, const int dstregbbox[9],
+                             const int srcregbbox[9], const cg_islab *slab, double time,
+                             int transport_operator, int centering, int order_space,
+                             int order_time, cg_stream_t stream) {
+  Transfer tr{};
+  int e = resolve_transfer(dst, srcs, times, ntimes, dstregbbox, srcregbbox, slab,
+                           time, transport_operator, centering, order_space,
+                           order_time, tr);
+  if (e)
+    return e;
+  if (!tr.tmp_ntl) // a single kernel on the caller's arrays: nothing to stage
+    return run_transfer(tr, 0, S(stream));
+  const unsigned long long need =
+      (unsigned long long)tr.tmp_ntl * (unsigned long long)tr.tmp_reg.size() * sizeof(double);
+  if (need > CG_NOALLOC_SCRATCH_BYTES)
+    return set_error(CG_ERR_SCRATCH,
+                     "%d time-level temporaries of %lld points need %llu bytes of scratch; the "
+                     "allocation-free transfer holds at most %llu",
+                     tr.tmp_ntl, tr.tmp_reg.size(), need, (unsigned long long)CG_NOALLOC_SCRATCH_BYTES);
+  double *tmp = nullptr;
+  if ((e = scratch_base(&tmp)))
+    return e;
+  return run_transfer_tmp_into(tr, tmp, S(stream));
 }
 
 } // extern "C"

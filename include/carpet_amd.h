@@ -61,7 +61,8 @@ enum {
   CG_ERR_NOMEM = -13,       /* allocation failed */
   CG_ERR_STATE = -14,       /* batch object used in the wrong state */
   CG_ERR_DEVICE = -15,      /* device kernel reported a failed check */
-  CG_ERR_SCHEDULE = -16     /* a dh region check failed (schedule build) */
+  CG_ERR_SCHEDULE = -16,    /* a dh region check failed (schedule build) */
+  CG_ERR_SCRATCH = -17      /* needs more than the allocation-free scratch */
 };
 
 /* transport operators: same numbering as CarpetLib's enum operator_type
@@ -455,6 +456,30 @@ int cg_transfer_from(const cg_gdata *dst, const cg_gdata *const *srcs,
                      const cg_islab *slab, double time, int transport_operator,
                      int centering, int order_space, int order_time,
                      cg_stream_t stream);
+
+/* No reference counterpart: cg_transfer_from for callers that cannot
+   allocate device memory (code under HIP graph capture, tight serving
+   loops, hosts that forbid hipMalloc).  Same arguments, validation and
+   results as cg_transfer_from.  The transfers that cg_transfer_from runs
+   through stream-ordered temporaries (prolongation with 4 or 5 time levels,
+   restriction and slabbed copies with time interpolation) stage them here
+   in one library-owned device array per GPU of CG_NOALLOC_SCRATCH_BYTES,
+   reserved when the library is loaded on that GPU and held until the
+   process ends; a transfer whose temporaries (time levels x region points
+   x 8 bytes) exceed it returns CG_ERR_SCRATCH before any GPU work.  Every
+   other transfer is one kernel on the caller's arrays.  Nothing is
+   allocated or freed (no hipMalloc, hipMallocAsync or hipFree).  All calls
+   on one GPU share its array: calls whose work overlaps on one GPU (on
+   different streams, or from several threads) race on it and must be
+   ordered by the caller.  No part of the array needs clearing: every
+   temporary point is written before it is read. */
+#define CG_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int cg_transfer_from_noalloc(const cg_gdata *dst, const cg_gdata *const *srcs,
+                             const double *times, int ntimes,
+                             const int dstregbbox[9], const int srcregbbox[9],
+                             const cg_islab *slab, double time, int transport_operator,
+                             int centering, int order_space, int order_time,
+                             cg_stream_t stream);
 
 /* ---- batched level step ----------------------------------------------- */
 /* Replaces the per-region walk of ggf::transfer_from_all

@@ -26,7 +26,7 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_error_strings():
-    for code in range(0, -17, -1):
+    for code in range(0, -18, -1):
         assert lib.cg_error_string(code).decode() != "unknown error"
 
 
@@ -178,6 +178,40 @@ def test_transfer_dispatch_validation():
     creg = cg.Box((8, 8, 8), (12, 12, 12), (2, 2, 2)).as_c()
     assert lib.cg_transfer_from(C.byref(c), csrc, tt, 1, creg, creg, None, 0.0, cg.OP_STAGGER111,
                                 cg.CELL_CENTERED, 3, 0, None) == -11
+
+
+def test_noalloc_rejects_temporaries_beyond_its_scratch():
+    """cg_transfer_from_noalloc stages time-level temporaries in a fixed
+    per-GPU array: a prolongation with 5 time levels onto a 120^3 region
+    needs 5 x 120^3 x 8 bytes, more than CG_NOALLOC_SCRATCH_BYTES (64 MiB),
+    and is refused before anything touches the GPU; so is every transfer
+    cg_transfer_from refuses."""
+    from carpet_amd._lib import GData
+    n = 120
+    c = GData()
+    c.storage = FAKE
+    nc = n // 2 + 8
+    c.padext[:] = [nc] * 3
+    c.ext[:] = [nc] * 3
+    c.bbox[:] = [-8, -8, -8, -8 + 2 * (nc - 1), -8 + 2 * (nc - 1), -8 + 2 * (nc - 1), 2, 2, 2]
+    f = GData()
+    f.storage = FAKE
+    f.padext[:] = [n] * 3
+    f.ext[:] = [n] * 3
+    f.bbox[:] = [0, 0, 0, n - 1, n - 1, n - 1, 1, 1, 1]
+    srcs = (C.POINTER(GData) * 5)(*[C.pointer(c)] * 5)
+    tt = (C.c_double * 5)(0.5, 0.25, 0.0, -0.25, -0.5)
+    reg = cg.Box((0, 0, 0), (n - 1,) * 3, (1, 1, 1)).as_c()
+    assert 5 * n**3 * 8 > 64 << 20
+    assert lib.cg_transfer_from_noalloc(C.byref(f), srcs, tt, 5, reg, reg, None, 0.1, cg.OP_LAGRANGE,
+                                        cg.VERTEX_CENTERED, 5, 4, None) == -17
+    assert "scratch" in lib.cg_last_error_message().decode()
+    assert lib.cg_error_string(-17).decode() != "unknown error"
+    # the same validation as cg_transfer_from comes first
+    assert lib.cg_transfer_from_noalloc(C.byref(f), srcs, tt, 5, reg, reg, None, 5.0, cg.OP_LAGRANGE,
+                                        cg.VERTEX_CENTERED, 5, 4, None) == -9
+    assert lib.cg_transfer_from_noalloc(C.byref(f), srcs, tt, 5, reg, reg, None, 0.1, cg.OP_LAGRANGE,
+                                        cg.VERTEX_CENTERED, 4, 4, None) == -7
 
 
 def test_batch_set_time_keeps_launch_classes():
