@@ -198,6 +198,8 @@ def run_config4(n_regions: int, n_gf: int, steps: int, warmup: int, plain_steps:
     desc = b.describe()
     print(f"# config4: {len(w.steps)} transfers, set up in {time.perf_counter() - t0:.1f} s\n" + desc,
           file=sys.stderr, flush=True)
+    if tune_cache and os.path.exists(tune_cache):
+        print("# tuner cache:\n" + open(tune_cache).read(), file=sys.stderr, flush=True)
     stats = b.stats()
     b.capture_graph(st)
     for _ in range(warmup):
@@ -225,23 +227,25 @@ def run_config4(n_regions: int, n_gf: int, steps: int, warmup: int, plain_steps:
     ms_retimed = e0.elapsed_ms(e1) / steps
     b.set_time(prol_phase, [0.0, -1.0, -2.0], -0.5)
     sample = w.outputs[:: max(1, len(w.outputs) // 64)]
-    b.launch_graph(st)
-    st.synchronize()
-    got_b = {k: dev[k].download() for k in sample}
-    for k in sample:  # back to the initial state (NaN, then the seeded values)
-        dev[k].fill(np.nan)
-        if w.arrays[k].seed is not None:
-            dev[k].fill_hash(w.arrays[k].seed)
-    st.synchronize()
-    b.run_plain(st)
-    st.synchronize()
-    same = all(np.array_equal(dev[k].download(), got_b[k], equal_nan=True) for k in sample)
-    e0.record(st)
-    for _ in range(plain_steps):
+    same, ms_p = None, float("nan")
+    if plain_steps > 0:
+        b.launch_graph(st)
+        st.synchronize()
+        got_b = {k: dev[k].download() for k in sample}
+        for k in sample:  # back to the initial state (NaN, then the seeded values)
+            dev[k].fill(np.nan)
+            if w.arrays[k].seed is not None:
+                dev[k].fill_hash(w.arrays[k].seed)
+        st.synchronize()
         b.run_plain(st)
-    e1.record(st)
-    st.synchronize()
-    ms_p = e0.elapsed_ms(e1) / plain_steps
+        st.synchronize()
+        same = all(np.array_equal(dev[k].download(), got_b[k], equal_nan=True) for k in sample)
+        e0.record(st)
+        for _ in range(plain_steps):
+            b.run_plain(st)
+        e1.record(st)
+        st.synchronize()
+        ms_p = e0.elapsed_ms(e1) / plain_steps
     b.close()
     w.free(dev)
     desc_bytes = int(desc.split("device descriptor bytes ")[1].split()[0]) if "descriptor bytes" in desc else None
@@ -256,7 +260,7 @@ def run_config4(n_regions: int, n_gf: int, steps: int, warmup: int, plain_steps:
            "retimed_step_ms": round(ms_retimed, 4),
            "plain_ms": round(ms_p, 4), "plain_gpoints_per_s": round(pts / ms_p / 1e6, 3),
            "plain_launches_per_step": stats["transfers"], "batched_vs_plain": round(ms_p / ms_b, 3),
-           "batched_equals_plain_on_sample": bool(same), "sample_destinations": len(sample)}
+           "batched_equals_plain_on_sample": same, "sample_destinations": len(sample)}
     if cpu_seconds > 0:
         out["cpu"] = cpu_baseline_config4(w, cpu_seconds)
     return out
@@ -519,7 +523,8 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n-gf", type=int, default=24)
-    ap.add_argument("--plain-steps", type=int, default=5)
+    ap.add_argument("--plain-steps", type=int, default=5,
+                    help="steps of the plain path (one launch per transfer); 0 skips it")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-config2", action="store_true",
                     help="skip the single-patch prolongation reported beside the headline line")
@@ -534,10 +539,19 @@ def main(argv=None):
     ap.add_argument("--n-regions", type=int, default=2000)
     ap.add_argument("--tune-cache", default=None,
                     help="batch tuner cache file: classes it names take its parameters, the others are "
-                         "searched (cg_batch_autotune) before the timed steps")
+                         "searched (cg_batch_autotune) before the timed steps; default: a fresh file, so "
+                         "every class is searched")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="time the default launch settings of cg_batch_finalize (no tuner search)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run one 1 GiB copy_3d first (HBM counter calibration under rocprofv3)")
     args = ap.parse_args(argv)
+    if args.no_tune:
+        args.tune_cache = None
+    elif args.tune_cache is None:
+        import tempfile
+        fd, args.tune_cache = tempfile.mkstemp(prefix="carpet_amd_tune_", suffix=".txt")
+        os.close(fd)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the transfer operators of a level step run on a single "
               "MI355X here (sharding across GPUs is a design note, not a measured path); "
@@ -635,7 +649,8 @@ def main(argv=None):
         line["single_patch_512"] = config2_line(args, side=True)
     if not args.no_config6:
         print("# config6 beside the headline line", file=sys.stderr, flush=True)
-        line["hierarchy_step_3_levels"] = run_config6(args.n_gf, 10, 2, 1)
+        line["hierarchy_step_3_levels"] = run_config6(args.n_gf, 10, 2, 1, tune_cache=args.tune_cache)
+    line["tuned"] = args.tune_cache is not None
     if cpu is not None:
         line["cpu_baseline"] = cpu
     print(json.dumps(line))

@@ -592,7 +592,8 @@ int cg_gdata_alloc(cg_gdata *out, const int bbox[9], const int owned_bbox[9],
     out->ext[d] = shape[d];
   }
   std::memcpy(out->bbox, bbox, sizeof out->bbox);
-  // remember the base pointer in front of the array for cg_gdata_free
+  // cg_gdata_free recovers the allocation from the storage pointer: the
+  // offset is under one 128-byte row and hipMalloc aligns to 256 bytes
   return CG_OK;
 }
 

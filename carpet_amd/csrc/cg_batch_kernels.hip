@@ -571,7 +571,7 @@ int batch_device_build(BatchDevice &dev, const std::vector<Transfer> &tr,
         }
         // ENO kinds (data-dependent stencils, several times the arithmetic
         // per plane) march slower and drift further apart: at most 64
-        // planes (8 variables of 192^3 -> 384^3 with ENO<3>: 1.56 ms at 64
+        // planes (8 variables of 192^3 -> 384^3 with ENO<3>: 1.64 ms at 64
         // against 1.74 ms at 128)
         const int first = fp64_only(g.kind) ? 1 : 0;
         const double best = *std::min_element(mk.begin() + first, mk.end());

@@ -544,7 +544,8 @@ int cg_batch_set_time(cg_batch *b, int phase, const double *times, int ntimes, d
 int cg_batch_set_storage(cg_batch *b, int n, const cg_gdata *from, const cg_gdata *to);
 /* Autotuner: times each launch of the finalized batch under candidate
    work-item sizes (tensor tiles: z chunk of 8/16/32/64/128/all coarse bases;
-   pointwise items: 2048..65536 points) in the class's default mode
+   pointwise items: 256, 512, 1024, 2048, 4096 or 16384 points) in the
+   class's default mode
    (persistent grid for tensor tiles, one workgroup per item for pointwise
    runs), then the best size in the other modes (one workgroup per item,
    persistent grids of resident capacity and of 2 workgroups per CU) and,
