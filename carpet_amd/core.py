@@ -73,7 +73,13 @@ class Box:
                    tuple(min(x, y) for x, y in zip(self.upper, b.upper)), self.stride)
 
     def as_c(self):
-        return I9(*self.lower, *self.upper, *self.stride)
+        # built once per box (the batch of a large hierarchy passes the same
+        # region boxes for every grid function)
+        c = self.__dict__.get("_c")
+        if c is None:
+            c = I9(*self.lower, *self.upper, *self.stride)
+            object.__setattr__(self, "_c", c)
+        return c
 
 
 def pad_shape(shape, offset, vec_size: int, linesize: int, align_interior: bool = True):

@@ -30,6 +30,12 @@
 
 namespace cg {
 
+// tensor transfers of at most this many tiles run whole on one XCD
+#ifndef CG_WHOLE_XCD_ITEMS
+#define CG_WHOLE_XCD_ITEMS 16
+#endif
+constexpr int WHOLE_XCD_ITEMS = CG_WHOLE_XCD_ITEMS;
+
 // default work items: pointwise = PT_DEFAULT consecutive destination points
 // (x fastest); tensor = one x/y tile over ZCHUNK_DEFAULT coarse z bases
 constexpr int PT_DEFAULT = 4096;
@@ -629,14 +635,26 @@ int batch_device_build(BatchDevice &dev, const std::vector<Transfer> &tr,
     // items are cut into 8 contiguous parts of equal cost (spatial slabs,
     // e.g. ranges of y tiles) and part x goes to XCD x, so all XCDs work on
     // the same transfer at the same time, as one launch per transfer would,
-    // with each slab's halos shared in one L2.  A transfer with fewer than
-    // 8 items gives each item to the XCD with the least work so far.
+    // with each slab's halos shared in one L2.  A tensor transfer of at
+    // most WHOLE_XCD_ITEMS tiles goes whole to the XCD with the least work
+    // so far: its tiles' stencil halos overlap, and on one XCD the rows two
+    // tiles share are fetched from HBM once, where striped over XCDs every
+    // tile fetches them into its own L2.  A pointwise transfer with fewer
+    // than 8 items gives each item to the least-loaded XCD.
     int segb[9];
     {
       std::vector<Item> part[8];
       double load[8] = {0, 0, 0, 0, 0, 0, 0, 0};
       for (size_t m = 0; m + 1 < first.size(); ++m) {
         const int i0 = first[m], i1 = first[m + 1];
+        if (g.is_tensor && i1 - i0 <= WHOLE_XCD_ITEMS) {
+          const int x = (int)(std::min_element(load, load + 8) - load);
+          for (int i = i0; i < i1; ++i) {
+            part[x].push_back(items[i]);
+            load[x] += cost[i];
+          }
+          continue;
+        }
         if (i1 - i0 < 8) {
           for (int i = i0; i < i1; ++i) {
             const int x = (int)(std::min_element(load, load + 8) - load);

@@ -17,6 +17,13 @@ template <class L, int NTL, int CX, int CY, int IPT> constexpr int persist_waves
   return (2 * CX * CY / IPT >= 512 && L::R <= 6) ? 4 : 1;
 }
 
+// planes whose windows are loaded ahead of their use (one register per
+// level and window element of a thread per plane); a compile-time knob
+#ifndef CG_BATCH_PF
+#define CG_BATCH_PF 1
+#endif
+constexpr int batch_pf() { return CG_BATCH_PF; }
+
 // NTL: levels of the passes; NSRC: source levels (NSRC > NTL: combined at
 // the window store)
 template <class L, int NTL, int CX, int CY, int IPT, int XQ, int PR, int NSRC>
@@ -33,7 +40,8 @@ __attribute__((amdgpu_waves_per_eu(persist_waves<L, NTL, CX, CY, IPT>()))) void 
       return;
     const Item w = items[it];
     const TensorArgs a = descs[w.desc];
-    tensor_tile<L, NTL, CX, CY, IPT, PR, 1, XQ, pipe_fits<L, NTL, CX, CY, XQ>(), NSRC>(a, (int)w.tile, smem);
+    tensor_tile<L, NTL, CX, CY, IPT, PR, batch_pf(), XQ, pipe_fits<L, NTL, CX, CY, XQ>(), NSRC>(a, (int)w.tile,
+                                                                                          smem);
   }
 }
 
@@ -46,8 +54,8 @@ __global__ __launch_bounds__(2 * CX * CY / IPT) void k_grid_tensor(
   if (it < 0)
     return;
   const Item w = items[it];
-  tensor_tile<L, NTL, CX, CY, IPT, PR, 1, XQ, pipe_fits<L, NTL, CX, CY, XQ>(), NSRC>(descs[w.desc], (int)w.tile,
-                                                                                   smem);
+  tensor_tile<L, NTL, CX, CY, IPT, PR, batch_pf(), XQ, pipe_fits<L, NTL, CX, CY, XQ>(), NSRC>(descs[w.desc],
+                                                                                            (int)w.tile, smem);
 }
 
 template <class L, int NSRC, int PR, int SH>

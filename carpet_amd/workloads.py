@@ -569,8 +569,69 @@ def nested_levels(ncoarse: int = 32, nlevels: int = 3, split=(2, 1, 1), centerin
     return levels
 
 
+def tiled_levels(counts=((4, 4, 4), (5, 5, 5), (7, 7, 6), (8, 8, 8), (10, 10, 10)), edge=(8, 96), seed: int = 7,
+                 ghost: int = 3, buffer: int = 6, order: int = 5, margin: int = 24):
+    """A nested vertex-centred hierarchy whose levels are tiled by boxes
+    with edges uniform in [edge[0], edge[1]] points per axis (counts[rl] =
+    boxes per axis on level rl; the default makes 1995 boxes on 5 levels).
+    Level rl's region is the union of its boxes, centred inside level rl-1's
+    region (refinement factor 2) with `margin` fine points to spare on every
+    side for ghost, buffer and prolongation stencil; level 0's region is the
+    domain.  Returns schedule.LevelDesc per level (vertex centring: level rl
+    on stride 2^(L-1-rl))."""
+    from .schedule import LevelDesc
+    rng = np.random.default_rng(seed)
+    nl = len(counts)
+    widths = []
+    for rl in range(nl):
+        for _ in range(1000):
+            w = [[int(v) for v in rng.integers(edge[0], edge[1] + 1, size=counts[rl][d])] for d in range(3)]
+            if rl == 0 or all(sum(w[d]) + 2 * margin <= 2 * (sum(widths[-1][d]) - 1) + 1 for d in range(3)):
+                break
+        else:
+            raise ValueError(f"level {rl}: no tiling fits inside level {rl - 1}")
+        widths.append(w)
+    levels = []
+    lo_prev = [0, 0, 0]
+    for rl in range(nl):
+        s = 2 ** (nl - 1 - rl)
+        w = widths[rl]
+        L = [sum(w[d]) for d in range(3)]
+        if rl == 0:
+            lo = [0, 0, 0]
+        else:
+            # level rl-1's region in level-rl points: 2*lo_prev .. 2*(lo_prev+Lp-1)
+            Lp = [sum(widths[rl - 1][d]) for d in range(3)]
+            lo = [2 * lo_prev[d] + (2 * (Lp[d] - 1) + 1 - L[d]) // 2 for d in range(3)]
+        comps = []
+        starts = []
+        for d in range(3):
+            acc, st = lo[d], []
+            for k in w[d]:
+                st.append(acc)
+                acc += k
+            starts.append(st)
+        for iz in range(counts[rl][2]):
+            for iy in range(counts[rl][1]):
+                for ix in range(counts[rl][0]):
+                    a = (starts[0][ix], starts[1][iy], starts[2][iz])
+                    b = (a[0] + w[0][ix] - 1, a[1] + w[1][iy] - 1, a[2] + w[2][iz] - 1)
+                    comps.append((tuple(s * v for v in a), tuple(s * v for v in b), (s,) * 3))
+        if rl == 0:
+            dom = ((0,) * 3, tuple(s * (L[d] - 1) for d in range(3)), (s,) * 3)
+        else:
+            dom = levels[0].domain
+            # each level's domain in its own stride: the level-0 domain refined
+            dom = ((0,) * 3, tuple(dom[1][d] for d in range(3)), (s,) * 3)
+        g = ((ghost,) * 3, (ghost,) * 3)
+        bw = ((buffer,) * 3, (buffer,) * 3) if rl > 0 else ((0,) * 3, (0,) * 3)
+        levels.append(LevelDesc(dom, comps, (2 ** rl,) * 3, g, bw, ((0,) * 3, (0,) * 3), order))
+        lo_prev = lo
+    return levels
+
+
 def hierarchy_step(levels, centering: int = VERTEX_CENTERED, n_gf: int = 2, seed: int = 6,
-                   boundary=((0, 0, 0), (0, 0, 0)), op: int = OP_LAGRANGE) -> Workload:
+                   boundary=((0, 0, 0), (0, 0, 0)), op: int = OP_LAGRANGE, only_used: bool = False) -> Workload:
     """One level step of a real hierarchy, from its communication schedule
     (schedule.Schedule, dh's lists): for each level rl in turn, restriction
     from rl+1 into rl (phase 3rl), boundary prolongation from rl-1 with
@@ -593,20 +654,29 @@ def hierarchy_step(levels, centering: int = VERTEX_CENTERED, n_gf: int = 2, seed
                 w.outputs.append(f"l{rl}c{c}g{g}t0")
     order = {rl: L.prolongation_order for rl, L in enumerate(levels)}
     for rl in range(len(levels)):
+        # the region boxes are the same for every grid function
+        rest = [(r, Box(*r.recv), Box(*r.send)) for r in sch.lists[rl][REF_REST]]
+        bnd = [(r, Box(*r.recv), Box(*r.send)) for r in sch.lists[rl][REF_BND_PROL]]
+        syn = [(r, Box(*r.recv), Box(*r.send)) for r in sch.lists[rl][SYNC]]
         for g in range(n_gf):
-            for r in sch.lists[rl][REF_REST]:  # fine rl+1 -> coarse rl
+            for r, recv, send in rest:  # fine rl+1 -> coarse rl
                 w.steps.append(Transfer(f"rest l{rl + 1}c{r.src}->l{rl}c{r.dst} g{g}", f"l{rl}c{r.dst}g{g}t0",
-                                        [f"l{rl + 1}c{r.src}g{g}t0"], [0.0], Box(*r.recv), Box(*r.send), 0.0, op,
+                                        [f"l{rl + 1}c{r.src}g{g}t0"], [0.0], recv, send, 0.0, op,
                                         centering, order[rl + 1], 0, phase=3 * rl))
-            for r in sch.lists[rl][REF_BND_PROL]:  # coarse rl-1 -> fine rl
+            for r, recv, send in bnd:  # coarse rl-1 -> fine rl
                 w.steps.append(Transfer(f"bnd l{rl - 1}c{r.src}->l{rl}c{r.dst} g{g}", f"l{rl}c{r.dst}g{g}t0",
                                         [f"l{rl - 1}c{r.src}g{g}t{tl}" for tl in range(3)], times,
-                                        Box(*r.recv), Box(*r.send), -0.5, op, centering, order[rl], 2,
+                                        recv, send, -0.5, op, centering, order[rl], 2,
                                         phase=3 * rl + 1))
-            for r in sch.lists[rl][SYNC]:
+            for r, recv, send in syn:
                 w.steps.append(Transfer(f"sync l{rl}c{r.src}->c{r.dst} g{g}", f"l{rl}c{r.dst}g{g}t0",
-                                        [f"l{rl}c{r.src}g{g}t0"], [0.0], Box(*r.recv), Box(*r.send), 0.0, op,
+                                        [f"l{rl}c{r.src}g{g}t0"], [0.0], recv, send, 0.0, op,
                                         centering, order[rl], 0, phase=3 * rl + 2))
+    if only_used:
+        # arrays no transfer reads or writes (the finest level's past time
+        # levels) are not allocated
+        used = {t.dst for t in w.steps} | {x for t in w.steps for x in t.srcs} | set(w.outputs)
+        w.arrays = {k: v for k, v in w.arrays.items() if k in used}
     w.schedule = sch
     return w
 
