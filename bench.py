@@ -740,7 +740,7 @@ def main(argv=None):
             # the dominant launch of the step (HIP events around each launch)
             "roofline": {"bound": "hbm", "achieved": dom["gbytes_per_s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(dom["gbytes_per_s"] / HBM_PEAK_GBS, 4),
-                         "traffic": _profile_traffic(f"config4_launch{dom['launch']}"),
+                         "traffic": _profile_traffic(f"level_step_2000_regions_x24_launch{dom['launch']}"),
                          "kernel": f"launch {dom['launch']} (phase {dom['phase']})", "ms": dom["ms"],
                          "algorithmic_bytes": dom["algorithmic_bytes"]}}
     if not args.no_config2:

@@ -197,15 +197,15 @@ static int save_cache(const char *path, const TuneMap &m) {
   return CG_OK;
 }
 
-static int time_launch(cg_batch *b, int idx, hipStream_t st, int reps, float *ms) {
+static int time_launch(BatchDevice &dev, int idx, hipStream_t st, int reps, float *ms) {
   hipEvent_t e0, e1;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
     return hip_error(hipGetLastError(), "hipEventCreate");
-  int e = batch_device_run_one(b->dev, idx, st); // warm-up
+  int e = batch_device_run_one(dev, idx, st); // warm-up
   float best = 1e30f;
   for (int r = 0; r < reps && !e; ++r) {
     (void)hipEventRecord(e0, st);
-    e = batch_device_run_one(b->dev, idx, st);
+    e = batch_device_run_one(dev, idx, st);
     (void)hipEventRecord(e1, st);
     if (!e && hipEventSynchronize(e1) == hipSuccess) {
       float t = 0;
@@ -247,14 +247,26 @@ extern "C" int cg_batch_autotune(cg_batch *b, cg_stream_t stream, int reps,
         cands.push_back({l.zchunk, p, l.wpc, -1});
     float best = 1e30f;
     Tune bestc = cands[0];
+    // each trial builds this launch alone into its own descriptor blob
+    // (launch i of the full build; the other launches are not rebuilt)
     auto trial_run = [&](const Tune &c) -> int {
       TuneMap trial = b->tuned;
       trial[l.key] = c;
-      int e = batch_device_build(b->dev, b->transfers, b->phase, b->prec, b->cls_ntl, b->tile_cfg, &trial);
-      if (e)
-        return e;
+      BatchDevice one;
+      int e = batch_device_build(one, b->transfers, b->phase, b->prec, b->cls_ntl, b->tile_cfg, &trial,
+                                 (int)i);
       float ms = 0;
-      if ((e = time_launch(b, (int)i, st, reps, &ms)))
+      if (!e && one.launches.size() != 1)
+        e = set_error(CG_ERR_STATE, "tuner: launch %zu did not build alone", i);
+      if (!e)
+        e = time_launch(one, 0, st, reps, &ms);
+      if (!e) {
+        hipError_t he = hipStreamSynchronize(st);
+        if (he != hipSuccess)
+          e = hip_error(he, "tuner trial");
+      }
+      batch_device_free(one);
+      if (e)
         return e;
       if (ms < best) {
         best = ms;

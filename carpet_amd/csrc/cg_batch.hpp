@@ -74,11 +74,12 @@ struct BatchDevice {
 // tile_cfg selects the z chunk of tensor tiles in coarse bases (0 = whole
 // region, < 0 = automatic); cls_ntl: per transfer the time levels of its
 // launch class (a tensor transfer resolved to fewer levels -- an exact time
-// match -- runs in its class with nlev = 1)
+// match -- runs in its class with nlev = 1); only >= 0 builds launch `only`
+// of the full build alone (the tuner's trials)
 int batch_device_build(BatchDevice &dev, const std::vector<Transfer> &tr,
                        const std::vector<int> &phase, const std::vector<int> &prec,
                        const std::vector<int> &cls_ntl, int tile_cfg,
-                       const TuneMap *tuned = nullptr);
+                       const TuneMap *tuned = nullptr, int only = -1);
 // the descriptor array of launch l from the current transfers, in the
 // layout batch_device_build uploads (the per-step parameter path rewrites
 // it in place: same items, new time weights and array pointers)

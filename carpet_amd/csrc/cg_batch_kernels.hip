@@ -389,7 +389,8 @@ void batch_launch_descs(const BatchLaunch &l, const std::vector<Transfer> &tr, c
 
 int batch_device_build(BatchDevice &dev, const std::vector<Transfer> &tr,
                        const std::vector<int> &phase, const std::vector<int> &prec,
-                       const std::vector<int> &cls_ntl, int tile_cfg, const TuneMap *tuned) {
+                       const std::vector<int> &cls_ntl, int tile_cfg, const TuneMap *tuned,
+                       int only) {
   batch_device_free(dev);
   // group transfers by (phase, kernel)
   std::vector<Group> groups;
@@ -441,7 +442,10 @@ int batch_device_build(BatchDevice &dev, const std::vector<Transfer> &tr,
       std::memcpy(host.data() + off, p, n);
     return off;
   };
-  for (const Group &g : groups) {
+  for (size_t gi = 0; gi < groups.size(); ++gi) {
+    const Group &g = groups[gi];
+    if (only >= 0 && (int)gi != only)
+      continue;
     if (g.prec != CG_PREC_F64 && (!g.is_tensor || fp64_only(g.kind)))
       return set_error(CG_ERR_OPERATOR, "mixed precision applies to Lagrange prolongation only");
     BatchLaunch l{};
