@@ -174,9 +174,9 @@ static void load_cache(const char *path, TuneMap &m) {
     const size_t bar = line.rfind('|');
     if (line.empty() || line[0] == '#' || bar == std::string::npos)
       continue;
-    int z = 0, p = 0, w = 0, sh = -1;
-    if (sscanf(line.c_str() + bar + 1, "%d %d %d %d", &z, &p, &w, &sh) >= 2)
-      m[line.substr(0, bar)] = Tune{z, p, w, sh};
+    int z = 0, p = 0, w = 0, sh = -1, ni = 1;
+    if (sscanf(line.c_str() + bar + 1, "%d %d %d %d %d", &z, &p, &w, &sh, &ni) >= 2)
+      m[line.substr(0, bar)] = Tune{z, p, w, sh, std::max(1, ni)};
   }
 }
 
@@ -190,10 +190,10 @@ static int save_cache(const char *path, const TuneMap &m) {
        "<workgroups per CU: 0 resident capacity, -1 one per item> "
        "<tile shape: 0 wide (32x8 coarse bases, 64x8 in the FP32-storage FP64 mode), 1 16x16, "
        "2 8x8 (FP64 regions thin in x and y or z, fixed), "
-       "-1 by region size>\n";
+       "-1 by region size> <256-point items per workgroup in grid mode>\n";
   for (const auto &kv : m)
     f << kv.first << "|" << kv.second.zchunk << " " << kv.second.pt << " " << kv.second.wpc << " "
-      << kv.second.shape << "\n";
+      << kv.second.shape << " " << kv.second.ni << "\n";
   return CG_OK;
 }
 
@@ -244,7 +244,7 @@ extern "C" int cg_batch_autotune(cg_batch *b, cg_stream_t stream, int reps,
         cands.push_back({z, l.pt, l.wpc, l.shape});
     else
       for (int p : {256, 512, 1024, 2048, 4096, 16384})
-        cands.push_back({l.zchunk, p, l.wpc, -1});
+        cands.push_back({l.zchunk, p, l.wpc, -1, l.ni});
     float best = 1e30f;
     Tune bestc = cands[0];
     // each trial builds this launch alone into its own descriptor blob
@@ -280,11 +280,19 @@ extern "C" int cg_batch_autotune(cg_batch *b, cg_stream_t stream, int reps,
     const Tune base = bestc;
     for (int w : {-1, 0, 2, -2})
       if (w != base.wpc && (w != -2 || l.is_tensor))
-        if (int e = trial_run(Tune{base.zchunk, base.pt, w, base.shape}))
+        if (int e = trial_run(Tune{base.zchunk, base.pt, w, base.shape, base.ni}))
           return e;
     if (l.is_tensor && l.shape < 2) // the other large tile shape, in the best mode
-      if (int e = trial_run(Tune{bestc.zchunk, bestc.pt, bestc.wpc, 1 - l.shape}))
+      if (int e = trial_run(Tune{bestc.zchunk, bestc.pt, bestc.wpc, 1 - l.shape, 1}))
         return e;
+    // 256-point items in grid mode: items per workgroup
+    if (!l.is_tensor && l.op != PO_TVD && bestc.pt == 256 && bestc.wpc == -1) {
+      const Tune b2 = bestc;
+      for (int ni : {1, 2, 4, 8})
+        if (ni != b2.ni)
+          if (int e = trial_run(Tune{b2.zchunk, b2.pt, b2.wpc, b2.shape, ni}))
+            return e;
+    }
     b->tuned[l.key] = bestc;
   }
   int e = batch_device_build(b->dev, b->transfers, b->phase, b->prec, b->cls_ntl, b->tile_cfg, &b->tuned);
@@ -306,7 +314,7 @@ extern "C" int cg_batch_describe(const cg_batch *b, char *buf, size_t len) {
   for (const BatchLaunch &l : b->dev.launches)
     o << "phase " << l.phase << " [" << l.key << "] items " << l.nitems << " grid " << l.grid
       << " zchunk " << l.zchunk << " pt " << l.pt << " wpc " << l.wpc << " shape " << l.shape
-      << "\n";
+      << " ni " << l.ni << "\n";
   const std::string s = o.str();
   const size_t n = std::min(len - 1, s.size());
   std::memcpy(buf, s.data(), n);

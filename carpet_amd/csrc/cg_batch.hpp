@@ -32,6 +32,7 @@ struct BatchLaunch {
                    // workgroup per item; -2 resident capacity taking items
                    // in fixed rounds (tensor tiles)
   int shape;       // tensor tile shape (0: 32 x 8 bases, 1: 16 x 16)
+  int ni;          // grid mode, 256-point pointwise items: items per workgroup
   std::string key; // tuner cache key (class + box-size histogram bucket)
   std::vector<int> members; // transfer indices in descriptor order
   size_t desc_bytes;        // bytes of the descriptor array
@@ -55,7 +56,7 @@ template <int KIND, int SH> const void *batch_tensor_fn_kind(const BatchLaunch &
 // tuned parameters of one launch class; wpc 0 = resident capacity;
 // shape -1 = chosen from the region sizes
 struct Tune {
-  int zchunk, pt, wpc, shape = -1;
+  int zchunk, pt, wpc, shape = -1, ni = 1;
 };
 using TuneMap = std::map<std::string, Tune>;
 

@@ -847,11 +847,8 @@ template <> struct TileShape<1, 1, 0> {
 template <> struct TileShape<2, 1, 0> {
   static constexpr int CX = 16, CY = 16, IPT = 1, XQ = 2;
 };
-#ifndef CG_NARROW3_CY
-#define CG_NARROW3_CY 16
-#endif
 template <> struct TileShape<3, 1, 0> {
-  static constexpr int CX = 16, CY = CG_NARROW3_CY, IPT = 1, XQ = 2;
+  static constexpr int CX = 16, CY = 16, IPT = 1, XQ = 2;
 };
 template <> struct TileShape<1, 0, 1> {
   static constexpr int CX = 64, CY = 8, IPT = 2, XQ = 2;

@@ -69,16 +69,20 @@ def test_batch_modes_match_oracle(make, tmp_path):
         # counters, wpc 0 and 2, and in fixed rounds, wpc -2, which point
         # classes run as wpc -1),
         # then the tuner's own search; each time from poisoned outputs
+        # 256-point items also with 2, 4 and 8 items per workgroup (ni)
         keys = re.findall(r"\[([^\]]+)\]", b.describe())
-        for wpc, zc, pt in ((-1, 16, 256), (0, 8, 512), (2, 0, 65536), (-2, 32, 1024)):
-            cache = tmp_path / f"tune_{wpc}.txt"
-            cache.write_text("".join(f"{k}|{zc} {pt} {wpc}\n" for k in keys))
+        for wpc, zc, pt, ni in ((-1, 16, 256, 1), (-1, 16, 256, 2), (-1, 16, 256, 4), (-1, 16, 256, 8),
+                                (0, 8, 512, 1), (2, 0, 65536, 1), (-2, 32, 1024, 1)):
+            cache = tmp_path / f"tune_{wpc}_{ni}.txt"
+            cache.write_text("".join(f"{k}|{zc} {pt} {wpc} -1 {ni}\n" for k in keys))
             b.autotune(st, reps=1, cache_path=str(cache))
             assert f"wpc {wpc}" in b.describe()
+            if ni > 1:
+                assert f"pt 256 wpc -1 shape 0 ni {ni}" in b.describe()
             _poison_outputs(w, dev, st)
             b.run(st)
             st.synchronize()
-            _compare(_outputs(w, dev), ref, f"persistent wpc {wpc}")
+            _compare(_outputs(w, dev), ref, f"persistent wpc {wpc} ni {ni}")
         cache = tmp_path / "tune_search.txt"
         b.autotune(st, reps=1, cache_path=str(cache))
         print("tuned:\n" + cache.read_text() + b.describe(), flush=True)

@@ -414,3 +414,34 @@ def test_regrid_schedule_matches_oracle(cent, seed):
         for k in L.comps:
             want |= k.active | k.overlaps
         assert np.array_equal(filled, want)
+
+
+def test_tiled_levels_make_a_valid_hierarchy():
+    """The 2000-box hierarchy of the level-step benchmark: 1995 boxes on 5
+    levels, every edge in [8,96] points, each level tiling its region
+    without gaps or overlaps, nested so that dh's consistency checks pass
+    (cg_schedule_build raises on a violation); every component has sync,
+    every finer level boundary prolongation and restriction lists."""
+    from carpet_amd.workloads import tiled_levels
+
+    lv = tiled_levels()
+    assert [len(L.components) for L in lv] == [64, 125, 294, 512, 1000]
+    for L in lv:
+        pts = 0
+        lo = [min(c[0][d] for c in L.components) for d in range(3)]
+        hi = [max(c[1][d] for c in L.components) for d in range(3)]
+        s = L.components[0][2][0]
+        for c in L.components:
+            for d in range(3):
+                e = (c[1][d] - c[0][d]) // s + 1
+                assert 8 <= e <= 96
+            pts += np.prod([(c[1][d] - c[0][d]) // s + 1 for d in range(3)])
+        assert pts == np.prod([(hi[d] - lo[d]) // s + 1 for d in range(3)])  # a tiling of the bounding box
+    sch = S.Schedule(1, ((0, 0, 0), (0, 0, 0)), lv)
+    for rl in range(5):
+        assert len(sch.lists[rl][S.SYNC]) > 0
+        if rl > 0:
+            assert len(sch.lists[rl][S.REF_BND_PROL]) > 0
+        if rl < 4:
+            assert len(sch.lists[rl][S.REF_REST]) > 0
+    sch.close()
