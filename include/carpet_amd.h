@@ -558,8 +558,11 @@ int cg_batch_set_storage(cg_batch *b, int n, const cg_gdata *from, const cg_gdat
    of the median region size, log2 of the region count) and kept in the
    text file cache_path (may be NULL; lines "<key>|<zchunk> <points>
    <workgroups per CU: -1 one per item, 0 resident capacity> <tile shape:
-   0 wide, 1 narrow, 2 small, -1 by region size>"): classes the file names take its
-   parameters, the others are searched.  The
+   0 wide, 1 narrow, 2 small, -1 by region size> <256-point items per
+   workgroup in grid mode: 1, 2, 4, 8>"): classes the file names take its
+   parameters, the others are searched (for pointwise classes of 256-point
+   items in grid mode also the items per workgroup).  Each trial builds its
+   launch alone into its own descriptor list.  The
    search executes each launch (one phase's class of transfers) reps times
    on its own, outside phase order, so call it before the step whose
    results are kept, or on scratch data.  Drops a captured graph. */
