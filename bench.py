@@ -82,6 +82,48 @@ def cpu_baseline(target_seconds: float = 12.0):
                       f"{n[0]}x{n[1]}x{n[2]} fine points, {reps} repetitions in {el:.1f} s"}
 
 
+def copy_ceiling(n: int = 640, reps: int = 40) -> dict:
+    """GB/s (bytes read + written) of flat copies on this GPU, the STREAM
+    copy ceiling beside the 8 TB/s peak: the library's flat stream copy
+    (cg_stream_copy_f64), the HIP runtime's device-to-device hipMemcpyAsync
+    and the library's copy_3d of a whole n^3 FP64 array into another; HIP
+    events on the stream they run on, over reps calls after a warmup."""
+    import ctypes as C
+
+    import carpet_amd as cg
+    from carpet_amd._lib import check, lib
+
+    box = cg.Box((0, 0, 0), (n - 1,) * 3, (1, 1, 1))
+    a, b = cg.DeviceArray(box), cg.DeviceArray(box)
+    nbytes = 8 * n**3
+    st = cg.Stream()
+    a.fill(1.0)
+    b.fill(0.0)
+    out = {}
+    for name in ("stream_copy", "hip_memcpy", "copy_3d"):
+        def once():
+            if name == "stream_copy":
+                check(lib.cg_stream_copy_f64(C.c_void_p(b.ptr), C.c_void_p(a.ptr), n**3, st.handle),
+                      "cg_stream_copy_f64")
+            elif name == "hip_memcpy":
+                check(lib.cg_memcpy_dtod(C.c_void_p(b.ptr), C.c_void_p(a.ptr), nbytes, st.handle),
+                      "cg_memcpy_dtod")
+            else:
+                cg.copy_3d(a, b, box, box, stream=st)
+        for _ in range(5):
+            once()
+        e0, e1 = cg.Event(), cg.Event()
+        e0.record(st)
+        for _ in range(reps):
+            once()
+        e1.record(st)
+        st.synchronize()
+        out[name] = round(2 * nbytes / (e0.elapsed_ms(e1) / reps) / 1e6, 1)
+    a.free()
+    b.free()
+    return out
+
+
 def calib_copy(n: int = 512):
     """One copy_3d of an n^3 FP64 array (known traffic: 8 n^3 bytes read and
     written) so that a profiled run can scale its HBM counters."""
@@ -747,6 +789,16 @@ def main(argv=None):
                          "traffic": _profile_traffic(f"level_step_2000_regions_x24_launch{dom['launch']}"),
                          "kernel": f"launch {dom['launch']} (phase {dom['phase']})", "ms": dom["ms"],
                          "algorithmic_bytes": dom["algorithmic_bytes"]}}
+    # the measured flat-copy ceiling of this GPU beside the 8 TB/s peak
+    cc = copy_ceiling()
+    ceil = max(cc.values())
+    line["copy_ceiling"] = {"gbytes_per_s": ceil, **cc,
+                            "step_frac": round(r["batched_gbytes_per_s"] / ceil, 4),
+                            "dominant_launch_frac": round(dom["gbytes_per_s"] / ceil, 4),
+                            "how": "the fastest of the library's flat stream copy (cg_stream_copy_f64), the HIP "
+                                   "runtime's device-to-device hipMemcpyAsync and copy_3d, each of a whole "
+                                   "640^3 FP64 array into another (16 B per point), HIP events over 40 calls"}
+    print(f"# copy ceiling: {cc} GB/s", file=sys.stderr, flush=True)
     if not args.no_config2:
         print("# config2 beside the headline line", file=sys.stderr, flush=True)
         line["single_patch_512"] = config2_line(args, side=True)

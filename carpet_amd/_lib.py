@@ -57,6 +57,7 @@ _decl("cg_memcpy_dtoh", i, vp, vp, C.c_size_t)
 _decl("cg_memcpy_dtod", i, vp, vp, C.c_size_t, vp)
 _decl("cg_memset_d8", i, vp, i, C.c_size_t, vp)
 _decl("cg_fill_f64", i, vp, dp, C.c_size_t, vp)
+_decl("cg_stream_copy_f64", i, vp, vp, C.c_size_t, vp)
 _decl("cg_fill_hash_f64", i, vp, ip, ip, C.c_ulonglong, dp, dp, vp)
 _decl("cg_fill_fourier_f64", i, C.POINTER(GData), dp, dp, i, ip, C.POINTER(C.c_double), vp)
 
@@ -116,6 +117,7 @@ _decl("cg_batch_capture_graph", i, vp, vp)
 _decl("cg_batch_launch_graph", i, vp, vp)
 _decl("cg_batch_set_time", i, vp, i, C.POINTER(C.c_double), i, dp)
 _decl("cg_batch_set_storage", i, vp, i, _GP, _GP)
+_decl("cg_batch_set_phase_mode", i, vp, i, i)
 _decl("cg_batch_launch_count", i, vp)
 _decl("cg_batch_run_launch", i, vp, i, vp)
 _decl("cg_batch_launch_stats", i, vp, i, ip, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong))

@@ -550,6 +550,11 @@ class Batch:
         to = (GData * max(1, n))(*[b.g for _, b in pairs])
         return check(lib.cg_batch_set_storage(self.handle, n, fr, to), "cg_batch_set_storage")
 
+    def set_phase_mode(self, phase: int, mode: int):
+        """How the launches of `phase` run: 1 side by side, 0 one after the
+        other, -1 automatic (cg_batch_set_phase_mode)."""
+        check(lib.cg_batch_set_phase_mode(self.handle, phase, mode), "cg_batch_set_phase_mode")
+
     def autotune(self, stream=None, reps: int = 3, cache_path: Optional[str] = None):
         """Searches the work-item sizes per launch class (cg_batch_autotune)."""
         check(lib.cg_batch_autotune(self.handle, _sh(stream), reps,

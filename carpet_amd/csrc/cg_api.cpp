@@ -644,6 +644,13 @@ int cg_fill_f64(double *dst, double value, size_t n, cg_stream_t s) {
     return set_error(CG_ERR_ARG, "null dst");
   return launch_fill(dst, value, n, S(s)) ? hip_error(hipGetLastError(), "fill") : CG_OK;
 }
+int cg_stream_copy_f64(double *dst, const double *src, size_t n, cg_stream_t s) {
+  if ((!dst || !src) && n)
+    return set_error(CG_ERR_ARG, "null pointer");
+  if ((reinterpret_cast<uintptr_t>(dst) | reinterpret_cast<uintptr_t>(src)) & 7)
+    return set_error(CG_ERR_ARG, "stream copy needs 8-byte aligned arrays");
+  return launch_stream_copy(dst, src, n, S(s)) ? hip_error(hipGetLastError(), "stream copy") : CG_OK;
+}
 int cg_fill_hash_f64(double *dst, const int padext[3], const int ext[3],
                      unsigned long long seed, double lo, double hi, cg_stream_t s) {
   if (!dst || !padext || !ext)

@@ -45,6 +45,12 @@ struct cg_batch {
   std::vector<BatchAddArgs> args;
   cg::BatchDevice dev;
   cg::TuneMap tuned;
+  // how the launches of a phase run (1 side by side, 0 one after the
+  // other): set per phase by the caller, chosen per phase (key: its launch
+  // classes) by the tuner; otherwise side by side exactly when every launch
+  // of the phase runs one workgroup per item
+  std::map<int, int> phase_mode;
+  std::map<std::string, int> tuned_side;
   int tile_cfg = -1;
   bool finalized = false;
   hipGraph_t graph = nullptr;
@@ -133,12 +139,48 @@ int cg_batch_add(cg_batch *b, int phase, const cg_gdata *dst,
 }
 
 // transfer -> launch map after a (re)build; nothing is pending after it
+// the launch classes of the phase group [i, j)
+static std::string group_key(const BatchDevice &dev, size_t i, size_t j) {
+  std::string k;
+  for (size_t m = i; m < j; ++m)
+    k += (m > i ? " + " : "") + dev.launches[m].key;
+  return k;
+}
+
+// A persistent launch sizes its grid to the resident capacity of the whole
+// GPU; beside another launch its workgroups wait for CUs the other holds,
+// and the phase takes longer than the two launches one after the other (the
+// finest-level prolongation of the 2000-box hierarchy step: 7.16 ms side by
+// side against 3.65 + 2.13 ms in turn).  Launches of one workgroup per item
+// fill each other's tails.
+static void apply_sides(cg_batch *b) {
+  BatchDevice &dev = b->dev;
+  dev.beside.assign(dev.launches.size(), 1);
+  for (size_t i = 0; i < dev.launches.size();) {
+    const size_t j = batch_phase_end(dev, i);
+    if (j - i > 1) {
+      int side = 1;
+      for (size_t m = i; m < j; ++m)
+        side = side && dev.launches[m].wpc == -1;
+      const auto t = b->tuned_side.find(group_key(dev, i, j));
+      if (t != b->tuned_side.end())
+        side = t->second;
+      const auto f = b->phase_mode.find(dev.launches[i].phase);
+      if (f != b->phase_mode.end())
+        side = f->second;
+      dev.beside[i] = (char)side;
+    }
+    i = j;
+  }
+}
+
 static void index_launches(cg_batch *b) {
   b->launch_of.assign(b->transfers.size(), -1);
   for (size_t l = 0; l < b->dev.launches.size(); ++l)
     for (int m : b->dev.launches[l].members)
       b->launch_of[m] = (int)l;
   b->dirty.assign(b->dev.launches.size(), 0);
+  apply_sides(b);
 }
 
 int cg_batch_finalize(cg_batch *b, int tile_cfg) {
@@ -165,7 +207,7 @@ static void drop_graph(cg_batch *b) {
   b->graph = nullptr;
 }
 
-static void load_cache(const char *path, TuneMap &m) {
+static void load_cache(const char *path, TuneMap &m, std::map<std::string, int> &sides) {
   if (!path)
     return;
   std::ifstream f(path);
@@ -174,13 +216,19 @@ static void load_cache(const char *path, TuneMap &m) {
     const size_t bar = line.rfind('|');
     if (line.empty() || line[0] == '#' || bar == std::string::npos)
       continue;
+    if (line.compare(0, 6, "phase ") == 0) {
+      int side = 1;
+      if (sscanf(line.c_str() + bar + 1, "%d", &side) == 1)
+        sides[line.substr(6, bar - 6)] = side ? 1 : 0;
+      continue;
+    }
     int z = 0, p = 0, w = 0, sh = -1, ni = 1;
     if (sscanf(line.c_str() + bar + 1, "%d %d %d %d %d", &z, &p, &w, &sh, &ni) >= 2)
       m[line.substr(0, bar)] = Tune{z, p, w, sh, std::max(1, ni)};
   }
 }
 
-static int save_cache(const char *path, const TuneMap &m) {
+static int save_cache(const char *path, const TuneMap &m, const std::map<std::string, int> &sides) {
   if (!path)
     return CG_OK;
   std::ofstream f(path);
@@ -190,10 +238,13 @@ static int save_cache(const char *path, const TuneMap &m) {
        "<workgroups per CU: 0 resident capacity, -1 one per item> "
        "<tile shape: 0 wide (32x8 coarse bases, 64x8 in the FP32-storage FP64 mode), 1 16x16, "
        "2 8x8 (FP64 regions thin in x and y or z, fixed), "
-       "-1 by region size> <256-point items per workgroup in grid mode>\n";
+       "-1 by region size> <256-point items per workgroup in grid mode>; "
+       "phase <class key> + <class key> ...|<1 side by side, 0 one after the other>\n";
   for (const auto &kv : m)
     f << kv.first << "|" << kv.second.zchunk << " " << kv.second.pt << " " << kv.second.wpc << " "
       << kv.second.shape << " " << kv.second.ni << "\n";
+  for (const auto &kv : sides)
+    f << "phase " << kv.first << "|" << kv.second << "\n";
   return CG_OK;
 }
 
@@ -219,6 +270,29 @@ static int time_launch(BatchDevice &dev, int idx, hipStream_t st, int reps, floa
   return e;
 }
 
+static int time_phase(BatchDevice &dev, size_t i, size_t j, bool side, hipStream_t st, int reps,
+                      float *ms) {
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+    return hip_error(hipGetLastError(), "hipEventCreate");
+  int e = batch_device_run_phase(dev, i, j, side, st); // warm-up
+  float best = 1e30f;
+  for (int r = 0; r < reps && !e; ++r) {
+    (void)hipEventRecord(e0, st);
+    e = batch_device_run_phase(dev, i, j, side, st);
+    (void)hipEventRecord(e1, st);
+    if (!e && hipEventSynchronize(e1) == hipSuccess) {
+      float t = 0;
+      (void)hipEventElapsedTime(&t, e0, e1);
+      best = std::min(best, t);
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *ms = best;
+  return e;
+}
+
 extern "C" int cg_batch_autotune(cg_batch *b, cg_stream_t stream, int reps,
                                  const char *cache_path) {
   if (!b)
@@ -230,7 +304,8 @@ extern "C" int cg_batch_autotune(cg_batch *b, cg_stream_t stream, int reps,
   // classes named in the cache take its parameters, every other class is
   // searched (the launches run on the batch's own arrays)
   b->tuned.clear();
-  load_cache(cache_path, b->tuned);
+  b->tuned_side.clear();
+  load_cache(cache_path, b->tuned, b->tuned_side);
   drop_graph(b);
   // current choices for every class, then search the classes the cache lacks
   for (size_t i = 0; i < b->dev.launches.size(); ++i) {
@@ -288,7 +363,7 @@ extern "C" int cg_batch_autotune(cg_batch *b, cg_stream_t stream, int reps,
     // 256-point items in grid mode: items per workgroup
     if (!l.is_tensor && l.op != PO_TVD && bestc.pt == 256 && bestc.wpc == -1) {
       const Tune b2 = bestc;
-      for (int ni : {1, 2, 4, 8})
+      for (int ni : {1, 2, 4, 8, 16})
         if (ni != b2.ni)
           if (int e = trial_run(Tune{b2.zchunk, b2.pt, b2.wpc, b2.shape, ni}))
             return e;
@@ -298,8 +373,24 @@ extern "C" int cg_batch_autotune(cg_batch *b, cg_stream_t stream, int reps,
   int e = batch_device_build(b->dev, b->transfers, b->phase, b->prec, b->cls_ntl, b->tile_cfg, &b->tuned);
   if (e)
     return e;
+  // phases of several launches: side by side or one after the other
+  for (size_t i = 0; i < b->dev.launches.size();) {
+    const size_t j = batch_phase_end(b->dev, i);
+    const std::string k = group_key(b->dev, i, j);
+    if (j - i > 1 && !b->tuned_side.count(k)) {
+      float ms[2] = {0, 0};
+      for (int side = 0; side < 2; ++side)
+        if (int e2 = time_phase(b->dev, i, j, side != 0, st, reps, &ms[side]))
+          return e2;
+      hipError_t he = hipStreamSynchronize(st);
+      if (he != hipSuccess)
+        return hip_error(he, "tuner phase trial");
+      b->tuned_side[k] = ms[1] <= ms[0] ? 1 : 0;
+    }
+    i = j;
+  }
   index_launches(b);
-  return save_cache(cache_path, b->tuned);
+  return save_cache(cache_path, b->tuned, b->tuned_side);
 }
 
 extern "C" int cg_batch_describe(const cg_batch *b, char *buf, size_t len) {
@@ -311,10 +402,19 @@ extern "C" int cg_batch_describe(const cg_batch *b, char *buf, size_t len) {
     items += l.nitems;
   o << "transfers " << b->transfers.size() << " launches " << b->dev.launches.size()
     << " work items " << items << " device descriptor bytes " << b->dev.blob_bytes << "\n";
-  for (const BatchLaunch &l : b->dev.launches)
-    o << "phase " << l.phase << " [" << l.key << "] items " << l.nitems << " grid " << l.grid
-      << " zchunk " << l.zchunk << " pt " << l.pt << " wpc " << l.wpc << " shape " << l.shape
-      << " ni " << l.ni << "\n";
+  for (size_t i = 0; i < b->dev.launches.size();) {
+    const size_t j = batch_phase_end(b->dev, i);
+    for (size_t m = i; m < j; ++m) {
+      const BatchLaunch &l = b->dev.launches[m];
+      o << "phase " << l.phase << " [" << l.key << "] items " << l.nitems << " grid " << l.grid
+        << " zchunk " << l.zchunk << " pt " << l.pt << " wpc " << l.wpc << " shape " << l.shape
+        << " ni " << l.ni;
+      if (j - i > 1)
+        o << (b->dev.beside[i] ? " side by side" : " in turn");
+      o << "\n";
+    }
+    i = j;
+  }
   const std::string s = o.str();
   const size_t n = std::min(len - 1, s.size());
   std::memcpy(buf, s.data(), n);
@@ -425,6 +525,21 @@ static int upload_pending(cg_batch *b, hipStream_t st) {
   if (he != hipSuccess)
     return hip_error(he, "hipEventRecord (descriptor staging)");
   b->stage_used = true;
+  return CG_OK;
+}
+
+int cg_batch_set_phase_mode(cg_batch *b, int phase, int mode) {
+  if (!b)
+    return set_error(CG_ERR_ARG, "null batch");
+  if (mode < -1 || mode > 1)
+    return set_error(CG_ERR_ARG, "phase mode %d (1 side by side, 0 one after the other, -1 automatic)", mode);
+  if (mode < 0)
+    b->phase_mode.erase(phase);
+  else
+    b->phase_mode[phase] = mode;
+  drop_graph(b);
+  if (b->finalized)
+    apply_sides(b);
   return CG_OK;
 }
 

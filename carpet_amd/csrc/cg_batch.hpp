@@ -70,6 +70,10 @@ struct BatchDevice {
   // the caller's stream and joined back (graph branches under capture)
   std::vector<hipStream_t> aux;
   std::vector<hipEvent_t> fork, join;
+  // per launch, read at the first launch of each phase: 1 = the phase's
+  // launches run side by side, 0 = one after the other on the caller's
+  // stream (empty: side by side)
+  std::vector<char> beside;
 };
 
 // tile_cfg selects the z chunk of tensor tiles in coarse bases (0 = whole
@@ -86,6 +90,12 @@ int batch_device_build(BatchDevice &dev, const std::vector<Transfer> &tr,
 // it in place: same items, new time weights and array pointers)
 void batch_launch_descs(const BatchLaunch &l, const std::vector<Transfer> &tr, char *out);
 int batch_device_run(BatchDevice &dev, hipStream_t st);
+// end of the phase group that starts at launch i (launches of one phase are
+// consecutive)
+size_t batch_phase_end(const BatchDevice &dev, size_t i);
+// runs the launches [i, j) of one phase, side by side or one after the
+// other, their work counters cleared first (the tuner's phase trials)
+int batch_device_run_phase(BatchDevice &dev, size_t i, size_t j, bool side, hipStream_t st);
 // runs launch idx alone (its counter cleared first)
 int batch_device_run_one(BatchDevice &dev, int idx, hipStream_t st);
 void batch_device_free(BatchDevice &dev);

@@ -36,6 +36,17 @@ namespace cg {
 #define CG_POINT_NI 4
 #endif
 
+// destination stores of the pointwise kernels: nontemporal (1) or plain (0)
+#ifndef CG_POINT_NT
+#define CG_POINT_NT 1
+#endif
+__device__ __forceinline__ void point_store(double v, double *p) {
+  if constexpr (CG_POINT_NT)
+    __builtin_nontemporal_store(v, p);
+  else
+    *p = v;
+}
+
 // tensor transfers of at most this many tiles run whole on one XCD
 #ifndef CG_WHOLE_XCD_ITEMS
 #define CG_WHOLE_XCD_ITEMS 16
@@ -134,7 +145,7 @@ __device__ __forceinline__ void point_item(const PointArgs &a, unsigned tile, in
 #pragma unroll
       for (int u = 0; u < PU; ++u)
         if (o[u] >= 0)
-          __builtin_nontemporal_store(v[u], a.dst + o[u]);
+          point_store(v[u], a.dst + o[u]);
     }
   }
 }
@@ -178,11 +189,10 @@ __global__ __launch_bounds__(256) void k_grid_point_n(const PointArgs *__restric
   const int x = blockIdx.x & 7;
   const int i0 = seg[x] + (int)(blockIdx.x >> 3) * NI, i1 = seg[x + 1];
   double v[NI];
-  double *d[NI];
-  long long o[NI];
+  double *q[NI]; // destination of each value, null outside the item
 #pragma unroll
   for (int u = 0; u < NI; ++u) {
-    o[u] = -1;
+    q[u] = nullptr;
     const int it = i0 + u;
     if (it < i1) {
       const Item w = items[it];
@@ -197,15 +207,14 @@ __global__ __launch_bounds__(256) void k_grid_point_n(const PointArgs *__restric
         const unsigned j = fastdiv(r, a.fx);
         const unsigned i = r - j * nx;
         v[u] = point_value<OP>(a, (int)i, (int)j, (int)k);
-        o[u] = (long long)i + a.dj * j + a.dk * k;
-        d[u] = a.dst;
+        q[u] = a.dst + ((long long)i + a.dj * j + a.dk * k);
       }
     }
   }
 #pragma unroll
   for (int u = 0; u < NI; ++u)
-    if (o[u] >= 0)
-      __builtin_nontemporal_store(v[u], d[u] + o[u]);
+    if (q[u])
+      point_store(v[u], q[u]);
 }
 
 // TVD items run per coarse cell with their own unrolling: one instance
@@ -574,7 +583,11 @@ int batch_device_build(BatchDevice &dev, const std::vector<Transfer> &tr,
         l.zchunk = it->second.zchunk;
         l.pt = it->second.pt;
         l.wpc = it->second.wpc;
-        l.ni = std::max(1, it->second.ni);
+        // items per workgroup: 1, 2, 4, 8 or 16 (the largest of these not
+        // above the cached value)
+        l.ni = 1;
+        while (l.ni < 16 && 2 * l.ni <= it->second.ni)
+          l.ni *= 2;
         if (!g.is_tensor && l.wpc == -2) // static rounds: tensor tiles only
           l.wpc = -1;
         if (g.is_tensor && !g.small && it->second.shape >= 0 && it->second.shape < 2)
@@ -808,6 +821,48 @@ static int ensure_aux(BatchDevice &dev, size_t n) {
   return CG_OK;
 }
 
+size_t batch_phase_end(const BatchDevice &dev, size_t i) {
+  size_t j = i + 1;
+  while (j < dev.launches.size() && dev.launches[j].phase == dev.launches[i].phase)
+    ++j;
+  return j;
+}
+
+// The launches of one phase write disjoint regions (dh's lists never send
+// one point twice), so the classes of a phase may run side by side: the
+// first on st, the others on helper streams forked from st by an event and
+// joined back before the next phase.  Under stream capture the fork and
+// join become branches of the graph.
+static int run_phase(BatchDevice &dev, size_t i, size_t j, bool side, hipStream_t st) {
+  const size_t extra = side ? j - i - 1 : 0;
+  if (extra > 0) {
+    if (int e = ensure_aux(dev, extra))
+      return e;
+    hipError_t he = hipEventRecord(dev.fork[0], st);
+    for (size_t k = 0; k < extra && he == hipSuccess; ++k)
+      he = hipStreamWaitEvent(dev.aux[k], dev.fork[0], 0);
+    if (he != hipSuccess)
+      return hip_error(he, "phase fork");
+  }
+  for (size_t k = i; k < j; ++k) {
+    int e = run_launch(dev, dev.launches[k], (k == i || !side) ? st : dev.aux[k - i - 1]);
+    if (e)
+      return e;
+  }
+  for (size_t k = 0; k < extra; ++k) {
+    hipError_t he = hipEventRecord(dev.join[k], dev.aux[k]);
+    if (he == hipSuccess)
+      he = hipStreamWaitEvent(st, dev.join[k], 0);
+    if (he != hipSuccess)
+      return hip_error(he, "phase join");
+  }
+  return CG_OK;
+}
+
+static bool phase_beside(const BatchDevice &dev, size_t i) {
+  return dev.beside.size() != dev.launches.size() || dev.beside[i] != 0;
+}
+
 int batch_device_run(BatchDevice &dev, hipStream_t st) {
   if (dev.launches.empty())
     return CG_OK;
@@ -818,41 +873,21 @@ int batch_device_run(BatchDevice &dev, hipStream_t st) {
   if (persistent)
     hipLaunchKernelGGL(k_zero_counters, dim3((8 * dev.ncounters + 255) / 256), dim3(256), 0, st,
                        dev.counters, 8 * dev.ncounters);
-  // The launches of one phase write disjoint regions (dh's lists never send
-  // one point twice), so the classes of a phase run side by side: the first
-  // on st, the others on helper streams forked from st by an event and joined
-  // back before the next phase.  Under stream capture the fork and join
-  // become branches of the graph.
-  const size_t nl = dev.launches.size();
-  for (size_t i = 0; i < nl;) {
-    size_t j = i + 1;
-    while (j < nl && dev.launches[j].phase == dev.launches[i].phase)
-      ++j;
-    const size_t extra = j - i - 1;
-    if (extra > 0) {
-      if (int e = ensure_aux(dev, extra))
-        return e;
-      hipError_t he = hipEventRecord(dev.fork[0], st);
-      for (size_t k = 0; k < extra && he == hipSuccess; ++k)
-        he = hipStreamWaitEvent(dev.aux[k], dev.fork[0], 0);
-      if (he != hipSuccess)
-        return hip_error(he, "phase fork");
-    }
-    for (size_t k = i; k < j; ++k) {
-      int e = run_launch(dev, dev.launches[k], k == i ? st : dev.aux[k - i - 1]);
-      if (e)
-        return e;
-    }
-    for (size_t k = 0; k < extra; ++k) {
-      hipError_t he = hipEventRecord(dev.join[k], dev.aux[k]);
-      if (he == hipSuccess)
-        he = hipStreamWaitEvent(st, dev.join[k], 0);
-      if (he != hipSuccess)
-        return hip_error(he, "phase join");
-    }
+  for (size_t i = 0; i < dev.launches.size();) {
+    const size_t j = batch_phase_end(dev, i);
+    if (int e = run_phase(dev, i, j, phase_beside(dev, i), st))
+      return e;
     i = j;
   }
   return CG_OK;
+}
+
+int batch_device_run_phase(BatchDevice &dev, size_t i, size_t j, bool side, hipStream_t st) {
+  if (i >= j || j > dev.launches.size())
+    return set_error(CG_ERR_ARG, "no launches [%zu, %zu)", i, j);
+  hipLaunchKernelGGL(k_zero_counters, dim3((8 * dev.ncounters + 255) / 256), dim3(256), 0, st,
+                     dev.counters, 8 * dev.ncounters);
+  return run_phase(dev, i, j, side, st);
 }
 
 int batch_device_run_one(BatchDevice &dev, int idx, hipStream_t st) {
@@ -888,7 +923,8 @@ static int run_launch(BatchDevice &dev, const BatchLaunch &l, hipStream_t st) {
           switch (l.ni) {
           case 2: hipLaunchKernelGGL((k_grid_point_n<OP, 2>), dim3(l.grid), dim3(256), 0, st, pd, items, seg); break;
           case 4: hipLaunchKernelGGL((k_grid_point_n<OP, 4>), dim3(l.grid), dim3(256), 0, st, pd, items, seg); break;
-          default: hipLaunchKernelGGL((k_grid_point_n<OP, 8>), dim3(l.grid), dim3(256), 0, st, pd, items, seg);
+          case 8: hipLaunchKernelGGL((k_grid_point_n<OP, 8>), dim3(l.grid), dim3(256), 0, st, pd, items, seg); break;
+          default: hipLaunchKernelGGL((k_grid_point_n<OP, 16>), dim3(l.grid), dim3(256), 0, st, pd, items, seg);
           }
           return 0;
         }

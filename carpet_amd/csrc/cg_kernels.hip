@@ -253,6 +253,34 @@ int launch_fill_fourier(double *d, long long sj, long long sk, const int *ext,
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
+// flat copy of n doubles: two per thread, 256 apart, both loads before the
+// stores.  Measured on 2 x 2.1 GB: 6.45 TB/s read + written, against 6.35
+// with one double per thread, 6.0 with four and 5.65-6.15 with 16-byte
+// accesses (one to four per thread, plain or nontemporal).
+__global__ __launch_bounds__(256) void k_stream_copy(const double *__restrict__ src,
+                                                     double *__restrict__ dst, size_t n) {
+  const size_t b = (size_t)blockIdx.x * 512 + threadIdx.x;
+  double v0 = 0, v1 = 0;
+  if (b < n)
+    v0 = src[b];
+  if (b + 256 < n)
+    v1 = src[b + 256];
+  if (b < n)
+    dst[b] = v0;
+  if (b + 256 < n)
+    dst[b + 256] = v1;
+}
+
+int launch_stream_copy(double *dst, const double *src, size_t n, hipStream_t s) {
+  if (n == 0)
+    return 0;
+  const size_t blocks = (n + 511) / 512;
+  if (blocks > 0x7fffffff)
+    return -1;
+  hipLaunchKernelGGL(k_stream_copy, dim3((unsigned)blocks), dim3(256), 0, s, src, dst, n);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
 int launch_fill(double *dst, double v, size_t n, hipStream_t s) {
   if (n == 0)
     return 0;

@@ -97,6 +97,7 @@ int launch_tensor(int kind, int order, int ntl, int prec, const TensorArgs &a,
 long long tensor_tiles(TensorArgs &a, const TileCfg &cfg);
 int launch_point(int op, const PointArgs &a, hipStream_t s);
 int launch_fill(double *dst, double v, size_t n, hipStream_t s);
+int launch_stream_copy(double *dst, const double *src, size_t n, hipStream_t s);
 int launch_fill_fourier(double *d, long long sj, long long sk, const int *ext,
                         const int *lower, const int *stride, double scale, double amp,
                         int nmodes, const int *kvec, const double *phase, hipStream_t s);

@@ -159,6 +159,10 @@ int cg_memcpy_dtoh(void *dst, const void *src, size_t bytes);
 int cg_memcpy_dtod(void *dst, const void *src, size_t bytes, cg_stream_t s);
 int cg_memset_d8(void *dst, int value, size_t bytes, cg_stream_t s);
 int cg_fill_f64(double *dst, double value, size_t n, cg_stream_t s);
+/* no reference counterpart: dst[i] = src[i] for n doubles as a flat stream
+   (both arrays 8-byte aligned, else CG_ERR_ARG), the copy ceiling a
+   benchmark measures the transfer kernels against */
+int cg_stream_copy_f64(double *dst, const double *src, size_t n, cg_stream_t s);
 /* no reference counterpart: seeded synthetic data for tests and benchmarks;
    element (i,j,k) of the extent (padding untouched) becomes
    lo + (hi-lo)*u with u the top 53 bits of splitmix64(seed + i +
@@ -559,19 +563,31 @@ int cg_batch_set_storage(cg_batch *b, int n, const cg_gdata *from, const cg_gdat
    text file cache_path (may be NULL; lines "<key>|<zchunk> <points>
    <workgroups per CU: -1 one per item, 0 resident capacity> <tile shape:
    0 wide, 1 narrow, 2 small, -1 by region size> <256-point items per
-   workgroup in grid mode: 1, 2, 4, 8>"): classes the file names take its
+   workgroup in grid mode: 1, 2, 4, 8, 16>"): classes the file names take its
    parameters, the others are searched (for pointwise classes of 256-point
    items in grid mode also the items per workgroup).  Each trial builds its
-   launch alone into its own descriptor list.  The
+   launch alone into its own descriptor list.  Then every phase of several
+   launches is timed with them side by side and one after the other (see
+   cg_batch_set_phase_mode) and keeps the faster, cached as lines
+   "phase <key> + <key> ...|<1 side by side, 0 one after the other>".  The
    search executes each launch (one phase's class of transfers) reps times
    on its own, outside phase order, so call it before the step whose
    results are kept, or on scratch data.  Drops a captured graph. */
 int cg_batch_autotune(cg_batch *b, cg_stream_t stream, int reps,
                       const char *cache_path);
+/* no reference counterpart: how the launches of one phase run.  They write
+   disjoint regions, so they may run side by side (helper streams forked
+   from the caller's stream, joined before the next phase; graph branches
+   under capture) or one after the other.  mode 1 side by side, 0 one after
+   the other, -1 automatic: the tuner's choice, else side by side exactly
+   when every launch of the phase runs one workgroup per item (a persistent
+   launch fills the GPU by itself).  Any time; applies from the next
+   cg_batch_run or capture and drops a captured graph. */
+int cg_batch_set_phase_mode(cg_batch *b, int phase, int mode);
 /* human-readable summary: a totals line (transfers, launches, work items,
    bytes of the device descriptor blob), then one line per launch (phase,
-   class key, work items, grid, tile choice, grid mode); returns the full
-   length */
+   class key, work items, grid, tile choice, grid mode; in phases of several
+   launches "side by side" or "in turn"); returns the full length */
 int cg_batch_describe(const cg_batch *b, char *buf, size_t len);
 /* no reference counterpart: the launches of a finalized batch one at a
    time, for per-kernel timing -- their number, launch idx run on its own

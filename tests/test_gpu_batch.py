@@ -69,16 +69,18 @@ def test_batch_modes_match_oracle(make, tmp_path):
         # counters, wpc 0 and 2, and in fixed rounds, wpc -2, which point
         # classes run as wpc -1),
         # then the tuner's own search; each time from poisoned outputs
-        # 256-point items also with 2, 4 and 8 items per workgroup (ni)
+        # 256-point items also with 2, 4, 8 and 16 items per workgroup (ni;
+        # a cached 3 runs as 2)
         keys = re.findall(r"\[([^\]]+)\]", b.describe())
-        for wpc, zc, pt, ni in ((-1, 16, 256, 1), (-1, 16, 256, 2), (-1, 16, 256, 4), (-1, 16, 256, 8),
-                                (0, 8, 512, 1), (2, 0, 65536, 1), (-2, 32, 1024, 1)):
+        for wpc, zc, pt, ni in ((-1, 16, 256, 1), (-1, 16, 256, 2), (-1, 16, 256, 3), (-1, 16, 256, 4),
+                                (-1, 16, 256, 8), (-1, 16, 256, 16), (0, 8, 512, 1), (2, 0, 65536, 1),
+                                (-2, 32, 1024, 1)):
             cache = tmp_path / f"tune_{wpc}_{ni}.txt"
             cache.write_text("".join(f"{k}|{zc} {pt} {wpc} -1 {ni}\n" for k in keys))
             b.autotune(st, reps=1, cache_path=str(cache))
             assert f"wpc {wpc}" in b.describe()
             if ni > 1:
-                assert f"pt 256 wpc -1 shape 0 ni {ni}" in b.describe()
+                assert f"pt 256 wpc -1 shape 0 ni {2 if ni == 3 else ni}" in b.describe()
             _poison_outputs(w, dev, st)
             b.run(st)
             st.synchronize()

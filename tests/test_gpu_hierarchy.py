@@ -51,6 +51,56 @@ def test_hierarchy_step_matches_oracle(cent, order, op):
     w.free(dev)
 
 
+@pytest.mark.parametrize("mode", [0, 1], ids=["in-turn", "side-by-side"])
+def test_phase_modes_match_oracle(mode, tmp_path):
+    """The launches of a phase (8x8-base and large tiles of one boundary
+    prolongation) run one after the other or side by side, direct and from a
+    captured graph; then the tuner chooses per phase (its choices cached)."""
+    import re
+    lv = nested_levels(ncoarse=32, nlevels=3, split=(2, 1, 1), centering=cg.VERTEX_CENTERED, buffer=6, order=5)
+    w = hierarchy_step(lv, cg.VERTEX_CENTERED, n_gf=2)
+    ref = run_oracle(w)
+    st = cg.Stream()
+    dev = w.allocate()
+    b = w.batch(dev)
+    b.finalize()
+    phases = [int(p) for p in re.findall(r"^phase (\d+) ", b.describe(), re.M)]
+    multi = sorted({p for p in phases if phases.count(p) > 1})
+    assert multi, b.describe()
+    for p in multi:
+        b.set_phase_mode(p, mode)
+    print(b.describe(), flush=True)
+    assert ("side by side" if mode else "in turn") in b.describe()
+    assert ("in turn" if mode else "side by side") not in b.describe()
+    for how in ("run", "graph", "tuned"):
+        for k in ref:  # back to the initial state
+            dev[k].fill(float("nan"))
+            dev[k].fill_hash(w.arrays[k].seed)
+        if how == "run":
+            b.run(st)
+        elif how == "graph":
+            b.capture_graph(st)
+            b.launch_graph(st)
+        else:
+            for p in multi:
+                b.set_phase_mode(p, -1)
+            cache = tmp_path / "tune.txt"
+            b.autotune(st, reps=1, cache_path=str(cache))
+            assert "phase " in cache.read_text()
+            print(b.describe(), flush=True)
+            for k in ref:  # the tuner ran the launches on these arrays
+                dev[k].fill(float("nan"))
+                dev[k].fill_hash(w.arrays[k].seed)
+            b.run(st)
+        st.synchronize()
+        for k in ref:
+            got = dev[k].download()
+            if not same(got, ref[k]):
+                raise AssertionError(f"{how}: {k} differs {first_diff(got, ref[k])}")
+    b.close()
+    w.free(dev)
+
+
 @pytest.mark.parametrize("cent,order", [(cg.VERTEX_CENTERED, 5), (cg.CELL_CENTERED, 3)], ids=["vertex-o5", "cell-o3"])
 def test_recompose_fill_matches_oracle(cent, order):
     """Regrid from a 2-level to a 3-level hierarchy with regrown fine

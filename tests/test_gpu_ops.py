@@ -201,6 +201,26 @@ def test_copy_parity():
         _check(exp, ddst, f"copy case {c}")
 
 
+@pytest.mark.parametrize("n", [1, 2, 1023, 2048, 2049, 100001])
+def test_stream_copy(n):
+    """The flat stream copy (the benchmark's copy ceiling) copies exactly n
+    doubles, odd n included, and nothing beyond them."""
+    import ctypes as C
+    from carpet_amd._lib import check, lib
+    m = n + 64
+    box = cg.Box((0, 0, 0), (m - 1, 0, 0), (1, 1, 1))
+    a, b = cg.DeviceArray(box, padext=(m, 1, 1)), cg.DeviceArray(box, padext=(m, 1, 1))
+    src = np.random.default_rng(n).standard_normal(m)
+    a.upload(src.reshape(1, 1, m))
+    b.fill(-7.0)
+    check(lib.cg_stream_copy_f64(C.c_void_p(b.ptr), C.c_void_p(a.ptr), n, None), "cg_stream_copy_f64")
+    got = b.download().reshape(-1)
+    assert np.array_equal(got[:n], src[:n])
+    assert np.all(got[n:] == -7.0)
+    a.free()
+    b.free()
+
+
 @pytest.mark.parametrize("ntl", [2, 3])
 def test_interpolate_parity(ntl):
     rng = np.random.default_rng(5 + ntl)

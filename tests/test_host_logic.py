@@ -133,6 +133,15 @@ def test_copy_validation():
     assert lib.cg_copy_3d_f64(FAKE, e, e, FAKE, e, e, box, box, empty, empty, None, None) == -5
 
 
+def test_stream_copy_validation():
+    """The flat stream copy takes 8-byte aligned arrays only; null arrays
+    are an error unless n is 0."""
+    assert lib.cg_stream_copy_f64(FAKE + 4, FAKE, 10, None) == -1
+    assert "8-byte" in lib.cg_last_error_message().decode()
+    assert lib.cg_stream_copy_f64(FAKE, FAKE + 12, 10, None) == -1
+    assert lib.cg_stream_copy_f64(None, FAKE, 10, None) == -1
+
+
 def test_transfer_dispatch_validation():
     """cg_transfer_from rejects operator/centering/order combinations the
     reference dispatch rejects (data.cc:584-800) before launching."""
@@ -248,6 +257,22 @@ def test_batch_set_time_keeps_launch_classes():
         assert lib.cg_batch_set_time(h, 1, nt, 3, 1.0) == 2  # exact matches: both stay in class
         assert lib.cg_batch_set_time(h, 2, nt, 3, 0.5) == 0  # no transfer in phase 2
         assert lib.cg_batch_set_time(h, 1, nt, 2, 0.5) == 0  # none added with 2 levels
+    finally:
+        lib.cg_batch_destroy(h)
+
+
+def test_batch_phase_mode_arguments():
+    """cg_batch_set_phase_mode takes 1 (side by side), 0 (one after the
+    other) or -1 (automatic) for any phase, before or after finalize; other
+    modes are CG_ERR_ARG.  Host logic only."""
+    h = C.c_void_p()
+    assert lib.cg_batch_create(C.byref(h)) == 0
+    try:
+        for mode in (1, 0, -1):
+            assert lib.cg_batch_set_phase_mode(h, 3, mode) == 0
+        assert lib.cg_batch_set_phase_mode(h, 3, 2) == -1
+        assert "phase mode" in lib.cg_last_error_message().decode()
+        assert lib.cg_batch_set_phase_mode(None, 3, 1) == -1
     finally:
         lib.cg_batch_destroy(h)
 
