@@ -653,6 +653,8 @@ def main(argv=None):
                     help="skip the hierarchy level step reported beside the headline line")
     ap.add_argument("--no-config7", action="store_true",
                     help="skip the 2000-box hierarchy level step reported beside the headline line")
+    ap.add_argument("--no-config5", action="store_true",
+                    help="skip the mixed-precision comparison reported beside the headline line")
     ap.add_argument("--cpu-seconds", type=float, default=16.0)
     ap.add_argument("--config", type=int, default=4, choices=(2, 3, 4, 5, 6, 7),
                     help="4: batched uneven level step (default, the headline line); 2: single large patch; "
@@ -809,6 +811,11 @@ def main(argv=None):
         # the same 2000 boxes as the regions of a real level step (dh's lists)
         print("# config7 beside the headline line", file=sys.stderr, flush=True)
         line["hierarchy_step_2000_boxes"] = run_config7(args.n_gf, 5, 1, 1, tune_cache=args.tune_cache)
+    if not args.no_config5:
+        # FP32 storage (FP64 arithmetic, and compensated FP32 sums) against
+        # batched FP64 on the 64^3 and 8 x 384^3 prolongation shapes
+        print("# config5 beside the headline line", file=sys.stderr, flush=True)
+        line["mixed_precision"] = run_config5(10, 2, tune_cache=args.tune_cache)
     line["tuned"] = args.tune_cache is not None
     if cpu is not None:
         line["cpu_baseline"] = cpu
