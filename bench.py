@@ -172,16 +172,18 @@ def run_config2(steps: int, warmup: int, n_gf: int, plain_steps: int, tune_cache
     per_launch = [ev[k].elapsed_ms(ev[k + 1]) for k in range(steps)]
     ms_step = sum(per_launch) / steps
     # plain path: one launch per transfer
-    for _ in range(max(1, warmup // 2)):
-        b.run_plain(st)
-    st.synchronize()
-    e0, e1 = cg.Event(), cg.Event()
-    e0.record(st)
-    for _ in range(plain_steps):
-        b.run_plain(st)
-    e1.record(st)
-    st.synchronize()
-    ms_plain = e0.elapsed_ms(e1) / plain_steps
+    ms_plain = float("nan")
+    if plain_steps > 0:
+        for _ in range(max(1, warmup // 2)):
+            b.run_plain(st)
+        st.synchronize()
+        e0, e1 = cg.Event(), cg.Event()
+        e0.record(st)
+        for _ in range(plain_steps):
+            b.run_plain(st)
+        e1.record(st)
+        st.synchronize()
+        ms_plain = e0.elapsed_ms(e1) / plain_steps
     b.close()
     for a in list(src.values()) + list(dst.values()):
         a.free()

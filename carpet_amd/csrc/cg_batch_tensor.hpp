@@ -12,13 +12,9 @@ namespace cg {
 template <int KIND, int SH> constexpr bool has_mixed() { return !fp64_only(KIND) && SH < 2; }
 
 // 512-thread workgroups of orders up to 5 are held to 128 VGPRs: two
-// workgroups (4 waves per SIMD) per CU instead of one; with one fine row
-// parity per thread (PS) to 80 VGPRs, three workgroups per CU
-#ifndef CG_PS_WAVES
-#define CG_PS_WAVES 6
-#endif
-template <class L, int NTL, int CX, int CY, int IPT, int PS> constexpr int persist_waves() {
-  return ((PS ? 4 : 2) * CX * CY / IPT >= 512 && L::R <= 6) ? (PS ? CG_PS_WAVES : 4) : 1;
+// workgroups (4 waves per SIMD) per CU instead of one
+template <class L, int NTL, int CX, int CY, int IPT> constexpr int persist_waves() {
+  return (2 * CX * CY / IPT >= 512 && L::R <= 6) ? 4 : 1;
 }
 
 // planes whose windows are loaded ahead of their use (one register per
@@ -29,9 +25,9 @@ template <int CX> constexpr int batch_pf() { return CX >= 32 ? 1 : 2; }
 
 // NTL: levels of the passes; NSRC: source levels (NSRC > NTL: combined at
 // the window store)
-template <class L, int NTL, int CX, int CY, int IPT, int XQ, int PR, int NSRC, int PS>
-__global__ __launch_bounds__((PS ? 4 : 2) * CX * CY / IPT)
-__attribute__((amdgpu_waves_per_eu(persist_waves<L, NTL, CX, CY, IPT, PS>()))) void k_persist_tensor(
+template <class L, int NTL, int CX, int CY, int IPT, int XQ, int PR, int NSRC>
+__global__ __launch_bounds__(2 * CX * CY / IPT)
+__attribute__((amdgpu_waves_per_eu(persist_waves<L, NTL, CX, CY, IPT>()))) void k_persist_tensor(
     const TensorArgs *__restrict__ descs, const Item *__restrict__ items,
     const int *__restrict__ seg, int *counter, int stat) {
   __shared__ __align__(16) double smem[tile_lds<L, NTL, CX, CY, XQ>()];
@@ -43,13 +39,13 @@ __attribute__((amdgpu_waves_per_eu(persist_waves<L, NTL, CX, CY, IPT, PS>()))) v
       return;
     const Item w = items[it];
     const TensorArgs a = descs[w.desc];
-    tensor_tile<L, NTL, CX, CY, IPT, PR, batch_pf<CX>(), XQ, pipe_fits<L, NTL, CX, CY, XQ>(), NSRC, PS>(
-        a, (int)w.tile, smem);
+    tensor_tile<L, NTL, CX, CY, IPT, PR, batch_pf<CX>(), XQ, pipe_fits<L, NTL, CX, CY, XQ>(), NSRC>(a, (int)w.tile,
+                                                                                          smem);
   }
 }
 
-template <class L, int NTL, int CX, int CY, int IPT, int XQ, int PR, int NSRC, int PS>
-__global__ __launch_bounds__((PS ? 4 : 2) * CX * CY / IPT) void k_grid_tensor(
+template <class L, int NTL, int CX, int CY, int IPT, int XQ, int PR, int NSRC>
+__global__ __launch_bounds__(2 * CX * CY / IPT) void k_grid_tensor(
     const TensorArgs *__restrict__ descs, const Item *__restrict__ items,
     const int *__restrict__ seg) {
   __shared__ __align__(16) double smem[tile_lds<L, NTL, CX, CY, XQ>()];
@@ -57,8 +53,8 @@ __global__ __launch_bounds__((PS ? 4 : 2) * CX * CY / IPT) void k_grid_tensor(
   if (it < 0)
     return;
   const Item w = items[it];
-  tensor_tile<L, NTL, CX, CY, IPT, PR, batch_pf<CX>(), XQ, pipe_fits<L, NTL, CX, CY, XQ>(), NSRC, PS>(
-      descs[w.desc], (int)w.tile, smem);
+  tensor_tile<L, NTL, CX, CY, IPT, PR, batch_pf<CX>(), XQ, pipe_fits<L, NTL, CX, CY, XQ>(), NSRC>(descs[w.desc],
+                                                                                            (int)w.tile, smem);
 }
 
 template <class L, int NSRC, int PR, int SH>
@@ -68,10 +64,10 @@ int batch_tensor_go(const BatchLaunch &l, const TensorArgs *td, const Item *item
   using T = TileShape<NTL, SH, PR>;
   const dim3 block(tile_threads<NTL, SH, PR>());
   if (l.wpc == -1)
-    hipLaunchKernelGGL((k_grid_tensor<L, NTL, T::CX, T::CY, T::IPT, T::XQ, PR, NSRC, T::PS>), dim3(l.grid), block,
+    hipLaunchKernelGGL((k_grid_tensor<L, NTL, T::CX, T::CY, T::IPT, T::XQ, PR, NSRC>), dim3(l.grid), block,
                        0, st, td, items, seg);
   else
-    hipLaunchKernelGGL((k_persist_tensor<L, NTL, T::CX, T::CY, T::IPT, T::XQ, PR, NSRC, T::PS>), dim3(l.grid),
+    hipLaunchKernelGGL((k_persist_tensor<L, NTL, T::CX, T::CY, T::IPT, T::XQ, PR, NSRC>), dim3(l.grid),
                        block, 0, st, td, items, seg, ctr, l.wpc == -2 ? 1 : 0);
   return 0;
 }
@@ -113,7 +109,7 @@ template <int KIND, int SH> const void *batch_tensor_fn_kind(const BatchLaunch &
       constexpr int NTL = pass_levels(NSRC, PR);
       using T = TileShape<NTL, SH, PR>;
       *threads = tile_threads<NTL, SH, PR>();
-      fn = reinterpret_cast<const void *>(&k_persist_tensor<L, NTL, T::CX, T::CY, T::IPT, T::XQ, PR, NSRC, T::PS>);
+      fn = reinterpret_cast<const void *>(&k_persist_tensor<L, NTL, T::CX, T::CY, T::IPT, T::XQ, PR, NSRC>);
     };
     auto by_prec = [&](auto ntlc) {
       if (l.prec == 0)

@@ -11,10 +11,10 @@ namespace cg {
 
 // NTL: levels of the passes; NSRC: source levels (NSRC > NTL: combined at
 // the window store)
-template <class L, int NTL, int CX, int CY, int IPT, int XQ, int PR, int NSRC, int PS>
-__global__ __launch_bounds__((PS ? 4 : 2) * CX * CY / IPT) void k_tensor(TensorArgs a) {
+template <class L, int NTL, int CX, int CY, int IPT, int XQ, int PR, int NSRC>
+__global__ __launch_bounds__(2 * CX * CY / IPT) void k_tensor(TensorArgs a) {
   __shared__ __align__(16) double smem[tile_lds<L, NTL, CX, CY, XQ>()];
-  tensor_tile<L, NTL, CX, CY, IPT, PR, 1, XQ, pipe_fits<L, NTL, CX, CY, XQ>(), NSRC, PS>(
+  tensor_tile<L, NTL, CX, CY, IPT, PR, 1, XQ, pipe_fits<L, NTL, CX, CY, XQ>(), NSRC>(
       a, xcd_remap(blockIdx.x, gridDim.x), smem);
 }
 
@@ -29,7 +29,7 @@ int launch_tensor_t(const TensorArgs &a0, const TileCfg &cfg0, hipStream_t s) {
   const long long nblk = tensor_tiles(a, cfg);
   if (nblk <= 0 || nblk > 0x7fffffffLL)
     return -1;
-  hipLaunchKernelGGL((k_tensor<L, NTL, T::CX, T::CY, T::IPT, T::XQ, PR, NSRC, T::PS>), dim3((unsigned)nblk),
+  hipLaunchKernelGGL((k_tensor<L, NTL, T::CX, T::CY, T::IPT, T::XQ, PR, NSRC>), dim3((unsigned)nblk),
                      dim3(tile_threads<NTL, 0, PR>()), 0, s, a);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }

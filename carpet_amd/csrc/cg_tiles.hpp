@@ -143,7 +143,7 @@ template <int I, int N, class F> __device__ __forceinline__ bool unroll_while(F 
 // the passes run once instead of once per level.
 // smem must be 16-byte aligned and hold TensorLds<..., XQ, PIPE ? 2 : 1>
 template <class L, int NTL, int CX, int CY, int IPT, int PR = 0, int PF = 1, int XQ = 1,
-          int PIPE = 0, int NSRC = NTL, int PS = 0>
+          int PIPE = 0, int NSRC = NTL>
 __device__ __forceinline__ void tensor_tile(const TensorArgs &a, int b, double *smem) {
   using TS = typename Prec<PR>::S; // storage
   using CT = typename Prec<PR>::C; // arithmetic of the passes
@@ -151,12 +151,8 @@ __device__ __forceinline__ void tensor_tile(const TensorArgs &a, int b, double *
   static_assert(!TF || (NTL == 1 && sizeof(CT) == 8), "time-first needs one pass level in FP64");
   using WT = std::conditional_t<TF, CT, TS>; // LDS window type
   constexpr int FX = 2 * CX;
-  // PS = 1: a thread holds one fine row parity of its bases (the two halves
-  // of the workgroup take parity 0 and 1), so its ring of y-pass results is
-  // half as large
-  constexpr int NPY = PS ? 1 : 2; // fine row parities per thread
-  constexpr int NT = FX * CY / IPT * (PS ? 2 : 1);
-  constexpr int NQY = CY / IPT;
+  constexpr int NT = FX * CY / IPT;
+  constexpr int NQY = NT / FX;
   static_assert(NQY * IPT == CY, "tile rows must divide among threads");
   static_assert(XQ == 1 || (XQ == 2 && CX % 2 == 0), "x-pass bases per thread: 1 or 2");
   constexpr int NB = PIPE ? 2 : 1;
@@ -200,8 +196,7 @@ __device__ __forceinline__ void tensor_tile(const TensorArgs &a, int b, double *
   const int c1 = min(qz1 + L::TMAXALL, a.fhi[2]);
 
   const int tid = threadIdx.x;
-  const int fx = tid % FX, qyl = (tid / FX) % NQY;
-  const int pyt = PS ? tid / (FX * NQY) : 0; // this thread's parity (PS = 1)
+  const int fx = tid % FX, qyl = tid / FX;
   const int gfx = 2 * qx0 + fx - a.off[0];
   const bool xin = gfx >= 0 && gfx < a.n[0];
 
@@ -231,16 +226,14 @@ __device__ __forceinline__ void tensor_tile(const TensorArgs &a, int b, double *
   };
 
   // ring of y-pass results: slot j holds the planes c with (c - c0) % R == j
-  CT ring[NTL][R][IPT][NPY];
+  CT ring[NTL][R][IPT][2];
 #pragma unroll
   for (int tl = 0; tl < NTL; ++tl)
 #pragma unroll
     for (int i = 0; i < R; ++i)
 #pragma unroll
       for (int m = 0; m < IPT; ++m)
-#pragma unroll
-        for (int py = 0; py < NPY; ++py)
-          ring[tl][i][m][py] = CT(0);
+        ring[tl][i][m][0] = ring[tl][i][m][1] = CT(0);
 
   auto prologue = [&](auto Sc) -> bool {
     constexpr int S = decltype(Sc)::value;
@@ -327,14 +320,8 @@ __device__ __forceinline__ void tensor_tile(const TensorArgs &a, int b, double *
 #pragma unroll
       for (int tl = 0; tl < NTL; ++tl) {
         const CT *col = &X[tl][qy - L::TMINALL][fx];
-        if constexpr (PS) {
-          // pyt is uniform over each wave (a half of the workgroup)
-          ring[tl][J][m][0] = pyt ? line_apply<L, 1, CT>([&](int t) { return col[(L::TMIN[1] + t) * FX]; })
-                                  : line_apply<L, 0, CT>([&](int t) { return col[(L::TMIN[0] + t) * FX]; });
-        } else {
-          ring[tl][J][m][0] = line_apply<L, 0, CT>([&](int t) { return col[(L::TMIN[0] + t) * FX]; });
-          ring[tl][J][m][1] = line_apply<L, 1, CT>([&](int t) { return col[(L::TMIN[1] + t) * FX]; });
-        }
+        ring[tl][J][m][0] = line_apply<L, 0, CT>([&](int t) { return col[(L::TMIN[0] + t) * FX]; });
+        ring[tl][J][m][1] = line_apply<L, 1, CT>([&](int t) { return col[(L::TMIN[1] + t) * FX]; });
       }
     }
     auto emit = [&](auto PZc) {
@@ -350,8 +337,8 @@ __device__ __forceinline__ void tensor_tile(const TensorArgs &a, int b, double *
 #pragma unroll
       for (int m = 0; m < IPT; ++m) {
 #pragma unroll
-        for (int py = 0; py < NPY; ++py) {
-          const int fy = 2 * (qy0 + qyl + NQY * m) + (PS ? pyt : py) - a.off[1];
+        for (int py = 0; py < 2; ++py) {
+          const int fy = 2 * (qy0 + qyl + NQY * m) + py - a.off[1];
           double v[NTL];
 #pragma unroll
           for (int tl = 0; tl < NTL; ++tl)
@@ -845,40 +832,32 @@ __device__ __forceinline__ void tvd_cell(const PointArgs &a, int qx, int qy, int
 // 64 bases across, so a fine row is 512 bytes of FP32 and the x halo is
 // half as large per point.
 template <int NTL, int SH = 0, int PR = 0> struct TileShape : TileShape<NTL, SH, 0> {};
-#ifndef CG_T3W_CY
-#define CG_T3W_CY 8
-#define CG_T3W_PS 0
-#endif
-#ifndef CG_T3N_CY
-#define CG_T3N_CY 16
-#define CG_T3N_PS 0
-#endif
 template <> struct TileShape<1, 0, 0> {
-  static constexpr int CX = 32, CY = 8, IPT = 2, XQ = 2, PS = 0;
+  static constexpr int CX = 32, CY = 8, IPT = 2, XQ = 2;
 };
 template <> struct TileShape<2, 0, 0> {
-  static constexpr int CX = 32, CY = 8, IPT = 1, XQ = 2, PS = 0;
+  static constexpr int CX = 32, CY = 8, IPT = 1, XQ = 2;
 };
 template <> struct TileShape<3, 0, 0> {
-  static constexpr int CX = 32, CY = CG_T3W_CY, IPT = 1, XQ = 2, PS = CG_T3W_PS;
+  static constexpr int CX = 32, CY = 8, IPT = 1, XQ = 2;
 };
 template <> struct TileShape<1, 1, 0> {
-  static constexpr int CX = 16, CY = 16, IPT = 2, XQ = 2, PS = 0;
+  static constexpr int CX = 16, CY = 16, IPT = 2, XQ = 2;
 };
 template <> struct TileShape<2, 1, 0> {
-  static constexpr int CX = 16, CY = 16, IPT = 1, XQ = 2, PS = 0;
+  static constexpr int CX = 16, CY = 16, IPT = 1, XQ = 2;
 };
 template <> struct TileShape<3, 1, 0> {
-  static constexpr int CX = 16, CY = CG_T3N_CY, IPT = 1, XQ = 2, PS = CG_T3N_PS;
+  static constexpr int CX = 16, CY = 16, IPT = 1, XQ = 2;
 };
 template <> struct TileShape<1, 0, 1> {
-  static constexpr int CX = 64, CY = 8, IPT = 2, XQ = 2, PS = 0;
+  static constexpr int CX = 64, CY = 8, IPT = 2, XQ = 2;
 };
 // small tiles (FP64 batch only) for regions a few fine points across in x
 // and in y or z: edges and corners of ghost and buffer shells, faces thin
 // in x, where a 32x8 or 16x16 tile keeps a tenth of its lanes busy or less
 template <int NTL> struct TileShape<NTL, 2, 0> {
-  static constexpr int CX = 8, CY = 8, IPT = 1, XQ = 2, PS = 0;
+  static constexpr int CX = 8, CY = 8, IPT = 1, XQ = 2;
 };
 // the pipelined schedule (double-buffered window and x-pass results) where
 // its LDS fits the 64 KB of a static allocation, else two barriers per plane
@@ -895,7 +874,7 @@ constexpr int pass_levels(int ntl, int prec) { return prec == 1 ? 1 : ntl; }
 
 template <int NTL, int SH = 0, int PR = 0> constexpr int tile_threads() {
   using T = TileShape<NTL, SH, PR>;
-  return (T::PS ? 4 : 2) * T::CX * T::CY / T::IPT;
+  return 2 * T::CX * T::CY / T::IPT;
 }
 // host view of the tile shape of a transfer class: ntl source levels,
 // precision mode prec, shape sh
