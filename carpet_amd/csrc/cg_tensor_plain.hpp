@@ -11,8 +11,11 @@ namespace cg {
 
 // NTL: levels of the passes; NSRC: source levels (NSRC > NTL: combined at
 // the window store)
+// 512-thread workgroups of orders up to 5 held to 128 VGPRs, two per CU
+// (the O=5, 3-level tile otherwise takes 130 and runs one per CU)
 template <class L, int NTL, int CX, int CY, int IPT, int XQ, int PR, int NSRC>
-__global__ __launch_bounds__(2 * CX * CY / IPT) void k_tensor(TensorArgs a) {
+__global__ __launch_bounds__(2 * CX * CY / IPT) __attribute__((amdgpu_waves_per_eu(
+    (2 * CX * CY / IPT >= 512 && L::R <= 6) ? 4 : 1))) void k_tensor(TensorArgs a) {
   __shared__ __align__(16) double smem[tile_lds<L, NTL, CX, CY, XQ>()];
   tensor_tile<L, NTL, CX, CY, IPT, PR, 1, XQ, pipe_fits<L, NTL, CX, CY, XQ>(), NSRC>(
       a, xcd_remap(blockIdx.x, gridDim.x), smem);
