@@ -237,7 +237,7 @@ static int save_cache(const char *path, const TuneMap &m, const std::map<std::st
   f << "# carpet_amd batch tuner cache: <class key>|<zchunk> <points per item> "
        "<workgroups per CU: 0 resident capacity, -1 one per item> "
        "<tile shape: 0 wide (32x8 coarse bases, 64x8 in the FP32-storage FP64 mode), 1 16x16, "
-       "2 8x8 (FP64 regions thin in x and y or z, fixed), "
+       "2 8x8 (FP64 regions thin in x and y or z, fixed), 3 4x16 (FP64 regions thin in x, fixed), "
        "-1 by region size> <256-point items per workgroup in grid mode>; "
        "phase <class key> + <class key> ...|<1 side by side, 0 one after the other>\n";
   for (const auto &kv : m)

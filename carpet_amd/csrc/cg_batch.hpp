@@ -46,8 +46,8 @@ struct Item {
 };
 
 // batched tensor launches per line kind (LK_*) and tile shape (0 32x8 wide,
-// 1 16x16 narrow, 2 8x8 small, FP64 only); defined in cg_batch_tensor.hpp,
-// instantiated in cg_batch_{vlag,clag,eno,veno}_{w,n,s}.hip
+// 1 16x16 narrow, 2 8x8 small and 3 4x16 slim, FP64 only); defined in
+// cg_batch_tensor.hpp, instantiated in cg_batch_{vlag,clag,eno,veno}_{w,n,s,t}.hip
 template <int KIND, int SH>
 int batch_tensor_kind(const BatchLaunch &l, const TensorArgs *td, const Item *items,
                       const int *seg, int *ctr, hipStream_t st);

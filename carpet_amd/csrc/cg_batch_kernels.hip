@@ -47,6 +47,11 @@ __device__ __forceinline__ void point_store(double v, double *p) {
     *p = v;
 }
 
+// regions thin in x and long in y may take the 4x16-base tiles
+#ifndef CG_SLIM_TILES
+#define CG_SLIM_TILES 1
+#endif
+
 // tensor transfers of at most this many tiles run whole on one XCD
 #ifndef CG_WHOLE_XCD_ITEMS
 #define CG_WHOLE_XCD_ITEMS 16
@@ -236,7 +241,7 @@ namespace {
 
 struct Group {
   int phase, is_tensor, kind, order, ntl, op, prec;
-  int small; // tensor regions routed to the 8x8-base tiles
+  int small; // tensor regions routed to the 8x8-base (2) or 4x16-base (3) tiles, else 0
   std::vector<int> members; // transfer indices
   bool same(const Group &g) const {
     return phase == g.phase && is_tensor == g.is_tensor && kind == g.kind &&
@@ -272,12 +277,19 @@ int line_span(int kind, int order) {
 // and 6-point shells, 1.14 ms against 1.61 ms without small tiles; 2000
 // regions of 8-96 points per axis 19.41 ms against 19.53 ms, and 19.83 ms
 // when routing at the break-even point 1.0.)
-bool small_tiles(const ProlongPlan &p, int ntl, int prec) {
+// Of the two small shapes: 4x16 for regions at most 4 bases across in x
+// (one tile wide) where it keeps fewer lanes busy than 8x8 -- ghost and
+// buffer faces normal to x -- else 8x8.  (Routing every region by the
+// lane count alone sent 2000 regions of 8-96 points per axis with
+// 5-12 bases in x to the 4x16 tiles: 18.59 ms per step against 17.46.)
+int small_tiles(const ProlongPlan &p, int ntl, int prec) {
   if (prec != CG_PREC_F64)
-    return false;
+    return 0;
   const int span = line_span(p.kind, p.order);
   const double big = std::min(tile_busy(p.t, ntl, prec, span, 0), tile_busy(p.t, ntl, prec, span, 1));
-  return tile_busy(p.t, ntl, prec, span, 2) * 1.3 <= big;
+  const double b2 = tile_busy(p.t, ntl, prec, span, 2), b3 = tile_busy(p.t, ntl, prec, span, 3);
+  const int sh = CG_SLIM_TILES && p.t.nq[0] <= 4 && b3 < b2 ? 3 : 2;
+  return (sh == 3 ? b3 : b2) * 1.3 <= big ? sh : 0;
 }
 
 // one transfer's descriptor in launch l: a tensor transfer resolved to one
@@ -370,6 +382,7 @@ const void *launch_fn(const BatchLaunch &l, int *threads) {
   case 0: return tensor_fn_sh<0>(l, threads);
   case 1: return tensor_fn_sh<1>(l, threads);
   case 2: return tensor_fn_sh<2>(l, threads);
+  case 3: return tensor_fn_sh<3>(l, threads);
   }
   return nullptr;
 }
@@ -460,7 +473,7 @@ int batch_device_build(BatchDevice &dev, const std::vector<Transfer> &tr,
     if (tr[i].is_prolong && tr[i].pr.kind != PK_TVD && tr[i].pr.kind != PK_MONO5 &&
         tr[i].pr.kind != PK_STAGGER) {
       g.is_tensor = 1;
-      g.small = small_tiles(tr[i].pr, cls_ntl[i], prec[i]) ? 1 : 0;
+      g.small = small_tiles(tr[i].pr, cls_ntl[i], prec[i]);
       g.kind = tr[i].pr.kind;
       g.order = tr[i].pr.order;
       g.ntl = cls_ntl[i];
@@ -526,7 +539,7 @@ int batch_device_build(BatchDevice &dev, const std::vector<Transfer> &tr,
       median_pts = pts[pts.size() / 2];
       char key[128];
       snprintf(key, sizeof key, "%s k%d o%d t%d p%d op%d h%d n%d",
-               !g.is_tensor ? "point" : g.small ? "tensor8" : "tensor",
+               !g.is_tensor ? "point" : g.small == 3 ? "tensor4" : g.small ? "tensor8" : "tensor",
                g.kind, g.order, g.ntl, g.prec, g.op, ilog2(median_pts),
                ilog2((long long)g.members.size()));
       l.key = key;
@@ -568,7 +581,7 @@ int batch_device_build(BatchDevice &dev, const std::vector<Transfer> &tr,
     // 512^3 patch: narrow 0.81x the busy lanes of wide, yet 4% slower)
     l.shape = 0;
     if (g.is_tensor && g.small) {
-      l.shape = 2;
+      l.shape = g.small;
     } else if (g.is_tensor) {
       const int span = line_span(g.kind, g.order);
       double busy[2] = {0, 0};
@@ -911,6 +924,7 @@ static int run_launch(BatchDevice &dev, const BatchLaunch &l, hipStream_t st) {
     case 0: rc = tensor_run_sh<0>(l, td, items, seg, ctr, st); break;
     case 1: rc = tensor_run_sh<1>(l, td, items, seg, ctr, st); break;
     case 2: rc = tensor_run_sh<2>(l, td, items, seg, ctr, st); break;
+    case 3: rc = tensor_run_sh<3>(l, td, items, seg, ctr, st); break;
     default: rc = -1;
     }
   } else {

@@ -1,6 +1,6 @@
 // cg_batch_tensor.hpp -- batched tensor prolongation kernels (grid and
 // persistent modes), one launch template per line kind and tile shape;
-// instantiated in cg_batch_{vlag,clag,eno,veno}_{w,n,s}.hip so they compile in
+// instantiated in cg_batch_{vlag,clag,eno,veno}_{w,n,s,t}.hip so they compile in
 // parallel.
 #pragma once
 
@@ -8,7 +8,8 @@
 
 namespace cg {
 
-// the small tile shape (2) is instantiated for FP64 storage only
+// the small and slim tile shapes (2, 3) are instantiated for FP64 storage
+// only
 template <int KIND, int SH> constexpr bool has_mixed() { return !fp64_only(KIND) && SH < 2; }
 
 // 512-thread workgroups of orders up to 5 are held to 128 VGPRs: two

@@ -859,6 +859,12 @@ template <> struct TileShape<1, 0, 1> {
 template <int NTL> struct TileShape<NTL, 2, 0> {
   static constexpr int CX = 8, CY = 8, IPT = 1, XQ = 2;
 };
+// slim tiles (FP64 batch only), 4 x 16 bases, for regions a few fine points
+// across in x and long in y: ghost and buffer faces normal to x, where an
+// 8x8 tile leaves most of its x lanes and window columns idle
+template <int NTL> struct TileShape<NTL, 3, 0> {
+  static constexpr int CX = 4, CY = 16, IPT = 1, XQ = 2;
+};
 // the pipelined schedule (double-buffered window and x-pass results) where
 // its LDS fits the 64 KB of a static allocation, else two barriers per plane
 template <class L, int NTL, int CX, int CY, int XQ> constexpr int pipe_fits() {
@@ -890,8 +896,10 @@ inline void tile_shape(int ntl, int prec, int *cx, int *cy, int *threads, int sh
       get(ntlc, std::integral_constant<int, 0>{}, prc);
     else if (sh == 1)
       get(ntlc, std::integral_constant<int, 1>{}, prc);
-    else
+    else if (sh == 2)
       get(ntlc, std::integral_constant<int, 2>{}, prc);
+    else
+      get(ntlc, std::integral_constant<int, 3>{}, prc);
   };
   auto by_prec = [&](auto ntlc) {
     if (prec == 1)

@@ -557,7 +557,9 @@ int cg_batch_set_storage(cg_batch *b, int n, const cg_gdata *from, const cg_gdat
    for CG_PREC_MIXED; narrow: 16x16),
    with HIP events, and keeps the fastest per launch class.  FP64 regions
    thin in x and in y or z (shell edges and corners) form classes of their
-   own on 8x8-base tiles, whose shape is fixed.
+   own on 8x8-base tiles, and regions at most 4 coarse bases across in x
+   and long in y (shell faces normal to x) on 4x16-base tiles; these two
+   shapes are fixed.
    Choices are keyed on (kernel class, order, time levels, precision, log2
    of the median region size, log2 of the region count) and kept in the
    text file cache_path (may be NULL; lines "<key>|<zchunk> <points>

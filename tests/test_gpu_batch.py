@@ -58,7 +58,8 @@ def test_batch_modes_match_oracle(make, tmp_path):
     print("stats", b.stats(), flush=True)
     print(b.describe(), flush=True)
     if make is _shells:
-        assert "[tensor8 " in b.describe()
+        # shell edges and corners on 8x8-base tiles, faces thin in x on 4x16
+        assert "[tensor8 " in b.describe() or "[tensor4 " in b.describe()
     # batched launches, one workgroup per item
     b.run(st)
     st.synchronize()
