@@ -24,6 +24,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
 import sys
 import time
 from pathlib import Path
@@ -35,13 +36,19 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "prolongate+restrict+sync Gpoints/s & HBM GB/s per level step, batched vs plain"
 
 
-def _profile_traffic(name: str):
+def _profile_traffic(name: str, key: str = None):
     """HBM bytes per launch measured with rocprofv3 FETCH_SIZE/WRITE_SIZE
-    (profiles/<name>.json), or None."""
+    (profiles/<name>.json; with key, the launch of that class key in its
+    "launches" list), or None."""
     p = ROOT / "profiles" / f"{name}.json"
     if p.exists():
         try:
-            return json.loads(p.read_text()).get("traffic_bytes_per_launch")
+            d = json.loads(p.read_text())
+            if key is None:
+                return d.get("traffic_bytes_per_launch")
+            for l in d.get("launches", []):
+                if l.get("key") == key:
+                    return l.get("traffic_bytes_per_launch")
         except Exception:
             return None
     return None
@@ -197,6 +204,8 @@ def _launch_times(b, st, steps: int):
     import carpet_amd as cg
 
     n = b.launch_count()
+    # class keys in launch order (cg_batch_describe lists the launches in order)
+    keys = re.findall(r"^phase -?\d+ \[([^\]]+)\]", b.describe(), re.M)
     ev = [cg.Event() for _ in range(n + 1)]
     tot = [0.0] * n
     for _ in range(steps):
@@ -210,7 +219,8 @@ def _launch_times(b, st, steps: int):
     out = []
     for i in range(n):
         ph, pts, alg = b.launch_stats(i)
-        out.append({"launch": i, "phase": ph, "ms": round(tot[i] / steps, 4), "dst_points": pts,
+        out.append({"launch": i, "phase": ph, "key": keys[i] if i < len(keys) else None,
+                    "ms": round(tot[i] / steps, 4), "dst_points": pts,
                     "algorithmic_bytes": alg, "gbytes_per_s": round(alg / (tot[i] / steps) / 1e6, 1)})
     return out
 
@@ -790,7 +800,7 @@ def main(argv=None):
             # the dominant launch of the step (HIP events around each launch)
             "roofline": {"bound": "hbm", "achieved": dom["gbytes_per_s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(dom["gbytes_per_s"] / HBM_PEAK_GBS, 4),
-                         "traffic": _profile_traffic(f"level_step_2000_regions_x24_launch{dom['launch']}"),
+                         "traffic": _profile_traffic("level_step_2000_regions_x24", dom["key"]),
                          "kernel": f"launch {dom['launch']} (phase {dom['phase']})", "ms": dom["ms"],
                          "algorithmic_bytes": dom["algorithmic_bytes"]}}
     # the measured flat-copy ceiling of this GPU beside the 8 TB/s peak
