@@ -649,6 +649,8 @@ int cg_stream_copy_f64(double *dst, const double *src, size_t n, cg_stream_t s) 
     return set_error(CG_ERR_ARG, "null pointer");
   if ((reinterpret_cast<uintptr_t>(dst) | reinterpret_cast<uintptr_t>(src)) & 7)
     return set_error(CG_ERR_ARG, "stream copy needs 8-byte aligned arrays");
+  if (n / 512 >= 0x7fffffffull)
+    return set_error(CG_ERR_ARG, "stream copy of %zu doubles exceeds one launch", n);
   return launch_stream_copy(dst, src, n, S(s)) ? hip_error(hipGetLastError(), "stream copy") : CG_OK;
 }
 int cg_fill_hash_f64(double *dst, const int padext[3], const int ext[3],

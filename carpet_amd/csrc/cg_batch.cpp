@@ -138,7 +138,6 @@ int cg_batch_add(cg_batch *b, int phase, const cg_gdata *dst,
   return (int)b->transfers.size() - 1;
 }
 
-// transfer -> launch map after a (re)build; nothing is pending after it
 // the launch classes of the phase group [i, j)
 static std::string group_key(const BatchDevice &dev, size_t i, size_t j) {
   std::string k;
@@ -174,6 +173,7 @@ static void apply_sides(cg_batch *b) {
   }
 }
 
+// transfer -> launch map after a (re)build; nothing is pending after it
 static void index_launches(cg_batch *b) {
   b->launch_of.assign(b->transfers.size(), -1);
   for (size_t l = 0; l < b->dev.launches.size(); ++l)
@@ -373,6 +373,7 @@ extern "C" int cg_batch_autotune(cg_batch *b, cg_stream_t stream, int reps,
   int e = batch_device_build(b->dev, b->transfers, b->phase, b->prec, b->cls_ntl, b->tile_cfg, &b->tuned);
   if (e)
     return e;
+  index_launches(b);
   // phases of several launches: side by side or one after the other
   for (size_t i = 0; i < b->dev.launches.size();) {
     const size_t j = batch_phase_end(b->dev, i);
@@ -389,7 +390,7 @@ extern "C" int cg_batch_autotune(cg_batch *b, cg_stream_t stream, int reps,
     }
     i = j;
   }
-  index_launches(b);
+  apply_sides(b);
   return save_cache(cache_path, b->tuned, b->tuned_side);
 }
 
@@ -410,7 +411,8 @@ extern "C" int cg_batch_describe(const cg_batch *b, char *buf, size_t len) {
         << " zchunk " << l.zchunk << " pt " << l.pt << " wpc " << l.wpc << " shape " << l.shape
         << " ni " << l.ni;
       if (j - i > 1)
-        o << (b->dev.beside[i] ? " side by side" : " in turn");
+        o << (b->dev.beside.size() != b->dev.launches.size() || b->dev.beside[i] ? " side by side"
+                                                                                 : " in turn");
       o << "\n";
     }
     i = j;
