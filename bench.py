@@ -408,6 +408,8 @@ def run_config6(n_gf: int, steps: int, warmup: int, plain_steps: int, ncoarse: i
           f"{time.perf_counter() - t0:.1f} s\n" + desc, file=sys.stderr, flush=True)
     stats = b.stats()
     ms_b = _time_graph(b, st, steps, warmup)
+    launches = _launch_times(b, st, 2)
+    print("# config6 launches: " + json.dumps(launches), file=sys.stderr, flush=True)
 
     def reseed():
         # later phases read what earlier ones write (restricted coarse points
@@ -441,7 +443,7 @@ def run_config6(n_gf: int, steps: int, warmup: int, plain_steps: int, ncoarse: i
     return {"transfers": stats["transfers"], "dst_points_per_step": pts, "algorithmic_bytes_per_step": alg,
             "batched_ms": round(ms_b, 4), "batched_gpoints_per_s": round(pts / ms_b / 1e6, 3),
             "batched_gbytes_per_s": round(alg / ms_b / 1e6, 1),
-            "batched_roofline_frac": round(alg / ms_b / 1e6 / HBM_PEAK_GBS, 4),
+            "batched_roofline_frac": round(alg / ms_b / 1e6 / HBM_PEAK_GBS, 4), "launches": launches,
             "plain_ms": round(ms_p, 4), "plain_gpoints_per_s": round(pts / ms_p / 1e6, 3),
             "batched_vs_plain": round(ms_p / ms_b, 3), "batched_equals_plain_on_sample": same,
             "sample_destinations": len(sample)}
