@@ -38,11 +38,11 @@ def _poison_outputs(w, dev, stream):
             a.upload(spec.init, fill=np.nan)
 
 
-def _thin_faces(cent, op, order):
-    """Prolongation (3 time levels, target between them) onto regions a few
-    fine points thick in x and long in y and z -- the shell faces normal to
-    x that go to the 4x16-base tiles -- at both parities, plus one thin in
-    y (large tiles) in the same phase."""
+def _thin_faces(cent, op, order, ntl=3):
+    """Prolongation (ntl time levels, target between them) onto regions a
+    few fine points thick in x and long in y and z -- the shell faces normal
+    to x that go to the 4x16-base tiles -- at both parities, plus one thin
+    in y (large tiles) in the same phase."""
     from carpet_amd.workloads import ArraySpec, Box, Transfer, Workload
     w = Workload(f"thin-faces c{cent} op{op} o{order}")
     # fine points at f (vertex, stride 1) or cells centred at 1 + 2f (cell,
@@ -50,28 +50,34 @@ def _thin_faces(cent, op, order):
     vc = cent == cg.VERTEX_CENTERED
     fs, cs, fo, co = (1, 2, 0, 0) if vc else (2, 4, 1, 2)
     cbox = Box((co - 6 * cs,) * 3, (co + 56 * cs,) * 3, (cs, cs, cs))
-    for tl in range(3):
+    for tl in range(ntl):
         w.arrays[f"c{tl}"] = ArraySpec(cbox, seed=900 + tl)
+    times = [0.0, -1.0, -2.0][:ntl]
+    target = -0.5 if ntl > 1 else 0.0
     regions = [((10, 4, 6), (12, 83, 71)), ((21, 5, 3), (26, 60, 44)), ((40, 2, 9), (42, 41, 88)),
                ((7, 30, 2), (80, 32, 50))]
     for r, (lo, up) in enumerate(regions):
         reg = Box(tuple(fo + fs * l for l in lo), tuple(fo + fs * u for u in up), (fs, fs, fs))
         fbox = reg.expand((3, 3, 3), (3, 3, 3))
         w.arrays[f"f{r}"] = ArraySpec(fbox, owned=reg, seed=950 + r)
-        w.steps.append(Transfer(f"prolong f{r}", f"f{r}", ["c0", "c1", "c2"], [0.0, -1.0, -2.0], reg, reg, -0.5,
-                                op, cent, order, 2, phase=0))
+        w.steps.append(Transfer(f"prolong f{r}", f"f{r}", [f"c{tl}" for tl in range(ntl)], times, reg, reg,
+                                target, op, cent, order, ntl - 1, phase=0))
         w.outputs.append(f"f{r}")
     return w
 
 
-@pytest.mark.parametrize("cent,op,order", [(cg.VERTEX_CENTERED, cg.OP_LAGRANGE, 5),
-                                           (cg.CELL_CENTERED, cg.OP_LAGRANGE, 3),
-                                           (cg.CELL_CENTERED, cg.OP_ENO, 5)],
-                         ids=["vertex-o5", "cell-o3", "cell-eno3"])
-def test_slim_tiles_match_oracle(cent, op, order):
+@pytest.mark.parametrize("cent,op,order,ntl", [(cg.VERTEX_CENTERED, cg.OP_LAGRANGE, 5, 3),
+                                               (cg.VERTEX_CENTERED, cg.OP_LAGRANGE, 5, 2),
+                                               (cg.VERTEX_CENTERED, cg.OP_LAGRANGE, 3, 1),
+                                               (cg.CELL_CENTERED, cg.OP_LAGRANGE, 3, 3),
+                                               (cg.CELL_CENTERED, cg.OP_ENO, 5, 3),
+                                               (cg.VERTEX_CENTERED, cg.OP_ENO, 5, 2)],
+                         ids=["vertex-o5-3tl", "vertex-o5-2tl", "vertex-o3-1tl", "cell-o3-3tl", "cell-eno3-3tl",
+                              "vertex-eno-2tl"])
+def test_slim_tiles_match_oracle(cent, op, order, ntl):
     """Regions thin in x run on the 4x16-base tiles (class tensor4), direct
     and from a captured graph, equal to the oracle bit for bit."""
-    w = _thin_faces(cent, op, order)
+    w = _thin_faces(cent, op, order, ntl)
     ref = run_oracle(w)
     st = cg.Stream()
     dev = w.allocate()
