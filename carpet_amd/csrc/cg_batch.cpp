@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <map>
@@ -306,6 +307,8 @@ extern "C" int cg_batch_autotune(cg_batch *b, cg_stream_t stream, int reps,
   b->tuned.clear();
   b->tuned_side.clear();
   load_cache(cache_path, b->tuned, b->tuned_side);
+  // CG_TUNE_LOG set: every trial's time on stderr
+  const bool tune_log = std::getenv("CG_TUNE_LOG") != nullptr;
   drop_graph(b);
   // current choices for every class, then search the classes the cache lacks
   for (size_t i = 0; i < b->dev.launches.size(); ++i) {
@@ -343,6 +346,9 @@ extern "C" int cg_batch_autotune(cg_batch *b, cg_stream_t stream, int reps,
       batch_device_free(one);
       if (e)
         return e;
+      if (tune_log)
+        fprintf(stderr, "tune [%s] zchunk %d pt %d wpc %d shape %d ni %d: %.4f ms\n", l.key.c_str(), c.zchunk,
+                c.pt, c.wpc, c.shape, c.ni, ms);
       if (ms < best) {
         best = ms;
         bestc = c;
@@ -387,6 +393,8 @@ extern "C" int cg_batch_autotune(cg_batch *b, cg_stream_t stream, int reps,
       if (he != hipSuccess)
         return hip_error(he, "tuner phase trial");
       b->tuned_side[k] = ms[1] <= ms[0] ? 1 : 0;
+      if (tune_log)
+        fprintf(stderr, "tune phase [%s]: in turn %.4f ms, side by side %.4f ms\n", k.c_str(), ms[0], ms[1]);
     }
     i = j;
   }

@@ -574,7 +574,9 @@ int cg_batch_set_storage(cg_batch *b, int n, const cg_gdata *from, const cg_gdat
    "phase <key> + <key> ...|<1 side by side, 0 one after the other>".  The
    search executes each launch (one phase's class of transfers) reps times
    on its own, outside phase order, so call it before the step whose
-   results are kept, or on scratch data.  Drops a captured graph. */
+   results are kept, or on scratch data.  Drops a captured graph.  With the
+   environment variable CG_TUNE_LOG set, every trial's time goes to
+   stderr. */
 int cg_batch_autotune(cg_batch *b, cg_stream_t stream, int reps,
                       const char *cache_path);
 /* no reference counterpart: how the launches of one phase run.  They write
