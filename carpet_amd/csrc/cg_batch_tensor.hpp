@@ -18,11 +18,11 @@ template <class L, int NTL, int CX, int CY, int IPT> constexpr int persist_waves
   return (2 * CX * CY / IPT >= 512 && L::R <= 6) ? 4 : 1;
 }
 
-// planes whose windows are loaded ahead of their use (one register per
-// level and window element of a thread per plane): two for the narrow and
-// small tiles (the 2000-region step 1.3-1.5% faster than one), one for the
-// wide tiles (the 512^3 patch 17.0 ms with two against 15.8 ms with one)
-template <int CX> constexpr int batch_pf() { return CX >= 32 ? 1 : 2; }
+// every tile shape loads each plane's window one plane ahead of its use:
+// two planes ahead was no faster on the narrow, small and slim tiles (the
+// 2000-region step equal, the 2000-box hierarchy step 0.6% slower) and
+// slower on the wide ones (the 512^3 patch 17.0 ms against 15.8 ms)
+constexpr int BATCH_PF = 1;
 
 // NTL: levels of the passes; NSRC: source levels (NSRC > NTL: combined at
 // the window store)
@@ -40,7 +40,7 @@ __attribute__((amdgpu_waves_per_eu(persist_waves<L, NTL, CX, CY, IPT>()))) void 
       return;
     const Item w = items[it];
     const TensorArgs a = descs[w.desc];
-    tensor_tile<L, NTL, CX, CY, IPT, PR, batch_pf<CX>(), XQ, pipe_fits<L, NTL, CX, CY, XQ>(), NSRC>(a, (int)w.tile,
+    tensor_tile<L, NTL, CX, CY, IPT, PR, BATCH_PF, XQ, pipe_fits<L, NTL, CX, CY, XQ>(), NSRC>(a, (int)w.tile,
                                                                                           smem);
   }
 }
@@ -54,7 +54,7 @@ __global__ __launch_bounds__(2 * CX * CY / IPT) void k_grid_tensor(
   if (it < 0)
     return;
   const Item w = items[it];
-  tensor_tile<L, NTL, CX, CY, IPT, PR, batch_pf<CX>(), XQ, pipe_fits<L, NTL, CX, CY, XQ>(), NSRC>(descs[w.desc],
+  tensor_tile<L, NTL, CX, CY, IPT, PR, BATCH_PF, XQ, pipe_fits<L, NTL, CX, CY, XQ>(), NSRC>(descs[w.desc],
                                                                                             (int)w.tile, smem);
 }
 
