@@ -97,6 +97,40 @@ def test_slim_tiles_match_oracle(cent, op, order, ntl):
     w.free(dev)
 
 
+def test_persistent_phase_runs_in_turn():
+    """A phase whose large-region class runs persistent (the default for
+    regions of 2^20 points and more) and whose thin regions run one
+    workgroup per item runs its launches one after the other by default,
+    and equals the oracle."""
+    from carpet_amd.workloads import ArraySpec, Box, Transfer
+    w = _thin_faces(cg.VERTEX_CENTERED, cg.OP_LAGRANGE, 5)
+    big = Box((2, 2, 2), (105, 105, 98), (1, 1, 1))  # 104 x 104 x 97 > 2^20 points
+    w.arrays["fbig"] = ArraySpec(big.expand((3, 3, 3), (3, 3, 3)), owned=big, seed=990)
+    w.steps.append(Transfer("prolong fbig", "fbig", ["c0", "c1", "c2"], [0.0, -1.0, -2.0], big, big, -0.5,
+                            cg.OP_LAGRANGE, cg.VERTEX_CENTERED, 5, 2, phase=0))
+    w.outputs.append("fbig")
+    ref = run_oracle(w)
+    st = cg.Stream()
+    dev = w.allocate()
+    b = w.batch(dev)
+    b.finalize()
+    d = b.describe()
+    print(d, flush=True)
+    assert re.search(r"\[tensor k0 o5 t3 [^\]]*\] .* wpc 0 .* in turn", d), d
+    assert "side by side" not in d
+    b.run(st)
+    st.synchronize()
+    _compare(_outputs(w, dev), ref, "in turn")
+    b.set_phase_mode(0, 1)
+    assert "side by side" in b.describe()
+    _poison_outputs(w, dev, st)
+    b.run(st)
+    st.synchronize()
+    _compare(_outputs(w, dev), ref, "side by side")
+    b.close()
+    w.free(dev)
+
+
 def _shells():
     # a 2-level vertex hierarchy's step: its boundary prolongation regions are
     # 3- and 6-point shells, which go to the 8x8-base tiles
