@@ -242,10 +242,12 @@ namespace {
 struct Group {
   int phase, is_tensor, kind, order, ntl, op, prec;
   int small; // tensor regions routed to the 8x8-base (2) or 4x16-base (3) tiles, else 0
+  int bigsh; // other tensor regions: the wide (0) or narrow (1) shape, per region
   std::vector<int> members; // transfer indices
   bool same(const Group &g) const {
     return phase == g.phase && is_tensor == g.is_tensor && kind == g.kind &&
-           order == g.order && ntl == g.ntl && op == g.op && prec == g.prec && small == g.small;
+           order == g.order && ntl == g.ntl && op == g.op && prec == g.prec && small == g.small &&
+           bigsh == g.bigsh;
   }
 };
 
@@ -474,6 +476,21 @@ int batch_device_build(BatchDevice &dev, const std::vector<Transfer> &tr,
         tr[i].pr.kind != PK_STAGGER) {
       g.is_tensor = 1;
       g.small = small_tiles(tr[i].pr, cls_ntl[i], prec[i]);
+      // the large shape that keeps fewer lanes busy on this region -- tiles x
+      // (planes + stencil span) x (tile width) x (tile rows + span), the work
+      // of the window loads and of the x- and y-passes -- with a 25% margin
+      // for the wide tile, which is faster per lane on large regions
+      // (single 512^3 patch: narrow 0.81x the busy lanes of wide, yet 4%
+      // slower).  Per region rather than per class: 0.6% faster on 2000
+      // regions of 8-96 points per axis, 1.8% and 1.2% on the level steps
+      // of the 3-level and the 2000-box hierarchies.
+      g.bigsh = 0;
+      if (!g.small) {
+        const int span = line_span(tr[i].pr.kind, tr[i].pr.order);
+        const double b0 = tile_busy(tr[i].pr.t, cls_ntl[i], prec[i], span, 0);
+        const double b1 = tile_busy(tr[i].pr.t, cls_ntl[i], prec[i], span, 1);
+        g.bigsh = b1 < 0.75 * b0 ? 1 : 0;
+      }
       g.kind = tr[i].pr.kind;
       g.order = tr[i].pr.order;
       g.ntl = cls_ntl[i];
@@ -539,7 +556,7 @@ int batch_device_build(BatchDevice &dev, const std::vector<Transfer> &tr,
       median_pts = pts[pts.size() / 2];
       char key[128];
       snprintf(key, sizeof key, "%s k%d o%d t%d p%d op%d h%d n%d",
-               !g.is_tensor ? "point" : g.small == 3 ? "tensor4" : g.small ? "tensor8" : "tensor",
+               !g.is_tensor ? "point" : g.small == 3 ? "tensor4" : g.small ? "tensor8" : g.bigsh == 1 ? "tensorN" : "tensor",
                g.kind, g.order, g.ntl, g.prec, g.op, ilog2(median_pts),
                ilog2((long long)g.members.size()));
       l.key = key;
@@ -574,22 +591,8 @@ int batch_device_build(BatchDevice &dev, const std::vector<Transfer> &tr,
     // hierarchy's shells one workgroup per item takes 0.45 ms less per
     // step for the small tiles and 0.09 ms less for the large ones.)
     l.wpc = g.is_tensor && !g.small && median_pts >= (1LL << 20) ? 0 : -1;
-    // tile shape: the one that keeps fewer lanes busy in total -- tiles x
-    // (planes + stencil span) x (tile width) x (tile rows + span), the work
-    // of the window loads and of the x- and y-passes -- with a 25% margin
-    // for the wide tile, which is faster per lane on large regions (single
-    // 512^3 patch: narrow 0.81x the busy lanes of wide, yet 4% slower)
-    l.shape = 0;
-    if (g.is_tensor && g.small) {
-      l.shape = g.small;
-    } else if (g.is_tensor) {
-      const int span = line_span(g.kind, g.order);
-      double busy[2] = {0, 0};
-      for (int sh = 0; sh < 2; ++sh)
-        for (int m : g.members)
-          busy[sh] += tile_busy(tr[m].pr.t, g.ntl, g.prec, span, sh);
-      l.shape = busy[1] < 0.75 * busy[0] ? 1 : 0;
-    }
+    // tile shape: the class's (its regions were routed by shape)
+    l.shape = !g.is_tensor ? 0 : g.small ? g.small : g.bigsh;
     if (tuned) {
       auto it = tuned->find(l.key);
       if (it != tuned->end()) {
